@@ -1,0 +1,194 @@
+/*
+ * sequoia.h — C ABI of the MI355X (gfx950) BVH build and ray-traversal library.
+ *
+ * Plain C types, host or device pointers and sizes; no framework types.
+ * All functions return SQ_OK (0) or a negative SQ_ERR_* code; the message
+ * of the last failure on the calling thread is available from
+ * sq_last_error(). No function aborts the process.
+ *
+ * Reference interfaces replaced (paths relative to the reference's src/):
+ *   Triangle / Node1 / Hierarchy ........ aabb.h:17-24, 34-42, 197-205
+ *   Ray<1> / HitAttributes<1> ........... ray.h:8-14, 25-30
+ *   Builder::build_hierarchy_* .......... builder.h:81-104 (called from main.cpp:258-277)
+ *   trace / trace_packet / trace_bundle . tracer.h:129-135 (called from main.cpp:279-300)
+ *   traverse ............................ tracer.h:118
+ *   radix_sort / bit_interleave_* ....... algorithms.h:89-116
+ * The C++ header sequoia_compat.h keeps the reference's own signatures on
+ * top of this ABI.
+ */
+#ifndef SEQUOIA_H
+#define SEQUOIA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define SQ_OK 0
+#define SQ_ERR_INVALID (-1)     /* bad argument */
+#define SQ_ERR_HIP (-2)         /* HIP runtime failure (no device, launch failure, ...) */
+#define SQ_ERR_NOMEM (-3)       /* device allocation failed */
+#define SQ_ERR_BUDGET (-4)      /* request exceeds the configured device-memory budget */
+#define SQ_ERR_UNSUPPORTED (-5) /* option combination not implemented */
+#define SQ_ERR_OVERFLOW (-6)    /* a traversal exceeded its stack bound; results invalid */
+
+/* ---- data layouts (byte-identical to the reference) ---- */
+typedef struct { float x, y, z; } sq_vec3;
+/* Triangle (aabb.h:17-24): 36 B, AoS. */
+typedef struct { sq_vec3 v0, v1, v2; } sq_triangle;
+/* Node1 (aabb.h:34-42): 32 B. Leaf: flags=1, child = first triangle index,
+ * count = triangles. Interior: flags=0, child = first of `count` consecutive
+ * child nodes. */
+typedef struct { sq_vec3 min; uint16_t count; uint16_t flags; sq_vec3 max; uint32_t child; } sq_node;
+/* Ray<1> (ray.h:8-14): origin, tmin, direction, tmax; 32 B. */
+typedef struct { sq_vec3 origin; float tmin; sq_vec3 dir; float tmax; } sq_ray;
+
+#define SQ_NODE_LEAF 1u
+#define SQ_MISS 0xFFFFFFFFu
+
+typedef struct sq_ctx sq_ctx;     /* one device + one HIP stream + workspace */
+typedef struct sq_scene sq_scene; /* device-resident triangles */
+typedef struct sq_bvh sq_bvh;     /* device-resident Node1[] built over a scene */
+
+/* ---- context ---- */
+const char* sq_last_error(void);
+const char* sq_version(void);
+/* Creates a context on `device` with its own HIP stream (default priority). */
+int sq_ctx_create(int device, sq_ctx** out);
+void sq_ctx_destroy(sq_ctx* ctx);
+int sq_ctx_sync(sq_ctx* ctx);
+/* Opaque hipStream_t of the context (work of this library is ordered on it). */
+void* sq_ctx_stream(sq_ctx* ctx);
+
+/* Per-phase device times (ms) of the last build/trace on this context,
+ * measured with HIP events recorded on the context stream. */
+typedef struct {
+    float bounds_ms, morton_ms, sort_ms, emit_ms, refit_ms, sah_ms, grid_ms, total_build_ms;
+    float trace_ms;
+    uint64_t node_tests, tri_tests; /* filled when the trace was asked to count work */
+    uint32_t sort_passes, tree_depth, top_levels, small_tasks;
+} sq_timings;
+int sq_get_timings(sq_ctx* ctx, sq_timings* out);
+
+/* ---- scenes ---- */
+int sq_scene_upload(sq_ctx* ctx, const sq_triangle* host_tris, uint32_t n, sq_scene** out);
+/* Wraps caller-owned device memory (not freed by sq_scene_free). */
+int sq_scene_wrap_device(sq_ctx* ctx, const sq_triangle* dev_tris, uint32_t n, sq_scene** out);
+void sq_scene_free(sq_scene* s);
+const sq_triangle* sq_scene_device_ptr(const sq_scene* s);
+uint32_t sq_scene_count(const sq_scene* s);
+
+/* ---- builders (builder.h:81-104) ---- */
+typedef enum {
+    SQ_BUILDER_RECURSIVE = 0,  /* build_hierarchy: binned SAH, Hoare partition, root 2n-1 */
+    SQ_BUILDER_HORIZONTAL = 1, /* build_hierarchy_horizontal: same tree as RECURSIVE */
+    SQ_BUILDER_GRID = 2,       /* build_hierarchy_grid: cell nodes carry the scene box */
+    SQ_BUILDER_GRID_SSE = 3,   /* build_hierarchy_grid_sse: stable partition, tight cell boxes */
+    SQ_BUILDER_GRID_AVX = 4,   /* build_hierarchy_grid_m128: same tree as GRID_SSE */
+    SQ_BUILDER_MORTON = 5,     /* build_hierarchy_morton: 30-bit codes */
+    SQ_BUILDER_MORTON2 = 6,    /* build_hierarchy_morton2: 60-bit codes */
+    SQ_BUILDER_MORTON63 = 7    /* 63-bit codes (21 bits per axis) */
+} sq_builder;
+
+#define SQ_LAYOUT_REFERENCE 0u /* reproduce the reference slot allocator exactly */
+#define SQ_LAYOUT_FIXED 1u     /* give a right singleton child its own slot */
+
+typedef struct {
+    uint32_t morton_bits_per_axis; /* 0 = builder default (10, 20 or 21) */
+    uint32_t radix_digit_bits;     /* 0 = 8 */
+    uint32_t num_bins;             /* 0 = 8 (reference NUM_BINS); 2..64 */
+    uint32_t grid_dim;             /* 0 = 4 (reference GRID_DIM); power of two <= 8 */
+    uint32_t layout;               /* SQ_LAYOUT_* */
+    uint32_t no_refit;             /* Morton builders: 1 = leave node boxes zero like the reference */
+    uint32_t subtree_threshold;    /* 0 = default; largest range one workgroup finishes alone */
+    uint32_t reserved;
+} sq_build_params;
+
+int sq_build(sq_ctx* ctx, const sq_scene* scene, sq_builder kind, const sq_build_params* params, sq_bvh** out);
+/* Wraps a host-built Node1[] (uploaded) so it can be traced on the device. */
+int sq_bvh_from_host(sq_ctx* ctx, const sq_scene* scene, const sq_node* nodes, uint32_t node_count, uint32_t root,
+                     sq_bvh** out);
+int sq_bvh_info(const sq_bvh* bvh, uint32_t* root, uint32_t* node_capacity, uint32_t* depth);
+int sq_bvh_download(const sq_bvh* bvh, sq_node* host_nodes, uint32_t capacity);
+const sq_node* sq_bvh_device_nodes(const sq_bvh* bvh);
+void sq_bvh_free(sq_bvh* bvh);
+
+/* ---- low-level kernels exported for tests and benchmarks ---- */
+/* bit_interleave_32/64 (algorithms.cpp:8-22) + setup_morton quantisation
+ * (builder.cpp:2204-2237): one code per triangle, written to host memory. */
+int sq_morton_codes(sq_ctx* ctx, const sq_scene* scene, uint32_t bits_per_axis, uint64_t* host_codes);
+/* Stable LSD radix sort of (key, value) pairs (algorithms.h:89-113) in host
+ * memory; key_bits selects the number of passes. */
+int sq_radix_sort_pairs(sq_ctx* ctx, uint64_t* host_keys, uint32_t* host_vals, uint32_t n, uint32_t key_bits,
+                        uint32_t digit_bits);
+/* Recomputes every node box of a device tree bottom-up. */
+int sq_refit(sq_ctx* ctx, sq_bvh* bvh);
+
+/* ---- tracing (tracer.h:129-135) ---- */
+/* Pinhole camera. tan_half_fov = 1 reproduces the reference trace() camera
+ * (tracer.cpp:202-216) bit-exactly; jitter = 1 replaces the pixel centre by
+ * hashed sub-pixel offsets; ray (px, py, s) is ray index (py*width+px)*spp+s. */
+typedef struct {
+    sq_vec3 origin;
+    float tan_half_fov;
+    uint32_t width, height, spp, jitter, seed;
+    float tmin, tmax;
+} sq_camera;
+
+typedef enum {
+    SQ_TRACE_EXHAUSTIVE = 0, /* reference semantics: every hit box visited; counts comparable */
+    SQ_TRACE_CLOSEST = 1     /* ordered traversal culling boxes beyond the closest hit */
+} sq_semantics;
+typedef enum {
+    SQ_HEATMAP_RAY = 0,    /* trace(): per-ray node tests */
+    SQ_HEATMAP_PACKET = 1, /* trace_packet<K>, K in {1,2,4,8} */
+    SQ_HEATMAP_BUNDLE = 2  /* trace_bundle<K>, K in {16,64} */
+} sq_heatmap;
+
+typedef struct {
+    uint32_t semantics;   /* sq_semantics */
+    uint32_t heatmap;     /* sq_heatmap */
+    uint32_t width;       /* packet/bundle K */
+    uint32_t block_size;  /* 0 = default; multiple of 64 */
+    uint32_t stack_slots; /* 0 = default; LDS stack entries per lane */
+    uint32_t count_work;  /* 1 = accumulate total node/triangle tests into sq_timings */
+    uint32_t reserved[2];
+} sq_trace_params;
+
+/* Outputs (any may be NULL). hit_id/hit_t: one per ray (SQ_MISS / +inf on
+ * miss). counts: one per pixel (camera) or per ray (ray list), node tests
+ * summed over samples. rgb: width*height*3 heatmap bytes
+ * uint8(min(1, tests/200)*255) (tracer.cpp:222-225). */
+typedef struct {
+    uint32_t* hit_id;
+    float* hit_t;
+    uint32_t* counts;
+    uint8_t* rgb;
+} sq_trace_out;
+
+/* out_on_device = 0: out pointers are host memory; 1: device memory. */
+int sq_trace_camera(sq_ctx* ctx, const sq_bvh* bvh, const sq_camera* cam, const sq_trace_params* params,
+                    const sq_trace_out* out, int out_on_device);
+/* rays: host (io_on_device = 0) or device (1) array of n rays. */
+int sq_trace_rays(sq_ctx* ctx, const sq_bvh* bvh, const sq_ray* rays, uint32_t n, const sq_trace_params* params,
+                  const sq_trace_out* out, int io_on_device);
+
+/* ---- deterministic workload generators (host) ---- */
+uint32_t sq_gen_icosphere_count(uint32_t target);
+int sq_gen_icosphere(uint32_t target, uint32_t seed, float amplitude, sq_triangle* out);
+int sq_gen_soup(uint32_t n, uint32_t seed, float sigma, sq_triangle* out);
+int sq_gen_clusters(uint32_t n, uint32_t clusters, uint32_t seed, sq_triangle* out);
+int sq_gen_secondary_rays(const sq_triangle* tris, uint32_t ntri, uint32_t nrays, uint32_t seed, float tmin,
+                          sq_ray* out);
+/* run()'s camera (main.cpp:311-318). */
+int sq_reference_camera(const sq_triangle* tris, uint32_t n, uint32_t w, uint32_t h, sq_camera* out);
+int sq_pinhole_camera(const sq_triangle* tris, uint32_t n, uint32_t w, uint32_t h, float fov_deg, float distance,
+                      uint32_t spp, uint32_t jitter, uint32_t seed, sq_camera* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,147 @@
+// morton.hip — scene bounds reduction and Morton-code generation.
+//
+// setup_morton / setup_morton2 (builder.cpp:2204-2279): the scene box is the
+// union of per-triangle boxes; each triangle's box centre is normalised to
+// the scene box and truncated to `bits` bits per axis, then interleaved
+// x -> bit 3i, y -> 3i+1, z -> 3i+2 (bit_interleave_32/64, algorithms.cpp:8-22).
+// Keys are written with values = triangle index for the stable sort.
+//
+// Algorithmic traffic: the bounds pass reads 36 B/triangle, the code pass
+// reads 36 B and writes key + value (8 B for 30-bit, 12 B for wider codes).
+#include "internal.h"
+
+namespace sq {
+
+constexpr int kBoundsBlock = 256;
+
+// Block-reduces the six box extremes into ordered-uint atomics in ord6[0..5]
+// (min x,y,z then max x,y,z). ord6 must be pre-set to (UINT_MAX x3, 0 x3).
+__global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle* __restrict__ tris, uint32_t n,
+                                                                uint32_t* __restrict__ ord6) {
+    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    uint32_t hi[3] = {0u, 0u, 0u};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float v[9];
+        sqd::load_tri(tris, i, v);
+        sqd::Box3 b = sqd::tri_box(v);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            uint32_t a = sqd::f2ord(b.mn[k]), c = sqd::f2ord(b.mx[k]);
+            lo[k] = a < lo[k] ? a : lo[k];
+            hi[k] = c > hi[k] ? c : hi[k];
+        }
+    }
+    // wave reduction
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            uint32_t a = __shfl_xor(lo[k], off, 64), c = __shfl_xor(hi[k], off, 64);
+            lo[k] = a < lo[k] ? a : lo[k];
+            hi[k] = c > hi[k] ? c : hi[k];
+        }
+    }
+    __shared__ uint32_t s[6][kBoundsBlock / 64];
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        for (int k = 0; k < 3; k++) { s[k][wave] = lo[k]; s[3 + k][wave] = hi[k]; }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        uint32_t r = s[k][0];
+        for (int w = 1; w < kBoundsBlock / 64; w++) r = (k < 3) ? (s[k][w] < r ? s[k][w] : r) : (s[k][w] > r ? s[k][w] : r);
+        if (k < 3) atomicMin(&ord6[k], r);
+        else atomicMax(&ord6[k], r);
+    }
+}
+
+__global__ void k_bounds_init(uint32_t* ord6) {
+    int k = threadIdx.x;
+    if (k < 6) ord6[k] = k < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+int scene_bounds(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t* d_ord6) {
+    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, c->stream, d_ord6);
+    uint32_t blocks = (n + kBoundsBlock - 1) / kBoundsBlock;
+    uint32_t cap = (uint32_t)c->num_cus * 8;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_scene_bounds, dim3(blocks), dim3(kBoundsBlock), 0, c->stream, tris, n, d_ord6);
+    SQ_CHECK_HIP(hipGetLastError());
+    return SQ_OK;
+}
+
+// Spread the low 10 bits of x to bits 0,3,...,27.
+__device__ __forceinline__ uint32_t expand10(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+// Spread the low 21 bits of x to bits 0,3,...,60.
+__device__ __forceinline__ uint64_t expand21(uint64_t x) {
+    x &= 0x1FFFFFull;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// Truncating float -> unsigned as the reference's x86-64 build does for the
+// values the quantiser produces; NaN (zero-extent axis) -> 0.
+__device__ __forceinline__ uint32_t quantise(float v) {
+    if (!(v == v) || v <= 0.0f) return 0u;
+    return (uint32_t)v;
+}
+
+template <bool KEY64>
+__global__ __launch_bounds__(256) void k_morton(const sq_triangle* __restrict__ tris, uint32_t n, uint32_t bits,
+                                                const uint32_t* __restrict__ ord6, void* __restrict__ keys_out,
+                                                uint32_t* __restrict__ vals_out) {
+    float smn[3], ext[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        smn[k] = sqd::ord2f(ord6[k]);
+        ext[k] = sqd::ord2f(ord6[3 + k]) - smn[k];
+    }
+    const float scale = (float)((1u << bits) - 1u);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float v[9];
+        sqd::load_tri(tris, i, v);
+        sqd::Box3 b = sqd::tri_box(v);
+        uint32_t q[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float cc = (b.mn[k] + b.mx[k]) * 0.5f;
+            cc = (cc - smn[k]) / ext[k];
+            q[k] = quantise(cc * scale);
+        }
+        if (KEY64) {
+            uint64_t code = expand21(q[0]) | (expand21(q[1]) << 1) | (expand21(q[2]) << 2);
+            ((uint64_t*)keys_out)[i] = code;
+        } else {
+            uint32_t code = expand10(q[0]) | (expand10(q[1]) << 1) | (expand10(q[2]) << 2);
+            ((uint32_t*)keys_out)[i] = code;
+        }
+        vals_out[i] = i;
+    }
+}
+
+int morton_codes(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t bits, const uint32_t* d_ord6, void* d_keys,
+                 uint32_t* d_vals, bool key64) {
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks == 0) blocks = 1;
+    if (key64)
+        hipLaunchKernelGGL(k_morton<true>, dim3(blocks), dim3(256), 0, c->stream, tris, n, bits, d_ord6, d_keys, d_vals);
+    else
+        hipLaunchKernelGGL(k_morton<false>, dim3(blocks), dim3(256), 0, c->stream, tris, n, bits, d_ord6, d_keys, d_vals);
+    SQ_CHECK_HIP(hipGetLastError());
+    return SQ_OK;
+}
+
+}  // namespace sq
