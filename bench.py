@@ -1,0 +1,174 @@
+"""Benchmark of the BVH build + traversal hot path on one MI355X.
+
+Workload (the plain GPU path): a 1,000,000-triangle fp32 soup (centroids
+U[0,1)^3, Gaussian edges sigma 0.01, seed 2) built as a 63-bit Morton LBVH,
+then 1920x1080 primary rays at 4 jittered samples per pixel (8,294,400 rays)
+from a 60-degree pinhole camera at distance 3, traced with the reference's
+exhaustive semantics (per-pixel node-visit heatmap + closest hit per ray).
+
+One step = build (Morton codes, radix sort, emission, refit) + trace, inputs
+resident in HBM. `value` is Mrays/s of the trace (HIP events on the library
+stream around the traversal kernel, summed over the timed steps).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_TRI = 1_000_000
+W, H, SPP = 1920, 1080, 4
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(tris, cam, gpu_rays_per_s, budget_s):
+    """Times the reference build + traverse on a bounded sample (host cores)."""
+    from oracle import binding as ob
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    kind = "reference" if ob.reference_available() else "port"
+    t0 = time.perf_counter()
+    if kind == "reference":
+        nodes, root = ob.ref_build(tris, "recursive")
+    else:
+        nodes, root = ob.build_sah(tris, 8)
+    build_s = time.perf_counter() - t0
+    rays = ob.camera_rays(cam)
+    # bounded sample: every k-th ray, sized from a small probe to ~budget_s
+    probe = rays[:: max(1, len(rays) // 20000)]
+    t0 = time.perf_counter()
+    if kind == "reference":
+        ob.ref_traverse_rays(nodes, root, tris, probe)
+    else:
+        ob.trace_rays(nodes, root, tris, probe)
+    probe_s = max(time.perf_counter() - t0, 1e-6)
+    per_ray = probe_s / len(probe)
+    n_sample = int(min(len(rays), max(len(probe), budget_s / per_ray)))
+    stride = max(1, len(rays) // n_sample)
+    sample = rays[::stride]
+    t0 = time.perf_counter()
+    if kind == "reference":
+        ob.ref_traverse_rays(nodes, root, tris, sample)
+    else:
+        ob.trace_rays(nodes, root, tris, sample)
+    trace_s = time.perf_counter() - t0
+    mrays = len(sample) / trace_s / 1e6
+    return {
+        "value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": kind,
+        "sample": "%d of %d config rays (every %d-th) traced exhaustively on the reference's 8-bin recursive SAH "
+                  "tree of the same 1M soup, %.1f s; that build took %.2f s (%.2f Mtri/s)"
+                  % (len(sample), len(rays), stride, trace_s, build_s, len(tris) / build_s / 1e6),
+        "build_mtri_s": round(len(tris) / build_s / 1e6, 3),
+        "gpu_over_cpu": round(gpu_rays_per_s / 1e6 / mrays, 1) if mrays > 0 else None,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--semantics", default="exhaustive", choices=["exhaustive", "closest"])
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import sequoia_amd as sq
+    log("generating %d-triangle soup" % N_TRI)
+    tris = sq.gen_soup(N_TRI, 2, 0.01)
+    cam = sq.pinhole_camera(tris, W, H, 60.0, 3.0, spp=SPP, jitter=1, seed=2)
+    nrays = W * H * SPP
+    ctx = sq.Context(0)
+    scene = ctx.upload(tris)
+    d_id = ctx.alloc(nrays * 4)
+    d_t = ctx.alloc(nrays * 4)
+    d_cnt = ctx.alloc(W * H * 4)
+    d_rgb = ctx.alloc(W * H * 3)
+    sem = sq.TRACE_CLOSEST if a.semantics == "closest" else sq.TRACE_EXHAUSTIVE
+
+    def step(count_work=False):
+        bvh = ctx.build(scene, "morton63")
+        bt = ctx.timings()
+        ctx.trace_camera_device(bvh, cam, d_id, d_t, d_cnt, d_rgb, semantics=sem, count_work=count_work)
+        tt = ctx.timings()
+        bvh.free()
+        return bt, tt
+
+    for i in range(a.warmup):
+        log("warmup", i)
+        step()
+    # work counters for the roofline (separate pass, outside the timed region)
+    _, tw = step(count_work=True)
+    node_tests, tri_tests = tw["node_tests"], tw["tri_tests"]
+
+    ctx.sync()
+    t0 = time.perf_counter()
+    builds, traces = [], []
+    for i in range(a.steps):
+        bt, tt = step()
+        builds.append(bt["total_build_ms"])
+        traces.append(tt["trace_ms"])
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    trace_ms = float(np.mean(traces))
+    build_ms = float(np.mean(builds))
+    rays_per_s = nrays / (trace_ms / 1e3)
+    alg_bytes = 32.0 * node_tests + 36.0 * tri_tests
+    achieved = alg_bytes / (trace_ms / 1e3) / 1e9
+
+    heat = ctx.to_host(d_cnt, W * H * 4, np.uint32)
+    line = {
+        "metric": "Mrays/s & build Mtri/s on 1 GPU; served p50/p99 latency, batched vs plain",
+        "value": round(rays_per_s / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall * 1e3 / a.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain GPU path: 1M-triangle soup (U[0,1)^3 centroids, N(0,0.01^2) edges, seed 2), "
+                        "63-bit Morton LBVH, 1920x1080x4 jittered primary rays (8,294,400), 60-deg pinhole at "
+                        "distance 3, exhaustive reference semantics with per-pixel heatmap",
+            "triangles": N_TRI, "rays": nrays, "builder": "morton63", "semantics": a.semantics,
+        },
+        "build_mtri_s": round(N_TRI / (build_ms / 1e3) / 1e6, 2),
+        "build_ms": round(build_ms, 3),
+        "trace_ms": round(trace_ms, 3),
+        "node_tests_per_ray": round(node_tests / nrays, 2),
+        "tri_tests_per_ray": round(tri_tests / nrays, 2),
+        "heatmap_mean_tests_per_pixel": round(float(heat.mean()) / SPP, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_trace<false>", "algorithmic_bytes_per_launch": alg_bytes},
+    }
+    if not a.no_cpu:
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(tris, cam, rays_per_s, a.cpu_budget)
+    for p in (d_id, d_t, d_cnt, d_rgb):
+        ctx.free(p)
+    scene.free()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

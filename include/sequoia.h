@@ -73,6 +73,12 @@ typedef struct {
 } sq_timings;
 int sq_get_timings(sq_ctx* ctx, sq_timings* out);
 
+/* ---- device memory for callers without their own allocator ---- */
+int sq_device_alloc(sq_ctx* ctx, size_t bytes, void** out);
+void sq_device_free(sq_ctx* ctx, void* ptr);
+/* Copies device -> host on the context stream and waits. */
+int sq_copy_to_host(sq_ctx* ctx, void* host_dst, const void* dev_src, size_t bytes);
+
 /* ---- scenes ---- */
 int sq_scene_upload(sq_ctx* ctx, const sq_triangle* host_tris, uint32_t n, sq_scene** out);
 /* Wraps caller-owned device memory (not freed by sq_scene_free). */

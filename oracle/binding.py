@@ -1,0 +1,270 @@
+"""ctypes binding of the CPU oracle (liboracle.so) and of the reference
+library built from the reference sources (_ref/libseqref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke entry and the
+bench's CPU-baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libseqref.so")
+
+NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+
+LAYOUT_REFERENCE, LAYOUT_FIXED = 0, 1
+PART_HOARE, PART_STABLE = 0, 1
+SA_SCALAR, SA_SSE, SA_M128 = 0, 1, 2
+CELLBOX_SCENE, CELLBOX_TIGHT = 0, 1
+
+
+class Camera(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("tan_half_fov", C.c_float), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("spp", C.c_uint32), ("jitter", C.c_uint32), ("seed", C.c_uint32),
+                ("tmin", C.c_float), ("tmax", C.c_float)]
+
+
+def camera_from(cam):
+    """Converts a sequoia_amd.Camera (same field order) to the oracle's struct."""
+    o = Camera()
+    o.origin[0], o.origin[1], o.origin[2] = cam.origin.x, cam.origin.y, cam.origin.z
+    for f in ("tan_half_fov", "width", "height", "spp", "jitter", "seed", "tmin", "tmax"):
+        setattr(o, f, getattr(cam, f))
+    return o
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+_o = None
+_r = None
+
+
+def oracle():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: make -C oracle")
+        _o = C.CDLL(ORACLE_SO)
+        V, U = C.c_void_p, C.c_uint32
+        sig = {
+            "sqo_interleave32": (U, [U, U, U]),
+            "sqo_interleave64": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
+            "sqo_scene_bounds": (None, [V, U, V, V]),
+            "sqo_morton_codes": (None, [V, U, U, V]),
+            "sqo_radix_sort32": (None, [V, V, U, U]),
+            "sqo_radix_sort64": (None, [V, V, U, U, U]),
+            "sqo_find_split": (U, [V, U]),
+            "sqo_emit_morton": (U, [V, V, U, U, V]),
+            "sqo_refit": (None, [V, U, U, V]),
+            "sqo_build_morton": (U, [V, U, U, U, U, V, V, V]),
+            "sqo_build_sah": (U, [V, U, U, U, U, U, V]),
+            "sqo_build_grid": (U, [V, U, U, U, U, U, U, U, V]),
+            "sqo_camera_ray": (None, [C.POINTER(Camera), U, U, U, V]),
+            "sqo_camera_rays": (None, [C.POINTER(Camera), V]),
+            "sqo_slab": (C.c_int, [V, V]),
+            "sqo_tri_test": (C.c_int, [V, V, V]),
+            "sqo_trace_rays": (None, [V, U, V, V, U, V, V, V, V]),
+            "sqo_trace_rays_closest": (None, [V, U, V, V, U, V, V]),
+            "sqo_trace_camera": (None, [V, U, V, C.POINTER(Camera), V, V, V, V]),
+            "sqo_trace_packet": (None, [V, U, V, C.POINTER(Camera), U, V, V]),
+            "sqo_trace_bundle": (None, [V, U, V, C.POINTER(Camera), U, V]),
+            "sqo_heat_byte": (C.c_uint8, [U, U]),
+        }
+        for k, (res, args) in sig.items():
+            f = getattr(_o, k)
+            f.restype, f.argtypes = res, args
+    return _o
+
+
+def reference_available():
+    return os.path.exists(REF_SO)
+
+
+def reference():
+    global _r
+    if _r is None:
+        if not reference_available():
+            raise RuntimeError("reference library not built (make -C oracle ref)")
+        _r = C.CDLL(REF_SO)
+        V, U = C.c_void_p, C.c_uint32
+        sig = {
+            "ref_sizeof_node": (U, []), "ref_sizeof_triangle": (U, []),
+            "ref_interleave32": (U, [U, U, U]),
+            "ref_interleave64": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
+            "ref_build": (C.c_int, [C.c_int, V, U, V, C.POINTER(U)]),
+            "ref_morton32": (None, [V, U, V, V, V]),
+            "ref_morton64": (None, [V, U, V, V, V]),
+            "ref_radix_sort32": (None, [V, U, V, V]),
+            "ref_trace": (C.c_int, [C.c_int, C.c_int, V, U, V, U, U, U, V, V]),
+            "ref_traverse_rays": (None, [V, U, V, U, V, U, V, V]),
+        }
+        for k, (res, args) in sig.items():
+            f = getattr(_r, k)
+            f.restype, f.argtypes = res, args
+    return _r
+
+
+# ---------------- oracle helpers ----------------
+def tris32(t):
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    assert t.ndim == 2 and t.shape[1] == 9
+    return t
+
+
+def morton_codes(tris, bits):
+    tris = tris32(tris)
+    out = np.empty(len(tris), np.uint64)
+    oracle().sqo_morton_codes(_p(tris), len(tris), bits, _p(out))
+    return out
+
+
+def radix_sort64(keys, vals, key_bits, digit_bits=8):
+    k = np.ascontiguousarray(keys, np.uint64).copy()
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    oracle().sqo_radix_sort64(_p(k), _p(v), len(k), digit_bits, key_bits)
+    return k, v
+
+
+def find_split(codes):
+    c = np.ascontiguousarray(codes, np.uint64)
+    return oracle().sqo_find_split(_p(c), len(c))
+
+
+def build_morton(tris, bits, layout=LAYOUT_REFERENCE, refit=True):
+    tris = tris32(tris)
+    n = len(tris)
+    nodes = np.zeros(2 * n, NODE_DTYPE)
+    codes = np.empty(n, np.uint64)
+    idx = np.empty(n, np.uint32)
+    root = oracle().sqo_build_morton(_p(tris), n, bits, layout, 1 if refit else 0, _p(nodes), _p(codes), _p(idx))
+    return nodes, root, codes, idx
+
+
+def build_sah(tris, num_bins=8, partition=PART_HOARE, sa_kind=SA_SCALAR, layout=LAYOUT_REFERENCE):
+    tris = tris32(tris)
+    n = len(tris)
+    nodes = np.zeros(2 * n, NODE_DTYPE)
+    root = oracle().sqo_build_sah(_p(tris), n, num_bins, partition, sa_kind, layout, _p(nodes))
+    return nodes, root
+
+
+def build_grid(tris, grid_dim=4, num_bins=8, partition=PART_STABLE, sa_kind=SA_SSE, cell_box=CELLBOX_TIGHT,
+               layout=LAYOUT_REFERENCE):
+    tris = tris32(tris)
+    n = len(tris)
+    nodes = np.zeros(2 * n + 2 * grid_dim ** 3, NODE_DTYPE)
+    root = oracle().sqo_build_grid(_p(tris), n, grid_dim, num_bins, partition, sa_kind, cell_box, layout, _p(nodes))
+    return nodes, root
+
+
+def refit(nodes, root, tris):
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE).copy()
+    tris = tris32(tris)
+    oracle().sqo_refit(_p(nodes), len(nodes), root, _p(tris))
+    return nodes
+
+
+def camera_rays(cam):
+    oc = camera_from(cam) if not isinstance(cam, Camera) else cam
+    out = np.empty(oc.width * oc.height * oc.spp, RAY_DTYPE)
+    oracle().sqo_camera_rays(C.byref(oc), _p(out))
+    return out
+
+
+def trace_rays(nodes, root, tris, rays):
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    tris = tris32(tris)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = len(rays)
+    counts = np.empty(n, np.uint32)
+    hid = np.empty(n, np.uint32)
+    ht = np.empty(n, np.float32)
+    anyh = np.empty(n, np.uint32)
+    oracle().sqo_trace_rays(_p(nodes), root, _p(tris), _p(rays), n, _p(counts), _p(hid), _p(ht), _p(anyh))
+    return {"counts": counts, "hit_id": hid, "hit_t": ht, "any_hit": anyh}
+
+
+def trace_camera(nodes, root, tris, cam):
+    oc = camera_from(cam) if not isinstance(cam, Camera) else cam
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    tris = tris32(tris)
+    npix = oc.width * oc.height
+    counts = np.empty(npix, np.uint32)
+    rgb = np.empty((oc.height, oc.width, 3), np.uint8)
+    hid = np.empty(npix * oc.spp, np.uint32)
+    ht = np.empty(npix * oc.spp, np.float32)
+    oracle().sqo_trace_camera(_p(nodes), root, _p(tris), C.byref(oc), _p(counts), _p(rgb), _p(hid), _p(ht))
+    return {"counts": counts, "rgb": rgb, "hit_id": hid, "hit_t": ht}
+
+
+def trace_packet(nodes, root, tris, cam, K):
+    oc = camera_from(cam) if not isinstance(cam, Camera) else cam
+    rgb = np.empty((oc.height, oc.width, 3), np.uint8)
+    oracle().sqo_trace_packet(_p(np.ascontiguousarray(nodes, NODE_DTYPE)), root, _p(tris32(tris)), C.byref(oc), K,
+                              None, _p(rgb))
+    return rgb
+
+
+def trace_bundle(nodes, root, tris, cam, K):
+    oc = camera_from(cam) if not isinstance(cam, Camera) else cam
+    rgb = np.empty((oc.height, oc.width, 3), np.uint8)
+    oracle().sqo_trace_bundle(_p(np.ascontiguousarray(nodes, NODE_DTYPE)), root, _p(tris32(tris)), C.byref(oc), K,
+                              _p(rgb))
+    return rgb
+
+
+# ---------------- reference helpers ----------------
+REF_KINDS = {"recursive": 0, "horizontal": 1, "grid": 2, "grid_sse": 3, "grid_avx": 4, "morton": 5, "morton2": 6}
+
+
+def ref_build(tris, kind):
+    tris = tris32(tris)
+    n = len(tris)
+    cap = 2 * n + (128 if kind.startswith("grid") else 0)
+    nodes = np.zeros(cap, NODE_DTYPE)
+    root = C.c_uint32()
+    rc = reference().ref_build(REF_KINDS[kind], _p(tris), n, _p(nodes), C.byref(root))
+    assert rc == 0
+    return nodes, root.value
+
+
+def ref_morton32(tris):
+    tris = tris32(tris)
+    n = len(tris)
+    cu, c, i = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.uint32)
+    reference().ref_morton32(_p(tris), n, _p(cu), _p(c), _p(i))
+    return cu, c, i
+
+
+def ref_morton64(tris):
+    tris = tris32(tris)
+    n = len(tris)
+    cu, c, i = np.empty(n, np.uint64), np.empty(n, np.uint64), np.empty(n, np.uint32)
+    reference().ref_morton64(_p(tris), n, _p(cu), _p(c), _p(i))
+    return cu, c, i
+
+
+def ref_trace(mode, K, nodes, root, tris, w, h, origin):
+    tris = tris32(tris)
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE).copy()
+    rgb = np.empty((h, w, 3), np.uint8)
+    o = np.asarray(origin, np.float32)
+    rc = reference().ref_trace(mode, K, _p(nodes), root, _p(tris), len(tris), w, h, _p(o), _p(rgb))
+    assert rc == 0
+    return rgb
+
+
+def ref_traverse_rays(nodes, root, tris, rays):
+    tris = tris32(tris)
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE).copy()
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = len(rays)
+    counts, hit = np.empty(n, np.uint32), np.empty(n, np.uint32)
+    reference().ref_traverse_rays(_p(nodes), root, _p(tris), len(tris), _p(rays), n, _p(counts), _p(hit))
+    return counts, hit

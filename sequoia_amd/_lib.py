@@ -1,0 +1,354 @@
+"""ctypes binding of libsequoia_amd.so (the C ABI declared in include/sequoia.h).
+
+The library is the product: if it is missing or fails to load, every entry
+point raises. There is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsequoia_amd.so")
+
+SQ_OK = 0
+ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "BUDGET", -5: "UNSUPPORTED", -6: "OVERFLOW"}
+
+BUILDERS = {
+    "recursive": 0, "horizontal": 1, "grid": 2, "grid_sse": 3, "grid_avx": 4,
+    "morton": 5, "morton2": 6, "morton63": 7,
+}
+LAYOUT_REFERENCE, LAYOUT_FIXED = 0, 1
+TRACE_EXHAUSTIVE, TRACE_CLOSEST = 0, 1
+HEATMAP_RAY, HEATMAP_PACKET, HEATMAP_BUNDLE = 0, 1, 2
+MISS = 0xFFFFFFFF
+
+NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+assert NODE_DTYPE.itemsize == 32 and RAY_DTYPE.itemsize == 32
+
+
+class SqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sequoia error %d (%s): %s" % (code, ERRORS.get(code, "?"), msg))
+        self.code = code
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("origin", Vec3), ("tan_half_fov", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("spp", C.c_uint32), ("jitter", C.c_uint32), ("seed", C.c_uint32), ("tmin", C.c_float),
+                ("tmax", C.c_float)]
+
+
+class BuildParams(C.Structure):
+    _fields_ = [("morton_bits_per_axis", C.c_uint32), ("radix_digit_bits", C.c_uint32), ("num_bins", C.c_uint32),
+                ("grid_dim", C.c_uint32), ("layout", C.c_uint32), ("no_refit", C.c_uint32),
+                ("subtree_threshold", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class TraceParams(C.Structure):
+    _fields_ = [("semantics", C.c_uint32), ("heatmap", C.c_uint32), ("width", C.c_uint32),
+                ("block_size", C.c_uint32), ("stack_slots", C.c_uint32), ("count_work", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class TraceOut(C.Structure):
+    _fields_ = [("hit_id", C.c_void_p), ("hit_t", C.c_void_p), ("counts", C.c_void_p), ("rgb", C.c_void_p)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("bounds_ms", C.c_float), ("morton_ms", C.c_float), ("sort_ms", C.c_float), ("emit_ms", C.c_float),
+                ("refit_ms", C.c_float), ("sah_ms", C.c_float), ("grid_ms", C.c_float),
+                ("total_build_ms", C.c_float), ("trace_ms", C.c_float), ("node_tests", C.c_uint64),
+                ("tri_tests", C.c_uint64), ("sort_passes", C.c_uint32), ("tree_depth", C.c_uint32),
+                ("top_levels", C.c_uint32), ("small_tasks", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_SIGS = {
+    "sq_last_error": (C.c_char_p, []),
+    "sq_version": (C.c_char_p, []),
+    "sq_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "sq_ctx_destroy": (None, [_P]),
+    "sq_ctx_sync": (C.c_int, [_P]),
+    "sq_ctx_stream": (_P, [_P]),
+    "sq_get_timings": (C.c_int, [_P, C.POINTER(Timings)]),
+    "sq_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "sq_device_free": (None, [_P, _P]),
+    "sq_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "sq_scene_upload": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
+    "sq_scene_wrap_device": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
+    "sq_scene_free": (None, [_P]),
+    "sq_scene_device_ptr": (_P, [_P]),
+    "sq_scene_count": (_U32, [_P]),
+    "sq_build": (C.c_int, [_P, _P, C.c_int, C.POINTER(BuildParams), C.POINTER(_P)]),
+    "sq_bvh_from_host": (C.c_int, [_P, _P, _P, _U32, _U32, C.POINTER(_P)]),
+    "sq_bvh_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32)]),
+    "sq_bvh_download": (C.c_int, [_P, _P, _U32]),
+    "sq_bvh_device_nodes": (_P, [_P]),
+    "sq_bvh_free": (None, [_P]),
+    "sq_morton_codes": (C.c_int, [_P, _P, _U32, _P]),
+    "sq_radix_sort_pairs": (C.c_int, [_P, _P, _P, _U32, _U32, _U32]),
+    "sq_refit": (C.c_int, [_P, _P]),
+    "sq_trace_camera": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
+    "sq_trace_rays": (C.c_int, [_P, _P, _P, _U32, C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
+    "sq_gen_icosphere_count": (_U32, [_U32]),
+    "sq_gen_icosphere": (C.c_int, [_U32, _U32, C.c_float, _P]),
+    "sq_gen_soup": (C.c_int, [_U32, _U32, C.c_float, _P]),
+    "sq_gen_clusters": (C.c_int, [_U32, _U32, _U32, _P]),
+    "sq_gen_secondary_rays": (C.c_int, [_P, _U32, _U32, _U32, C.c_float, _P]),
+    "sq_reference_camera": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Camera)]),
+    "sq_pinhole_camera": (C.c_int, [_P, _U32, _U32, _U32, C.c_float, C.c_float, _U32, _U32, _U32,
+                                    C.POINTER(Camera)]),
+}
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsequoia_amd.so is not built: run python -m sequoia_amd.build")
+        l = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGS)
+
+
+def _check(rc):
+    if rc != SQ_OK:
+        raise SqError(rc, lib().sq_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def as_tris(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    assert a.ndim == 2 and a.shape[1] == 9, a.shape
+    return a
+
+
+# ---- generators (host) ----
+def gen_icosphere(target, seed=1, amplitude=0.1):
+    n = lib().sq_gen_icosphere_count(target)
+    out = np.empty((n, 9), np.float32)
+    r = lib().sq_gen_icosphere(target, seed, amplitude, _ptr(out))
+    if r < 0:
+        _check(r)
+    return out
+
+
+def gen_soup(n, seed=2, sigma=0.01):
+    out = np.empty((n, 9), np.float32)
+    r = lib().sq_gen_soup(n, seed, sigma, _ptr(out))
+    if r < 0:
+        _check(r)
+    return out
+
+
+def gen_clusters(n, clusters=64, seed=3):
+    out = np.empty((n, 9), np.float32)
+    r = lib().sq_gen_clusters(n, clusters, seed, _ptr(out))
+    if r < 0:
+        _check(r)
+    return out
+
+
+def gen_secondary_rays(tris, nrays, seed=3, tmin=1e-4):
+    tris = as_tris(tris)
+    out = np.empty(nrays, RAY_DTYPE)
+    r = lib().sq_gen_secondary_rays(_ptr(tris), len(tris), nrays, seed, tmin, _ptr(out))
+    if r < 0:
+        _check(r)
+    return out
+
+
+def reference_camera(tris, w, h):
+    tris = as_tris(tris)
+    cam = Camera()
+    _check(lib().sq_reference_camera(_ptr(tris), len(tris), w, h, C.byref(cam)))
+    return cam
+
+
+def pinhole_camera(tris, w, h, fov_deg=60.0, distance=3.0, spp=1, jitter=0, seed=0):
+    tris = as_tris(tris)
+    cam = Camera()
+    _check(lib().sq_pinhole_camera(_ptr(tris), len(tris), w, h, fov_deg, distance, spp, jitter, seed, C.byref(cam)))
+    return cam
+
+
+# ---- device objects ----
+class Scene:
+    def __init__(self, ctx, handle, n, keep=None):
+        self.ctx, self.h, self.n, self._keep = ctx, handle, n, keep
+
+    def free(self):
+        if self.h:
+            lib().sq_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Bvh:
+    def __init__(self, ctx, handle, scene):
+        self.ctx, self.h, self.scene = ctx, handle, scene
+
+    def info(self):
+        r, cap, d = _U32(), _U32(), _U32()
+        _check(lib().sq_bvh_info(self.h, C.byref(r), C.byref(cap), C.byref(d)))
+        return r.value, cap.value, d.value
+
+    @property
+    def root(self):
+        return self.info()[0]
+
+    def download(self):
+        _, cap, _ = self.info()
+        out = np.empty(cap, NODE_DTYPE)
+        _check(lib().sq_bvh_download(self.h, _ptr(out), cap))
+        return out
+
+    def free(self):
+        if self.h:
+            lib().sq_bvh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().sq_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().sq_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        _check(lib().sq_ctx_sync(self.h))
+
+    def timings(self):
+        t = Timings()
+        _check(lib().sq_get_timings(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        _check(lib().sq_device_alloc(self.h, nbytes, C.byref(p)))
+        return p
+
+    def free(self, p):
+        lib().sq_device_free(self.h, p)
+
+    def to_host(self, dev, nbytes, dtype):
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+        _check(lib().sq_copy_to_host(self.h, _ptr(out), dev, nbytes))
+        return out
+
+    def trace_camera_device(self, bvh, cam, hit_id=None, hit_t=None, counts=None, rgb=None,
+                            semantics=TRACE_EXHAUSTIVE, stack_slots=0, count_work=False):
+        """Trace with outputs already in device memory (no host copies)."""
+        out = TraceOut(hit_id, hit_t, counts, rgb)
+        p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work)
+        _check(lib().sq_trace_camera(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 1))
+
+    def upload(self, tris):
+        tris = as_tris(tris)
+        h = C.c_void_p()
+        _check(lib().sq_scene_upload(self.h, _ptr(tris), len(tris), C.byref(h)))
+        return Scene(self, h, len(tris))
+
+    def build(self, scene, kind="morton", morton_bits=0, digit_bits=0, num_bins=0, grid_dim=0,
+              layout=LAYOUT_REFERENCE, refit=True, subtree_threshold=0):
+        p = BuildParams(morton_bits, digit_bits, num_bins, grid_dim, layout, 0 if refit else 1, subtree_threshold, 0)
+        h = C.c_void_p()
+        _check(lib().sq_build(self.h, scene.h, BUILDERS[kind] if isinstance(kind, str) else kind, C.byref(p),
+                              C.byref(h)))
+        return Bvh(self, h, scene)
+
+    def bvh_from_host(self, scene, nodes, root):
+        nodes = np.ascontiguousarray(nodes, dtype=NODE_DTYPE)
+        h = C.c_void_p()
+        _check(lib().sq_bvh_from_host(self.h, scene.h, _ptr(nodes), len(nodes), root, C.byref(h)))
+        return Bvh(self, h, scene)
+
+    def morton_codes(self, scene, bits_per_axis):
+        out = np.empty(scene.n, np.uint64)
+        _check(lib().sq_morton_codes(self.h, scene.h, bits_per_axis, _ptr(out)))
+        return out
+
+    def radix_sort(self, keys, vals, key_bits, digit_bits=8):
+        k = np.ascontiguousarray(keys, dtype=np.uint64).copy()
+        v = np.ascontiguousarray(vals, dtype=np.uint32).copy()
+        _check(lib().sq_radix_sort_pairs(self.h, _ptr(k), _ptr(v), len(k), key_bits, digit_bits))
+        return k, v
+
+    def refit(self, bvh):
+        _check(lib().sq_refit(self.h, bvh.h))
+
+    @staticmethod
+    def _params(semantics, heatmap, width, stack_slots, count_work):
+        return TraceParams(semantics, heatmap, width, 0, stack_slots, 1 if count_work else 0, (C.c_uint32 * 2)())
+
+    def trace_camera(self, bvh, cam, semantics=TRACE_EXHAUSTIVE, heatmap=HEATMAP_RAY, width=0, want_hits=True,
+                     want_counts=True, want_rgb=True, stack_slots=0, count_work=False):
+        npix = cam.width * cam.height
+        nray = npix * cam.spp
+        res = {}
+        hid = np.empty(nray, np.uint32) if want_hits else None
+        ht = np.empty(nray, np.float32) if want_hits else None
+        cnt = np.empty(npix, np.uint32) if want_counts else None
+        rgb = np.empty((cam.height, cam.width, 3), np.uint8) if want_rgb else None
+        out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), _ptr(rgb))
+        p = self._params(semantics, heatmap, width, stack_slots, count_work)
+        _check(lib().sq_trace_camera(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 0))
+        res.update(hit_id=hid, hit_t=ht, counts=cnt, rgb=rgb)
+        return res
+
+    def trace_rays(self, bvh, rays, semantics=TRACE_EXHAUSTIVE, want_counts=True, stack_slots=0, count_work=False):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        n = len(rays)
+        hid = np.empty(n, np.uint32)
+        ht = np.empty(n, np.float32)
+        cnt = np.empty(n, np.uint32) if want_counts else None
+        out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), None)
+        p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work)
+        _check(lib().sq_trace_rays(self.h, bvh.h, _ptr(rays), n, C.byref(p), C.byref(out), 0))
+        return {"hit_id": hid, "hit_t": ht, "counts": cnt}

@@ -156,6 +156,26 @@ int sq_get_timings(sq_ctx* c, sq_timings* out) {
     return SQ_OK;
 }
 
+int sq_device_alloc(sq_ctx* c, size_t bytes, void** out) {
+    if (!c || !out) SQ_RET(SQ_ERR_INVALID, "sq_device_alloc: NULL argument");
+    *out = nullptr;
+    SQ_CHECK_HIP(hipSetDevice(c->device));
+    SQ_CHECK_HIP(hipMalloc(out, bytes ? bytes : 1));
+    return SQ_OK;
+}
+
+void sq_device_free(sq_ctx* c, void* p) {
+    if (c) (void)hipSetDevice(c->device);
+    if (p) (void)hipFree(p);
+}
+
+int sq_copy_to_host(sq_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src) SQ_RET(SQ_ERR_INVALID, "sq_copy_to_host: NULL argument");
+    SQ_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SQ_OK;
+}
+
 int sq_scene_upload(sq_ctx* c, const sq_triangle* host, uint32_t n, sq_scene** out) {
     if (!c || !host || !out || n == 0) SQ_RET(SQ_ERR_INVALID, "sq_scene_upload: bad argument");
     SQ_CHECK_HIP(hipSetDevice(c->device));

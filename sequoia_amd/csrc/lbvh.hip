@@ -42,9 +42,11 @@ struct EmitArgs {
     sq_node* nodes;
     uint32_t layout;
     uint32_t refit;
+    uint32_t cap;  // node capacity: writes at or beyond it are dropped
 };
 
 __device__ __forceinline__ void write_leaf(const EmitArgs& a, uint32_t slot, uint32_t tri, sqd::Box3* box) {
+    if (slot >= a.cap) return;
     if (a.refit) {
         float v[9];
         sqd::load_tri(a.tris, tri, v);
@@ -80,10 +82,12 @@ __device__ void emit_children(const EmitArgs& a, uint32_t b, uint32_t e, uint32_
             write_leaf(a, lfree + i, a.idx[b + i], &lb);
             if (i == 0) u = lb; else sqd::box_grow(u, lb);
         }
-        if (a.refit) sqd::store_node(a.nodes, lslot, u, lfree, L, 0);
-        else sqd::store_node_links(a.nodes, lslot, lfree, L, 0);
+        if (lslot < a.cap) {
+            if (a.refit) sqd::store_node(a.nodes, lslot, u, lfree, L, 0);
+            else sqd::store_node_links(a.nodes, lslot, lfree, L, 0);
+        }
     } else {
-        sqd::store_node_links(a.nodes, lslot, lfree, 2, 0);
+        if (lslot < a.cap) sqd::store_node_links(a.nodes, lslot, lfree, 2, 0);
         push(b, b + L, lfree, lslot);
     }
     // right child
@@ -95,18 +99,22 @@ __device__ void emit_children(const EmitArgs& a, uint32_t b, uint32_t e, uint32_
             write_leaf(a, rfree + i, a.idx[b + L + i], &lb);
             if (i == 0) u = lb; else sqd::box_grow(u, lb);
         }
-        if (a.refit) sqd::store_node(a.nodes, rslot, u, rfree, R, 0);
-        else sqd::store_node_links(a.nodes, rslot, rfree, R, 0);
+        if (rslot < a.cap) {
+            if (a.refit) sqd::store_node(a.nodes, rslot, u, rfree, R, 0);
+            else sqd::store_node_links(a.nodes, rslot, rfree, R, 0);
+        }
     } else {
-        sqd::store_node_links(a.nodes, rslot, rfree, 2, 0);
+        if (rslot < a.cap) sqd::store_node_links(a.nodes, rslot, rfree, 2, 0);
         push(b + L, e, rfree, rslot);
     }
 }
 
 // Union of a node's child list, in list order (interior refit).
-__device__ __forceinline__ void refit_node(sq_node* nodes, uint32_t self) {
+__device__ __forceinline__ void refit_node(sq_node* nodes, uint32_t self, uint32_t cap) {
+    if (self >= cap) return;
     const uint32_t* w = (const uint32_t*)(nodes + self);
     uint32_t cnt = w[3] & 0xFFFFu, child = w[7];
+    if (cnt == 0 || child >= cap || cnt > cap - child) return;
     sqd::Box3 u = sqd::load_node_box(nodes, child);
     for (uint32_t j = 1; j < cnt; j++) sqd::box_grow(u, sqd::load_node_box(nodes, child + j));
     sqd::store_node_box(nodes, self, u);
@@ -279,7 +287,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(SmallArgs a) {
         if (a.e.refit) {
             for (int lv = (int)nlev - 1; lv >= 0; lv--) {
                 for (uint32_t i = s_lvl[lv] + threadIdx.x; i < s_lvl[lv + 1]; i += kSmallThreads)
-                    refit_node(a.e.nodes, s_tasks[i].self);
+                    refit_node(a.e.nodes, s_tasks[i].self, a.e.cap);
                 __syncthreads();
             }
         }
@@ -288,13 +296,14 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(SmallArgs a) {
     if (threadIdx.x == 0 && max_lev) atomicMax(&a.counters[3], max_lev);
 }
 
-__global__ __launch_bounds__(kTopThreads) void k_lbvh_top_refit(sq_node* nodes, const uint32_t* top_list,
+__global__ __launch_bounds__(kTopThreads) void k_lbvh_top_refit(sq_node* nodes, uint32_t cap, const uint32_t* top_list,
                                                                  const uint32_t* top_level_off,
                                                                  const uint32_t* counters) {
-    const uint32_t levels = counters[2] < (uint32_t)kMaxLevels ? counters[2] : (uint32_t)kMaxLevels;
+    if (counters[2] >= (uint32_t)kMaxLevels) return;
+    const uint32_t levels = counters[2];
     for (int lv = (int)levels - 1; lv >= 0; lv--) {
         uint32_t lo = top_level_off[lv], hi = top_level_off[lv + 1];
-        for (uint32_t i = lo + threadIdx.x; i < hi; i += kTopThreads) refit_node(nodes, top_list[i]);
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += kTopThreads) refit_node(nodes, top_list[i], cap);
         __syncthreads();
     }
 }
@@ -320,7 +329,7 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
     uint32_t* counters = c->counters.as<uint32_t>();
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, c->stream, counters, 16);
 
-    EmitArgs ea{tris, d_idx, d_nodes, layout, refit ? 1u : 0u};
+    EmitArgs ea{tris, d_idx, d_nodes, layout, refit ? 1u : 0u, 2 * n};
     TopArgs ta;
     ta.e = ea;
     ta.codes = d_codes;
@@ -348,7 +357,7 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
     if (refit) {
         phase_end(c, PH_EMIT);
         phase_begin(c, PH_REFIT);
-        hipLaunchKernelGGL(k_lbvh_top_refit, dim3(1), dim3(kTopThreads), 0, c->stream, d_nodes, ta.top_list,
+        hipLaunchKernelGGL(k_lbvh_top_refit, dim3(1), dim3(kTopThreads), 0, c->stream, d_nodes, 2 * n, ta.top_list,
                            ta.top_level_off, counters);
         SQ_CHECK_HIP(hipGetLastError());
         phase_end(c, PH_REFIT);

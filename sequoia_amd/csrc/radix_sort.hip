@@ -169,8 +169,10 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
         if (idx < n) {
             uint32_t d = digit_of(key[r], shift, mask);
             uint32_t pos = s_start[d] + s_cnt[wave][d] + rank[r];
-            s_keys[pos] = key[r];
-            s_vals[pos] = val[r];
+            if (pos < (uint32_t)kRsTile) {
+                s_keys[pos] = key[r];
+                s_vals[pos] = val[r];
+            }
         }
     }
     __syncthreads();
@@ -179,8 +181,10 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
         K k = s_keys[i];
         uint32_t d = digit_of(k, shift, mask);
         size_t out = (size_t)s_gbase[d] + (i - s_start[d]);
-        keys_out[out] = k;
-        vals_out[out] = s_vals[i];
+        if (out < n) {
+            keys_out[out] = k;
+            vals_out[out] = s_vals[i];
+        }
     }
 }
 

@@ -1,0 +1,490 @@
+// trace.hip — wave64 ray traversal over Node1 trees.
+//
+// Semantics follow traverse / trace (tracer.cpp:47-65, 195-230):
+//   * every visited node costs one box test (intersect_ray_aabb,
+//     tracer.h:13-35: inv = 1/d, slab with the reference's ternary min/max,
+//     hit = back >= front && back <= tmax);
+//   * EXHAUSTIVE visits the children of every hit interior node, so the
+//     per-ray test count equals the reference's num_tests for any visiting
+//     order;
+//   * triangles of hit leaves are tested with the reference Moller-Trumbore
+//     (tracer.h:83-116, same fused-multiply-add pattern as its x86 build);
+//     the closest valid hit (t > max(1e-9, tmin), t < tmax; ties -> lower
+//     triangle id) is reported, which the reference computes but discards;
+//   * CLOSEST culls lists whose entry distance exceeds the best hit by more
+//     than 2^-20 relative and descends nearer children first.
+//
+// Execution: persistent workgroups; each wave pulls a 64-item tile (8x8
+// pixels or 64 rays) from a global counter. Each lane keeps a stack of child
+// lists: the newest `S` entries live in LDS (slot-major, bank-conflict
+// free), older ones in a per-thread spill area in HBM. An entry packs
+// (first child << 4 | count-1) or, for lists longer than 15, the parent
+// node index with tag 15 (refetched on pop).
+#include "internal.h"
+
+namespace sq {
+
+struct TraceArgs {
+    const sq_node* nodes;
+    const sq_triangle* tris;
+    uint32_t root;
+    uint32_t node_count;  // reads at or beyond these bounds are skipped
+    uint32_t tri_count;
+    uint32_t camera;  // 1: camera rays, 0: ray list
+    float ox, oy, oz, tanh;
+    uint32_t w, h, spp, jitter, seed;
+    float tmin, tmax;
+    const sq_ray* rays;
+    uint32_t nrays;
+    uint32_t* hit_id;
+    float* hit_t;
+    uint32_t* counts;
+    uint8_t* rgb;
+    uint32_t slots;       // LDS entries per lane (power of two)
+    uint32_t* spill;      // per-thread spill areas
+    uint32_t spill_cap;   // entries per thread in the spill area
+    uint32_t* work;       // [0] tile counter, [1] overflow flag
+    unsigned long long* stats;  // [0] node tests, [1] triangle tests
+    uint32_t num_tiles;
+    uint32_t tiles_x;
+};
+
+__device__ __forceinline__ uint32_t hash_u32(uint32_t v) {
+    uint32_t s = v * 747796405u + 2891336453u;
+    uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+
+struct RayF {
+    float ox, oy, oz, dx, dy, dz, tmin, tmax;
+};
+
+// tracer.cpp:202-216 generalised: ndc = fma((p + j)/dim, 2, -1), x scaled by
+// w/h, (x, y) scaled by tan(fov/2), p = (x, y, 1), |p|^2 = fma(z,z,fma(x,x,y*y)).
+__device__ __forceinline__ RayF camera_ray(const TraceArgs& a, uint32_t px, uint32_t py, uint32_t s) {
+    float jx = 0.5f, jy = 0.5f;
+    if (a.jitter) {
+        uint32_t pix = py * a.w + px;
+        uint32_t h1 = hash_u32(a.seed ^ hash_u32(pix * a.spp + s));
+        uint32_t h2 = hash_u32(h1 ^ 0x9E3779B9u);
+        jx = (float)(h1 >> 8) * (1.0f / 16777216.0f);
+        jy = (float)(h2 >> 8) * (1.0f / 16777216.0f);
+    }
+    float w = (float)a.w, h = (float)a.h;
+    float nx = __fmaf_rn(((float)px + jx) / w, 2.0f, -1.0f);
+    float ny = __fmaf_rn(((float)py + jy) / h, 2.0f, -1.0f);
+    nx = nx * (w / h);
+    float tx = nx * a.tanh, ty = ny * a.tanh, tz = 1.0f;
+    float len = sqrtf(__fmaf_rn(tz, tz, __fmaf_rn(tx, tx, ty * ty)));
+    RayF r;
+    r.ox = a.ox; r.oy = a.oy; r.oz = a.oz;
+    r.dx = tx / len; r.dy = ty / len; r.dz = tz / len;
+    r.tmin = a.tmin; r.tmax = a.tmax;
+    return r;
+}
+
+// intersect_ray_triangle<1> (tracer.h:83-116).
+__device__ __forceinline__ bool tri_test(const float v[9], const RayF& r, float& t) {
+    const float eps = 0.000000001f;
+    float e1x = v[3] - v[0], e1y = v[4] - v[1], e1z = v[5] - v[2];
+    float e2x = v[6] - v[0], e2y = v[7] - v[1], e2z = v[8] - v[2];
+    float hx = __fmaf_rn(-e2y, r.dz, r.dy * e2z);
+    float hy = __fmaf_rn(-e2z, r.dx, r.dz * e2x);
+    float hz = __fmaf_rn(-e2x, r.dy, r.dx * e2y);
+    float av = __fmaf_rn(e1z, hz, __fmaf_rn(e1y, hy, e1x * hx));
+    bool miss = (av > -eps) & (av < eps);
+    float f = 1.0f / av;
+    float sx = r.ox - v[0], sy = r.oy - v[1], sz = r.oz - v[2];
+    float u = f * __fmaf_rn(sz, hz, __fmaf_rn(sy, hy, sx * hx));
+    miss |= (u < 0.0f) | (u > 1.0f);
+    float qx = __fmaf_rn(-e1y, sz, sy * e1z);
+    float qy = __fmaf_rn(-e1z, sx, sz * e1x);
+    float qz = __fmaf_rn(-e1x, sy, sx * e1y);
+    float vv = f * __fmaf_rn(r.dz, qz, __fmaf_rn(r.dy, qy, r.dx * qx));
+    miss |= (vv < 0.0f) | ((u + vv) > 1.0f);
+    t = f * __fmaf_rn(e2z, qz, __fmaf_rn(e2y, qy, e2x * qx));
+    return (!miss) & (t > eps);
+}
+
+struct NodeV {
+    float mn[3], mx[3];
+    uint32_t count, leaf, child;
+};
+
+__device__ __forceinline__ NodeV load_node(const sq_node* __restrict__ nodes, uint32_t i) {
+    const float4* p = (const float4*)(nodes + i);
+    float4 a = p[0], b = p[1];
+    NodeV n;
+    n.mn[0] = a.x; n.mn[1] = a.y; n.mn[2] = a.z;
+    n.mx[0] = b.x; n.mx[1] = b.y; n.mx[2] = b.z;
+    uint32_t cf = __float_as_uint(a.w);
+    n.count = cf & 0xFFFFu;
+    n.leaf = (cf >> 16) & 1u;
+    n.child = __float_as_uint(b.w);
+    return n;
+}
+
+// Slab test with the reference's operation order and selects.
+__device__ __forceinline__ bool slab(const NodeV& n, const RayF& r, float ix, float iy, float iz, float& front) {
+    float t1x = (n.mn[0] - r.ox) * ix, t2x = (n.mx[0] - r.ox) * ix;
+    float t1y = (n.mn[1] - r.oy) * iy, t2y = (n.mx[1] - r.oy) * iy;
+    float t1z = (n.mn[2] - r.oz) * iz, t2z = (n.mx[2] - r.oz) * iz;
+    float tnx = sqd::simd_min(t1x, t2x), tny = sqd::simd_min(t1y, t2y), tnz = sqd::simd_min(t1z, t2z);
+    float txx = sqd::simd_max(t1x, t2x), txy = sqd::simd_max(t1y, t2y), txz = sqd::simd_max(t1z, t2z);
+    front = sqd::simd_max(sqd::simd_max(tnx, tny), tnz);
+    float back = sqd::simd_min(sqd::simd_min(txx, txy), txz);
+    return (back >= front) & (back <= r.tmax);
+}
+
+struct Stack {
+    uint32_t* lds;     // this lane's column: entry k at lds[(k & mask) * stride]
+    float* lds_t;      // closest mode: entry distances
+    uint32_t stride, mask, S;
+    uint32_t* spill;   // this thread's spill area (closest: 2 words per entry)
+    uint32_t cap;
+    uint32_t sp;
+    bool overflow;
+};
+
+template <bool CLOSEST>
+__device__ __forceinline__ void push(Stack& st, uint32_t e, float t) {
+    const uint32_t slot = (st.sp & st.mask) * st.stride;
+    if (st.sp >= st.S) {
+        uint32_t k = st.sp - st.S;
+        if (k >= st.cap) { st.overflow = true; return; }
+        if (CLOSEST) {
+            st.spill[2 * k] = st.lds[slot];
+            st.spill[2 * k + 1] = __float_as_uint(st.lds_t[slot]);
+        } else {
+            st.spill[k] = st.lds[slot];
+        }
+    }
+    st.lds[slot] = e;
+    if (CLOSEST) st.lds_t[slot] = t;
+    st.sp++;
+}
+
+template <bool CLOSEST>
+__device__ __forceinline__ uint32_t pop(Stack& st, float& t) {
+    st.sp--;
+    const uint32_t slot = (st.sp & st.mask) * st.stride;
+    uint32_t e = st.lds[slot];
+    if (CLOSEST) t = st.lds_t[slot];
+    if (st.sp >= st.S) {
+        uint32_t k = st.sp - st.S;
+        if (CLOSEST) {
+            st.lds[slot] = st.spill[2 * k];
+            st.lds_t[slot] = __uint_as_float(st.spill[2 * k + 1]);
+        } else {
+            st.lds[slot] = st.spill[k];
+        }
+    }
+    return e;
+}
+
+__device__ __forceinline__ uint32_t encode_list(uint32_t parent, const NodeV& n) {
+    if (n.count >= 1 && n.count <= 15 && n.child < (1u << 28)) return (n.child << 4) | (n.count - 1);
+    return (parent << 4) | 15u;
+}
+
+template <bool CLOSEST>
+__device__ void traverse(const TraceArgs& a, const RayF& r, Stack& st, uint32_t& count, uint32_t& tri_tests,
+                         uint32_t& best_id, float& best_t) {
+    const float ix = 1.0f / r.dx, iy = 1.0f / r.dy, iz = 1.0f / r.dz;
+    const float tlo = r.tmin > 0.000000001f ? r.tmin : 0.000000001f;
+    uint32_t first = a.root, cnt = 1;
+    st.sp = 0;
+    while (true) {
+        uint32_t c0 = 0, c1 = 0, ncand = 0;
+        float f0 = 0.f, f1 = 0.f;
+        for (uint32_t i = 0; i < cnt; i++) {
+            const uint32_t ni = first + i;
+            if (ni >= a.node_count) { st.overflow = true; break; }
+            NodeV n = load_node(a.nodes, ni);
+            count++;
+            float front;
+            bool hit = slab(n, r, ix, iy, iz, front);
+            if (CLOSEST) hit = hit && !(front > best_t + best_t * (1.0f / 1048576.0f));
+            if (!hit) continue;
+            if (n.leaf) {
+                for (uint32_t j = 0; j < n.count; j++) {
+                    const uint32_t tid = n.child + j;
+                    if (tid >= a.tri_count) { st.overflow = true; break; }
+                    float v[9];
+                    sqd::load_tri(a.tris, tid, v);
+                    float t;
+                    tri_tests++;
+                    bool m = tri_test(v, r, t);
+                    if (m && t > r.tmin && t < r.tmax && (t < best_t || (t == best_t && tid < best_id))) {
+                        best_t = t;
+                        best_id = tid;
+                    }
+                }
+                (void)tlo;
+            } else {
+                uint32_t e = encode_list(ni, n);
+                if (CLOSEST) {
+                    if (ncand == 0) { c0 = e; f0 = front; ncand = 1; }
+                    else if (ncand == 1) { c1 = e; f1 = front; ncand = 2; }
+                    else push<true>(st, e, front);
+                } else {
+                    push<false>(st, e, 0.f);
+                }
+            }
+        }
+        if (CLOSEST && ncand) {
+            // Nearer list is taken next; the farther one waits on the stack.
+            if (ncand == 2) {
+                if (f1 < f0) { uint32_t te = c0; c0 = c1; c1 = te; float tf = f0; f0 = f1; f1 = tf; }
+                push<true>(st, c1, f1);
+            }
+            push<true>(st, c0, f0);
+        }
+        if (st.overflow) return;
+        // pop the next list
+        bool found = false;
+        while (st.sp > 0) {
+            float te = 0.f;
+            uint32_t e = pop<CLOSEST>(st, te);
+            if (CLOSEST && te > best_t + best_t * (1.0f / 1048576.0f)) continue;
+            if ((e & 15u) != 15u) {
+                first = e >> 4;
+                cnt = (e & 15u) + 1;
+            } else {
+                if ((e >> 4) >= a.node_count) { st.overflow = true; return; }
+                NodeV p = load_node(a.nodes, e >> 4);
+                first = p.child;
+                cnt = p.count;
+            }
+            found = true;
+            break;
+        }
+        if (!found) return;
+    }
+}
+
+__device__ __forceinline__ uint8_t heat_byte(uint32_t count, uint32_t spp) {
+    float c = (float)count;
+    if (spp > 1) c = c / (float)spp;
+    float v = c / 200.0f;
+    v = (v < 1.0f) ? v : 1.0f;
+    return (uint8_t)(v * 255.0f);
+}
+
+template <bool CLOSEST>
+__global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
+    extern __shared__ uint32_t s_dyn[];
+    const uint32_t lane = threadIdx.x & 63;
+    Stack st;
+    st.stride = blockDim.x;
+    st.S = a.slots;
+    st.mask = a.slots - 1;
+    st.lds = s_dyn + threadIdx.x;
+    st.lds_t = CLOSEST ? (float*)(s_dyn + a.slots * blockDim.x) + threadIdx.x : nullptr;
+    const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    st.cap = a.spill_cap;
+    st.spill = a.spill + gtid * (size_t)a.spill_cap * (CLOSEST ? 2 : 1);
+    st.overflow = false;
+    uint32_t tests_total = 0, tris_total = 0;
+    while (true) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(&a.work[0], 1u);
+        tile = __shfl(tile, 0, 64);
+        if (tile >= a.num_tiles) break;
+        if (a.camera) {
+            const uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+            const uint32_t px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+            if (px < a.w && py < a.h) {
+                uint32_t sum = 0;
+                for (uint32_t s = 0; s < a.spp; s++) {
+                    RayF r = camera_ray(a, px, py, s);
+                    uint32_t cnt = 0, best_id = SQ_MISS;
+                    float best_t = __int_as_float(0x7f800000);
+                    traverse<CLOSEST>(a, r, st, cnt, tris_total, best_id, best_t);
+                    sum += cnt;
+                    size_t ri = ((size_t)py * a.w + px) * a.spp + s;
+                    if (a.hit_id) a.hit_id[ri] = best_id;
+                    if (a.hit_t) a.hit_t[ri] = best_t;
+                }
+                tests_total += sum;
+                size_t p = (size_t)py * a.w + px;
+                if (a.counts) a.counts[p] = sum;
+                if (a.rgb) {
+                    uint8_t b = heat_byte(sum, a.spp);
+                    a.rgb[3 * p] = b; a.rgb[3 * p + 1] = b; a.rgb[3 * p + 2] = b;
+                }
+            }
+        } else {
+            const size_t ri = (size_t)tile * 64 + lane;
+            if (ri < a.nrays) {
+                const sq_ray q = a.rays[ri];
+                RayF r;
+                r.ox = q.origin.x; r.oy = q.origin.y; r.oz = q.origin.z;
+                r.dx = q.dir.x; r.dy = q.dir.y; r.dz = q.dir.z;
+                r.tmin = q.tmin; r.tmax = q.tmax;
+                uint32_t cnt = 0, best_id = SQ_MISS;
+                float best_t = __int_as_float(0x7f800000);
+                traverse<CLOSEST>(a, r, st, cnt, tris_total, best_id, best_t);
+                tests_total += cnt;
+                if (a.counts) a.counts[ri] = cnt;
+                if (a.hit_id) a.hit_id[ri] = best_id;
+                if (a.hit_t) a.hit_t[ri] = best_t;
+            }
+        }
+        if (st.overflow) atomicOr(&a.work[1], 1u);
+        st.overflow = false;
+    }
+    if (a.stats) {
+        unsigned long long t0 = tests_total, t1 = tris_total;
+        for (int off = 32; off > 0; off >>= 1) {
+            t0 += __shfl_xor(t0, off, 64);
+            t1 += __shfl_xor(t1, off, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.stats[0], t0);
+            atomicAdd(&a.stats[1], t1);
+        }
+    }
+}
+
+__global__ void k_trace_init(uint32_t* work, unsigned long long* stats) {
+    if (threadIdx.x < 4) work[threadIdx.x] = 0;
+    if (stats && threadIdx.x < 2) stats[threadIdx.x] = 0;
+}
+
+static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace_params* p) {
+    const bool closest = p && p->semantics == SQ_TRACE_CLOSEST;
+    uint32_t slots = p && p->stack_slots ? p->stack_slots : 16;
+    uint32_t pow2 = 1;
+    while (pow2 < slots) pow2 <<= 1;
+    slots = pow2 > 64 ? 64 : pow2;
+    const uint32_t block = 128;
+    const uint32_t blocks_per_cu = 8;
+    const uint32_t grid = (uint32_t)c->num_cus * blocks_per_cu;
+    const uint32_t depth = b->depth ? b->depth : 64;
+    uint32_t cap = 2 * depth + 16;
+    cap = cap > slots ? cap - slots + 8 : 8;
+    const size_t words = (size_t)grid * block * cap * (closest ? 2 : 1);
+    SQ_TRY(c->spill.ensure(words * 4));
+    SQ_TRY(c->counters.ensure(64));
+    SQ_TRY(c->stats.ensure(64));
+    a.slots = slots;
+    a.spill = c->spill.as<uint32_t>();
+    a.spill_cap = cap;
+    a.work = c->counters.as<uint32_t>();
+    a.stats = (p && p->count_work) ? c->stats.as<unsigned long long>() : nullptr;
+    hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, c->stream, a.work, a.stats);
+    const size_t lds = (size_t)slots * block * 4 * (closest ? 2 : 1);
+    phase_begin(c, PH_TRACE);
+    if (closest) hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(block), lds, c->stream, a);
+    else hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(block), lds, c->stream, a);
+    phase_end(c, PH_TRACE);
+    SQ_CHECK_HIP(hipGetLastError());
+    return SQ_OK;
+}
+
+static int trace_finish(sq_ctx* c, const sq_trace_params* p) {
+    SQ_TRY(pinned_ensure(c, 64));
+    uint32_t* h = (uint32_t*)c->pinned;
+    SQ_CHECK_HIP(hipMemcpyAsync(h, c->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
+    if (p && p->count_work)
+        SQ_CHECK_HIP(hipMemcpyAsync(h + 8, c->stats.p, 16, hipMemcpyDeviceToHost, c->stream));
+    SQ_TRY(phases_collect(c));
+    if (p && p->count_work) {
+        unsigned long long* s = (unsigned long long*)(h + 8);
+        c->last.node_tests = s[0];
+        c->last.tri_tests = s[1];
+    }
+    if (h[1]) SQ_RET(SQ_ERR_OVERFLOW, "trace: a ray exceeded the traversal stack bound");
+    return SQ_OK;
+}
+
+}  // namespace sq
+
+using namespace sq;
+
+extern "C" {
+
+int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_trace_params* p,
+                    const sq_trace_out* out, int out_on_device) {
+    if (!c || !b || !cam || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: NULL argument");
+    if (cam->width == 0 || cam->height == 0 || cam->spp == 0) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: empty image");
+    if (p && p->heatmap != SQ_HEATMAP_RAY) SQ_RET(SQ_ERR_UNSUPPORTED, "sq_trace_camera: heatmap mode %u", p->heatmap);
+    SQ_CHECK_HIP(hipSetDevice(c->device));
+    phases_reset(c);
+    const size_t npix = (size_t)cam->width * cam->height, nray = npix * cam->spp;
+    TraceArgs a{};
+    a.nodes = b->d_nodes;
+    a.tris = b->scene->d_tris;
+    a.root = b->root;
+    a.node_count = b->capacity;
+    a.tri_count = b->scene->n;
+    a.camera = 1;
+    a.ox = cam->origin.x; a.oy = cam->origin.y; a.oz = cam->origin.z;
+    a.tanh = cam->tan_half_fov;
+    a.w = cam->width; a.h = cam->height; a.spp = cam->spp; a.jitter = cam->jitter; a.seed = cam->seed;
+    a.tmin = cam->tmin; a.tmax = cam->tmax;
+    a.tiles_x = (cam->width + 7) / 8;
+    a.num_tiles = a.tiles_x * ((cam->height + 7) / 8);
+    if (out_on_device) {
+        a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts; a.rgb = out->rgb;
+        SQ_TRY(trace_launch(c, b, a, p));
+        return trace_finish(c, p);
+    }
+    // Host outputs: stage through workspace buffers.
+    size_t off_id = 0, off_t = off_id + (out->hit_id ? nray * 4 : 0);
+    size_t off_c = off_t + (out->hit_t ? nray * 4 : 0), off_rgb = off_c + (out->counts ? npix * 4 : 0);
+    size_t total = off_rgb + (out->rgb ? npix * 3 : 0);
+    SQ_TRY(c->scratch.ensure(total + 256));
+    char* base = c->scratch.as<char>();
+    a.hit_id = out->hit_id ? (uint32_t*)(base + off_id) : nullptr;
+    a.hit_t = out->hit_t ? (float*)(base + off_t) : nullptr;
+    a.counts = out->counts ? (uint32_t*)(base + off_c) : nullptr;
+    a.rgb = out->rgb ? (uint8_t*)(base + off_rgb) : nullptr;
+    SQ_TRY(trace_launch(c, b, a, p));
+    if (out->hit_id) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_id, a.hit_id, nray * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out->hit_t) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_t, a.hit_t, nray * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out->counts) SQ_CHECK_HIP(hipMemcpyAsync(out->counts, a.counts, npix * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out->rgb) SQ_CHECK_HIP(hipMemcpyAsync(out->rgb, a.rgb, npix * 3, hipMemcpyDeviceToHost, c->stream));
+    return trace_finish(c, p);
+}
+
+int sq_trace_rays(sq_ctx* c, const sq_bvh* b, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
+                  const sq_trace_out* out, int io_on_device) {
+    if (!c || !b || !rays || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_rays: NULL argument");
+    if (p && p->heatmap != SQ_HEATMAP_RAY) SQ_RET(SQ_ERR_UNSUPPORTED, "sq_trace_rays: heatmap mode %u", p->heatmap);
+    if (n == 0) return SQ_OK;
+    SQ_CHECK_HIP(hipSetDevice(c->device));
+    phases_reset(c);
+    TraceArgs a{};
+    a.nodes = b->d_nodes;
+    a.tris = b->scene->d_tris;
+    a.root = b->root;
+    a.node_count = b->capacity;
+    a.tri_count = b->scene->n;
+    a.camera = 0;
+    a.nrays = n;
+    a.num_tiles = (n + 63) / 64;
+    if (io_on_device) {
+        a.rays = rays;
+        a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts;
+        SQ_TRY(trace_launch(c, b, a, p));
+        return trace_finish(c, p);
+    }
+    const size_t nr = n;
+    size_t off_r = 0, off_id = nr * sizeof(sq_ray), off_t = off_id + nr * 4, off_c = off_t + nr * 4;
+    SQ_TRY(c->scratch.ensure(off_c + nr * 4 + 256));
+    char* base = c->scratch.as<char>();
+    SQ_CHECK_HIP(hipMemcpyAsync(base + off_r, rays, nr * sizeof(sq_ray), hipMemcpyHostToDevice, c->stream));
+    a.rays = (const sq_ray*)(base + off_r);
+    a.hit_id = out->hit_id ? (uint32_t*)(base + off_id) : nullptr;
+    a.hit_t = out->hit_t ? (float*)(base + off_t) : nullptr;
+    a.counts = out->counts ? (uint32_t*)(base + off_c) : nullptr;
+    SQ_TRY(trace_launch(c, b, a, p));
+    if (out->hit_id) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_id, a.hit_id, nr * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out->hit_t) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_t, a.hit_t, nr * 4, hipMemcpyDeviceToHost, c->stream));
+    if (out->counts) SQ_CHECK_HIP(hipMemcpyAsync(out->counts, a.counts, nr * 4, hipMemcpyDeviceToHost, c->stream));
+    return trace_finish(c, p);
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+"""GPU Morton codes, radix sort and LBVH emission/refit against the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+from conftest import golden, nodes_equal
+from oracle import binding as ob
+from scenes import GOLDEN_SCENES
+
+import sequoia_amd as sq
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("bits", [10, 20, 21])
+@pytest.mark.parametrize("name", ["ico1280", "clus3000", "ico69632"])
+def test_morton_codes(ctx, name, bits):
+    say("morton codes", name, bits)
+    tris = GOLDEN_SCENES[name]()
+    scene = ctx.upload(tris)
+    got = ctx.morton_codes(scene, bits)
+    assert np.array_equal(got, ob.morton_codes(tris, bits))
+
+
+@pytest.mark.parametrize("key_bits,n", [(30, 1), (30, 4095), (30, 4097), (32, 100000), (63, 250000), (64, 70000)])
+def test_radix_sort_is_stable(ctx, key_bits, n):
+    say("radix sort", key_bits, n)
+    rng = np.random.default_rng(key_bits * 1000 + n)
+    hi = (1 << 12)
+    keys = rng.integers(0, hi, n, dtype=np.uint64)
+    keys = (keys * np.uint64(0x9E3779B97F4A7C15 >> (64 - key_bits)) if key_bits < 64 else keys * np.uint64(0x9E3779B97F4A7C15))
+    keys &= np.uint64((1 << key_bits) - 1) if key_bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    vals = np.arange(n, dtype=np.uint32)
+    k, v = ctx.radix_sort(keys, vals, key_bits)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order])
+    assert np.array_equal(v, vals[order])
+
+
+def test_radix_sort_matches_reference_fixture(ctx):
+    g = golden("radix_sort32")
+    k, v = ctx.radix_sort(g["keys"].astype(np.uint64), np.arange(len(g["keys"]), dtype=np.uint32), 32)
+    assert np.array_equal(k.astype(np.uint32), g["sorted_keys"]) and np.array_equal(v, g["order"])
+
+
+@pytest.mark.parametrize("digit", [4, 6, 8])
+def test_radix_sort_digit_widths(ctx, digit):
+    rng = np.random.default_rng(digit)
+    keys = rng.integers(0, 1 << 30, 50000, dtype=np.uint64)
+    k, v = ctx.radix_sort(keys, np.arange(len(keys), dtype=np.uint32), 30, digit)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(v, order.astype(np.uint32))
+
+
+BUILD_CASES = [
+    ("ico1280", "morton", 10), ("soup2000", "morton", 10), ("clus3000", "morton2", 20),
+    ("ico69632", "morton", 10), ("ico69632", "morton63", 21), ("clus3000", "morton63", 21),
+]
+
+
+@pytest.mark.parametrize("layout", [sq.LAYOUT_REFERENCE, sq.LAYOUT_FIXED])
+@pytest.mark.parametrize("threshold", [0, 64])
+@pytest.mark.parametrize("name,kind,bits", BUILD_CASES)
+def test_morton_build_bit_exact(ctx, name, kind, bits, layout, threshold):
+    say("morton build", name, kind, layout, threshold)
+    tris = GOLDEN_SCENES[name]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, kind, layout=layout, subtree_threshold=threshold)
+    got = bvh.download()
+    want, root, _, _ = ob.build_morton(tris, bits, layout=layout, refit=True)
+    assert bvh.root == root
+    assert len(nodes_equal(got, want)) == 0
+
+
+@pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000"])
+def test_morton_topology_matches_reference_fixture(ctx, name):
+    tris = GOLDEN_SCENES[name]()
+    scene = ctx.upload(tris)
+    g = golden(name)
+    for kind in ("morton", "morton2"):
+        got = ctx.build(scene, kind, refit=False).download()
+        links = np.stack([got["count"].astype(np.uint32), got["flags"].astype(np.uint32), got["child"]], 1)
+        assert np.array_equal(links, g["links_" + kind])
+        assert not got["min"].any() and not got["max"].any()
+
+
+def test_small_and_degenerate_scenes(ctx):
+    rng = np.random.default_rng(5)
+    for n in (1, 2, 7, 8, 9, 15, 64, 2047, 2048, 2049):
+        tris = rng.random((n, 9), dtype=np.float32)
+        say("tiny build", n)
+        scene = ctx.upload(tris)
+        for layout in (sq.LAYOUT_REFERENCE, sq.LAYOUT_FIXED):
+            got = ctx.build(scene, "morton63", layout=layout).download()
+            want, _, _, _ = ob.build_morton(tris, 21, layout=layout)
+            assert len(nodes_equal(got, want)) == 0
+    # all centroids equal -> midpoint splits
+    tris = np.tile(np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32), (5000, 1))
+    scene = ctx.upload(tris)
+    got = ctx.build(scene, "morton63").download()
+    want, _, _, _ = ob.build_morton(tris, 21)
+    assert len(nodes_equal(got, want)) == 0
+
+
+def test_large_soup_morton63(ctx):
+    say("1M soup morton63")
+    tris = sq.gen_soup(1_000_000, 2, 0.01)
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    got = bvh.download()
+    want, _, _, _ = ob.build_morton(tris, 21)
+    assert len(nodes_equal(got, want)) == 0
+    t = ctx.timings()
+    say("timings", {k: round(v, 3) if isinstance(v, float) else v for k, v in t.items()})

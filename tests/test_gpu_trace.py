@@ -1,0 +1,117 @@
+"""GPU traversal against the oracle: per-ray node-test counts and heatmaps
+bit-exact on identical trees; closest hits (id bit-exact, t bit-exact)."""
+import hashlib
+
+import numpy as np
+import pytest
+
+from oracle import binding as ob
+from scenes import GOLDEN_SCENES
+
+import sequoia_amd as sq
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def check_hits(got_id, got_t, want_id, want_t):
+    same = got_id == want_id
+    # ids may differ only where two hits lie within 1 ulp in t
+    bad = ~same
+    if bad.any():
+        gt, wt = got_t[bad], want_t[bad]
+        ulp = np.abs(gt.view(np.int32).astype(np.int64) - wt.view(np.int32).astype(np.int64))
+        assert (ulp <= 1).all(), (np.nonzero(bad)[0][:5], got_id[bad][:5], want_id[bad][:5])
+    hit = same & (want_id != sq.MISS)
+    assert np.array_equal(got_t[hit].view(np.uint32), want_t[hit].view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000", "ico69632"])
+def test_reference_camera_on_reference_tree(ctx, name, manifest):
+    say("reference camera", name)
+    tris = GOLDEN_SCENES[name]()
+    tr = manifest["traces"][name]
+    nodes, root = ob.build_sah(tris, 8)       # equals the reference's recursive tree (pinned)
+    scene = ctx.upload(tris)
+    bvh = ctx.bvh_from_host(scene, nodes, root)
+    cam = sq.reference_camera(tris, tr["w"], tr["h"])
+    got = ctx.trace_camera(bvh, cam)
+    assert sha(got["counts"]) == tr["counts_sha256"]
+    assert sha(got["rgb"]) == tr["basic_rgb_sha256"]
+    want = ob.trace_camera(nodes, root, tris, cam)
+    check_hits(got["hit_id"], got["hit_t"], want["hit_id"], want["hit_t"])
+
+
+@pytest.mark.parametrize("kind", ["morton63", "morton"])
+@pytest.mark.parametrize("name", ["ico1280", "clus3000", "ico69632"])
+def test_jittered_pinhole_on_gpu_tree(ctx, name, kind):
+    say("jittered pinhole", name, kind)
+    tris = GOLDEN_SCENES[name]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, kind)
+    nodes = bvh.download()
+    cam = sq.pinhole_camera(tris, 200, 120, 60.0, 3.0, spp=4, jitter=1, seed=9)
+    got = ctx.trace_camera(bvh, cam)
+    want = ob.trace_camera(nodes, bvh.root, tris, cam)
+    assert np.array_equal(got["counts"], want["counts"])
+    assert np.array_equal(got["rgb"], want["rgb"])
+    check_hits(got["hit_id"], got["hit_t"], want["hit_id"], want["hit_t"])
+
+
+@pytest.mark.parametrize("name", ["ico69632", "clus3000"])
+def test_closest_mode_matches_exhaustive_hits(ctx, name):
+    say("closest", name)
+    tris = GOLDEN_SCENES[name]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63", layout=sq.LAYOUT_FIXED)
+    cam = sq.pinhole_camera(tris, 256, 160, 60.0, 3.0, spp=2, jitter=1, seed=3)
+    ex = ctx.trace_camera(bvh, cam)
+    cl = ctx.trace_camera(bvh, cam, semantics=sq.TRACE_CLOSEST)
+    check_hits(cl["hit_id"], cl["hit_t"], ex["hit_id"], ex["hit_t"])
+    assert cl["counts"].sum() <= ex["counts"].sum()
+
+
+def test_ray_list_secondary_rays(ctx):
+    say("secondary rays")
+    tris = sq.gen_clusters(20000, 64, 3)
+    rays = sq.gen_secondary_rays(tris, 20000, seed=3, tmin=1e-4)
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    nodes = bvh.download()
+    got = ctx.trace_rays(bvh, rays)
+    want = ob.trace_rays(nodes, bvh.root, tris, rays)
+    assert np.array_equal(got["counts"], want["counts"])
+    check_hits(got["hit_id"], got["hit_t"], want["hit_id"], want["hit_t"])
+    cl = ctx.trace_rays(bvh, rays, semantics=sq.TRACE_CLOSEST)
+    check_hits(cl["hit_id"], cl["hit_t"], want["hit_id"], want["hit_t"])
+
+
+def test_camera_rays_equal_uploaded_oracle_rays(ctx):
+    """GPU-generated camera rays produce the same results as the oracle's ray bits."""
+    tris = GOLDEN_SCENES["ico1280"]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    cam = sq.pinhole_camera(tris, 64, 48, 60.0, 3.0, spp=2, jitter=1, seed=1)
+    a = ctx.trace_camera(bvh, cam, want_rgb=False)
+    b = ctx.trace_rays(bvh, ob.camera_rays(cam))
+    assert np.array_equal(a["hit_id"], b["hit_id"])
+    assert np.array_equal(a["hit_t"].view(np.uint32), b["hit_t"].view(np.uint32))
+    assert np.array_equal(a["counts"], b["counts"].reshape(-1, 2).sum(1))
+
+
+def test_small_stack_spills_correctly(ctx):
+    say("spill")
+    tris = GOLDEN_SCENES["ico69632"]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    cam = sq.pinhole_camera(tris, 128, 96, 60.0, 3.0)
+    a = ctx.trace_camera(bvh, cam, stack_slots=2)
+    b = ctx.trace_camera(bvh, cam, stack_slots=32)
+    assert np.array_equal(a["counts"], b["counts"]) and np.array_equal(a["hit_id"], b["hit_id"])

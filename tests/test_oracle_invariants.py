@@ -1,0 +1,186 @@
+"""Pins of the oracle beyond the reference's outputs: closed forms, brute
+force and structural invariants for the parts the reference never computes
+(closest hit, box refit, 63-bit codes, the fixed layout)."""
+import numpy as np
+import pytest
+
+from oracle import binding as ob
+from scenes import GOLDEN_SCENES
+
+
+def reachable(nodes, root):
+    seen, leaves, st = set(), [], [root]
+    while st:
+        i = st.pop()
+        seen.add(i)
+        n = nodes[i]
+        if n["flags"] & 1:
+            leaves.extend(range(n["child"], n["child"] + n["count"]))
+        else:
+            st.extend(range(n["child"], n["child"] + n["count"]))
+    return seen, leaves
+
+
+def test_interleave_bruteforce():
+    rng = np.random.default_rng(0)
+    o = ob.oracle()
+    for _ in range(200):
+        x, y, z = (int(v) for v in rng.integers(0, 1 << 21, 3))
+        want = 0
+        for i in range(21):
+            want |= ((x >> i) & 1) << (3 * i) | ((y >> i) & 1) << (3 * i + 1) | ((z >> i) & 1) << (3 * i + 2)
+        assert o.sqo_interleave64(x, y, z) == want
+
+
+def test_morton63_quantisation_closed_form():
+    tris = GOLDEN_SCENES["soup2000"]()
+    codes = ob.morton_codes(tris, 21)
+    b = tris.reshape(-1, 3, 3)
+    mn, mx = b.min(1), b.max(1)
+    c = (mn + mx) * np.float32(0.5)
+    smn, smx = mn.min(0), mx.max(0)
+    q = (((c - smn) / (smx - smn)) * np.float32(2097151.0)).astype(np.uint64)
+    o = ob.oracle()
+    want = np.array([o.sqo_interleave64(int(a), int(b_), int(d)) for a, b_, d in q], np.uint64)
+    assert np.array_equal(codes, want)
+    assert codes.max() < (1 << 63)
+
+
+def test_find_split_cases():
+    assert ob.find_split(np.zeros(9, np.uint64)) == 4           # all equal -> n>>1
+    assert ob.find_split(np.array([0] * 8 + [1], np.uint64)) == 7
+    assert ob.find_split(np.array([0] + [4] * 8, np.uint64)) == 0
+    c = np.array([0, 1, 2, 3, 8, 9, 10, 11], np.uint64)
+    assert ob.find_split(c) == 3
+
+
+def test_right_singleton_quirk_and_fixed_layout():
+    # 9 primitives: 8 with code 0 and one with the top code -> split L=8, R=1.
+    codes = np.array([0] * 8 + [7], np.uint64)
+    idx = np.arange(9, dtype=np.uint32)
+    nodes = np.zeros(18, ob.NODE_DTYPE)
+    ob.oracle().sqo_emit_morton(codes.ctypes.data, idx.ctypes.data, 9, ob.LAYOUT_REFERENCE, nodes.ctypes.data)
+    assert (nodes[0]["child"], nodes[0]["count"]) == (1, 2)
+    # left node at 1 has children from 2; slot 2 is its first child, the right leaf is orphaned at 1+2*8-1=16
+    assert nodes[1]["child"] == 2 and nodes[1]["count"] == 2
+    assert nodes[16]["flags"] == 1 and nodes[16]["child"] == 8
+    _, leaves = reachable(nodes, 0)
+    assert 8 not in leaves
+    fixed = np.zeros(18, ob.NODE_DTYPE)
+    ob.oracle().sqo_emit_morton(codes.ctypes.data, idx.ctypes.data, 9, ob.LAYOUT_FIXED, fixed.ctypes.data)
+    _, leaves = reachable(fixed, 0)
+    assert sorted(leaves) == list(range(9))
+
+
+@pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000"])
+@pytest.mark.parametrize("bits", [10, 21])
+def test_fixed_layout_reaches_every_triangle_once(name, bits):
+    tris = GOLDEN_SCENES[name]()
+    nodes, root, _, _ = ob.build_morton(tris, bits, layout=ob.LAYOUT_FIXED)
+    _, leaves = reachable(nodes, root)
+    assert sorted(leaves) == list(range(len(tris)))
+    nodes, root = ob.build_sah(tris, 16, layout=ob.LAYOUT_FIXED)
+    _, leaves = reachable(nodes, root)
+    assert sorted(leaves) == list(range(len(tris)))
+
+
+@pytest.mark.parametrize("name", ["ico1280", "clus3000"])
+def test_refit_boxes_are_brute_force_unions(name):
+    tris = GOLDEN_SCENES[name]()
+    nodes, root, _, _ = ob.build_morton(tris, 21)
+    b = tris.reshape(-1, 3, 3)
+    tmin, tmax = b.min(1), b.max(1)
+
+    def subtree(i):
+        n = nodes[i]
+        if n["flags"] & 1:
+            return [n["child"]]
+        out = []
+        for c in range(n["child"], n["child"] + n["count"]):
+            out += subtree(c)
+        return out
+
+    seen, _ = reachable(nodes, root)
+    for i in list(seen)[:3000]:
+        ids = subtree(i)
+        assert np.array_equal(nodes[i]["min"], tmin[ids].min(0))
+        assert np.array_equal(nodes[i]["max"], tmax[ids].max(0))
+
+
+def test_sah_children_cover_parent():
+    tris = GOLDEN_SCENES["soup2000"]()
+    for nb in (8, 16, 32):
+        nodes, root = ob.build_sah(tris, nb)
+        seen, leaves = reachable(nodes, root)
+        for i in seen:
+            n = nodes[i]
+            if n["flags"] & 1:
+                continue
+            ch = nodes[n["child"]:n["child"] + n["count"]]
+            assert (ch["min"] >= n["min"]).all() and (ch["max"] <= n["max"]).all()
+
+
+@pytest.mark.parametrize("name", ["ico1280", "soup2000"])
+def test_closest_hit_equals_brute_force(name):
+    """BVH closest hit == argmin over all triangles (fixed layout: all reachable)."""
+    import sequoia_amd as sq
+    tris = GOLDEN_SCENES[name]()
+    cam = sq.pinhole_camera(tris, 48, 32, 60.0, 3.0)
+    rays = ob.camera_rays(cam)
+    nodes, root, _, _ = ob.build_morton(tris, 21, layout=ob.LAYOUT_FIXED)
+    out = ob.trace_rays(nodes, root, tris, rays)
+    o = ob.oracle()
+    t = np.zeros(1, np.float32)
+    for r in range(0, len(rays), 7):
+        best, bid = np.inf, 0xFFFFFFFF
+        for i in range(len(tris)):
+            if o.sqo_tri_test(tris[i].ctypes.data, rays[r:r + 1].ctypes.data, t.ctypes.data):
+                if rays[r]["tmin"] < t[0] < rays[r]["tmax"] and (t[0] < best or (t[0] == best and i < bid)):
+                    best, bid = t[0], i
+        assert out["hit_id"][r] == bid
+        if bid != 0xFFFFFFFF:
+            assert out["hit_t"][r] == np.float32(best)
+    cl = np.empty(len(rays), np.uint32), np.empty(len(rays), np.float32)
+    o.sqo_trace_rays_closest(nodes.ctypes.data, root, tris.ctypes.data, rays.ctypes.data, len(rays),
+                             cl[0].ctypes.data, cl[1].ctypes.data)
+    assert np.array_equal(cl[0], out["hit_id"])
+
+
+def test_counts_are_order_independent_sum():
+    """count = 1 + sum of child counts of hit interior nodes (closed form)."""
+    import sequoia_amd as sq
+    tris = GOLDEN_SCENES["clus3000"]()
+    nodes, root = ob.build_sah(tris, 8)
+    cam = sq.reference_camera(tris, 24, 16)
+    rays = ob.camera_rays(cam)
+    out = ob.trace_rays(nodes, root, tris, rays)
+    o = ob.oracle()
+    for r in range(0, len(rays), 5):
+        total, st = 1, [root]
+        while st:
+            i = st.pop()
+            if o.sqo_slab(nodes[i:i + 1].ctypes.data, rays[r:r + 1].ctypes.data) and not nodes[i]["flags"] & 1:
+                total += int(nodes[i]["count"])
+                st.extend(range(nodes[i]["child"], nodes[i]["child"] + nodes[i]["count"]))
+        assert out["counts"][r] == total
+
+
+def test_slab_nan_semantics():
+    """Zero direction component + origin on a slab plane gives 0*inf = NaN;
+    the reference ternaries keep the first operand, IEEE fminf would not."""
+    node = np.zeros(1, ob.NODE_DTYPE)
+    node["min"] = [0, 0, 0]
+    node["max"] = [1, 1, 1]
+    ray = np.zeros(1, ob.RAY_DTYPE)
+    ray["origin"] = [0, 0.5, -1]
+    ray["dir"] = [0, 0, 1]
+    ray["tmax"] = 1e4
+    # x slab: t1 = (0-0)*inf = NaN, t2 = (1-0)*inf = inf; min -> NaN kept as first operand
+    assert ob.oracle().sqo_slab(node.ctypes.data, ray.ctypes.data) == 1
+
+
+def test_heat_byte():
+    o = ob.oracle()
+    assert o.sqo_heat_byte(0, 1) == 0 and o.sqo_heat_byte(200, 1) == 255 and o.sqo_heat_byte(1000, 1) == 255
+    assert o.sqo_heat_byte(100, 1) == int(np.float32(0.5) * np.float32(255))
+    assert o.sqo_heat_byte(400, 4) == o.sqo_heat_byte(100, 1)
