@@ -1,0 +1,103 @@
+"""GPU binned-SAH and grid builders against the oracle (bit-exact Node1[])."""
+import numpy as np
+import pytest
+
+from conftest import golden, nodes_equal
+from oracle import binding as ob
+from scenes import GOLDEN_SCENES
+
+import sequoia_amd as sq
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+ORACLE = {
+    "recursive": lambda t, nb, lay: ob.build_sah(t, nb, ob.PART_HOARE, ob.SA_SCALAR, lay),
+    "horizontal": lambda t, nb, lay: ob.build_sah(t, nb, ob.PART_HOARE, ob.SA_SCALAR, lay),
+    "grid": lambda t, nb, lay: ob.build_grid(t, 4, nb, ob.PART_HOARE, ob.SA_SCALAR, ob.CELLBOX_SCENE, lay),
+    "grid_sse": lambda t, nb, lay: ob.build_grid(t, 4, nb, ob.PART_STABLE, ob.SA_SSE, ob.CELLBOX_TIGHT, lay),
+    "grid_avx": lambda t, nb, lay: ob.build_grid(t, 4, nb, ob.PART_STABLE, ob.SA_M128, ob.CELLBOX_TIGHT, lay),
+}
+
+
+@pytest.mark.parametrize("kind", list(ORACLE))
+@pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000", "ico69632"])
+def test_sah_reference_bins_bit_exact(ctx, name, kind):
+    say("sah", name, kind)
+    tris = GOLDEN_SCENES[name]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, kind, num_bins=8)
+    got = bvh.download()
+    want, root = ORACLE[kind](tris, 8, ob.LAYOUT_REFERENCE)
+    assert bvh.root == root
+    assert len(nodes_equal(got, want)) == 0
+
+
+@pytest.mark.parametrize("nb", [16, 32])
+@pytest.mark.parametrize("layout", [sq.LAYOUT_REFERENCE, sq.LAYOUT_FIXED])
+def test_sah_bins_and_layouts(ctx, nb, layout):
+    say("sah bins", nb, layout)
+    tris = sq.gen_soup(50000, 4, 0.01)
+    scene = ctx.upload(tris)
+    for kind in ("recursive", "grid_sse"):
+        got = ctx.build(scene, kind, num_bins=nb, layout=layout).download()
+        want, _ = ORACLE[kind](tris, nb, layout)
+        assert len(nodes_equal(got, want)) == 0
+
+
+@pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000"])
+def test_sah_matches_reference_fixture(ctx, name):
+    tris = GOLDEN_SCENES[name]()
+    g = golden(name)
+    scene = ctx.upload(tris)
+    for kind in ("recursive", "grid", "grid_sse"):
+        key = "nodes_" + kind
+        if key not in g:
+            continue
+        got = ctx.build(scene, kind).download()
+        assert len(nodes_equal(got, g[key])) == 0, kind
+
+
+def test_sah_degenerate_inputs(ctx):
+    rng = np.random.default_rng(3)
+    cases = {
+        "tiny": rng.random((5, 9), dtype=np.float32),
+        "nine": rng.random((9, 9), dtype=np.float32),
+        "collinear": (np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32) +
+                      np.arange(40, dtype=np.float32)[:, None] * np.array([0, 0, 0.01] * 3, np.float32)),
+        "duplicates": np.tile(rng.random((1, 9), dtype=np.float32), (3000, 1)),
+    }
+    for name, tris in cases.items():
+        say("degenerate", name)
+        scene = ctx.upload(tris)
+        for kind in ("recursive", "grid_sse", "grid"):
+            got = ctx.build(scene, kind).download()
+            want, _ = ORACLE[kind](tris, 8, ob.LAYOUT_REFERENCE)
+            assert len(nodes_equal(got, want)) == 0, (name, kind)
+
+
+def test_sah_one_million(ctx):
+    say("sah 1M soup, 32 bins")
+    tris = sq.gen_soup(1_000_000, 2, 0.01)
+    scene = ctx.upload(tris)
+    got = ctx.build(scene, "recursive", num_bins=32).download()
+    t = ctx.timings()
+    say("sah timings", round(t["sah_ms"], 3), "depth", t["tree_depth"])
+    want, _ = ob.build_sah(tris, 32)
+    assert len(nodes_equal(got, want)) == 0
+
+
+def test_trace_on_gpu_sah_tree(ctx, manifest):
+    name = "ico69632"
+    tris = GOLDEN_SCENES[name]()
+    tr = manifest["traces"][name]
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "recursive")
+    cam = sq.reference_camera(tris, tr["w"], tr["h"])
+    got = ctx.trace_camera(bvh, cam, want_hits=False)
+    import hashlib
+    assert hashlib.sha256(got["rgb"].tobytes()).hexdigest() == tr["basic_rgb_sha256"]
