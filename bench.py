@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--semantics", default="exhaustive", choices=["exhaustive", "closest"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of MIB MiB (byte-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -98,6 +100,12 @@ def main():
     d_cnt = ctx.alloc(W * H * 4)
     d_rgb = ctx.alloc(W * H * 3)
     sem = sq.TRACE_CLOSEST if a.semantics == "closest" else sq.TRACE_EXHAUSTIVE
+    if a.copy_calibration:
+        nb = a.copy_calibration << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        ctx.copy(dst, src, nb)
+        ctx.free(src)
+        ctx.free(dst)
 
     def step(count_work=False):
         bvh = ctx.build(scene, "morton63")

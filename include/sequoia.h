@@ -76,6 +76,9 @@ int sq_get_timings(sq_ctx* ctx, sq_timings* out);
 /* ---- device memory for callers without their own allocator ---- */
 int sq_device_alloc(sq_ctx* ctx, size_t bytes, void** out);
 void sq_device_free(sq_ctx* ctx, void* ptr);
+/* Device-to-device copy by a plain streaming kernel (also the reference
+ * kernel for calibrating hardware byte counters). */
+int sq_device_copy(sq_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Copies device -> host on the context stream and waits. */
 int sq_copy_to_host(sq_ctx* ctx, void* host_dst, const void* dev_src, size_t bytes);
 

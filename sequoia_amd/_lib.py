@@ -86,6 +86,7 @@ _SIGS = {
     "sq_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "sq_device_free": (None, [_P, _P]),
     "sq_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "sq_device_copy": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sq_scene_upload": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
     "sq_scene_wrap_device": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
     "sq_scene_free": (None, [_P]),
@@ -276,6 +277,9 @@ class Context:
 
     def free(self, p):
         lib().sq_device_free(self.h, p)
+
+    def copy(self, dst, src, nbytes):
+        _check(lib().sq_device_copy(self.h, dst, src, nbytes))
 
     def to_host(self, dev, nbytes, dtype):
         out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)

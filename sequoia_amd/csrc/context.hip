@@ -92,6 +92,12 @@ int launch_fill_u32(sq_ctx* c, void* dst, uint32_t value, size_t words) {
     return SQ_OK;
 }
 
+__global__ void k_copy_u32x4(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace sq
 
 using namespace sq;
@@ -167,6 +173,18 @@ int sq_device_alloc(sq_ctx* c, size_t bytes, void** out) {
 void sq_device_free(sq_ctx* c, void* p) {
     if (c) (void)hipSetDevice(c->device);
     if (p) (void)hipFree(p);
+}
+
+int sq_device_copy(sq_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        SQ_RET(SQ_ERR_INVALID, "sq_device_copy: pointers and size must be 16-byte aligned");
+    size_t n16 = bytes / 16, blocks = (n16 + 255) / 256, cap = (size_t)c->num_cus * 16;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) return SQ_OK;
+    hipLaunchKernelGGL(k_copy_u32x4, dim3((uint32_t)blocks), dim3(256), 0, c->stream, (uint4*)dst, (const uint4*)src, n16);
+    SQ_CHECK_HIP(hipGetLastError());
+    SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SQ_OK;
 }
 
 int sq_copy_to_host(sq_ctx* c, void* dst, const void* src, size_t bytes) {

@@ -8,8 +8,8 @@
 //   upsweep   each 4096-key tile builds its digit histogram; each wave counts
 //             its own 1024 keys with ballot-matched peer groups into a
 //             wave-private LDS row (no atomics),
-//   scan      one workgroup turns the digit-major [digit][tile] counts into
-//             exclusive global offsets,
+//   scan      one workgroup per digit scans that digit's tile counts, then
+//             one workgroup scans the 2^b digit totals,
 //   downsweep each tile ranks its keys stably (wave-private running counts +
 //             per-digit prefix over waves), stages the tile in LDS in digit
 //             order and writes contiguous runs per digit.
@@ -71,30 +71,54 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_upsweep(const K* __restrict__
     }
 }
 
-// Exclusive scan of `count` words in place, one workgroup of 1024 threads.
-constexpr int kScanThreads = 1024;
-__global__ __launch_bounds__(kScanThreads) void k_scan_exclusive(uint32_t* __restrict__ data, uint32_t count) {
-    __shared__ uint32_t s_part[kScanThreads];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (count + kScanThreads - 1) / kScanThreads;
-    const uint32_t lo = t * per, hi = (lo + per < count) ? lo + per : count;
-    uint32_t sum = 0;
-    for (uint32_t i = lo; i < hi; i++) sum += data[i];
-    s_part[t] = sum;
+// Offsets of the digit-major [digit][tile] counts in two steps: each digit
+// row is scanned over its tiles by its own workgroup (row prefix + row
+// total), then one workgroup scans the row totals into digit bases.
+constexpr int kScanThreads = 256;
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_rows(uint32_t* __restrict__ hist, uint32_t num_tiles,
+                                                            uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_w[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t* row = hist + (size_t)blockIdx.x * num_tiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < num_tiles; base += kScanThreads) {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = i < num_tiles ? row[i] : 0u;
+        uint32_t incl = wave_incl_scan(v, lane);
+        if (lane == 63) s_w[wave] = incl;
+        __syncthreads();
+        uint32_t wb = 0, tot = 0;
+        for (int w = 0; w < kScanThreads / 64; w++) {
+            uint32_t t = s_w[w];
+            wb += w < wave ? t : 0u;
+            tot += t;
+        }
+        if (i < num_tiles) row[i] = carry + wb + incl - v;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_totals(uint32_t* __restrict__ totals, uint32_t radix) {
+    __shared__ uint32_t s_w[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t v = threadIdx.x < radix ? totals[threadIdx.x] : 0u;
+    uint32_t incl = wave_incl_scan(v, lane);
+    if (lane == 63) s_w[wave] = incl;
     __syncthreads();
-    // Hillis-Steele inclusive scan over the 1024 partial sums.
-    for (uint32_t off = 1; off < kScanThreads; off <<= 1) {
-        uint32_t v = t >= off ? s_part[t - off] : 0u;
-        __syncthreads();
-        s_part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = t ? s_part[t - 1] : 0u;
-    for (uint32_t i = lo; i < hi; i++) {
-        uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
+    uint32_t wb = 0;
+    for (int w = 0; w < wave; w++) wb += s_w[w];
+    if (threadIdx.x < radix) totals[threadIdx.x] = wb + incl - v;
 }
 
 template <typename K>
@@ -102,7 +126,8 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
                                                              const uint32_t* __restrict__ vals_in,
                                                              K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                              uint32_t n, uint32_t shift, uint32_t bits,
-                                                             const uint32_t* __restrict__ offsets, uint32_t num_tiles) {
+                                                             const uint32_t* __restrict__ offsets,
+                                                             const uint32_t* __restrict__ digit_base, uint32_t num_tiles) {
     __shared__ uint32_t s_cnt[kRsWaves][kRadixMax];  // per-wave running counts, then per-wave prefix
     __shared__ uint32_t s_start[kRadixMax];          // tile-local start of each digit
     __shared__ uint32_t s_gbase[kRadixMax];          // global offset of each digit for this tile
@@ -112,7 +137,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
     const uint32_t radix = 1u << bits, mask = radix - 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (uint32_t i = threadIdx.x; i < kRsWaves * kRadixMax; i += kRsThreads) (&s_cnt[0][0])[i] = 0;
-    for (uint32_t d = threadIdx.x; d < radix; d += kRsThreads) s_gbase[d] = offsets[(size_t)d * num_tiles + blockIdx.x];
+    for (uint32_t d = threadIdx.x; d < radix; d += kRsThreads) s_gbase[d] = digit_base[d] + offsets[(size_t)d * num_tiles + blockIdx.x];
     __syncthreads();
 
     const size_t tile_base = (size_t)blockIdx.x * kRsTile;
@@ -192,7 +217,8 @@ template <typename K>
 static int radix_sort_t(sq_ctx* c, uint32_t n, uint32_t key_bits, uint32_t bits, int in, int* out_slot) {
     const uint32_t num_tiles = (n + kRsTile - 1) / kRsTile;
     const uint32_t radix = 1u << bits;
-    SQ_TRY(c->hist.ensure((size_t)radix * num_tiles * 4));
+    SQ_TRY(c->hist.ensure((size_t)radix * num_tiles * 4 + kRadixMax * 4));
+    uint32_t* totals = c->hist.as<uint32_t>() + (size_t)radix * num_tiles;
     const uint32_t passes = (key_bits + bits - 1) / bits;
     int cur = in;
     for (uint32_t p = 0; p < passes; p++) {
@@ -203,10 +229,11 @@ static int radix_sort_t(sq_ctx* c, uint32_t n, uint32_t key_bits, uint32_t bits,
         uint32_t* vout = c->vals[cur ^ 1].as<uint32_t>();
         hipLaunchKernelGGL(k_rs_upsweep<K>, dim3(num_tiles), dim3(kRsThreads), 0, c->stream, kin, n, shift, bits,
                            c->hist.as<uint32_t>(), num_tiles);
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(kScanThreads), 0, c->stream, c->hist.as<uint32_t>(),
-                           radix * num_tiles);
+        hipLaunchKernelGGL(k_scan_rows, dim3(radix), dim3(kScanThreads), 0, c->stream, c->hist.as<uint32_t>(),
+                           num_tiles, totals);
+        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, c->stream, totals, radix);
         hipLaunchKernelGGL(k_rs_downsweep<K>, dim3(num_tiles), dim3(kRsThreads), 0, c->stream, kin, vin, kout, vout, n,
-                           shift, bits, c->hist.as<uint32_t>(), num_tiles);
+                           shift, bits, c->hist.as<uint32_t>(), totals, num_tiles);
         SQ_CHECK_HIP(hipGetLastError());
         cur ^= 1;
     }

@@ -124,11 +124,21 @@ __device__ __forceinline__ NodeV load_node(const sq_node* __restrict__ nodes, ui
     return n;
 }
 
-// Slab test with the reference's operation order and selects.
-__device__ __forceinline__ bool slab(const NodeV& n, const RayF& r, float ix, float iy, float iz, float& front) {
+// Slab test with the reference's operation order and selects. When every
+// reciprocal direction component is finite no NaN can arise (|1/d| >= 1, so
+// (b - o) * inv is finite or +-inf), and IEEE min/max then differ from the
+// reference's ternaries only in the sign of a zero, which no comparison
+// below can observe; that path uses the hardware min3/max3 forms.
+__device__ __forceinline__ bool slab(const NodeV& n, const RayF& r, float ix, float iy, float iz, bool fast,
+                                     float& front) {
     float t1x = (n.mn[0] - r.ox) * ix, t2x = (n.mx[0] - r.ox) * ix;
     float t1y = (n.mn[1] - r.oy) * iy, t2y = (n.mx[1] - r.oy) * iy;
     float t1z = (n.mn[2] - r.oz) * iz, t2z = (n.mx[2] - r.oz) * iz;
+    if (fast) {
+        front = fmaxf(fmaxf(fminf(t1x, t2x), fminf(t1y, t2y)), fminf(t1z, t2z));
+        float back = fminf(fminf(fmaxf(t1x, t2x), fmaxf(t1y, t2y)), fmaxf(t1z, t2z));
+        return (back >= front) & (back <= r.tmax);
+    }
     float tnx = sqd::simd_min(t1x, t2x), tny = sqd::simd_min(t1y, t2y), tnz = sqd::simd_min(t1z, t2z);
     float txx = sqd::simd_max(t1x, t2x), txy = sqd::simd_max(t1y, t2y), txz = sqd::simd_max(t1z, t2z);
     front = sqd::simd_max(sqd::simd_max(tnx, tny), tnz);
@@ -187,80 +197,106 @@ __device__ __forceinline__ uint32_t encode_list(uint32_t parent, const NodeV& n)
     return (parent << 4) | 15u;
 }
 
-template <bool CLOSEST>
-__device__ void traverse(const TraceArgs& a, const RayF& r, Stack& st, uint32_t& count, uint32_t& tri_tests,
-                         uint32_t& best_id, float& best_t) {
-    const float ix = 1.0f / r.dx, iy = 1.0f / r.dy, iz = 1.0f / r.dz;
-    const float tlo = r.tmin > 0.000000001f ? r.tmin : 0.000000001f;
-    uint32_t first = a.root, cnt = 1;
+// Per-lane traversal state. One loop iteration tests up to two nodes of the
+// lane's current child list, so lanes advance in lock step through binary
+// lists; a lane whose ray is finished takes the next ray of the wave's tile
+// (dynamic refill), which keeps lanes busy while rays of very different
+// length share a wave.
+struct LaneRay {
+    RayF r;
+    float ix, iy, iz;
+    bool fast;
+    uint32_t first, cnt;  // remaining part of the current child list
+    uint32_t tests;
+    uint32_t best_id;
+    float best_t;
+};
+
+__device__ __forceinline__ void lane_begin(LaneRay& L, const RayF& r, uint32_t root, Stack& st) {
+    L.r = r;
+    L.ix = 1.0f / r.dx; L.iy = 1.0f / r.dy; L.iz = 1.0f / r.dz;
+    L.fast = isfinite(L.ix) && isfinite(L.iy) && isfinite(L.iz) && isfinite(r.ox) && isfinite(r.oy) && isfinite(r.oz);
+    L.first = root;
+    L.cnt = 1;
+    L.tests = 0;
+    L.best_id = SQ_MISS;
+    L.best_t = __int_as_float(0x7f800000);
     st.sp = 0;
-    while (true) {
-        uint32_t c0 = 0, c1 = 0, ncand = 0;
-        float f0 = 0.f, f1 = 0.f;
-        for (uint32_t i = 0; i < cnt; i++) {
-            const uint32_t ni = first + i;
-            if (ni >= a.node_count) { st.overflow = true; break; }
-            NodeV n = load_node(a.nodes, ni);
-            count++;
-            float front;
-            bool hit = slab(n, r, ix, iy, iz, front);
-            if (CLOSEST) hit = hit && !(front > best_t + best_t * (1.0f / 1048576.0f));
-            if (!hit) continue;
-            if (n.leaf) {
-                for (uint32_t j = 0; j < n.count; j++) {
-                    const uint32_t tid = n.child + j;
-                    if (tid >= a.tri_count) { st.overflow = true; break; }
-                    float v[9];
-                    sqd::load_tri(a.tris, tid, v);
-                    float t;
-                    tri_tests++;
-                    bool m = tri_test(v, r, t);
-                    if (m && t > r.tmin && t < r.tmax && (t < best_t || (t == best_t && tid < best_id))) {
-                        best_t = t;
-                        best_id = tid;
-                    }
-                }
-                (void)tlo;
-            } else {
-                uint32_t e = encode_list(ni, n);
-                if (CLOSEST) {
-                    if (ncand == 0) { c0 = e; f0 = front; ncand = 1; }
-                    else if (ncand == 1) { c1 = e; f1 = front; ncand = 2; }
-                    else push<true>(st, e, front);
-                } else {
-                    push<false>(st, e, 0.f);
-                }
-            }
+}
+
+__device__ __forceinline__ void leaf_tris(const TraceArgs& a, const NodeV& n, LaneRay& L, uint32_t& tri_tests,
+                                          bool& bad) {
+    for (uint32_t j = 0; j < n.count; j++) {
+        const uint32_t tid = n.child + j;
+        if (tid >= a.tri_count) { bad = true; break; }
+        float v[9];
+        sqd::load_tri(a.tris, tid, v);
+        float t;
+        tri_tests++;
+        bool m = tri_test(v, L.r, t);
+        if (m && t > L.r.tmin && t < L.r.tmax && (t < L.best_t || (t == L.best_t && tid < L.best_id))) {
+            L.best_t = t;
+            L.best_id = tid;
         }
-        if (CLOSEST && ncand) {
-            // Nearer list is taken next; the farther one waits on the stack.
-            if (ncand == 2) {
-                if (f1 < f0) { uint32_t te = c0; c0 = c1; c1 = te; float tf = f0; f0 = f1; f1 = tf; }
-                push<true>(st, c1, f1);
-            }
-            push<true>(st, c0, f0);
-        }
-        if (st.overflow) return;
-        // pop the next list
-        bool found = false;
-        while (st.sp > 0) {
-            float te = 0.f;
-            uint32_t e = pop<CLOSEST>(st, te);
-            if (CLOSEST && te > best_t + best_t * (1.0f / 1048576.0f)) continue;
-            if ((e & 15u) != 15u) {
-                first = e >> 4;
-                cnt = (e & 15u) + 1;
-            } else {
-                if ((e >> 4) >= a.node_count) { st.overflow = true; return; }
-                NodeV p = load_node(a.nodes, e >> 4);
-                first = p.child;
-                cnt = p.count;
-            }
-            found = true;
-            break;
-        }
-        if (!found) return;
     }
+}
+
+// One step: pop a list if the current one is exhausted, then test up to two
+// of its nodes. Returns false when the ray is finished.
+template <bool CLOSEST>
+__device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+    const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
+    while (L.cnt == 0) {
+        if (st.sp == 0) return false;
+        float te = 0.f;
+        uint32_t e = pop<CLOSEST>(st, te);
+        if (CLOSEST && te > cull) continue;
+        if ((e & 15u) != 15u) {
+            L.first = e >> 4;
+            L.cnt = (e & 15u) + 1;
+        } else {
+            if ((e >> 4) >= a.node_count) { st.overflow = true; return false; }
+            NodeV p = load_node(a.nodes, e >> 4);
+            L.first = p.child;
+            L.cnt = p.count;
+        }
+    }
+    const uint32_t i0 = L.first;
+    const bool two = L.cnt >= 2;
+    if (i0 >= a.node_count || (two && i0 + 1 >= a.node_count)) { st.overflow = true; return false; }
+    NodeV n0 = load_node(a.nodes, i0);
+    NodeV n1;
+    if (two) n1 = load_node(a.nodes, i0 + 1);
+    L.first += two ? 2 : 1;
+    L.cnt -= two ? 2 : 1;
+    L.tests += two ? 2 : 1;
+    float f0, f1 = 0.f;
+    bool h0 = slab(n0, L.r, L.ix, L.iy, L.iz, L.fast, f0);
+    bool h1 = two && slab(n1, L.r, L.ix, L.iy, L.iz, L.fast, f1);
+    if (CLOSEST) {
+        h0 = h0 && !(f0 > cull);
+        h1 = h1 && !(f1 > cull);
+    }
+    bool bad = false;
+    if (h0 && n0.leaf) leaf_tris(a, n0, L, tri_tests, bad);
+    if (h1 && n1.leaf) leaf_tris(a, n1, L, tri_tests, bad);
+    if (bad) { st.overflow = true; return false; }
+    const bool c0 = h0 && !n0.leaf, c1 = h1 && !n1.leaf;
+    if (c0 || c1) {
+        uint32_t e0 = c0 ? encode_list(i0, n0) : 0u, e1 = c1 ? encode_list(i0 + 1, n1) : 0u;
+        if (c0 && c1) {
+            // the nearer list is pushed last and popped first
+            bool swap = CLOSEST && f1 < f0;
+            uint32_t ea = swap ? e0 : e1, eb = swap ? e1 : e0;
+            float fa = swap ? f0 : f1, fb = swap ? f1 : f0;
+            push<CLOSEST>(st, ea, fa);
+            push<CLOSEST>(st, eb, fb);
+        } else {
+            push<CLOSEST>(st, c0 ? e0 : e1, c0 ? f0 : f1);
+        }
+        if (st.overflow) return false;
+    }
+    return true;
 }
 
 __device__ __forceinline__ uint8_t heat_byte(uint32_t count, uint32_t spp) {
@@ -271,69 +307,114 @@ __device__ __forceinline__ uint8_t heat_byte(uint32_t count, uint32_t spp) {
     return (uint8_t)(v * 255.0f);
 }
 
+constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode
+
 template <bool CLOSEST>
 __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
     extern __shared__ uint32_t s_dyn[];
-    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Stack st;
     st.stride = blockDim.x;
     st.S = a.slots;
     st.mask = a.slots - 1;
     st.lds = s_dyn + threadIdx.x;
     st.lds_t = CLOSEST ? (float*)(s_dyn + a.slots * blockDim.x) + threadIdx.x : nullptr;
+    uint32_t* s_cnt = s_dyn + a.slots * blockDim.x * (CLOSEST ? 2 : 1) + wave * 64;
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     st.cap = a.spill_cap;
     st.spill = a.spill + gtid * (size_t)a.spill_cap * (CLOSEST ? 2 : 1);
     st.overflow = false;
-    uint32_t tests_total = 0, tris_total = 0;
+    uint32_t tests_total = 0, tris_total = 0, overflow = 0;
     while (true) {
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(&a.work[0], 1u);
         tile = __shfl(tile, 0, 64);
         if (tile >= a.num_tiles) break;
+        uint32_t tx = 0, ty = 0, tile_rays;
         if (a.camera) {
-            const uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+            tx = tile % a.tiles_x;
+            ty = tile / a.tiles_x;
+            tile_rays = 64 * a.spp;
+            s_cnt[lane] = 0;
+        } else {
+            size_t base = (size_t)tile * kRayTile;
+            tile_rays = (uint32_t)((a.nrays - base) < kRayTile ? (a.nrays - base) : kRayTile);
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t next = 0;
+        bool active = false;
+        uint32_t my_ray = 0;
+        LaneRay L;
+        while (true) {
+            const uint64_t need = __ballot(!active);
+            if (need != 0 && next < tile_rays) {
+                const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+                const uint32_t r = next + rank;
+                if (!active && r < tile_rays) {
+                    if (a.camera) {
+                        const uint32_t p = r & 63u, smp = r >> 6;
+                        const uint32_t px = tx * 8 + (p & 7), py = ty * 8 + (p >> 3);
+                        if (px < a.w && py < a.h) {
+                            lane_begin(L, camera_ray(a, px, py, smp), a.root, st);
+                            my_ray = r;
+                            active = true;
+                        }
+                    } else {
+                        const size_t ri = (size_t)tile * kRayTile + r;
+                        const sq_ray q = a.rays[ri];
+                        RayF rf;
+                        rf.ox = q.origin.x; rf.oy = q.origin.y; rf.oz = q.origin.z;
+                        rf.dx = q.dir.x; rf.dy = q.dir.y; rf.dz = q.dir.z;
+                        rf.tmin = q.tmin; rf.tmax = q.tmax;
+                        lane_begin(L, rf, a.root, st);
+                        my_ray = r;
+                        active = true;
+                    }
+                }
+                const uint32_t taken = (uint32_t)__popcll(need);
+                next = (next + taken < tile_rays) ? next + taken : tile_rays;
+            }
+            if (__ballot(active) == 0) {
+                if (next >= tile_rays) break;
+                continue;
+            }
+            if (active) {
+                if (!lane_step<CLOSEST>(a, L, st, tris_total)) {
+                    // ray finished
+                    if (st.overflow) { overflow = 1; st.overflow = false; }
+                    tests_total += L.tests;
+                    size_t ri;
+                    if (a.camera) {
+                        const uint32_t p = my_ray & 63u, smp = my_ray >> 6;
+                        const uint32_t px = tx * 8 + (p & 7), py = ty * 8 + (p >> 3);
+                        ri = ((size_t)py * a.w + px) * a.spp + smp;
+                        atomicAdd(&s_cnt[p], L.tests);
+                    } else {
+                        ri = (size_t)tile * kRayTile + my_ray;
+                        if (a.counts) a.counts[ri] = L.tests;
+                    }
+                    if (a.hit_id) a.hit_id[ri] = L.best_id;
+                    if (a.hit_t) a.hit_t[ri] = L.best_t;
+                    active = false;
+                }
+            }
+        }
+        if (a.camera) {
+            __builtin_amdgcn_wave_barrier();
             const uint32_t px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
             if (px < a.w && py < a.h) {
-                uint32_t sum = 0;
-                for (uint32_t s = 0; s < a.spp; s++) {
-                    RayF r = camera_ray(a, px, py, s);
-                    uint32_t cnt = 0, best_id = SQ_MISS;
-                    float best_t = __int_as_float(0x7f800000);
-                    traverse<CLOSEST>(a, r, st, cnt, tris_total, best_id, best_t);
-                    sum += cnt;
-                    size_t ri = ((size_t)py * a.w + px) * a.spp + s;
-                    if (a.hit_id) a.hit_id[ri] = best_id;
-                    if (a.hit_t) a.hit_t[ri] = best_t;
-                }
-                tests_total += sum;
-                size_t p = (size_t)py * a.w + px;
+                const uint32_t sum = s_cnt[lane];
+                const size_t p = (size_t)py * a.w + px;
                 if (a.counts) a.counts[p] = sum;
                 if (a.rgb) {
                     uint8_t b = heat_byte(sum, a.spp);
                     a.rgb[3 * p] = b; a.rgb[3 * p + 1] = b; a.rgb[3 * p + 2] = b;
                 }
             }
-        } else {
-            const size_t ri = (size_t)tile * 64 + lane;
-            if (ri < a.nrays) {
-                const sq_ray q = a.rays[ri];
-                RayF r;
-                r.ox = q.origin.x; r.oy = q.origin.y; r.oz = q.origin.z;
-                r.dx = q.dir.x; r.dy = q.dir.y; r.dz = q.dir.z;
-                r.tmin = q.tmin; r.tmax = q.tmax;
-                uint32_t cnt = 0, best_id = SQ_MISS;
-                float best_t = __int_as_float(0x7f800000);
-                traverse<CLOSEST>(a, r, st, cnt, tris_total, best_id, best_t);
-                tests_total += cnt;
-                if (a.counts) a.counts[ri] = cnt;
-                if (a.hit_id) a.hit_id[ri] = best_id;
-                if (a.hit_t) a.hit_t[ri] = best_t;
-            }
+            __builtin_amdgcn_wave_barrier();
         }
-        if (st.overflow) atomicOr(&a.work[1], 1u);
-        st.overflow = false;
     }
+    if (overflow) atomicOr(&a.work[1], 1u);
     if (a.stats) {
         unsigned long long t0 = tests_total, t1 = tris_total;
         for (int off = 32; off > 0; off >>= 1) {
@@ -359,8 +440,14 @@ static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace
     while (pow2 < slots) pow2 <<= 1;
     slots = pow2 > 64 ? 64 : pow2;
     const uint32_t block = 128;
-    const uint32_t blocks_per_cu = 8;
-    const uint32_t grid = (uint32_t)c->num_cus * blocks_per_cu;
+    const size_t lds_bytes = ((size_t)slots * (closest ? 2 : 1) + 1) * block * 4;
+    int per_cu = 0;
+    if (closest)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true>, block, lds_bytes);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false>, block, lds_bytes);
+    if (per_cu < 1) per_cu = 8;
+    const uint32_t grid = (uint32_t)c->num_cus * (uint32_t)per_cu;
     const uint32_t depth = b->depth ? b->depth : 64;
     uint32_t cap = 2 * depth + 16;
     cap = cap > slots ? cap - slots + 8 : 8;
@@ -374,7 +461,7 @@ static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace
     a.work = c->counters.as<uint32_t>();
     a.stats = (p && p->count_work) ? c->stats.as<unsigned long long>() : nullptr;
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, c->stream, a.work, a.stats);
-    const size_t lds = (size_t)slots * block * 4 * (closest ? 2 : 1);
+    const size_t lds = lds_bytes;
     phase_begin(c, PH_TRACE);
     if (closest) hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(block), lds, c->stream, a);
     else hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(block), lds, c->stream, a);
@@ -464,7 +551,7 @@ int sq_trace_rays(sq_ctx* c, const sq_bvh* b, const sq_ray* rays, uint32_t n, co
     a.tri_count = b->scene->n;
     a.camera = 0;
     a.nrays = n;
-    a.num_tiles = (n + 63) / 64;
+    a.num_tiles = (n + kRayTile - 1) / kRayTile;
     if (io_on_device) {
         a.rays = rays;
         a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts;
