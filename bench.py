@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 N_TRI = 1_000_000
 W, H, SPP = 1920, 1080, 4
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "trace_traffic_soup1m.json")
 
 
 def log(*a):
@@ -138,6 +139,10 @@ def main():
     achieved = alg_bytes / (trace_ms / 1e3) / 1e9
 
     heat = ctx.to_host(d_cnt, W * H * 4, np.uint32)
+    traffic = None
+    if a.semantics == "exhaustive" and os.path.exists(TRAFFIC_PROFILE):
+        with open(TRAFFIC_PROFILE) as f:
+            traffic = json.load(f).get("traffic_bytes_per_launch")
     line = {
         "metric": "Mrays/s & build Mtri/s on 1 GPU; served p50/p99 latency, batched vs plain",
         "value": round(rays_per_s / 1e6, 2),
@@ -164,7 +169,7 @@ def main():
         "tri_tests_per_ray": round(tri_tests / nrays, 2),
         "heatmap_mean_tests_per_pixel": round(float(heat.mean()) / SPP, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_trace<false>", "algorithmic_bytes_per_launch": alg_bytes},
     }
     if not a.no_cpu:
