@@ -69,6 +69,12 @@ def build_library(jobs=8, verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lgomp"])
     os.replace(tmp, LIB)
+    # the reference-API demo caller used by the drop-in test
+    demo = os.path.join(HERE, "compat_demo")
+    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "compat", "compat_demo.cpp"), "-o", demo + ".tmp",
+          "-L" + HERE, "-lsequoia_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(demo + ".tmp", demo)
     return LIB
 
 

@@ -115,3 +115,34 @@ def test_small_stack_spills_correctly(ctx):
     a = ctx.trace_camera(bvh, cam, stack_slots=2)
     b = ctx.trace_camera(bvh, cam, stack_slots=32)
     assert np.array_equal(a["counts"], b["counts"]) and np.array_equal(a["hit_id"], b["hit_id"])
+
+
+@pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000", "ico69632"])
+def test_packet_and_bundle_heatmaps_match_reference(ctx, name, manifest):
+    say("packet/bundle", name)
+    tris = GOLDEN_SCENES[name]()
+    tr = manifest["traces"][name]
+    nodes, root = ob.build_sah(tris, 8)
+    scene = ctx.upload(tris)
+    bvh = ctx.bvh_from_host(scene, nodes, root)
+    cam = sq.reference_camera(tris, tr["w"], tr["h"])
+    for K in (1, 2, 4, 8):
+        got = ctx.trace_camera(bvh, cam, heatmap=sq.HEATMAP_PACKET, width=K, want_hits=False)
+        assert sha(got["rgb"]) == tr["packet%d_rgb_sha256" % K], K
+    for K in (16, 64):
+        got = ctx.trace_camera(bvh, cam, heatmap=sq.HEATMAP_BUNDLE, width=K, want_hits=False)
+        assert sha(got["rgb"]) == tr["bundle%d_rgb_sha256" % K], K
+
+
+def test_packet_and_bundle_on_gpu_morton_tree(ctx):
+    tris = GOLDEN_SCENES["ico69632"]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    nodes = bvh.download()
+    cam = sq.pinhole_camera(tris, 320, 200, 60.0, 3.0)
+    for K in (1, 8):
+        got = ctx.trace_camera(bvh, cam, heatmap=sq.HEATMAP_PACKET, width=K, want_hits=False)
+        assert np.array_equal(got["rgb"], ob.trace_packet(nodes, bvh.root, tris, cam, K))
+    for K in (16, 64):
+        got = ctx.trace_camera(bvh, cam, heatmap=sq.HEATMAP_BUNDLE, width=K, want_hits=False)
+        assert np.array_equal(got["rgb"], ob.trace_bundle(nodes, bvh.root, tris, cam, K))
