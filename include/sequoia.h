@@ -185,6 +185,63 @@ int sq_trace_camera(sq_ctx* ctx, const sq_bvh* bvh, const sq_camera* cam, const 
 int sq_trace_rays(sq_ctx* ctx, const sq_bvh* bvh, const sq_ray* rays, uint32_t n, const sq_trace_params* params,
                   const sq_trace_out* out, int io_on_device);
 
+/* ---- batches: many scenes built and traced together ----
+ * No counterpart in the reference, which builds and traces one Hierarchy per
+ * call (Builder::build_hierarchy_morton, builder.cpp:2474-2508; trace,
+ * tracer.cpp:195-230); every scene's tree and outputs equal what those calls
+ * produce for it alone (and what sq_build / sq_trace_camera produce).
+ *
+ * Triangles of all scenes live in one device buffer (scene s = counts[s]
+ * consecutive triangles). A build runs the Morton pipeline once over the
+ * concatenation (per-scene bounds and codes, one radix sort by (scene, code),
+ * per-scene emission and refit in shared launches). A trace runs one
+ * persistent launch whose waves steal 8x8-pixel tiles of every scene. */
+typedef struct sq_batch sq_batch;
+
+int sq_batch_create(sq_ctx* ctx, const sq_triangle* tris, const uint32_t* counts, uint32_t nscenes,
+                    int tris_on_device, sq_batch** out);
+void sq_batch_free(sq_batch* b);
+uint32_t sq_batch_count(const sq_batch* b);
+
+/* Morton builders only (MORTON, MORTON2, MORTON63); at most 2048 scenes. */
+int sq_batch_build(sq_ctx* ctx, sq_batch* b, sq_builder kind, const sq_build_params* params);
+/* Scene s's tree, owned by the batch (sq_bvh_free ignores it); usable with
+ * sq_bvh_info / sq_bvh_download / sq_trace_camera / sq_trace_rays. */
+int sq_batch_bvh(sq_batch* b, uint32_t scene, const sq_bvh** out);
+
+#define SQ_OUT_HIT_ID 1u
+#define SQ_OUT_HIT_T 2u
+#define SQ_OUT_COUNTS 4u
+#define SQ_OUT_RGB 8u
+/* One camera per scene; allocates batch-owned device outputs (mask of
+ * SQ_OUT_*). params: semantics, block_size, stack_slots, count_work. */
+int sq_batch_set_cameras(sq_ctx* ctx, sq_batch* b, const sq_camera* cams, const sq_trace_params* params,
+                         uint32_t outputs);
+/* Traces every scene's camera in one launch (needs a build and cameras). */
+int sq_batch_trace(sq_ctx* ctx, sq_batch* b);
+/* Build + trace as one HIP graph: captured on the first call (or after the
+ * builder, parameters or cameras change), replayed afterwards. */
+int sq_batch_run(sq_ctx* ctx, sq_batch* b, sq_builder kind, const sq_build_params* params);
+/* Device pointers of scene s's outputs (NULL where not requested). */
+int sq_batch_outputs(const sq_batch* b, uint32_t scene, sq_trace_out* device_out);
+/* Copies scene s's outputs to host arrays (any may be NULL). */
+int sq_batch_download(sq_ctx* ctx, const sq_batch* b, uint32_t scene, const sq_trace_out* host_out);
+/* Device time (ms) of the last sq_batch_build / sq_batch_trace / sq_batch_run. */
+int sq_batch_times(const sq_batch* b, float* build_ms, float* trace_ms, float* run_ms);
+
+/* Autotuner: times candidate radix digit widths (4..8) and subtree
+ * thresholds (512..2048) for sq_batch_build, and stack slots (8..32) and
+ * workgroup sizes (64, 128) for sq_batch_trace, on this batch (which must
+ * have cameras set), keeps the fastest in the batch and reports them. None
+ * of them changes any result. */
+typedef struct {
+    uint32_t radix_digit_bits, subtree_threshold, stack_slots, block_size;
+    float build_ms, trace_ms;           /* best times */
+    float default_build_ms, default_trace_ms;
+    uint32_t candidates_tried;
+} sq_tune_result;
+int sq_batch_autotune(sq_ctx* ctx, sq_batch* b, sq_builder kind, uint32_t repeats, sq_tune_result* out);
+
 /* ---- deterministic workload generators (host) ---- */
 uint32_t sq_gen_icosphere_count(uint32_t target);
 int sq_gen_icosphere(uint32_t target, uint32_t seed, float amplitude, sq_triangle* out);

@@ -60,6 +60,15 @@ class TraceOut(C.Structure):
     _fields_ = [("hit_id", C.c_void_p), ("hit_t", C.c_void_p), ("counts", C.c_void_p), ("rgb", C.c_void_p)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("radix_digit_bits", C.c_uint32), ("subtree_threshold", C.c_uint32), ("stack_slots", C.c_uint32),
+                ("block_size", C.c_uint32), ("build_ms", C.c_float), ("trace_ms", C.c_float),
+                ("default_build_ms", C.c_float), ("default_trace_ms", C.c_float), ("candidates_tried", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Timings(C.Structure):
     _fields_ = [("bounds_ms", C.c_float), ("morton_ms", C.c_float), ("sort_ms", C.c_float), ("emit_ms", C.c_float),
                 ("refit_ms", C.c_float), ("sah_ms", C.c_float), ("grid_ms", C.c_float),
@@ -103,6 +112,18 @@ _SIGS = {
     "sq_refit": (C.c_int, [_P, _P]),
     "sq_trace_camera": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
     "sq_trace_rays": (C.c_int, [_P, _P, _P, _U32, C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
+    "sq_batch_create": (C.c_int, [_P, _P, _P, _U32, C.c_int, C.POINTER(_P)]),
+    "sq_batch_free": (None, [_P]),
+    "sq_batch_count": (_U32, [_P]),
+    "sq_batch_build": (C.c_int, [_P, _P, C.c_int, C.POINTER(BuildParams)]),
+    "sq_batch_bvh": (C.c_int, [_P, _U32, C.POINTER(_P)]),
+    "sq_batch_set_cameras": (C.c_int, [_P, _P, _P, C.POINTER(TraceParams), _U32]),
+    "sq_batch_trace": (C.c_int, [_P, _P]),
+    "sq_batch_run": (C.c_int, [_P, _P, C.c_int, C.POINTER(BuildParams)]),
+    "sq_batch_outputs": (C.c_int, [_P, _U32, C.POINTER(TraceOut)]),
+    "sq_batch_download": (C.c_int, [_P, _P, _U32, C.POINTER(TraceOut)]),
+    "sq_batch_times": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "sq_batch_autotune": (C.c_int, [_P, _P, C.c_int, _U32, C.POINTER(TuneResult)]),
     "sq_gen_icosphere_count": (_U32, [_U32]),
     "sq_gen_icosphere": (C.c_int, [_U32, _U32, C.c_float, _P]),
     "sq_gen_soup": (C.c_int, [_U32, _U32, C.c_float, _P]),
@@ -243,6 +264,101 @@ class Bvh:
             self.free()
         except Exception:
             pass
+
+
+OUT_HIT_ID, OUT_HIT_T, OUT_COUNTS, OUT_RGB = 1, 2, 4, 8
+OUT_ALL = 15
+
+
+class BatchBvh(Bvh):
+    """Tree of one scene of a batch (storage owned by the batch)."""
+
+    def free(self):
+        self.h = None
+
+
+class Batch:
+    """Many scenes built and traced together (sq_batch_*)."""
+
+    def __init__(self, ctx, scenes):
+        self.ctx = ctx
+        arrs = [as_tris(t) for t in scenes]
+        self.counts = np.array([len(a) for a in arrs], np.uint32)
+        self._tris = np.ascontiguousarray(np.concatenate(arrs))
+        h = C.c_void_p()
+        _check(lib().sq_batch_create(ctx.h, _ptr(self._tris), _ptr(self.counts), len(arrs), 0, C.byref(h)))
+        self.h = h
+        self.cams = None
+
+    def __len__(self):
+        return len(self.counts)
+
+    def free(self):
+        if self.h:
+            lib().sq_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _bp(kind, digit_bits, subtree_threshold, layout, morton_bits=0):
+        return BuildParams(morton_bits, digit_bits, 0, 0, layout, 0, subtree_threshold, 0)
+
+    def build(self, kind="morton63", digit_bits=0, subtree_threshold=0, layout=LAYOUT_REFERENCE):
+        p = self._bp(kind, digit_bits, subtree_threshold, layout)
+        _check(lib().sq_batch_build(self.ctx.h, self.h, BUILDERS[kind], C.byref(p)))
+
+    def bvh(self, i):
+        h = C.c_void_p()
+        _check(lib().sq_batch_bvh(self.h, i, C.byref(h)))
+        return BatchBvh(self.ctx, h, None)
+
+    def set_cameras(self, cams, semantics=TRACE_EXHAUSTIVE, outputs=OUT_ALL, stack_slots=0, block_size=0,
+                    count_work=False):
+        arr = (Camera * len(cams))(*cams)
+        p = TraceParams(semantics, HEATMAP_RAY, 0, block_size, stack_slots, 1 if count_work else 0,
+                        (C.c_uint32 * 2)())
+        _check(lib().sq_batch_set_cameras(self.ctx.h, self.h, arr, C.byref(p), outputs))
+        self.cams = list(cams)
+        self._cam_arr = arr
+
+    def trace(self):
+        _check(lib().sq_batch_trace(self.ctx.h, self.h))
+
+    def run(self, kind="morton63", digit_bits=0, subtree_threshold=0, layout=LAYOUT_REFERENCE):
+        p = self._bp(kind, digit_bits, subtree_threshold, layout)
+        _check(lib().sq_batch_run(self.ctx.h, self.h, BUILDERS[kind], C.byref(p)))
+
+    def times(self):
+        b, t, r = C.c_float(), C.c_float(), C.c_float()
+        _check(lib().sq_batch_times(self.h, C.byref(b), C.byref(t), C.byref(r)))
+        return {"build_ms": b.value, "trace_ms": t.value, "run_ms": r.value}
+
+    def outputs(self, i):
+        o = TraceOut()
+        _check(lib().sq_batch_outputs(self.h, i, C.byref(o)))
+        return o
+
+    def download(self, i, hits=True, counts=True, rgb=True):
+        cam = self.cams[i]
+        npix = cam.width * cam.height
+        nray = npix * cam.spp
+        hid = np.empty(nray, np.uint32) if hits else None
+        ht = np.empty(nray, np.float32) if hits else None
+        cnt = np.empty(npix, np.uint32) if counts else None
+        im = np.empty((cam.height, cam.width, 3), np.uint8) if rgb else None
+        out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), _ptr(im))
+        _check(lib().sq_batch_download(self.ctx.h, self.h, i, C.byref(out)))
+        return {"hit_id": hid, "hit_t": ht, "counts": cnt, "rgb": im}
+
+    def autotune(self, kind="morton63", repeats=2):
+        r = TuneResult()
+        _check(lib().sq_batch_autotune(self.ctx.h, self.h, BUILDERS[kind], repeats, C.byref(r)))
+        return r.as_dict()
 
 
 class Context:

@@ -405,7 +405,7 @@ int sq_bvh_download(const sq_bvh* b, sq_node* host, uint32_t capacity) {
 const sq_node* sq_bvh_device_nodes(const sq_bvh* b) { return b ? b->d_nodes : nullptr; }
 
 void sq_bvh_free(sq_bvh* b) {
-    if (!b) return;
+    if (!b || b->view) return;
     if (b->d_nodes) (void)hipFree(b->d_nodes);
     delete b;
 }

@@ -47,9 +47,24 @@ constexpr uint32_t kNoTask = 0xFFFFFFFFu;
 
 // A range of the sorted primitive array that still has to be split.
 struct Task {
-    uint32_t begin, end;  // [begin, end) in sorted order
-    uint32_t free;        // first free node slot for this range's children
-    uint32_t self;        // node slot of the range's own interior node
+    uint32_t begin, end;  // [begin, end) in sorted order (positions in the batch)
+    uint32_t free;        // first free node slot for this range's children (scene-local)
+    uint32_t self;        // node slot of the range's own interior node (scene-local)
+    uint32_t scene;       // scene of the batch
+};
+
+// One scene of a batched Morton build. Positions index the batch's sorted
+// key/value arrays; node slots and leaf triangle ids written into the
+// scene's Node1[] are scene-local, as in a plain build.
+struct SceneDesc {
+    uint32_t seg_begin;   // first sorted position of the scene
+    uint32_t n;           // triangles
+    uint32_t tri_base;    // first triangle of the scene in the batch's triangle array
+    uint32_t node_base;   // first node of the scene in the batch's node array
+    uint32_t q_base;      // offset of the scene's top-level task queues
+    uint32_t top_base;    // offset of the scene's top node list
+    uint32_t lvl_base;    // offset of the scene's level table
+    uint32_t pad;
 };
 
 // Events bracketing the phases of the last build/trace.
@@ -85,6 +100,7 @@ struct sq_bvh {
     uint32_t capacity = 0;
     uint32_t root = 0;
     uint32_t depth = 0;  // levels from the root to the deepest node (root = 1)
+    bool view = false;   // part of a batch: storage owned by the batch
 };
 
 namespace sq {
@@ -102,9 +118,28 @@ int scene_bounds(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t* d_ord
 int morton_codes(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t bits, const uint32_t* d_ord6,
                  void* d_keys, uint32_t* d_vals, bool key64);
 // Sorts (keys, vals) in c->keys/c->vals slot `in`; returns the slot holding the result.
+struct SortBufs {
+    hipStream_t stream;
+    void* keys[2];
+    uint32_t* vals[2];
+    uint32_t* hist;  // sort_hist_bytes(n, digit_bits)
+};
+size_t sort_hist_bytes(uint32_t n, uint32_t digit_bits);
+uint32_t sort_segment_passes(uint32_t nseg, uint32_t digit_bits);
+// Key passes, then (d_seg_off != NULL, nseg > 1) segment passes that group
+// the pairs by the segment holding each value (nseg <= 2048).
+int radix_sort_bufs(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_t digit_bits, bool key64,
+                    const uint32_t* d_seg_off, uint32_t nseg, int in, int* out_slot, uint32_t* passes);
 int radix_sort(sq_ctx* c, uint32_t n, uint32_t key_bits, uint32_t digit_bits, bool key64, int in, int* out_slot);
 int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_codes, const uint32_t* d_idx,
               bool key64, uint32_t layout, bool refit, uint32_t threshold, sq_node* d_nodes, uint32_t* depth);
+// Batched emission: scenes[0..nscenes) on device; no host synchronisation.
+// `ws` must hold lbvh_batch_workspace(...) bytes.
+size_t lbvh_batch_workspace(uint32_t total_tris, uint32_t nscenes, uint32_t threshold);
+void lbvh_fill_descs(SceneDesc* descs, const uint32_t* counts, uint32_t nscenes, uint32_t threshold);
+int lbvh_emit_batch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total_tris, const void* d_codes,
+                    const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
+                    uint32_t threshold, sq_node* d_nodes, void* ws, uint32_t* d_depth_out);
 int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_params* p, sq_builder kind,
               sq_node* d_nodes, uint32_t* root, uint32_t* depth);
 int refit_tree(sq_ctx* c, sq_bvh* bvh);
@@ -185,6 +220,49 @@ __device__ __forceinline__ Box3 load_node_box(const sq_node* nodes, uint32_t i) 
     b.mn[0] = a.x; b.mn[1] = a.y; b.mn[2] = a.z;
     b.mx[0] = c.x; b.mx[1] = c.y; b.mx[2] = c.z;
     return b;
+}
+
+// Spread the low 10 bits of x to bits 0,3,...,27.
+__device__ __forceinline__ uint32_t expand10(uint32_t x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+// Spread the low 21 bits of x to bits 0,3,...,60.
+__device__ __forceinline__ uint64_t expand21(uint64_t x) {
+    x &= 0x1FFFFFull;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// Truncating float -> unsigned as the reference's x86-64 build does for the
+// values the quantiser produces; NaN (zero-extent axis) -> 0.
+__device__ __forceinline__ uint32_t quantise(float v) {
+    if (!(v == v) || v <= 0.0f) return 0u;
+    return (uint32_t)v;
+}
+
+// setup_morton's code of one triangle: box centre normalised to the scene
+// box (smn, ext = max - min) and truncated to `scale` = 2^bits - 1.
+template <bool KEY64>
+__device__ __forceinline__ uint64_t morton_code(const float v[9], const float smn[3], const float ext[3], float scale) {
+    Box3 b = tri_box(v);
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float cc = (b.mn[k] + b.mx[k]) * 0.5f;
+        cc = (cc - smn[k]) / ext[k];
+        q[k] = quantise(cc * scale);
+    }
+    if (KEY64) return expand21(q[0]) | (expand21(q[1]) << 1) | (expand21(q[2]) << 2);
+    return expand10(q[0]) | (expand10(q[1]) << 1) | (expand10(q[2]) << 2);
 }
 
 }  // namespace sqd

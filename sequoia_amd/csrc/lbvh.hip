@@ -13,13 +13,17 @@
 // Boxes (no reference counterpart, the reference writes none): leaf = the
 // triangle's bounds; interior = union of its child list in list order.
 //
-// Three kernels, no cross-workgroup waiting:
-//   top    one workgroup splits every range larger than T level by level
-//          (one wave per range, 65-ary search on the sorted codes) and
-//          queues ranges of 8..T primitives,
-//   small  one workgroup per queued range: codes in LDS, the reference's
-//          binary search per thread, level by level, then refit bottom-up,
-//   refit  one workgroup refits the nodes written by `top`, deepest first.
+// Three kernels, no cross-workgroup waiting, any number of scenes per launch
+// (scene s: sorted range [seg_begin, seg_begin + n), nodes at node_base,
+// leaf child = triangle index - tri_base):
+//   top    one workgroup per scene splits every range larger than T level
+//          by level (one wave per range, 65-ary search on the sorted codes)
+//          and queues ranges of 8..T primitives,
+//   small  persistent workgroups take queued ranges of every scene: codes in
+//          LDS, the reference's binary search per thread, level by level,
+//          then refit bottom-up,
+//   finish one workgroup per scene refits the nodes written by `top`,
+//          deepest first, and records the scene's depth.
 #include "internal.h"
 
 namespace sq {
@@ -37,29 +41,31 @@ __device__ __forceinline__ int clz_k(K x) {
 }
 
 struct EmitArgs {
-    const sq_triangle* tris;
-    const uint32_t* idx;
-    sq_node* nodes;
+    const sq_triangle* tris;  // the batch's triangles (global ids)
+    const uint32_t* idx;      // sorted global triangle ids
+    sq_node* nodes;           // the scene's Node1[] (scene-local slots)
     uint32_t layout;
     uint32_t refit;
-    uint32_t cap;  // node capacity: writes at or beyond it are dropped
+    uint32_t cap;             // node capacity of the scene: writes at or beyond it are dropped
+    uint32_t tri_base;        // first global triangle id of the scene
 };
 
-__device__ __forceinline__ void write_leaf(const EmitArgs& a, uint32_t slot, uint32_t tri, sqd::Box3* box) {
+__device__ __forceinline__ void write_leaf(const EmitArgs& a, uint32_t slot, uint32_t gtri, sqd::Box3* box) {
     if (slot >= a.cap) return;
+    const uint32_t local = gtri - a.tri_base;
     if (a.refit) {
         float v[9];
-        sqd::load_tri(a.tris, tri, v);
+        sqd::load_tri(a.tris, gtri, v);
         sqd::Box3 b = sqd::tri_box(v);
-        sqd::store_node(a.nodes, slot, b, tri, 1, 1);
+        sqd::store_node(a.nodes, slot, b, local, 1, 1);
         if (box) *box = b;
     } else {
-        sqd::store_node_links(a.nodes, slot, tri, 1, 1);
+        sqd::store_node_links(a.nodes, slot, local, 1, 1);
     }
 }
 
-// Writes the two children of range [b, e) split after `split`; returns the
-// child ranges that still need splitting (count >= 8) through the callback.
+// Writes the two children of range [b, e) split after `split`; hands the
+// child ranges that still need splitting (count >= 8) to the callback.
 template <typename Push>
 __device__ void emit_children(const EmitArgs& a, uint32_t b, uint32_t e, uint32_t f, uint32_t split, Push push) {
     const uint32_t count = e - b;
@@ -165,37 +171,57 @@ __device__ uint32_t find_split_ref(const K* codes, uint32_t count) {
     return (uint32_t)split;
 }
 
-struct TopArgs {
-    EmitArgs e;
+struct LbvhArgs {
+    const sq_triangle* tris;
+    const uint32_t* idx;
     const void* codes;
-    uint32_t n, threshold;
-    Task* q[2];
-    Task* small;
-    uint32_t* counters;  // [0] small count, [1] top node count, [2] levels
-    uint32_t* top_list;
-    uint32_t* top_level_off;
+    sq_node* nodes;
+    const SceneDesc* scenes;
+    uint32_t layout, refit, threshold;
+    Task* qa;                  // per-scene regions at SceneDesc::q_base
+    Task* qb;
+    Task* small;               // ranges of 8..T primitives of every scene
+    uint32_t* top_list;        // per-scene regions at SceneDesc::top_base
+    uint32_t* level_off;       // per-scene tables at SceneDesc::lvl_base
+    uint32_t* scene_levels;    // per scene: number of top levels
+    uint32_t* scene_small;     // per scene: most levels of one small range
+    uint32_t* depth_out;       // per scene: tree depth (root = 1), may be null
+    uint32_t* counters;        // [0] small-task count, [1] max small-range levels
 };
 
+__device__ __forceinline__ EmitArgs scene_emit(const LbvhArgs& a, const SceneDesc& d) {
+    return EmitArgs{a.tris, a.idx, a.nodes + d.node_base, a.layout, a.refit, 2 * d.n, d.tri_base};
+}
+
+// One workgroup per scene: splits every range larger than T level by level.
 template <typename K>
-__global__ __launch_bounds__(kTopThreads) void k_lbvh_top(TopArgs a) {
-    __shared__ uint32_t s_ncur, s_nnext, s_nsmall, s_ntop, s_cur;
+__global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
+    __shared__ uint32_t s_ncur, s_nnext, s_ntop, s_cur;
+    const uint32_t scene = blockIdx.x;
+    const SceneDesc d = a.scenes[scene];
+    const EmitArgs e = scene_emit(a, d);
     const K* codes = (const K*)a.codes;
+    Task* q[2] = {a.qa + d.q_base, a.qb + d.q_base};
+    uint32_t* top_list = a.top_list + d.top_base;
+    uint32_t* level_off = a.level_off + d.lvl_base;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = kTopThreads / 64;
     if (threadIdx.x == 0) {
-        s_ncur = 0; s_nnext = 0; s_nsmall = 0; s_ntop = 0; s_cur = 0;
-        if (a.n < 8) {
+        s_ncur = 0; s_nnext = 0; s_ntop = 0; s_cur = 0;
+        if (d.n < 8) {
             sqd::Box3 u, lb;
-            for (uint32_t i = 0; i < a.n; i++) {
-                write_leaf(a.e, 1 + i, a.e.idx[i], &lb);
+            for (uint32_t i = 0; i < d.n; i++) {
+                write_leaf(e, 1 + i, a.idx[d.seg_begin + i], &lb);
                 if (i == 0) u = lb; else sqd::box_grow(u, lb);
             }
-            if (a.e.refit) sqd::store_node(a.e.nodes, 0, u, 1, a.n, 0);
-            else sqd::store_node_links(a.e.nodes, 0, 1, a.n, 0);
+            if (d.n > 0) {
+                if (a.refit) sqd::store_node(e.nodes, 0, u, 1, d.n, 0);
+                else sqd::store_node_links(e.nodes, 0, 1, d.n, 0);
+            }
         } else {
-            sqd::store_node_links(a.e.nodes, 0, 1, 2, 0);
-            Task t{0, a.n, 1, 0};
-            if (a.n > a.threshold) { a.q[0][0] = t; s_ncur = 1; }
-            else { a.small[0] = t; s_nsmall = 1; }
+            sqd::store_node_links(e.nodes, 0, 1, 2, 0);
+            Task t{d.seg_begin, d.seg_begin + d.n, 1, 0, scene};
+            if (d.n > a.threshold) { q[0][0] = t; s_ncur = 1; }
+            else a.small[atomicAdd(&a.counters[0], 1u)] = t;
         }
     }
     __syncthreads();
@@ -203,21 +229,21 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(TopArgs a) {
     while (true) {
         const uint32_t ncur = s_ncur;
         if (ncur == 0) break;
-        Task* cur = a.q[s_cur];
-        Task* nxt = a.q[s_cur ^ 1];
-        if (threadIdx.x == 0 && level < kMaxLevels) a.top_level_off[level] = s_ntop;
+        Task* cur = q[s_cur];
+        Task* nxt = q[s_cur ^ 1];
+        if (threadIdx.x == 0 && level < kMaxLevels) level_off[level] = s_ntop;
         __syncthreads();
         for (uint32_t t = wave; t < ncur; t += nwaves) {
             Task tk = cur[t];
             uint32_t split = wave_find_split<K>(codes, tk.begin, tk.end, lane);
             if (lane == 0) {
-                emit_children(a.e, tk.begin, tk.end, tk.free, split,
-                              [&](uint32_t b, uint32_t e, uint32_t f, uint32_t self) {
-                                  Task c{b, e, f, self};
-                                  if (e - b > a.threshold) nxt[atomicAdd(&s_nnext, 1u)] = c;
-                                  else a.small[atomicAdd(&s_nsmall, 1u)] = c;
+                emit_children(e, tk.begin, tk.end, tk.free, split,
+                              [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
+                                  Task c{b, en, f, self, scene};
+                                  if (en - b > a.threshold) nxt[atomicAdd(&s_nnext, 1u)] = c;
+                                  else a.small[atomicAdd(&a.counters[0], 1u)] = c;
                               });
-                a.top_list[atomicAdd(&s_ntop, 1u)] = tk.self;
+                top_list[atomicAdd(&s_ntop, 1u)] = tk.self;
             }
         }
         __syncthreads();
@@ -230,22 +256,15 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(TopArgs a) {
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (level < kMaxLevels) a.top_level_off[level] = s_ntop;
-        a.counters[0] = s_nsmall;
-        a.counters[1] = s_ntop;
-        a.counters[2] = level;
+        if (level < kMaxLevels) level_off[level] = s_ntop;
+        a.scene_levels[scene] = level;
     }
 }
 
-struct SmallArgs {
-    EmitArgs e;
-    const void* codes;
-    const Task* small;
-    uint32_t* counters;  // [3] max small depth
-};
-
+// Persistent: one workgroup finishes one queued range at a time (codes in
+// LDS, the reference's binary search per thread), then refits it.
 template <typename K>
-__global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(SmallArgs a) {
+__global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
     __shared__ K s_codes[kMaxThreshold];
     __shared__ Task s_tasks[kMaxThreshold / 4 + 2];
     __shared__ uint32_t s_lvl[kMaxLevels + 1];
@@ -253,9 +272,10 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(SmallArgs a) {
     const K* codes = (const K*)a.codes;
     const uint32_t ntasks = a.counters[0];
     uint32_t max_lev = 0;
-    // Persistent loop: the number of queued ranges is only known on the device.
     for (uint32_t ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
         const Task root = a.small[ti];
+        const SceneDesc d = a.scenes[root.scene];
+        const EmitArgs e = scene_emit(a, d);
         const uint32_t base = root.begin, count = root.end - root.begin;
         for (uint32_t i = threadIdx.x; i < count; i += kSmallThreads) s_codes[i] = codes[base + i];
         if (threadIdx.x == 0) {
@@ -269,9 +289,9 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(SmallArgs a) {
             for (uint32_t i = lb + threadIdx.x; i < le; i += kSmallThreads) {
                 Task tk = s_tasks[i];
                 uint32_t split = find_split_ref<K>(s_codes + (tk.begin - base), tk.end - tk.begin);
-                emit_children(a.e, tk.begin, tk.end, tk.free, split,
-                              [&](uint32_t b, uint32_t e, uint32_t f, uint32_t self) {
-                                  s_tasks[atomicAdd(&s_total, 1u)] = Task{b, e, f, self};
+                emit_children(e, tk.begin, tk.end, tk.free, split,
+                              [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
+                                  s_tasks[atomicAdd(&s_total, 1u)] = Task{b, en, f, self, root.scene};
                               });
             }
             __syncthreads();
@@ -283,96 +303,171 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(SmallArgs a) {
         if (nlev > (uint32_t)kMaxLevels) nlev = kMaxLevels;
         if (threadIdx.x == 0) s_lvl[nlev] = le;
         max_lev = nlev > max_lev ? nlev : max_lev;
+        if (threadIdx.x == 0) atomicMax(&a.scene_small[root.scene], nlev);
         __syncthreads();
-        if (a.e.refit) {
+        if (a.refit) {
             for (int lv = (int)nlev - 1; lv >= 0; lv--) {
                 for (uint32_t i = s_lvl[lv] + threadIdx.x; i < s_lvl[lv + 1]; i += kSmallThreads)
-                    refit_node(a.e.nodes, s_tasks[i].self, a.e.cap);
+                    refit_node(e.nodes, s_tasks[i].self, e.cap);
                 __syncthreads();
             }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0 && max_lev) atomicMax(&a.counters[3], max_lev);
+    if (threadIdx.x == 0 && max_lev) atomicMax(&a.counters[1], max_lev);
 }
 
-__global__ __launch_bounds__(kTopThreads) void k_lbvh_top_refit(sq_node* nodes, uint32_t cap, const uint32_t* top_list,
-                                                                 const uint32_t* top_level_off,
-                                                                 const uint32_t* counters) {
-    if (counters[2] >= (uint32_t)kMaxLevels) return;
-    const uint32_t levels = counters[2];
+// One workgroup per scene: refits the nodes the top phase wrote, deepest
+// first, and records the scene's depth.
+__global__ __launch_bounds__(kTopThreads) void k_lbvh_finish(LbvhArgs a) {
+    const uint32_t scene = blockIdx.x;
+    const SceneDesc d = a.scenes[scene];
+    const uint32_t levels = a.scene_levels[scene];
+    if (threadIdx.x == 0 && a.depth_out) a.depth_out[scene] = levels + a.scene_small[scene] + 2;
+    if (!a.refit || levels >= (uint32_t)kMaxLevels) return;
+    sq_node* nodes = a.nodes + d.node_base;
+    const uint32_t cap = 2 * d.n;
+    const uint32_t* top_list = a.top_list + d.top_base;
+    const uint32_t* level_off = a.level_off + d.lvl_base;
     for (int lv = (int)levels - 1; lv >= 0; lv--) {
-        uint32_t lo = top_level_off[lv], hi = top_level_off[lv + 1];
+        uint32_t lo = level_off[lv], hi = level_off[lv + 1];
         for (uint32_t i = lo + threadIdx.x; i < hi; i += kTopThreads) refit_node(nodes, top_list[i], cap);
         __syncthreads();
     }
 }
 
-__global__ void k_zero_counters(uint32_t* c, int n) {
-    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+__global__ void k_zero_counters(uint32_t* c, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = 0;
+}
+
+static uint32_t clamp_threshold(uint32_t threshold) {
+    uint32_t T = threshold ? threshold : kDefaultThreshold;
+    if (T < 8) T = 8;
+    if (T > kMaxThreshold) T = kMaxThreshold;
+    return T;
+}
+
+// Scene descriptors for a concatenation of scenes: scene s covers triangles
+// (and sorted positions) [tri_base, tri_base + n) and nodes [2 tri_base,
+// 2 (tri_base + n)); its queue, top-list and level-table regions are prefix
+// sums of n/T + 4, 2 n/T + 8 and kMaxLevels + 2 entries.
+void lbvh_fill_descs(SceneDesc* d, const uint32_t* counts, uint32_t nscenes, uint32_t threshold) {
+    const uint32_t T = clamp_threshold(threshold);
+    uint32_t tri = 0, q = 0, top = 0;
+    for (uint32_t s = 0; s < nscenes; s++) {
+        const uint32_t n = counts[s];
+        d[s] = SceneDesc{tri, n, tri, 2 * tri, q, top, s * (uint32_t)(kMaxLevels + 2), 0};
+        tri += n;
+        q += n / T + 4;
+        top += 2 * (n / T) + 8;
+    }
+}
+
+struct WsLayout {
+    size_t qa, qb, small, top, lvl, counters, scene_levels, scene_small, total;
+};
+static WsLayout ws_layout(uint32_t total, uint32_t nscenes, uint32_t T) {
+    WsLayout w;
+    size_t q = (size_t)total / T + 4 * (size_t)nscenes + 4;
+    size_t sm = (size_t)total / 8 + nscenes + 4;
+    size_t tp = 2 * ((size_t)total / T) + 8 * (size_t)nscenes + 8;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    w.qa = take(q * sizeof(Task));
+    w.qb = take(q * sizeof(Task));
+    w.small = take(sm * sizeof(Task));
+    w.top = take(tp * 4);
+    w.lvl = take((size_t)nscenes * (kMaxLevels + 2) * 4);
+    // counters, scene_levels and scene_small are contiguous (zeroed together)
+    w.counters = off;
+    off += 64;
+    w.scene_levels = off;
+    off += (size_t)nscenes * 4;
+    w.scene_small = off;
+    off += (size_t)nscenes * 4;
+    off = (off + 255) & ~(size_t)255;
+    w.total = off;
+    return w;
+}
+
+size_t lbvh_batch_workspace(uint32_t total, uint32_t nscenes, uint32_t threshold) {
+    return ws_layout(total, nscenes, clamp_threshold(threshold)).total;
+}
+
+static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total, const void* d_codes,
+                       const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
+                       bool refit, uint32_t T, sq_node* d_nodes, void* ws, uint32_t* d_depth, bool split_refit_phase) {
+    WsLayout w = ws_layout(total, nscenes, T);
+    char* base = (char*)ws;
+    LbvhArgs a;
+    a.tris = tris;
+    a.idx = d_idx;
+    a.codes = d_codes;
+    a.nodes = d_nodes;
+    a.scenes = d_scenes;
+    a.layout = layout;
+    a.refit = refit ? 1u : 0u;
+    a.threshold = T;
+    a.qa = (Task*)(base + w.qa);
+    a.qb = (Task*)(base + w.qb);
+    a.small = (Task*)(base + w.small);
+    a.top_list = (uint32_t*)(base + w.top);
+    a.level_off = (uint32_t*)(base + w.lvl);
+    a.scene_levels = (uint32_t*)(base + w.scene_levels);
+    a.scene_small = (uint32_t*)(base + w.scene_small);
+    a.depth_out = d_depth;
+    a.counters = (uint32_t*)(base + w.counters);
+    const uint32_t zero_words = 16 + 2 * nscenes;
+    uint32_t zb = (zero_words + 255) / 256;
+    zb = zb > 64 ? 64 : zb;
+    hipLaunchKernelGGL(k_zero_counters, dim3(zb), dim3(256), 0, stream, a.counters, zero_words);
+    if (key64) hipLaunchKernelGGL(k_lbvh_top<uint64_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+    else hipLaunchKernelGGL(k_lbvh_top<uint32_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+    uint32_t max_tasks = (uint32_t)(total / 8 + nscenes);
+    uint32_t grid = (uint32_t)c->num_cus * 8;
+    if (grid > max_tasks) grid = max_tasks;
+    if (grid == 0) grid = 1;
+    if (key64) hipLaunchKernelGGL(k_lbvh_small<uint64_t>, dim3(grid), dim3(kSmallThreads), 0, stream, a);
+    else hipLaunchKernelGGL(k_lbvh_small<uint32_t>, dim3(grid), dim3(kSmallThreads), 0, stream, a);
+    if (refit && split_refit_phase) { phase_end(c, PH_EMIT); phase_begin(c, PH_REFIT); }
+    hipLaunchKernelGGL(k_lbvh_finish, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+    if (refit && split_refit_phase) { phase_end(c, PH_REFIT); phase_begin(c, PH_EMIT); }
+    SQ_CHECK_HIP(hipGetLastError());
+    return SQ_OK;
+}
+
+int lbvh_emit_batch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total, const void* d_codes,
+                    const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
+                    uint32_t threshold, sq_node* d_nodes, void* ws, uint32_t* d_depth_out) {
+    return lbvh_launch(c, stream, tris, total, d_codes, d_idx, key64, d_scenes, nscenes, layout, true,
+                       clamp_threshold(threshold), d_nodes, ws, d_depth_out, false);
 }
 
 int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_codes, const uint32_t* d_idx, bool key64,
               uint32_t layout, bool refit, uint32_t threshold, sq_node* d_nodes, uint32_t* depth) {
-    uint32_t T = threshold ? threshold : kDefaultThreshold;
-    if (T < 8) T = 8;
-    if (T > kMaxThreshold) T = kMaxThreshold;
-    const size_t qcap = n / T + 4;
-    const size_t scap = n / 8 + 4;
-    const size_t tcap = 2 * (n / T) + 8;
-    SQ_TRY(c->tasks[0].ensure(qcap * sizeof(Task)));
-    SQ_TRY(c->tasks[1].ensure(qcap * sizeof(Task)));
-    SQ_TRY(c->tasks[2].ensure(scap * sizeof(Task)));
-    SQ_TRY(c->aux[1].ensure(tcap * 4));
-    SQ_TRY(c->aux[2].ensure((kMaxLevels + 2) * 4));
-    SQ_TRY(c->counters.ensure(64));
-    uint32_t* counters = c->counters.as<uint32_t>();
-    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, c->stream, counters, 16);
-
-    EmitArgs ea{tris, d_idx, d_nodes, layout, refit ? 1u : 0u, 2 * n};
-    TopArgs ta;
-    ta.e = ea;
-    ta.codes = d_codes;
-    ta.n = n;
-    ta.threshold = T;
-    ta.q[0] = c->tasks[0].as<Task>();
-    ta.q[1] = c->tasks[1].as<Task>();
-    ta.small = c->tasks[2].as<Task>();
-    ta.counters = counters;
-    ta.top_list = c->aux[1].as<uint32_t>();
-    ta.top_level_off = c->aux[2].as<uint32_t>();
-    if (key64) hipLaunchKernelGGL(k_lbvh_top<uint64_t>, dim3(1), dim3(kTopThreads), 0, c->stream, ta);
-    else hipLaunchKernelGGL(k_lbvh_top<uint32_t>, dim3(1), dim3(kTopThreads), 0, c->stream, ta);
-    SQ_CHECK_HIP(hipGetLastError());
-
-    {
-        SmallArgs sa{ea, d_codes, c->tasks[2].as<Task>(), counters};
-        uint32_t max_tasks = (uint32_t)(n / 8 + 1);
-        uint32_t grid = (uint32_t)c->num_cus * 8;
-        if (grid > max_tasks) grid = max_tasks;
-        if (key64) hipLaunchKernelGGL(k_lbvh_small<uint64_t>, dim3(grid), dim3(kSmallThreads), 0, c->stream, sa);
-        else hipLaunchKernelGGL(k_lbvh_small<uint32_t>, dim3(grid), dim3(kSmallThreads), 0, c->stream, sa);
-        SQ_CHECK_HIP(hipGetLastError());
-    }
-    if (refit) {
-        phase_end(c, PH_EMIT);
-        phase_begin(c, PH_REFIT);
-        hipLaunchKernelGGL(k_lbvh_top_refit, dim3(1), dim3(kTopThreads), 0, c->stream, d_nodes, 2 * n, ta.top_list,
-                           ta.top_level_off, counters);
-        SQ_CHECK_HIP(hipGetLastError());
-        phase_end(c, PH_REFIT);
-        phase_begin(c, PH_EMIT);
-    }
-    SQ_TRY(pinned_ensure(c, 64));
+    const uint32_t T = clamp_threshold(threshold);
+    WsLayout w = ws_layout(n, 1, T);
+    SQ_TRY(c->aux[1].ensure(w.total + 256));
+    SQ_TRY(c->aux[2].ensure(sizeof(SceneDesc)));
+    SQ_TRY(pinned_ensure(c, 256));
+    SceneDesc* hd = (SceneDesc*)((char*)c->pinned + 128);
+    *hd = SceneDesc{0, n, 0, 0, 0, 0, 0, 0};
+    SQ_CHECK_HIP(hipMemcpyAsync(c->aux[2].p, hd, sizeof(SceneDesc), hipMemcpyHostToDevice, c->stream));
+    char* base = c->aux[1].as<char>();
+    uint32_t* d_depth = (uint32_t*)(base + w.total);
+    SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit, T,
+                       d_nodes, c->aux[1].p, d_depth, true));
     uint32_t* h = (uint32_t*)c->pinned;
-    SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 16, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 8, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipMemcpyAsync(h + 2, base + w.scene_levels, 4, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipMemcpyAsync(h + 3, d_depth, 4, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
     const uint32_t nsmall = h[0], top_levels = h[2];
     if (top_levels >= (uint32_t)kMaxLevels) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: more than %d top levels", kMaxLevels);
     c->last.top_levels = top_levels;
     c->last.small_tasks = nsmall;
     // levels of split ranges + fan/leaf level + root
-    *depth = top_levels + h[3] + 2;
+    *depth = h[3];
     return SQ_OK;
 }
 

@@ -72,33 +72,6 @@ int scene_bounds(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t* d_ord
     return SQ_OK;
 }
 
-// Spread the low 10 bits of x to bits 0,3,...,27.
-__device__ __forceinline__ uint32_t expand10(uint32_t x) {
-    x &= 0x3FFu;
-    x = (x | (x << 16)) & 0x030000FFu;
-    x = (x | (x << 8)) & 0x0300F00Fu;
-    x = (x | (x << 4)) & 0x030C30C3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-}
-// Spread the low 21 bits of x to bits 0,3,...,60.
-__device__ __forceinline__ uint64_t expand21(uint64_t x) {
-    x &= 0x1FFFFFull;
-    x = (x | (x << 32)) & 0x1F00000000FFFFull;
-    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
-    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
-    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
-    x = (x | (x << 2)) & 0x1249249249249249ull;
-    return x;
-}
-
-// Truncating float -> unsigned as the reference's x86-64 build does for the
-// values the quantiser produces; NaN (zero-extent axis) -> 0.
-__device__ __forceinline__ uint32_t quantise(float v) {
-    if (!(v == v) || v <= 0.0f) return 0u;
-    return (uint32_t)v;
-}
-
 template <bool KEY64>
 __global__ __launch_bounds__(256) void k_morton(const sq_triangle* __restrict__ tris, uint32_t n, uint32_t bits,
                                                 const uint32_t* __restrict__ ord6, void* __restrict__ keys_out,
@@ -113,21 +86,8 @@ __global__ __launch_bounds__(256) void k_morton(const sq_triangle* __restrict__ 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float v[9];
         sqd::load_tri(tris, i, v);
-        sqd::Box3 b = sqd::tri_box(v);
-        uint32_t q[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            float cc = (b.mn[k] + b.mx[k]) * 0.5f;
-            cc = (cc - smn[k]) / ext[k];
-            q[k] = quantise(cc * scale);
-        }
-        if (KEY64) {
-            uint64_t code = expand21(q[0]) | (expand21(q[1]) << 1) | (expand21(q[2]) << 2);
-            ((uint64_t*)keys_out)[i] = code;
-        } else {
-            uint32_t code = expand10(q[0]) | (expand10(q[1]) << 1) | (expand10(q[2]) << 2);
-            ((uint32_t*)keys_out)[i] = code;
-        }
+        if (KEY64) ((uint64_t*)keys_out)[i] = sqd::morton_code<true>(v, smn, ext, scale);
+        else ((uint32_t*)keys_out)[i] = (uint32_t)sqd::morton_code<false>(v, smn, ext, scale);
         vals_out[i] = i;
     }
 }

@@ -15,6 +15,12 @@
 //             order and writes contiguous runs per digit.
 // Algorithmic traffic per pass: read keys (upsweep) + read keys/values +
 // write keys/values (downsweep).
+//
+// Segment passes (batched builds): the digit is taken from the segment that
+// holds the pair's value (values are indices into a concatenation of
+// segments, offsets in LDS, binary search). Run after the key passes they
+// group the pairs by segment while keeping each segment's key order, i.e.
+// sort by (segment, key, value) with the keys carried along.
 #include "internal.h"
 
 namespace sq {
@@ -25,6 +31,26 @@ constexpr int kRsRounds = 16;                          // 64-key rounds per wave
 constexpr int kRsWaveKeys = 64 * kRsRounds;            // 1024 keys per wave
 constexpr int kRsTile = kRsWaveKeys * kRsWaves;        // 4096 keys per tile
 constexpr int kRadixMax = 256;
+constexpr uint32_t kMaxSegs = 2048;  // segments per segment pass (LDS table)
+
+// Digit source of a pass: the key, or the segment of the value.
+struct SegTable {
+    const uint32_t* off;  // nseg + 1 ascending offsets, off[0] = 0
+    uint32_t nseg;
+};
+
+__device__ __forceinline__ void load_seg_lds(uint32_t* s_off, const SegTable& t) {
+    for (uint32_t i = threadIdx.x; i <= t.nseg; i += blockDim.x) s_off[i] = t.off[i];
+}
+// Largest s with off[s] <= v (v < off[nseg]).
+__device__ __forceinline__ uint32_t seg_of(const uint32_t* s_off, uint32_t nseg, uint32_t v) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 
 template <typename K>
 __device__ __forceinline__ uint32_t digit_of(K key, uint32_t shift, uint32_t mask) {
@@ -41,14 +67,17 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint32_t bits, uint6
     return peers;
 }
 
-template <typename K>
-__global__ __launch_bounds__(kRsThreads) void k_rs_upsweep(const K* __restrict__ keys, uint32_t n, uint32_t shift,
-                                                           uint32_t bits, uint32_t* __restrict__ hist,
-                                                           uint32_t num_tiles) {
+template <typename K, bool SEG>
+__global__ __launch_bounds__(kRsThreads) void k_rs_upsweep(const K* __restrict__ keys,
+                                                           const uint32_t* __restrict__ vals, uint32_t n,
+                                                           uint32_t shift, uint32_t bits, uint32_t* __restrict__ hist,
+                                                           uint32_t num_tiles, SegTable seg) {
     __shared__ uint32_t s_cnt[kRsWaves][kRadixMax];
+    __shared__ uint32_t s_off[SEG ? kMaxSegs + 1 : 1];
     const uint32_t radix = 1u << bits, mask = radix - 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (uint32_t i = threadIdx.x; i < kRsWaves * kRadixMax; i += kRsThreads) (&s_cnt[0][0])[i] = 0;
+    if (SEG) load_seg_lds(s_off, seg);
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * kRsTile + (size_t)wave * kRsWaveKeys;
     for (int r = 0; r < kRsRounds; r++) {
@@ -56,7 +85,8 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_upsweep(const K* __restrict__
         bool valid = idx < n;
         uint64_t active = __ballot(valid);
         if (active == 0) break;
-        uint32_t d = valid ? digit_of(keys[idx], shift, mask) : 0u;
+        uint32_t d = 0;
+        if (valid) d = SEG ? (seg_of(s_off, seg.nseg, vals[idx]) >> shift) & mask : digit_of(keys[idx], shift, mask);
         uint64_t peers = match_digit(d, bits, active);
         if (valid) {
             uint64_t lower = peers & ((1ull << lane) - 1ull);
@@ -121,29 +151,32 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_totals(uint32_t* __restri
     if (threadIdx.x < radix) totals[threadIdx.x] = wb + incl - v;
 }
 
-template <typename K>
+template <typename K, bool SEG>
 __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict__ keys_in,
                                                              const uint32_t* __restrict__ vals_in,
                                                              K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
                                                              uint32_t n, uint32_t shift, uint32_t bits,
                                                              const uint32_t* __restrict__ offsets,
-                                                             const uint32_t* __restrict__ digit_base, uint32_t num_tiles) {
+                                                             const uint32_t* __restrict__ digit_base, uint32_t num_tiles,
+                                                             SegTable seg) {
     __shared__ uint32_t s_cnt[kRsWaves][kRadixMax];  // per-wave running counts, then per-wave prefix
     __shared__ uint32_t s_start[kRadixMax];          // tile-local start of each digit
     __shared__ uint32_t s_gbase[kRadixMax];          // global offset of each digit for this tile
     __shared__ K s_keys[kRsTile];
     __shared__ uint32_t s_vals[kRsTile];
+    __shared__ uint32_t s_off[SEG ? kMaxSegs + 1 : 1];
 
     const uint32_t radix = 1u << bits, mask = radix - 1;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (uint32_t i = threadIdx.x; i < kRsWaves * kRadixMax; i += kRsThreads) (&s_cnt[0][0])[i] = 0;
+    if (SEG) load_seg_lds(s_off, seg);
     for (uint32_t d = threadIdx.x; d < radix; d += kRsThreads) s_gbase[d] = digit_base[d] + offsets[(size_t)d * num_tiles + blockIdx.x];
     __syncthreads();
 
     const size_t tile_base = (size_t)blockIdx.x * kRsTile;
     const size_t base = tile_base + (size_t)wave * kRsWaveKeys;
     K key[kRsRounds];
-    uint32_t val[kRsRounds], rank[kRsRounds];
+    uint32_t val[kRsRounds], rank[kRsRounds], dig[kRsRounds];
 #pragma unroll
     for (int r = 0; r < kRsRounds; r++) {
         size_t idx = base + (size_t)r * 64 + lane;
@@ -156,7 +189,9 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
         size_t idx = base + (size_t)r * 64 + lane;
         bool valid = idx < n;
         uint64_t active = __ballot(valid);
-        uint32_t d = digit_of(key[r], shift, mask);
+        uint32_t d = SEG ? (valid ? (seg_of(s_off, seg.nseg, val[r]) >> shift) & mask : 0u)
+                         : digit_of(key[r], shift, mask);
+        dig[r] = d;
         rank[r] = 0;
         if (active == 0) continue;
         uint64_t peers = match_digit(d, bits, active);
@@ -192,7 +227,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
     for (int r = 0; r < kRsRounds; r++) {
         size_t idx = base + (size_t)r * 64 + lane;
         if (idx < n) {
-            uint32_t d = digit_of(key[r], shift, mask);
+            uint32_t d = dig[r];
             uint32_t pos = s_start[d] + s_cnt[wave][d] + rank[r];
             if (pos < (uint32_t)kRsTile) {
                 s_keys[pos] = key[r];
@@ -204,7 +239,7 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
     const uint32_t tile_count = (uint32_t)((n - tile_base) < (size_t)kRsTile ? (n - tile_base) : (size_t)kRsTile);
     for (uint32_t i = threadIdx.x; i < tile_count; i += kRsThreads) {
         K k = s_keys[i];
-        uint32_t d = digit_of(k, shift, mask);
+        uint32_t d = SEG ? (seg_of(s_off, seg.nseg, s_vals[i]) >> shift) & mask : digit_of(k, shift, mask);
         size_t out = (size_t)s_gbase[d] + (i - s_start[d]);
         if (out < n) {
             keys_out[out] = k;
@@ -214,38 +249,74 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
 }
 
 template <typename K>
-static int radix_sort_t(sq_ctx* c, uint32_t n, uint32_t key_bits, uint32_t bits, int in, int* out_slot) {
+static int sort_passes(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_t bits, uint32_t seg_passes,
+                       SegTable seg, int in, int* out_slot, uint32_t* passes_out) {
     const uint32_t num_tiles = (n + kRsTile - 1) / kRsTile;
     const uint32_t radix = 1u << bits;
-    SQ_TRY(c->hist.ensure((size_t)radix * num_tiles * 4 + kRadixMax * 4));
-    uint32_t* totals = c->hist.as<uint32_t>() + (size_t)radix * num_tiles;
-    const uint32_t passes = (key_bits + bits - 1) / bits;
+    uint32_t* hist = b.hist;
+    uint32_t* totals = hist + (size_t)radix * num_tiles;
+    const uint32_t kpasses = (key_bits + bits - 1) / bits;
     int cur = in;
-    for (uint32_t p = 0; p < passes; p++) {
-        uint32_t shift = p * bits;
-        const K* kin = c->keys[cur].as<K>();
-        const uint32_t* vin = c->vals[cur].as<uint32_t>();
-        K* kout = c->keys[cur ^ 1].as<K>();
-        uint32_t* vout = c->vals[cur ^ 1].as<uint32_t>();
-        hipLaunchKernelGGL(k_rs_upsweep<K>, dim3(num_tiles), dim3(kRsThreads), 0, c->stream, kin, n, shift, bits,
-                           c->hist.as<uint32_t>(), num_tiles);
-        hipLaunchKernelGGL(k_scan_rows, dim3(radix), dim3(kScanThreads), 0, c->stream, c->hist.as<uint32_t>(),
-                           num_tiles, totals);
-        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, c->stream, totals, radix);
-        hipLaunchKernelGGL(k_rs_downsweep<K>, dim3(num_tiles), dim3(kRsThreads), 0, c->stream, kin, vin, kout, vout, n,
-                           shift, bits, c->hist.as<uint32_t>(), totals, num_tiles);
+    for (uint32_t p = 0; p < kpasses + seg_passes; p++) {
+        const bool segp = p >= kpasses;
+        const uint32_t shift = (segp ? p - kpasses : p) * bits;
+        const K* kin = (const K*)b.keys[cur];
+        const uint32_t* vin = b.vals[cur];
+        K* kout = (K*)b.keys[cur ^ 1];
+        uint32_t* vout = b.vals[cur ^ 1];
+        if (segp)
+            hipLaunchKernelGGL((k_rs_upsweep<K, true>), dim3(num_tiles), dim3(kRsThreads), 0, b.stream, kin, vin, n,
+                               shift, bits, hist, num_tiles, seg);
+        else
+            hipLaunchKernelGGL((k_rs_upsweep<K, false>), dim3(num_tiles), dim3(kRsThreads), 0, b.stream, kin, vin, n,
+                               shift, bits, hist, num_tiles, seg);
+        hipLaunchKernelGGL(k_scan_rows, dim3(radix), dim3(kScanThreads), 0, b.stream, hist, num_tiles, totals);
+        hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(kScanThreads), 0, b.stream, totals, radix);
+        if (segp)
+            hipLaunchKernelGGL((k_rs_downsweep<K, true>), dim3(num_tiles), dim3(kRsThreads), 0, b.stream, kin, vin,
+                               kout, vout, n, shift, bits, hist, totals, num_tiles, seg);
+        else
+            hipLaunchKernelGGL((k_rs_downsweep<K, false>), dim3(num_tiles), dim3(kRsThreads), 0, b.stream, kin, vin,
+                               kout, vout, n, shift, bits, hist, totals, num_tiles, seg);
         SQ_CHECK_HIP(hipGetLastError());
         cur ^= 1;
     }
-    c->last.sort_passes = passes;
+    if (passes_out) *passes_out = kpasses + seg_passes;
     *out_slot = cur;
     return SQ_OK;
 }
 
+size_t sort_hist_bytes(uint32_t n, uint32_t digit_bits) {
+    const size_t num_tiles = (n + kRsTile - 1) / kRsTile;
+    return ((size_t)1 << digit_bits) * num_tiles * 4 + kRadixMax * 4;
+}
+
+uint32_t sort_segment_passes(uint32_t nseg, uint32_t digit_bits) {
+    uint32_t segbits = 0;
+    while ((1u << segbits) < nseg) segbits++;
+    return (segbits + digit_bits - 1) / digit_bits;
+}
+
+int radix_sort_bufs(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_t digit_bits, bool key64,
+                    const uint32_t* d_seg_off, uint32_t nseg, int in, int* out_slot, uint32_t* passes) {
+    if (n == 0) { *out_slot = in; if (passes) *passes = 0; return SQ_OK; }
+    if (digit_bits < 1 || digit_bits > 8) SQ_RET(SQ_ERR_INVALID, "radix digit bits %u outside 1..8", digit_bits);
+    if (nseg > kMaxSegs) SQ_RET(SQ_ERR_INVALID, "radix sort: %u segments exceed %u", nseg, kMaxSegs);
+    SegTable seg{d_seg_off, nseg};
+    const uint32_t sp = (d_seg_off && nseg > 1) ? sort_segment_passes(nseg, digit_bits) : 0;
+    if (key64) return sort_passes<uint64_t>(b, n, key_bits, digit_bits, sp, seg, in, out_slot, passes);
+    return sort_passes<uint32_t>(b, n, key_bits, digit_bits, sp, seg, in, out_slot, passes);
+}
+
 int radix_sort(sq_ctx* c, uint32_t n, uint32_t key_bits, uint32_t digit_bits, bool key64, int in, int* out_slot) {
     if (n == 0) { *out_slot = in; return SQ_OK; }
-    if (key64) return radix_sort_t<uint64_t>(c, n, key_bits, digit_bits, in, out_slot);
-    return radix_sort_t<uint32_t>(c, n, key_bits, digit_bits, in, out_slot);
+    SQ_TRY(c->hist.ensure(sort_hist_bytes(n, digit_bits)));
+    SortBufs b{c->stream, {c->keys[0].p, c->keys[1].p}, {c->vals[0].as<uint32_t>(), c->vals[1].as<uint32_t>()},
+               c->hist.as<uint32_t>()};
+    uint32_t passes = 0;
+    SQ_TRY(radix_sort_bufs(b, n, key_bits, digit_bits, key64, nullptr, 0, in, out_slot, &passes));
+    c->last.sort_passes = passes;
+    return SQ_OK;
 }
 
 }  // namespace sq

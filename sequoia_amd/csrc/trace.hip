@@ -128,7 +128,101 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
 
 constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode
 
+// One tile of job J: the wave's lanes take the tile's rays one after
+// another (per-lane refill) until all are traced.
 template <bool CLOSEST>
+__device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, uint32_t lane, Stack& st,
+                                           uint32_t* s_cnt, uint32_t& tests_total, uint32_t& tris_total,
+                                           uint32_t& overflow) {
+    uint32_t tx = 0, ty = 0, tile_rays;
+    if (a.camera) {
+        tx = tile % a.tiles_x;
+        ty = tile / a.tiles_x;
+        tile_rays = 64 * a.spp;
+        s_cnt[lane] = 0;
+    } else {
+        size_t base = (size_t)tile * kRayTile;
+        tile_rays = (uint32_t)((a.nrays - base) < kRayTile ? (a.nrays - base) : kRayTile);
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t next = 0;
+    bool active = false;
+    uint32_t my_ray = 0;
+    LaneRay L;
+    while (true) {
+        const uint64_t need = __ballot(!active);
+        if (need != 0 && next < tile_rays) {
+            const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
+            const uint32_t r = next + rank;
+            if (!active && r < tile_rays) {
+                if (a.camera) {
+                    const uint32_t p = r & 63u, smp = r >> 6;
+                    const uint32_t px = tx * 8 + (p & 7), py = ty * 8 + (p >> 3);
+                    if (px < a.w && py < a.h) {
+                        lane_begin(L, camera_ray(a, px, py, smp), a.root, st);
+                        my_ray = r;
+                        active = true;
+                    }
+                } else {
+                    const size_t ri = (size_t)tile * kRayTile + r;
+                    const sq_ray q = a.rays[ri];
+                    RayF rf;
+                    rf.ox = q.origin.x; rf.oy = q.origin.y; rf.oz = q.origin.z;
+                    rf.dx = q.dir.x; rf.dy = q.dir.y; rf.dz = q.dir.z;
+                    rf.tmin = q.tmin; rf.tmax = q.tmax;
+                    lane_begin(L, rf, a.root, st);
+                    my_ray = r;
+                    active = true;
+                }
+            }
+            const uint32_t taken = (uint32_t)__popcll(need);
+            next = (next + taken < tile_rays) ? next + taken : tile_rays;
+        }
+        if (__ballot(active) == 0) {
+            if (next >= tile_rays) break;
+            continue;
+        }
+        if (active) {
+            if (!lane_step<CLOSEST>(a, L, st, tris_total)) {
+                // ray finished
+                if (st.overflow) { overflow = 1; st.overflow = false; }
+                tests_total += L.tests;
+                size_t ri;
+                if (a.camera) {
+                    const uint32_t p = my_ray & 63u, smp = my_ray >> 6;
+                    const uint32_t px = tx * 8 + (p & 7), py = ty * 8 + (p >> 3);
+                    ri = ((size_t)py * a.w + px) * a.spp + smp;
+                    atomicAdd(&s_cnt[p], L.tests);
+                } else {
+                    ri = (size_t)tile * kRayTile + my_ray;
+                    if (a.counts) a.counts[ri] = L.tests;
+                }
+                if (a.hit_id) a.hit_id[ri] = L.best_id;
+                if (a.hit_t) a.hit_t[ri] = L.best_t;
+                active = false;
+            }
+        }
+    }
+    if (a.camera) {
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+        if (px < a.w && py < a.h) {
+            const uint32_t sum = s_cnt[lane];
+            const size_t p = (size_t)py * a.w + px;
+            if (a.counts) a.counts[p] = sum;
+            if (a.rgb) {
+                uint8_t b = heat_byte(sum, a.spp);
+                a.rgb[3 * p] = b; a.rgb[3 * p + 1] = b; a.rgb[3 * p + 2] = b;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// MULTI: the launch covers a.njobs jobs (tree + rays + outputs each); tile t
+// of the launch is tile t - job_tiles[j] of the job j with job_tiles[j] <= t
+// < job_tiles[j + 1]. Waves steal tiles from one counter across all jobs.
+template <bool CLOSEST, bool MULTI>
 __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
     extern __shared__ uint32_t s_dyn[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -149,88 +243,29 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
         if (lane == 0) tile = atomicAdd(&a.work[0], 1u);
         tile = __shfl(tile, 0, 64);
         if (tile >= a.num_tiles) break;
-        uint32_t tx = 0, ty = 0, tile_rays;
-        if (a.camera) {
-            tx = tile % a.tiles_x;
-            ty = tile / a.tiles_x;
-            tile_rays = 64 * a.spp;
-            s_cnt[lane] = 0;
+        if (MULTI) {
+            // The job table is read-only here: constant-address-space loads
+            // keep the wave-uniform job in scalar registers.
+#if defined(__HIP_DEVICE_COMPILE__)
+            typedef const __attribute__((address_space(4))) TraceArgs CJob;
+            typedef const __attribute__((address_space(4))) uint32_t CU32;
+#else
+            typedef const TraceArgs CJob;  // host pass: never executed
+            typedef const uint32_t CU32;
+#endif
+            const CJob* jobs = (const CJob*)a.jobs;
+            const CU32* jt = (const CU32*)a.job_tiles;
+            tile = __builtin_amdgcn_readfirstlane(tile);
+            uint32_t lo = 0, hi = a.njobs;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (jt[mid] <= tile) lo = mid;
+                else hi = mid;
+            }
+            const TraceArgs J = jobs[lo];
+            trace_tile<CLOSEST>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, overflow);
         } else {
-            size_t base = (size_t)tile * kRayTile;
-            tile_rays = (uint32_t)((a.nrays - base) < kRayTile ? (a.nrays - base) : kRayTile);
-        }
-        __builtin_amdgcn_wave_barrier();
-        uint32_t next = 0;
-        bool active = false;
-        uint32_t my_ray = 0;
-        LaneRay L;
-        while (true) {
-            const uint64_t need = __ballot(!active);
-            if (need != 0 && next < tile_rays) {
-                const uint32_t rank = (uint32_t)__popcll(need & ((1ull << lane) - 1ull));
-                const uint32_t r = next + rank;
-                if (!active && r < tile_rays) {
-                    if (a.camera) {
-                        const uint32_t p = r & 63u, smp = r >> 6;
-                        const uint32_t px = tx * 8 + (p & 7), py = ty * 8 + (p >> 3);
-                        if (px < a.w && py < a.h) {
-                            lane_begin(L, camera_ray(a, px, py, smp), a.root, st);
-                            my_ray = r;
-                            active = true;
-                        }
-                    } else {
-                        const size_t ri = (size_t)tile * kRayTile + r;
-                        const sq_ray q = a.rays[ri];
-                        RayF rf;
-                        rf.ox = q.origin.x; rf.oy = q.origin.y; rf.oz = q.origin.z;
-                        rf.dx = q.dir.x; rf.dy = q.dir.y; rf.dz = q.dir.z;
-                        rf.tmin = q.tmin; rf.tmax = q.tmax;
-                        lane_begin(L, rf, a.root, st);
-                        my_ray = r;
-                        active = true;
-                    }
-                }
-                const uint32_t taken = (uint32_t)__popcll(need);
-                next = (next + taken < tile_rays) ? next + taken : tile_rays;
-            }
-            if (__ballot(active) == 0) {
-                if (next >= tile_rays) break;
-                continue;
-            }
-            if (active) {
-                if (!lane_step<CLOSEST>(a, L, st, tris_total)) {
-                    // ray finished
-                    if (st.overflow) { overflow = 1; st.overflow = false; }
-                    tests_total += L.tests;
-                    size_t ri;
-                    if (a.camera) {
-                        const uint32_t p = my_ray & 63u, smp = my_ray >> 6;
-                        const uint32_t px = tx * 8 + (p & 7), py = ty * 8 + (p >> 3);
-                        ri = ((size_t)py * a.w + px) * a.spp + smp;
-                        atomicAdd(&s_cnt[p], L.tests);
-                    } else {
-                        ri = (size_t)tile * kRayTile + my_ray;
-                        if (a.counts) a.counts[ri] = L.tests;
-                    }
-                    if (a.hit_id) a.hit_id[ri] = L.best_id;
-                    if (a.hit_t) a.hit_t[ri] = L.best_t;
-                    active = false;
-                }
-            }
-        }
-        if (a.camera) {
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
-            if (px < a.w && py < a.h) {
-                const uint32_t sum = s_cnt[lane];
-                const size_t p = (size_t)py * a.w + px;
-                if (a.counts) a.counts[p] = sum;
-                if (a.rgb) {
-                    uint8_t b = heat_byte(sum, a.spp);
-                    a.rgb[3 * p] = b; a.rgb[3 * p + 1] = b; a.rgb[3 * p + 2] = b;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
+            trace_tile<CLOSEST>(a, tile, lane, st, s_cnt, tests_total, tris_total, overflow);
         }
     }
     if (overflow) atomicOr(&a.work[1], 1u);
@@ -252,41 +287,59 @@ __global__ void k_trace_init(uint32_t* work, unsigned long long* stats) {
     if (stats && threadIdx.x < 2) stats[threadIdx.x] = 0;
 }
 
-static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace_params* p) {
-    const bool closest = p && p->semantics == SQ_TRACE_CLOSEST;
+int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, TracePlan* out) {
+    TracePlan L;
+    L.closest = p && p->semantics == SQ_TRACE_CLOSEST;
     uint32_t slots = p && p->stack_slots ? p->stack_slots : 16;
     uint32_t pow2 = 1;
     while (pow2 < slots) pow2 <<= 1;
-    slots = pow2 > 64 ? 64 : pow2;
-    const uint32_t block = 128;
-    const size_t lds_bytes = ((size_t)slots * (closest ? 2 : 1) + 1) * block * 4;
+    L.slots = pow2 > 64 ? 64 : pow2;
+    L.block = p && p->block_size ? p->block_size : 128;
+    if (L.block != 64 && L.block != 128) SQ_RET(SQ_ERR_INVALID, "trace block size %u (64 or 128)", L.block);
+    L.lds_bytes = ((size_t)L.slots * (L.closest ? 2 : 1) + 1) * L.block * 4;
     int per_cu = 0;
-    if (closest)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true>, block, lds_bytes);
+    if (L.closest)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false>, L.block, L.lds_bytes);
     else
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false>, block, lds_bytes);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, L.block, L.lds_bytes);
     if (per_cu < 1) per_cu = 8;
-    const uint32_t grid = (uint32_t)c->num_cus * (uint32_t)per_cu;
-    const uint32_t depth = b->depth ? b->depth : 64;
+    L.grid = (uint32_t)c->num_cus * (uint32_t)per_cu;
+    if (depth == 0) depth = 64;
     uint32_t cap = 2 * depth + 16;
-    cap = cap > slots ? cap - slots + 8 : 8;
-    const size_t words = (size_t)grid * block * cap * (closest ? 2 : 1);
-    SQ_TRY(c->spill.ensure(words * 4));
-    SQ_TRY(c->counters.ensure(64));
-    SQ_TRY(c->stats.ensure(64));
-    a.slots = slots;
-    a.spill = c->spill.as<uint32_t>();
-    a.spill_cap = cap;
-    a.work = c->counters.as<uint32_t>();
-    a.stats = (p && p->count_work) ? c->stats.as<unsigned long long>() : nullptr;
-    hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, c->stream, a.work, a.stats);
-    const size_t lds = lds_bytes;
-    phase_begin(c, PH_TRACE);
-    if (closest) hipLaunchKernelGGL(k_trace<true>, dim3(grid), dim3(block), lds, c->stream, a);
-    else hipLaunchKernelGGL(k_trace<false>, dim3(grid), dim3(block), lds, c->stream, a);
-    phase_end(c, PH_TRACE);
+    L.spill_cap = cap > L.slots ? cap - L.slots + 8 : 8;
+    L.spill_words = (size_t)L.grid * L.block * L.spill_cap * (L.closest ? 2 : 1);
+    *out = L;
+    return SQ_OK;
+}
+
+int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi) {
+    hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
+    if (multi) {
+        if (L.closest) hipLaunchKernelGGL((k_trace<true, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        else hipLaunchKernelGGL((k_trace<false, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+    } else {
+        if (L.closest) hipLaunchKernelGGL((k_trace<true, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        else hipLaunchKernelGGL((k_trace<false, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+    }
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
+}
+
+static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace_params* p) {
+    TracePlan L;
+    SQ_TRY(trace_plan(c, p, b->depth, &L));
+    SQ_TRY(c->spill.ensure(L.spill_words * 4));
+    SQ_TRY(c->counters.ensure(64));
+    SQ_TRY(c->stats.ensure(64));
+    a.slots = L.slots;
+    a.spill = c->spill.as<uint32_t>();
+    a.spill_cap = L.spill_cap;
+    a.work = c->counters.as<uint32_t>();
+    a.stats = (p && p->count_work) ? c->stats.as<unsigned long long>() : nullptr;
+    phase_begin(c, PH_TRACE);
+    int rc = trace_launch_plan(c->stream, L, a, false);
+    phase_end(c, PH_TRACE);
+    return rc;
 }
 
 static int trace_finish(sq_ctx* c, const sq_trace_params* p) {

@@ -29,6 +29,10 @@ struct TraceArgs {
     unsigned long long* stats;  // [0] node tests, [1] triangle tests
     uint32_t num_tiles;
     uint32_t tiles_x;
+    // multi-job launches: job table and its tile prefix (njobs + 1 entries)
+    const TraceArgs* jobs;
+    const uint32_t* job_tiles;
+    uint32_t njobs;
 };
 
 __device__ __forceinline__ uint32_t hash_u32(uint32_t v) {
@@ -187,6 +191,16 @@ __device__ __forceinline__ uint8_t heat_byte(uint32_t count, uint32_t spp) {
     return (uint8_t)(v * 255.0f);
 }
 
+
+// Launch geometry and stack sizing of the traversal kernel.
+struct TracePlan {
+    bool closest;
+    uint32_t slots, block, grid, spill_cap;
+    size_t lds_bytes, spill_words;
+};
+int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, TracePlan* out);
+// Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).
+int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi);
 
 // Packet / bundle heatmap launch (heatmap.hip).
 int heat_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& t, const sq_trace_params* p);
