@@ -74,6 +74,100 @@ def cpu_baseline(tris, cam, gpu_rays_per_s, budget_s):
     }
 
 
+BATCH_SCENES = 256
+
+
+def batched_workload(sq):
+    """256 icosphere scenes (3-octave value noise, amplitude 0.1, seeds 4000+i)
+    with triangle counts log-uniform in [1k, 500k] and 4-sample jittered
+    16:9 cameras whose ray counts are log-uniform in [64k, 2M] (draws: numpy
+    default_rng(4))."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(4)
+    ntri = np.exp(rng.uniform(np.log(1e3), np.log(5e5), BATCH_SCENES)).astype(np.int64)
+    nray = np.exp(rng.uniform(np.log(65536.0), np.log(2097152.0), BATCH_SCENES)).astype(np.int64)
+    with ThreadPoolExecutor(16) as ex:
+        scenes = list(ex.map(lambda i: sq.gen_icosphere(int(ntri[i]), 4000 + i, 0.1), range(BATCH_SCENES)))
+    cams = []
+    for i, t in enumerate(scenes):
+        npix = nray[i] / 4.0
+        h = max(8, int(round(np.sqrt(npix * 9.0 / 16.0))))
+        w = max(8, int(round(npix / h)))
+        cams.append(sq.pinhole_camera(t, w, h, 60.0, 3.0, spp=4, jitter=1, seed=4000 + i))
+    return scenes, cams
+
+
+def batched_section(sq, ctx, steps, warmup):
+    """Batched (one captured graph: build all + trace all) against 256
+    sequential plain calls (sq_build + sq_trace_camera per scene), both with
+    triangles resident in HBM and outputs written to device memory."""
+    log("batched: generating %d scenes" % BATCH_SCENES)
+    scenes, cams = batched_workload(sq)
+    ntri = sum(len(t) for t in scenes)
+    nrays = sum(c.width * c.height * c.spp for c in cams)
+    out_mask = sq.OUT_HIT_ID | sq.OUT_HIT_T | sq.OUT_COUNTS | sq.OUT_RGB
+    batch = sq.Batch(ctx, scenes)
+    batch.set_cameras(cams, outputs=out_mask)
+    log("batched: autotune")
+    tune = batch.autotune("morton63", repeats=1)
+    for _ in range(warmup):
+        batch.run("morton63")
+    ctx.sync()
+    runs, t0 = [], time.perf_counter()
+    for _ in range(steps):
+        batch.run("morton63")
+        runs.append(batch.times()["run_ms"])
+    batch_wall = (time.perf_counter() - t0) / steps
+    batch.build("morton63")
+    batch.trace()
+    eager = batch.times()
+
+    # plain path: one build + one trace call per scene
+    log("batched: plain per-scene calls")
+    resident = [ctx.upload(t) for t in scenes]
+    maxr = max(c.width * c.height * c.spp for c in cams)
+    maxp = max(c.width * c.height for c in cams)
+    d_id, d_t, d_cnt, d_rgb = ctx.alloc(maxr * 4), ctx.alloc(maxr * 4), ctx.alloc(maxp * 4), ctx.alloc(maxp * 3)
+
+    def plain_all(check=None):
+        for i, sc in enumerate(resident):
+            bvh = ctx.build(sc, "morton63")
+            ctx.trace_camera_device(bvh, cams[i], d_id, d_t, d_cnt, d_rgb)
+            if check is not None and i in check:
+                npix = cams[i].width * cams[i].height
+                check[i] = ctx.to_host(d_cnt, npix * 4, np.uint32)
+            bvh.free()
+
+    check = {i: None for i in range(0, BATCH_SCENES, 37)}
+    plain_all(check)
+    parity = all(np.array_equal(check[i], batch.download(i, hits=False, rgb=False)["counts"]) for i in check)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain_all()
+    ctx.sync()
+    plain_wall = (time.perf_counter() - t0) / steps
+    for p in (d_id, d_t, d_cnt, d_rgb):
+        ctx.free(p)
+    for sc in resident:
+        sc.free()
+    batch.free()
+    run_ms = float(np.median(runs))
+    return {
+        "workload": "256 icosphere scenes (value-noise amplitude 0.1), triangles log-uniform [1k, 500k], "
+                    "4-spp jittered cameras with rays log-uniform [64k, 2M]; morton63 build + exhaustive trace",
+        "scenes": BATCH_SCENES, "triangles": int(ntri), "rays": int(nrays),
+        "batched_ms": round(run_ms, 2), "batched_wall_ms": round(batch_wall * 1e3, 2),
+        "batched_mrays_s": round(nrays / (batch_wall) / 1e6, 1),
+        "batched_build_ms": round(eager["build_ms"], 2), "batched_trace_ms": round(eager["trace_ms"], 2),
+        "plain_wall_ms": round(plain_wall * 1e3, 2),
+        "plain_mrays_s": round(nrays / plain_wall / 1e6, 1),
+        "speedup_vs_plain": round(plain_wall / batch_wall, 2),
+        "counts_equal_plain": bool(parity), "checked_scenes": len(check),
+        "autotune": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +176,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--semantics", default="exhaustive", choices=["exhaustive", "closest"])
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-scene batched-vs-plain comparison")
+    ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (byte-counter calibration)")
     a = ap.parse_args()
@@ -172,6 +268,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_trace<false>", "algorithmic_bytes_per_launch": alg_bytes},
     }
+    if not a.no_batched:
+        line["batched"] = batched_section(sq, ctx, a.batched_steps, 1)
     if not a.no_cpu:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(tris, cam, rays_per_s, a.cpu_budget)
