@@ -168,6 +168,160 @@ def batched_section(sq, ctx, steps, warmup):
     }
 
 
+SERVED_REQUESTS = 240
+SERVED_RATES = (100.0, 300.0, 1000.0)
+
+
+def served_workload(sq):
+    """Request mix (numpy default_rng(5)): 70% build+trace of a new icosphere
+    scene (triangles log-uniform [1k, 2M], value-noise amplitude 0.1), 30%
+    trace-only on an already built scene picked Zipf(s=1.1) by build order;
+    rays log-uniform [4k, 2M], half primary (16:9 jittered camera, 1-4 spp),
+    half cosine-weighted secondary rays from the scene surface (tmin 1e-4)."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(5)
+    plan, built = [], []
+    for i in range(SERVED_REQUESTS):
+        nray = int(np.exp(rng.uniform(np.log(4096.0), np.log(2097152.0))))
+        primary = rng.random() < 0.5
+        if built and rng.random() < 0.3:
+            ranks = np.arange(1, len(built) + 1, dtype=np.float64)
+            p = ranks ** -1.1
+            key = built[int(rng.choice(len(built), p=p / p.sum()))]
+            plan.append((key, None, nray, primary, 7000 + i))
+        else:
+            n = int(np.exp(rng.uniform(np.log(1e3), np.log(2e6))))
+            key = len(built)
+            built.append(key)
+            plan.append((key, n, nray, primary, 7000 + i))
+    sizes = {k: n for (k, n, _, _, _) in plan if n}
+    with ThreadPoolExecutor(16) as ex:
+        scene = dict(zip(sizes, ex.map(lambda k: sq.gen_icosphere(sizes[k], 6000 + k, 0.1), list(sizes))))
+    reqs = []
+    for key, n, nray, primary, seed in plan:
+        tris = scene[key]
+        if primary:
+            spp = 1 + seed % 4
+            npix = max(64, nray // spp)
+            h = max(8, int(round(np.sqrt(npix * 9.0 / 16.0))))
+            w = max(8, int(round(npix / h)))
+            req = dict(key=key, build=n is not None, camera=sq.pinhole_camera(tris, w, h, 60.0, 3.0, spp=spp, jitter=1,
+                                                                                 seed=seed), rays=None)
+        else:
+            req = dict(key=key, build=n is not None, camera=None,
+                       rays=sq.gen_secondary_rays(tris, nray, seed=seed, tmin=1e-4))
+        reqs.append(req)
+    return reqs, scene
+
+
+def _arrivals(rate, n, seed):
+    return np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n))
+
+
+def _served_run(sq, reqs, scene, arrivals):
+    import threading
+    srv = sq.Server(0)
+    ids = [None] * len(reqs)
+
+    def client():
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            d = arrivals[i] - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids[i] = srv.submit(r["key"], tris=scene[r["key"]] if r["build"] else None, camera=r["camera"],
+                                rays=r["rays"])
+
+    th = threading.Thread(target=client)
+    t0 = time.perf_counter()
+    th.start()
+    th.join()
+    srv.drain()
+    wall = time.perf_counter() - t0
+    lat, failed = [], []
+    for k, i in enumerate(ids):
+        st, _ = srv.wait(i)
+        if st["status"] != 0:
+            failed.append((k, st["status"], st["fault"]))
+            continue
+        lat.append(st["latency_ms"])
+    stats = srv.stats()
+    stats["failed_requests"] = failed
+    srv.close()
+    return np.array(lat), wall, stats
+
+
+def _plain_run(sq, ctx, reqs, scene, arrivals):
+    """One request at a time in arrival order with the plain calls (upload +
+    sq_build morton63 for new scenes, sq_trace_camera / sq_trace_rays with
+    host outputs); built trees stay resident for trace-only requests."""
+    cache, lat = {}, []
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        now = time.perf_counter() - t0
+        if arrivals[i] > now:
+            time.sleep(arrivals[i] - now)
+        if r["build"]:
+            sc = ctx.upload(scene[r["key"]])
+            cache[r["key"]] = (sc, ctx.build(sc, "morton63"))
+        bvh = cache[r["key"]][1]
+        try:
+            if r["rays"] is not None:
+                ctx.trace_rays(bvh, r["rays"])
+            else:
+                ctx.trace_camera(bvh, r["camera"])
+        except sq.SqError as e:
+            log("plain: request %d failed: %s (tree depth %d)" % (i, e, bvh.info()[2]))
+            continue
+        lat.append((time.perf_counter() - t0 - arrivals[i]) * 1e3)
+    wall = time.perf_counter() - t0
+    for sc, b in cache.values():
+        b.free()
+        sc.free()
+    return np.array(lat), wall
+
+
+def served_section(sq, ctx):
+    log("served: generating %d requests" % SERVED_REQUESTS)
+    reqs, scene = served_workload(sq)
+    nrays = sum((r["rays"].shape[0] if r["rays"] is not None else r["camera"].width * r["camera"].height *
+                 r["camera"].spp) for r in reqs)
+    nbuild = sum(1 for r in reqs if r["build"])
+    # warm both paths once on a few requests
+    _served_run(sq, reqs[:8], scene, np.zeros(8))
+    _plain_run(sq, ctx, reqs[:8], scene, np.zeros(8))
+    points = []
+    for rate in SERVED_RATES:
+        arr = _arrivals(rate, len(reqs), 55)
+        log("served: rate %.0f/s" % rate)
+        sl, sw, st = _served_run(sq, reqs, scene, arr)
+        for k, code, fault in st["failed_requests"]:
+            r = reqs[k]
+            log("served: request %d failed (status %d, fault %d): build=%s n=%d rays=%s" % (
+                k, code, fault, r["build"], len(scene[r["key"]]),
+                len(r["rays"]) if r["rays"] is not None else "camera"))
+        pl, pw = _plain_run(sq, ctx, reqs, scene, arr)
+        points.append({
+            "offered_req_s": rate,
+            "served": {"req_s": round(len(reqs) / sw, 1), "mrays_s": round(nrays / sw / 1e6, 1),
+                       "p50_ms": round(float(np.percentile(sl, 50)), 2),
+                       "p99_ms": round(float(np.percentile(sl, 99)), 2),
+                       "waves": int(st["waves"]), "mean_wave_requests": round(st["mean_wave_requests"], 2),
+                       "failed": len(st["failed_requests"])},
+            "plain": {"req_s": round(len(reqs) / pw, 1), "mrays_s": round(nrays / pw / 1e6, 1),
+                      "p50_ms": round(float(np.percentile(pl, 50)), 2),
+                      "p99_ms": round(float(np.percentile(pl, 99)), 2)},
+        })
+    return {
+        "workload": "%d requests: %d build+trace of icosphere scenes (log-uniform [1k, 2M] triangles), %d "
+                    "trace-only Zipf(1.1) repeats; rays log-uniform [4k, 2M], half primary camera, half cosine "
+                    "secondary lists; Poisson arrivals; host inputs and outputs (PCIe included); budget 200 GB"
+                    % (len(reqs), nbuild, len(reqs) - nbuild),
+        "requests": len(reqs), "rays": int(nrays), "triangles_built": int(sum(len(scene[k]) for k in scene)),
+        "points": points,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +332,8 @@ def main():
     ap.add_argument("--semantics", default="exhaustive", choices=["exhaustive", "closest"])
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-scene batched-vs-plain comparison")
     ap.add_argument("--batched-steps", type=int, default=3)
+    ap.add_argument("--served", action="store_true",
+                    help="also run the Poisson-stream served-vs-plain comparison (off by default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (byte-counter calibration)")
     a = ap.parse_args()
@@ -270,6 +426,8 @@ def main():
     }
     if not a.no_batched:
         line["batched"] = batched_section(sq, ctx, a.batched_steps, 1)
+    if a.served:
+        line["served"] = served_section(sq, ctx)
     if not a.no_cpu:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(tris, cam, rays_per_s, a.cpu_budget)

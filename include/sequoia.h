@@ -34,6 +34,8 @@ extern "C" {
 #define SQ_ERR_BUDGET (-4)      /* request exceeds the configured device-memory budget */
 #define SQ_ERR_UNSUPPORTED (-5) /* option combination not implemented */
 #define SQ_ERR_OVERFLOW (-6)    /* a traversal exceeded its stack bound; results invalid */
+#define SQ_ERR_TIMEOUT (-7)     /* a served request expired in the queue, or a wait timed out */
+#define SQ_ERR_CANCELLED (-8)   /* a served request was cancelled before it ran */
 
 /* ---- data layouts (byte-identical to the reference) ---- */
 typedef struct { float x, y, z; } sq_vec3;
@@ -241,6 +243,84 @@ typedef struct {
     uint32_t candidates_tried;
 } sq_tune_result;
 int sq_batch_autotune(sq_ctx* ctx, sq_batch* b, sq_builder kind, uint32_t repeats, sq_tune_result* out);
+
+/* ---- serving: a continuous-batching request loop ----
+ * No counterpart in the reference (its main.cpp runs one build and one trace
+ * per process). A server owns one device, a scheduler thread and a pool of
+ * device memory. Requests (build + trace of a new scene, or trace of a
+ * resident scene) queue in arrival order; whenever one of the in-flight wave
+ * slots is free, the scheduler admits every queued request that fits the
+ * wave limits and the memory budget into the next wave: the wave's new
+ * scenes are built together (the batched Morton pipeline) and all its traces
+ * run in one persistent launch, on the slot's own stream, overlapping the
+ * other slot's wave. Built trees stay resident in HBM under their key for
+ * later trace-only requests; unreferenced scenes are evicted LRU when the
+ * budget needs room. Every result equals the single-scene calls' on the same
+ * input (morton63 tree, sq_trace_camera / sq_trace_rays). */
+typedef struct sq_server sq_server;
+
+typedef struct {
+    uint64_t budget_bytes;      /* device bytes for resident scenes + in-flight waves; 0 = 200 GB */
+    uint32_t max_wave_requests; /* 0 = 256 */
+    uint32_t max_wave_rays;     /* 0 = 64 Mi */
+    uint32_t max_wave_tris;     /* 0 = 32 Mi */
+    uint32_t semantics;         /* sq_semantics of every trace */
+    uint32_t inflight_waves;    /* wave slots on the device at once, 1 or 2; 0 = 2 */
+    uint32_t reserved[3];
+} sq_server_config;
+
+/* A request. Host buffers (tris, rays, out) must stay valid until it completes. */
+typedef struct {
+    uint64_t key;              /* scene key */
+    const sq_triangle* tris;   /* non-NULL: build this scene under `key`; NULL: trace resident scene `key` */
+    uint32_t ntris;
+    uint32_t nrays;            /* > 0: trace the ray list `rays`; 0: trace `camera` */
+    const sq_ray* rays;
+    sq_camera camera;
+    sq_trace_out out;          /* host destinations, any may be NULL (counts: per pixel / per listed ray) */
+    float timeout_ms;          /* > 0: fail with SQ_ERR_TIMEOUT if not admitted within this time */
+    uint32_t reserved;
+} sq_request;
+
+#define SQ_REQ_QUEUED 0
+#define SQ_REQ_RUNNING 1
+#define SQ_REQ_DONE 2
+#define SQ_REQ_FAILED 3
+
+typedef struct {
+    int32_t state;      /* SQ_REQ_* */
+    int32_t status;     /* SQ_OK or the failure (BUDGET: larger than the budget; TIMEOUT; CANCELLED;
+                           INVALID: trace of a scene that is not resident) */
+    float latency_ms;   /* submit -> outputs in host memory */
+    float queue_ms;     /* submit -> admitted into a wave */
+    uint64_t wave;
+    uint32_t fault;     /* status OVERFLOW: 1 stack bound, 2 node index, 4 triangle index */
+    uint32_t reserved;
+} sq_request_status;
+
+typedef struct {
+    uint64_t submitted, completed, failed, waves, evictions;
+    uint64_t resident_scenes, resident_bytes, inflight_bytes, pool_reserved_bytes;
+    uint64_t rays_traced, tris_built;
+    float p50_ms, p99_ms, mean_ms, max_ms; /* latency over completed requests */
+    float span_s;                          /* first submit -> last completion */
+    float mean_wave_requests;
+} sq_server_stats;
+
+int sq_server_create(int device, const sq_server_config* cfg, sq_server** out);
+/* Finishes in-flight waves, fails queued requests (CANCELLED), frees everything. */
+void sq_server_destroy(sq_server* s);
+int sq_server_submit(sq_server* s, const sq_request* r, uint64_t* id);
+/* Waits up to timeout_ms (< 0: forever) for request `id`; SQ_ERR_TIMEOUT if still running. */
+int sq_server_wait(sq_server* s, uint64_t id, float timeout_ms, sq_request_status* st);
+/* Cancels a queued request (SQ_ERR_INVALID once it is in a wave or finished). */
+int sq_server_cancel(sq_server* s, uint64_t id);
+/* Waits until every submitted request has finished. */
+int sq_server_drain(sq_server* s);
+/* Drops resident scene `key` (SQ_ERR_INVALID if unknown or in use). */
+int sq_server_evict(sq_server* s, uint64_t key);
+int sq_server_get_stats(sq_server* s, sq_server_stats* out);
+int sq_server_reset_stats(sq_server* s);
 
 /* ---- deterministic workload generators (host) ---- */
 uint32_t sq_gen_icosphere_count(uint32_t target);

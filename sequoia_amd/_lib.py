@@ -12,7 +12,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsequoia_amd.so")
 
 SQ_OK = 0
-ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "BUDGET", -5: "UNSUPPORTED", -6: "OVERFLOW"}
+ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "BUDGET", -5: "UNSUPPORTED", -6: "OVERFLOW", -7: "TIMEOUT",
+          -8: "CANCELLED"}
+ERR_INVALID, ERR_BUDGET, ERR_TIMEOUT, ERR_CANCELLED = -1, -4, -7, -8
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 BUILDERS = {
     "recursive": 0, "horizontal": 1, "grid": 2, "grid_sse": 3, "grid_avx": 4,
@@ -64,6 +67,34 @@ class TuneResult(C.Structure):
     _fields_ = [("radix_digit_bits", C.c_uint32), ("subtree_threshold", C.c_uint32), ("stack_slots", C.c_uint32),
                 ("block_size", C.c_uint32), ("build_ms", C.c_float), ("trace_ms", C.c_float),
                 ("default_build_ms", C.c_float), ("default_trace_ms", C.c_float), ("candidates_tried", C.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("max_wave_requests", C.c_uint32), ("max_wave_rays", C.c_uint32),
+                ("max_wave_tris", C.c_uint32), ("semantics", C.c_uint32), ("inflight_waves", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
+
+
+class Request(C.Structure):
+    _fields_ = [("key", C.c_uint64), ("tris", C.c_void_p), ("ntris", C.c_uint32), ("nrays", C.c_uint32),
+                ("rays", C.c_void_p), ("camera", Camera), ("out", TraceOut), ("timeout_ms", C.c_float),
+                ("reserved", C.c_uint32)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("latency_ms", C.c_float), ("queue_ms", C.c_float),
+                ("wave", C.c_uint64), ("fault", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64), ("waves", C.c_uint64),
+                ("evictions", C.c_uint64), ("resident_scenes", C.c_uint64), ("resident_bytes", C.c_uint64),
+                ("inflight_bytes", C.c_uint64), ("pool_reserved_bytes", C.c_uint64), ("rays_traced", C.c_uint64),
+                ("tris_built", C.c_uint64), ("p50_ms", C.c_float), ("p99_ms", C.c_float), ("mean_ms", C.c_float),
+                ("max_ms", C.c_float), ("span_s", C.c_float), ("mean_wave_requests", C.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -124,6 +155,15 @@ _SIGS = {
     "sq_batch_download": (C.c_int, [_P, _P, _U32, C.POINTER(TraceOut)]),
     "sq_batch_times": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "sq_batch_autotune": (C.c_int, [_P, _P, C.c_int, _U32, C.POINTER(TuneResult)]),
+    "sq_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "sq_server_destroy": (None, [_P]),
+    "sq_server_submit": (C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    "sq_server_wait": (C.c_int, [_P, C.c_uint64, C.c_float, C.POINTER(RequestStatus)]),
+    "sq_server_cancel": (C.c_int, [_P, C.c_uint64]),
+    "sq_server_drain": (C.c_int, [_P]),
+    "sq_server_evict": (C.c_int, [_P, C.c_uint64]),
+    "sq_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "sq_server_reset_stats": (C.c_int, [_P]),
     "sq_gen_icosphere_count": (_U32, [_U32]),
     "sq_gen_icosphere": (C.c_int, [_U32, _U32, C.c_float, _P]),
     "sq_gen_soup": (C.c_int, [_U32, _U32, C.c_float, _P]),
@@ -359,6 +399,95 @@ class Batch:
         r = TuneResult()
         _check(lib().sq_batch_autotune(self.ctx.h, self.h, BUILDERS[kind], repeats, C.byref(r)))
         return r.as_dict()
+
+
+class Server:
+    """Continuous-batching request loop (sq_server_*). Inputs and output
+    arrays of a request are kept alive here until wait() returns for it."""
+
+    def __init__(self, device=0, budget_bytes=0, max_wave_requests=0, max_wave_rays=0, max_wave_tris=0,
+                 semantics=TRACE_EXHAUSTIVE, inflight_waves=0):
+        cfg = ServerConfig(budget_bytes, max_wave_requests, max_wave_rays, max_wave_tris, semantics, inflight_waves,
+                           (C.c_uint32 * 3)())
+        h = C.c_void_p()
+        _check(lib().sq_server_create(device, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def close(self):
+        if self.h:
+            lib().sq_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, key, tris=None, camera=None, rays=None, hits=True, counts=True, rgb=True, timeout_ms=0.0):
+        """Build `tris` under `key` (or trace the resident scene `key` when
+        tris is None) and trace `camera` or the ray list `rays`."""
+        keep = []
+        r = Request()
+        r.key = key
+        if tris is not None:
+            tris = as_tris(tris)
+            keep.append(tris)
+            r.tris, r.ntris = _ptr(tris), len(tris)
+        if rays is not None:
+            rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+            keep.append(rays)
+            r.rays, r.nrays = _ptr(rays), len(rays)
+            nray, npix = len(rays), 0
+        else:
+            r.camera = camera
+            npix = camera.width * camera.height
+            nray = npix * camera.spp
+        out = {"hit_id": np.empty(nray, np.uint32) if hits else None,
+               "hit_t": np.empty(nray, np.float32) if hits else None,
+               "counts": np.empty(nray if rays is not None else npix, np.uint32) if counts else None,
+               "rgb": np.empty((camera.height, camera.width, 3), np.uint8) if (rgb and rays is None) else None}
+        r.out = TraceOut(_ptr(out["hit_id"]), _ptr(out["hit_t"]), _ptr(out["counts"]), _ptr(out["rgb"]))
+        r.timeout_ms = timeout_ms
+        rid = C.c_uint64()
+        _check(lib().sq_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._live[rid.value] = (keep, out)
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1.0):
+        """Returns (status dict, outputs); raises SqError(TIMEOUT) if the wait times out."""
+        st = RequestStatus()
+        rc = lib().sq_server_wait(self.h, rid, timeout_ms, C.byref(st))
+        if rc != SQ_OK:
+            _check(rc)
+        _, out = self._live.pop(rid)
+        return {"state": st.state, "status": st.status, "latency_ms": st.latency_ms, "queue_ms": st.queue_ms,
+                "wave": st.wave, "fault": st.fault}, out
+
+    def cancel(self, rid):
+        _check(lib().sq_server_cancel(self.h, rid))
+
+    def drain(self):
+        _check(lib().sq_server_drain(self.h))
+
+    def evict(self, key):
+        _check(lib().sq_server_evict(self.h, key))
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().sq_server_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        _check(lib().sq_server_reset_stats(self.h))
 
 
 class Context:

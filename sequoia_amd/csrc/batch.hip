@@ -19,6 +19,7 @@
 // the order within a scene is (code, index) in both, emission is per scene
 // with scene-local slots and triangle indices, traversal is per ray.
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <vector>
 
@@ -117,6 +118,37 @@ __global__ __launch_bounds__(256) void k_batch_morton(const sq_triangle* __restr
     vals_out[i] = i;
 }
 
+int enqueue_batch_build(sq_ctx* c, hipStream_t stream, const BatchBuild& j) {
+    const bool key64 = j.bits > 10;
+    SQ_TRY(launch_fill_u32_on(stream, c->num_cus, j.nodes, 0u, (size_t)2 * j.total * 8));
+    const uint32_t ib = (6 * j.nscenes + 255) / 256;
+    hipLaunchKernelGGL(k_batch_bounds_init, dim3(ib), dim3(256), 0, stream, j.d_ord, j.nscenes);
+    const uint32_t tiles = (j.total + kBbTile - 1) / kBbTile;
+    hipLaunchKernelGGL(k_batch_bounds, dim3(tiles), dim3(kBbThreads), 0, stream, j.tris, j.d_off, j.nscenes, j.total,
+                       j.d_ord);
+    const uint32_t mb = (j.total + 255) / 256;
+    if (key64)
+        hipLaunchKernelGGL(k_batch_morton<true>, dim3(mb), dim3(256), 0, stream, j.tris, j.d_off, j.nscenes, j.total,
+                           j.bits, j.d_ord, j.keys[0], j.vals[0]);
+    else
+        hipLaunchKernelGGL(k_batch_morton<false>, dim3(mb), dim3(256), 0, stream, j.tris, j.d_off, j.nscenes, j.total,
+                           j.bits, j.d_ord, j.keys[0], j.vals[0]);
+    SQ_CHECK_HIP(hipGetLastError());
+    SortBufs sb{stream, {j.keys[0], j.keys[1]}, {j.vals[0], j.vals[1]}, j.hist};
+    int slot = 0;
+    uint32_t passes = 0;
+    SQ_TRY(radix_sort_bufs(sb, j.total, 3 * j.bits, j.digit, key64, j.d_off, j.nscenes, 0, &slot, &passes));
+    SQ_TRY(lbvh_emit_batch(c, stream, j.tris, j.total, j.keys[slot], j.vals[slot], key64, j.d_desc, j.nscenes,
+                           j.layout, j.threshold, j.nodes, j.ws, j.d_depth));
+    return SQ_OK;
+}
+
+uint32_t morton_depth_bound(uint32_t bits, uint32_t n) {
+    uint32_t lg = 0;
+    while ((1ull << lg) < n) lg++;
+    return 3 * bits + lg + 3;
+}
+
 }  // namespace sq
 
 using namespace sq;
@@ -165,6 +197,7 @@ struct sq_batch {
     size_t spill_bytes = 0;
     uint32_t* d_work = nullptr;
     unsigned long long* d_stats = nullptr;
+    uint32_t* d_job_fail = nullptr;  // fault bits per scene of the last trace
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -218,14 +251,6 @@ int morton_bits(sq_builder kind, const sq_build_params* p, uint32_t* bits) {
     return SQ_OK;
 }
 
-// Upper bound of a Morton tree's depth: at most 3*bits splits on a code bit,
-// then midpoint splits of an all-equal range, a fan level and the root.
-uint32_t morton_depth_bound(uint32_t bits, uint32_t n) {
-    uint32_t lg = 0;
-    while ((1ull << lg) < n) lg++;
-    return 3 * bits + lg + 3;
-}
-
 int configure_build(sq_batch* b, sq_builder kind, const sq_build_params* p) {
     uint32_t bits = 0;
     SQ_TRY(morton_bits(kind, p, &bits));
@@ -276,28 +301,26 @@ int configure_build(sq_batch* b, sq_builder kind, const sq_build_params* p) {
 
 // Enqueues the whole build on the context stream (no host synchronisation).
 int enqueue_build(sq_ctx* c, sq_batch* b) {
-    const bool key64 = b->bits > 10;
-    SQ_TRY(launch_fill_u32(c, b->d_nodes, 0u, (size_t)2 * b->total * 8));
-    uint32_t ib = (6 * b->nscenes + 255) / 256;
-    hipLaunchKernelGGL(k_batch_bounds_init, dim3(ib), dim3(256), 0, c->stream, b->d_ord, b->nscenes);
-    const uint32_t tiles = (b->total + kBbTile - 1) / kBbTile;
-    hipLaunchKernelGGL(k_batch_bounds, dim3(tiles), dim3(kBbThreads), 0, c->stream, b->d_tris, b->d_off, b->nscenes,
-                       b->total, b->d_ord);
-    const uint32_t mb = (b->total + 255) / 256;
-    if (key64)
-        hipLaunchKernelGGL(k_batch_morton<true>, dim3(mb), dim3(256), 0, c->stream, b->d_tris, b->d_off, b->nscenes,
-                           b->total, b->bits, b->d_ord, b->d_keys[0], b->d_vals[0]);
-    else
-        hipLaunchKernelGGL(k_batch_morton<false>, dim3(mb), dim3(256), 0, c->stream, b->d_tris, b->d_off, b->nscenes,
-                           b->total, b->bits, b->d_ord, b->d_keys[0], b->d_vals[0]);
-    SQ_CHECK_HIP(hipGetLastError());
-    SortBufs sb{c->stream, {b->d_keys[0], b->d_keys[1]}, {b->d_vals[0], b->d_vals[1]}, b->d_hist};
-    int slot = 0;
-    uint32_t passes = 0;
-    SQ_TRY(radix_sort_bufs(sb, b->total, 3 * b->bits, b->digit, key64, b->d_off, b->nscenes, 0, &slot, &passes));
-    SQ_TRY(lbvh_emit_batch(c, c->stream, b->d_tris, b->total, b->d_keys[slot], b->d_vals[slot], key64, b->d_desc,
-                           b->nscenes, b->layout, b->threshold, b->d_nodes, b->d_ws, b->d_depth));
-    return SQ_OK;
+    BatchBuild j;
+    j.tris = b->d_tris;
+    j.total = b->total;
+    j.nscenes = b->nscenes;
+    j.d_off = b->d_off;
+    j.d_desc = b->d_desc;
+    j.d_ord = b->d_ord;
+    j.keys[0] = b->d_keys[0];
+    j.keys[1] = b->d_keys[1];
+    j.vals[0] = b->d_vals[0];
+    j.vals[1] = b->d_vals[1];
+    j.hist = b->d_hist;
+    j.ws = b->d_ws;
+    j.d_depth = b->d_depth;
+    j.nodes = b->d_nodes;
+    j.bits = b->bits;
+    j.digit = b->digit;
+    j.layout = b->layout;
+    j.threshold = b->threshold;
+    return enqueue_batch_build(c, c->stream, j);
 }
 
 int plan_trace(sq_ctx* c, sq_batch* b, TracePlan* L) {
@@ -316,6 +339,8 @@ int enqueue_trace(sq_ctx* c, sq_batch* b, const TracePlan& L) {
     a.jobs = b->d_jobs;
     a.job_tiles = b->d_job_tiles;
     a.njobs = b->nscenes;
+    a.job_fail = b->d_job_fail;
+    SQ_TRY(launch_fill_u32_on(c->stream, c->num_cus, b->d_job_fail, 0u, b->nscenes));
     return trace_launch_plan(c->stream, L, a, true);
 }
 
@@ -349,7 +374,15 @@ int check_trace(sq_ctx* c, sq_batch* b) {
         c->last.node_tests = st[0];
         c->last.tri_tests = st[1];
     }
-    if (h[1]) SQ_RET(SQ_ERR_OVERFLOW, "batch trace: a ray exceeded the traversal stack bound");
+    if (h[1]) {
+        std::vector<uint32_t> f(b->nscenes);
+        SQ_CHECK_HIP(hipMemcpy(f.data(), b->d_job_fail, 4ull * b->nscenes, hipMemcpyDeviceToHost));
+        uint32_t first = 0;
+        while (first < b->nscenes && !f[first]) first++;
+        char what[64];
+        snprintf(what, sizeof(what), "batch trace (first failing scene %u)", first);
+        return trace_fault(h[1], what);
+    }
     return SQ_OK;
 }
 
@@ -406,6 +439,7 @@ int sq_batch_create(sq_ctx* c, const sq_triangle* tris, const uint32_t* counts, 
     if ((rc = balloc_t(b, 2ull * total, &b->d_nodes)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, 16, &b->d_work)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, 4, &b->d_stats)) != SQ_OK) return fail(rc);
+    if ((rc = balloc_t(b, nscenes, &b->d_job_fail)) != SQ_OK) return fail(rc);
     hipError_t e = hipMemcpy(b->d_off, b->off.data(), 4ull * (nscenes + 1), hipMemcpyHostToDevice);
     if (e != hipSuccess) { set_error("batch upload: %s", hipGetErrorString(e)); return fail(SQ_ERR_HIP); }
     b->scenes.resize(nscenes);

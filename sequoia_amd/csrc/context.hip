@@ -82,14 +82,18 @@ __global__ void k_fill_u32(uint32_t* __restrict__ dst, uint32_t value, size_t wo
     for (; i < words; i += stride) dst[i] = value;
 }
 
-int launch_fill_u32(sq_ctx* c, void* dst, uint32_t value, size_t words) {
+int launch_fill_u32_on(hipStream_t stream, int num_cus, void* dst, uint32_t value, size_t words) {
     if (words == 0) return SQ_OK;
     size_t blocks = (words + 255) / 256;
-    size_t cap = (size_t)c->num_cus * 16;
+    size_t cap = (size_t)num_cus * 16;
     if (blocks > cap) blocks = cap;
-    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, c->stream, (uint32_t*)dst, value, words);
+    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(256), 0, stream, (uint32_t*)dst, value, words);
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
+}
+
+int launch_fill_u32(sq_ctx* c, void* dst, uint32_t value, size_t words) {
+    return launch_fill_u32_on(c->stream, c->num_cus, dst, value, words);
 }
 
 __global__ void k_copy_u32x4(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {

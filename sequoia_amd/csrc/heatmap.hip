@@ -63,10 +63,10 @@ __device__ uint32_t packet_walk(const TraceArgs& a, const RayF& r, float ix, flo
     while (st.sp > base_sp) {
         float te;
         uint32_t e = pop<false>(st, te), first, cnt;
-        if (!decode_entry(a, e, first, cnt)) { st.overflow = true; return tests; }
+        if (!decode_entry(a, e, first, cnt)) { st.fault |= kFaultNode; return tests; }
         for (uint32_t i = 0; i < cnt; i++) {
             const uint32_t ni = first + i;
-            if (ni >= a.node_count) { st.overflow = true; return tests; }
+            if (ni >= a.node_count) { st.fault |= kFaultNode; return tests; }
             NodeV n = load_node(a.nodes, ni);
             float f;
             bool h = slab(n, r, ix, iy, iz, fast, f);
@@ -74,7 +74,7 @@ __device__ uint32_t packet_walk(const TraceArgs& a, const RayF& r, float ix, flo
             tests++;
             if (g != 0 && !n.leaf) {
                 push<false>(st, encode_list(ni, n), 0.f);
-                if (st.overflow) return tests;
+                if (st.fault) return tests;
             }
         }
     }
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(128) void k_heat(HeatArgs h) {
         const bool fast = isfinite(ix) && isfinite(iy) && isfinite(iz) && isfinite(r.ox) && isfinite(r.oy) &&
                           isfinite(r.oz);
         st.sp = 0;
-        st.overflow = false;
+        st.fault = 0;
         float value = 0.f;
         uint32_t count = 0;
         if (valid) {
@@ -179,13 +179,13 @@ __global__ __launch_bounds__(128) void k_heat(HeatArgs h) {
                 uint32_t btests = 0, ptests = 0;
                 const uint32_t pgroup = lane / 8;
                 push<false>(st, a.root < (1u << 28) ? (a.root << 4) : 15u, 0.f);
-                while (st.sp > 0 && !st.overflow) {
+                while (st.sp > 0 && !st.fault) {
                     float te;
                     uint32_t e = pop<false>(st, te), first, cnt;
-                    if (!decode_entry(a, e, first, cnt)) { st.overflow = true; break; }
-                    for (uint32_t i = 0; i < cnt && !st.overflow; i++) {
+                    if (!decode_entry(a, e, first, cnt)) { st.fault |= kFaultNode; break; }
+                    for (uint32_t i = 0; i < cnt && !st.fault; i++) {
                         const uint32_t ni = first + i;
-                        if (ni >= a.node_count) { st.overflow = true; break; }
+                        if (ni >= a.node_count) { st.fault |= kFaultNode; break; }
                         NodeV n = load_node(a.nodes, ni);
                         bool mid;
                         bool any = bundle_test(bb, n, mid);
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(128) void k_heat(HeatArgs h) {
                 count = btests;
                 value = ((float)btests / (float)K + (float)ptests / 8.0f) / 200.0f;
             }
-            if (st.overflow) overflow = 1;
+            overflow |= st.fault;
             if (px < a.w && py < a.h) {
                 const size_t p = (size_t)py * a.w + px;
                 float v = (value < 1.0f) ? value : 1.0f;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(128) void k_heat(HeatArgs h) {
             }
         }
     }
-    if (overflow) atomicOr(&a.work[1], 1u);
+    if (overflow) atomicOr(&a.work[1], overflow);
 }
 
 int heat_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& t, const sq_trace_params* p) {

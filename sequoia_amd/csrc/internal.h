@@ -140,6 +140,30 @@ void lbvh_fill_descs(SceneDesc* descs, const uint32_t* counts, uint32_t nscenes,
 int lbvh_emit_batch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total_tris, const void* d_codes,
                     const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
                     uint32_t threshold, sq_node* d_nodes, void* ws, uint32_t* d_depth_out);
+// Batched Morton pipeline over a concatenation of scenes (bounds, codes,
+// sort by (scene, code), emission, refit), enqueued on `stream` with no host
+// synchronisation. Buffers: keys 8 B x total x 2, vals 4 B x total x 2,
+// hist sort_hist_bytes(total, digit), ws lbvh_batch_workspace(...), ord
+// 6 x nscenes words, nodes 2 x total, depth nscenes words.
+struct BatchBuild {
+    const sq_triangle* tris;
+    uint32_t total, nscenes;
+    const uint32_t* d_off;  // nscenes + 1 triangle offsets
+    const SceneDesc* d_desc;
+    uint32_t* d_ord;
+    void* keys[2];
+    uint32_t* vals[2];
+    uint32_t* hist;
+    void* ws;
+    uint32_t* d_depth;
+    sq_node* nodes;
+    uint32_t bits, digit, layout, threshold;
+};
+int enqueue_batch_build(sq_ctx* c, hipStream_t stream, const BatchBuild& j);
+// Upper bound of a Morton tree's depth: at most 3*bits splits on a code bit,
+// then midpoint splits of an all-equal range, a fan level and the root.
+uint32_t morton_depth_bound(uint32_t bits, uint32_t n);
+int launch_fill_u32_on(hipStream_t stream, int num_cus, void* dst, uint32_t value, size_t words);
 int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_params* p, sq_builder kind,
               sq_node* d_nodes, uint32_t* root, uint32_t* depth);
 int refit_tree(sq_ctx* c, sq_bvh* bvh);

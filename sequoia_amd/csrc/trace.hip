@@ -82,7 +82,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
             L.first = e >> 4;
             L.cnt = (e & 15u) + 1;
         } else {
-            if ((e >> 4) >= a.node_count) { st.overflow = true; return false; }
+            if ((e >> 4) >= a.node_count) { st.fault |= kFaultNode; return false; }
             NodeV p = load_node(a.nodes, e >> 4);
             L.first = p.child;
             L.cnt = p.count;
@@ -90,7 +90,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
     }
     const uint32_t i0 = L.first;
     const bool two = L.cnt >= 2;
-    if (i0 >= a.node_count || (two && i0 + 1 >= a.node_count)) { st.overflow = true; return false; }
+    if (i0 >= a.node_count || (two && i0 + 1 >= a.node_count)) { st.fault |= kFaultNode; return false; }
     NodeV n0 = load_node(a.nodes, i0);
     NodeV n1;
     if (two) n1 = load_node(a.nodes, i0 + 1);
@@ -107,7 +107,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
     bool bad = false;
     if (h0 && n0.leaf) leaf_tris(a, n0, L, tri_tests, bad);
     if (h1 && n1.leaf) leaf_tris(a, n1, L, tri_tests, bad);
-    if (bad) { st.overflow = true; return false; }
+    if (bad) { st.fault |= kFaultTri; return false; }
     const bool c0 = h0 && !n0.leaf, c1 = h1 && !n1.leaf;
     if (c0 || c1) {
         uint32_t e0 = c0 ? encode_list(i0, n0) : 0u, e1 = c1 ? encode_list(i0 + 1, n1) : 0u;
@@ -121,7 +121,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
         } else {
             push<CLOSEST>(st, c0 ? e0 : e1, c0 ? f0 : f1);
         }
-        if (st.overflow) return false;
+        if (st.fault) return false;
     }
     return true;
 }
@@ -133,7 +133,7 @@ constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode
 template <bool CLOSEST>
 __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, uint32_t lane, Stack& st,
                                            uint32_t* s_cnt, uint32_t& tests_total, uint32_t& tris_total,
-                                           uint32_t& overflow) {
+                                           uint32_t& faults, uint32_t* job_fail) {
     uint32_t tx = 0, ty = 0, tile_rays;
     if (a.camera) {
         tx = tile % a.tiles_x;
@@ -185,7 +185,11 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
         if (active) {
             if (!lane_step<CLOSEST>(a, L, st, tris_total)) {
                 // ray finished
-                if (st.overflow) { overflow = 1; st.overflow = false; }
+                if (st.fault) {
+                    faults |= st.fault;
+                    if (job_fail) atomicOr(job_fail, st.fault);
+                    st.fault = 0;
+                }
                 tests_total += L.tests;
                 size_t ri;
                 if (a.camera) {
@@ -236,8 +240,8 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     st.cap = a.spill_cap;
     st.spill = a.spill + gtid * (size_t)a.spill_cap * (CLOSEST ? 2 : 1);
-    st.overflow = false;
-    uint32_t tests_total = 0, tris_total = 0, overflow = 0;
+    st.fault = 0;
+    uint32_t tests_total = 0, tris_total = 0, faults = 0;
     while (true) {
         uint32_t tile = 0;
         if (lane == 0) tile = atomicAdd(&a.work[0], 1u);
@@ -263,12 +267,13 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
                 else hi = mid;
             }
             const TraceArgs J = jobs[lo];
-            trace_tile<CLOSEST>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, overflow);
+            trace_tile<CLOSEST>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, faults,
+                                a.job_fail ? a.job_fail + lo : nullptr);
         } else {
-            trace_tile<CLOSEST>(a, tile, lane, st, s_cnt, tests_total, tris_total, overflow);
+            trace_tile<CLOSEST>(a, tile, lane, st, s_cnt, tests_total, tris_total, faults, nullptr);
         }
     }
-    if (overflow) atomicOr(&a.work[1], 1u);
+    if (faults) atomicOr(&a.work[1], faults);
     if (a.stats) {
         unsigned long long t0 = tests_total, t1 = tris_total;
         for (int off = 32; off > 0; off >>= 1) {
@@ -285,6 +290,12 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
 __global__ void k_trace_init(uint32_t* work, unsigned long long* stats) {
     if (threadIdx.x < 4) work[threadIdx.x] = 0;
     if (stats && threadIdx.x < 2) stats[threadIdx.x] = 0;
+}
+
+int trace_fault(uint32_t bits, const char* what) {
+    SQ_RET(SQ_ERR_OVERFLOW, "%s: rays abandoned:%s%s%s", what, (bits & kFaultStack) ? " stack bound exceeded" : "",
+           (bits & kFaultNode) ? " child index outside the node array" : "",
+           (bits & kFaultTri) ? " triangle index outside the scene" : "");
 }
 
 int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, TracePlan* out) {
@@ -354,7 +365,7 @@ static int trace_finish(sq_ctx* c, const sq_trace_params* p) {
         c->last.node_tests = s[0];
         c->last.tri_tests = s[1];
     }
-    if (h[1]) SQ_RET(SQ_ERR_OVERFLOW, "trace: a ray exceeded the traversal stack bound");
+    if (h[1]) return trace_fault(h[1], "trace");
     return SQ_OK;
 }
 

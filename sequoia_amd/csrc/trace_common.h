@@ -25,7 +25,7 @@ struct TraceArgs {
     uint32_t slots;       // LDS entries per lane (power of two)
     uint32_t* spill;      // per-thread spill areas
     uint32_t spill_cap;   // entries per thread in the spill area
-    uint32_t* work;       // [0] tile counter, [1] overflow flag
+    uint32_t* work;       // [0] tile counter, [1] fault bits (kFault*)
     unsigned long long* stats;  // [0] node tests, [1] triangle tests
     uint32_t num_tiles;
     uint32_t tiles_x;
@@ -33,7 +33,13 @@ struct TraceArgs {
     const TraceArgs* jobs;
     const uint32_t* job_tiles;
     uint32_t njobs;
+    uint32_t* job_fail;   // optional: fault bits per job (njobs entries, zeroed by the caller)
 };
+
+// Why a ray was abandoned (its results are then invalid).
+constexpr uint32_t kFaultStack = 1;  // deeper than LDS slots + spill capacity
+constexpr uint32_t kFaultNode = 2;   // child index outside the node array
+constexpr uint32_t kFaultTri = 4;    // leaf triangle index outside the scene
 
 __device__ __forceinline__ uint32_t hash_u32(uint32_t v) {
     uint32_t s = v * 747796405u + 2891336453u;
@@ -139,7 +145,7 @@ struct Stack {
     uint32_t* spill;   // this thread's spill area (closest: 2 words per entry)
     uint32_t cap;
     uint32_t sp;
-    bool overflow;
+    uint32_t fault;  // kFault* bits of the current ray
 };
 
 template <bool CLOSEST>
@@ -147,7 +153,7 @@ __device__ __forceinline__ void push(Stack& st, uint32_t e, float t) {
     const uint32_t slot = (st.sp & st.mask) * st.stride;
     if (st.sp >= st.S) {
         uint32_t k = st.sp - st.S;
-        if (k >= st.cap) { st.overflow = true; return; }
+        if (k >= st.cap) { st.fault |= kFaultStack; return; }
         if (CLOSEST) {
             st.spill[2 * k] = st.lds[slot];
             st.spill[2 * k + 1] = __float_as_uint(st.lds_t[slot]);
@@ -199,6 +205,8 @@ struct TracePlan {
     size_t lds_bytes, spill_words;
 };
 int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, TracePlan* out);
+// SQ_ERR_OVERFLOW with a message naming the fault bits.
+int trace_fault(uint32_t bits, const char* what);
 // Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).
 int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi);
 

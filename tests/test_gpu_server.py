@@ -1,0 +1,169 @@
+"""Serving loop: each request's results are the same alone and in a mix of
+concurrent requests (and equal to the single-scene calls, hence the oracle);
+admission under a small budget evicts, rejects and fails as specified;
+queued requests can time out or be cancelled."""
+import time
+
+import numpy as np
+import pytest
+
+import sequoia_amd as sq
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def same(a, b):
+    if a is None or b is None:
+        return a is None and b is None
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
+
+
+def plain(ctx, tris, camera=None, rays=None, semantics=sq.TRACE_EXHAUSTIVE):
+    s = ctx.upload(tris)
+    b = ctx.build(s, "morton63")
+    if rays is not None:
+        r = ctx.trace_rays(b, rays, semantics=semantics)
+        r["rgb"] = None
+    else:
+        r = ctx.trace_camera(b, camera, semantics=semantics)
+    b.free()
+    s.free()
+    return r
+
+
+def mix_requests():
+    """Builds of uneven size with camera or secondary-ray traces, then
+    trace-only repeats of resident scenes."""
+    rng = np.random.default_rng(5)
+    reqs = []
+    scenes = {}
+    for k in range(14):
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(3e5))))
+        tris = sq.gen_icosphere(n, 500 + k, 0.1) if k % 3 == 0 else sq.gen_soup(n, 500 + k, 0.02)
+        scenes[k] = tris
+        if k % 2 == 0:
+            w = int(rng.integers(16, 200))
+            h = int(rng.integers(16, 120))
+            cam = sq.pinhole_camera(tris, w, h, 60.0, 3.0, spp=int(rng.integers(1, 4)), jitter=1, seed=k)
+            reqs.append((k, tris, cam, None))
+        else:
+            rays = sq.gen_secondary_rays(tris, int(rng.integers(1000, 60000)), seed=k, tmin=1e-4)
+            reqs.append((k, tris, None, rays))
+    for j in range(10):
+        k = int(rng.integers(0, 14))
+        tris = scenes[k]
+        if j % 2:
+            cam = sq.pinhole_camera(tris, 64, 40, 50.0, 2.5, spp=2, jitter=1, seed=100 + j)
+            reqs.append((k, None, cam, None))
+        else:
+            rays = sq.gen_secondary_rays(tris, 5000 + 100 * j, seed=200 + j, tmin=1e-4)
+            reqs.append((k, None, None, rays))
+    return reqs, scenes
+
+
+@pytest.mark.parametrize("semantics", [sq.TRACE_EXHAUSTIVE, sq.TRACE_CLOSEST])
+def test_mix_equals_alone(ctx, semantics):
+    reqs, scenes = mix_requests()
+    say("server mix", len(reqs), semantics)
+    with sq.Server(0, semantics=semantics) as srv:
+        ids = [srv.submit(k, tris=t, camera=c, rays=r) for (k, t, c, r) in reqs]
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    say("stats", st)
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    for (k, t, c, r), (status, out) in zip(reqs, got):
+        assert status["state"] == sq.REQ_DONE and status["status"] == 0
+        want = plain(ctx, scenes[k], camera=c, rays=r, semantics=semantics)
+        for f in ("hit_id", "hit_t", "counts", "rgb"):
+            assert same(out[f], want[f]), (k, f)
+    # the same requests one at a time on a fresh server
+    with sq.Server(0, semantics=semantics) as srv:
+        for (k, t, c, r), (_, mixed) in zip(reqs[:6], got[:6]):
+            rid = srv.submit(k, tris=t if t is not None else scenes[k], camera=c, rays=r)
+            _, alone = srv.wait(rid)
+            for f in ("hit_id", "hit_t", "counts", "rgb"):
+                assert same(alone[f], mixed[f]), (k, f)
+
+
+def test_batched_waves_overlap():
+    """Requests submitted together share waves (continuous batching)."""
+    tris = [sq.gen_soup(2000 + 100 * i, 50 + i, 0.02) for i in range(40)]
+    with sq.Server(0) as srv:
+        ids = []
+        for i, t in enumerate(tris):
+            cam = sq.pinhole_camera(t, 32, 32, 60.0, 3.0)
+            ids.append(srv.submit(i, tris=t, camera=cam))
+        waves = {srv.wait(i)[0]["wave"] for i in ids}
+        st = srv.stats()
+    assert st["completed"] == 40
+    assert len(waves) < 40 and st["mean_wave_requests"] > 1.0
+
+
+def test_budget_eviction_and_rejection(ctx):
+    tris = [sq.gen_soup(20000, 70 + i, 0.02) for i in range(6)]
+    # a 20000-triangle scene holds ~2.0 MB resident and its build + 40x30 trace
+    # wave ~3.4 MB: 9.5 MB leaves room for three resident scenes and one wave
+    with sq.Server(0, budget_bytes=9_500_000, inflight_waves=1, max_wave_requests=1) as srv:
+        ids = []
+        for i, t in enumerate(tris):
+            cam = sq.pinhole_camera(t, 40, 30, 60.0, 3.0)
+            ids.append(srv.submit(i, tris=t, camera=cam))
+        res = [srv.wait(i) for i in ids]
+        assert all(r[0]["state"] == sq.REQ_DONE for r in res)
+        st = srv.stats()
+        assert st["evictions"] >= 2 and st["resident_scenes"] <= 4
+        # scene 0 was evicted first: a trace-only request on it fails
+        cam = sq.pinhole_camera(tris[0], 40, 30, 60.0, 3.0)
+        s0, _ = srv.wait(srv.submit(0, camera=cam))
+        assert s0["state"] == sq.REQ_FAILED and s0["status"] == sq.ERR_INVALID
+        # the most recent scene is still resident and traces like the plain path
+        s5, out = srv.wait(srv.submit(5, camera=sq.pinhole_camera(tris[5], 40, 30, 60.0, 3.0)))
+        assert s5["state"] == sq.REQ_DONE
+        want = plain(ctx, tris[5], camera=sq.pinhole_camera(tris[5], 40, 30, 60.0, 3.0))
+        assert same(out["counts"], want["counts"])
+        # a request larger than the whole budget is rejected
+        big = sq.gen_soup(200000, 9, 0.02)
+        sb, _ = srv.wait(srv.submit(99, tris=big, camera=sq.pinhole_camera(big, 16, 16, 60.0, 3.0)))
+        assert sb["state"] == sq.REQ_FAILED and sb["status"] == sq.ERR_BUDGET
+        # explicit eviction
+        srv.evict(5)
+        with pytest.raises(sq.SqError):
+            srv.evict(5)
+
+
+def test_timeout_and_cancel():
+    big = sq.gen_soup(1000000, 2, 0.01)
+    cam = sq.pinhole_camera(big, 1920, 1080, 60.0, 3.0, spp=4, jitter=1, seed=2)
+    small = sq.gen_soup(1000, 3, 0.02)
+    scam = sq.pinhole_camera(small, 16, 16, 60.0, 3.0)
+    with sq.Server(0, inflight_waves=1) as srv:
+        a = srv.submit(1, tris=big, camera=cam, hits=False, rgb=False)
+        # wait until the long wave is on the device, then queue behind it
+        lib = sq.lib()
+        st = sq._lib.RequestStatus()
+        for _ in range(100000):
+            rc = lib.sq_server_wait(srv.h, a, 0.0, st)
+            if st.state != sq.REQ_QUEUED:
+                break
+            time.sleep(0.0001)
+        assert st.state in (sq.REQ_RUNNING, sq.REQ_DONE)
+        if st.state == sq.REQ_DONE:
+            pytest.skip("the long request finished before the queued ones could be tested")
+        b = srv.submit(2, tris=small, camera=scam, timeout_ms=0.001)
+        c = srv.submit(3, tris=small, camera=scam)
+        srv.cancel(c)
+        d = srv.submit(4, tris=small, camera=scam)
+        sa, _ = srv.wait(a)
+        sb, _ = srv.wait(b)
+        sc, _ = srv.wait(c)
+        sd, _ = srv.wait(d)
+        assert sa["state"] == sq.REQ_DONE
+        assert sb["state"] == sq.REQ_FAILED and sb["status"] == sq.ERR_TIMEOUT
+        assert sc["state"] == sq.REQ_FAILED and sc["status"] == sq.ERR_CANCELLED
+        assert sd["state"] == sq.REQ_DONE
+        with pytest.raises(sq.SqError):
+            srv.cancel(d)
