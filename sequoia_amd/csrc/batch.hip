@@ -208,6 +208,14 @@ struct sq_batch {
 
 namespace {
 
+// Host-to-device copy ordered on the context stream, the stream of every
+// kernel that reads the data (then waits, so the host source may go away).
+int upload(sq_ctx* c, void* dst, const void* src, size_t bytes) {
+    SQ_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SQ_OK;
+}
+
 int balloc(sq_batch* b, size_t bytes, void** out) {
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
@@ -266,7 +274,7 @@ int configure_build(sq_batch* b, sq_builder kind, const sq_build_params* p) {
     if (changed || !b->d_ws) {
         // descriptors and emission workspace depend on the threshold
         lbvh_fill_descs(b->desc.data(), b->counts.data(), b->nscenes, thr);
-        SQ_CHECK_HIP(hipMemcpy(b->d_desc, b->desc.data(), b->desc.size() * sizeof(SceneDesc), hipMemcpyHostToDevice));
+        SQ_TRY(upload(b->ctx, b->d_desc, b->desc.data(), b->desc.size() * sizeof(SceneDesc)));
         size_t ws = lbvh_batch_workspace(b->total, b->nscenes, thr);
         if (ws > b->ws_bytes) {
             bfree(b, b->d_ws);
@@ -361,6 +369,8 @@ int finish_depths(sq_ctx* c, sq_batch* b) {
     SQ_CHECK_HIP(hipMemcpyAsync(depth.data(), b->d_depth, 4ull * b->nscenes, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
     for (uint32_t s = 0; s < b->nscenes; s++) b->bvhs[s].depth = depth[s];
+    for (uint32_t s = 0; s < b->nscenes; s++)
+        if (depth[s] == 0) SQ_RET(SQ_ERR_OVERFLOW, "batch build: emission workspace overflow (scene %u)", s);
     return SQ_OK;
 }
 
@@ -376,7 +386,8 @@ int check_trace(sq_ctx* c, sq_batch* b) {
     }
     if (h[1]) {
         std::vector<uint32_t> f(b->nscenes);
-        SQ_CHECK_HIP(hipMemcpy(f.data(), b->d_job_fail, 4ull * b->nscenes, hipMemcpyDeviceToHost));
+        SQ_CHECK_HIP(hipMemcpyAsync(f.data(), b->d_job_fail, 4ull * b->nscenes, hipMemcpyDeviceToHost, c->stream));
+        SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
         uint32_t first = 0;
         while (first < b->nscenes && !f[first]) first++;
         char what[64];
@@ -429,8 +440,7 @@ int sq_batch_create(sq_ctx* c, const sq_triangle* tris, const uint32_t* counts, 
     } else {
         if ((rc = balloc_t(b, total, &b->d_tris)) != SQ_OK) return fail(rc);
         b->own_tris = true;
-        hipError_t e = hipMemcpy(b->d_tris, tris, total * sizeof(sq_triangle), hipMemcpyHostToDevice);
-        if (e != hipSuccess) { set_error("batch upload: %s", hipGetErrorString(e)); return fail(SQ_ERR_HIP); }
+        if ((rc = upload(c, b->d_tris, tris, total * sizeof(sq_triangle))) != SQ_OK) return fail(rc);
     }
     if ((rc = balloc_t(b, nscenes + 1, &b->d_off)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, nscenes, &b->d_desc)) != SQ_OK) return fail(rc);
@@ -440,8 +450,7 @@ int sq_batch_create(sq_ctx* c, const sq_triangle* tris, const uint32_t* counts, 
     if ((rc = balloc_t(b, 16, &b->d_work)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, 4, &b->d_stats)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, nscenes, &b->d_job_fail)) != SQ_OK) return fail(rc);
-    hipError_t e = hipMemcpy(b->d_off, b->off.data(), 4ull * (nscenes + 1), hipMemcpyHostToDevice);
-    if (e != hipSuccess) { set_error("batch upload: %s", hipGetErrorString(e)); return fail(SQ_ERR_HIP); }
+    if ((rc = upload(c, b->d_off, b->off.data(), 4ull * (nscenes + 1))) != SQ_OK) return fail(rc);
     b->scenes.resize(nscenes);
     b->bvhs.resize(nscenes);
     for (uint32_t s = 0; s < nscenes; s++) {
@@ -558,8 +567,8 @@ int sq_batch_set_cameras(sq_ctx* c, sq_batch* b, const sq_camera* cams, const sq
     }
     SQ_TRY(balloc_t(b, jobs.size(), &b->d_jobs));
     SQ_TRY(balloc_t(b, job_tiles.size(), &b->d_job_tiles));
-    SQ_CHECK_HIP(hipMemcpy(b->d_jobs, jobs.data(), jobs.size() * sizeof(TraceArgs), hipMemcpyHostToDevice));
-    SQ_CHECK_HIP(hipMemcpy(b->d_job_tiles, job_tiles.data(), job_tiles.size() * 4, hipMemcpyHostToDevice));
+    SQ_TRY(upload(c, b->d_jobs, jobs.data(), jobs.size() * sizeof(TraceArgs)));
+    SQ_TRY(upload(c, b->d_job_tiles, job_tiles.data(), job_tiles.size() * 4));
     b->jobs = std::move(jobs);
     b->job_tiles = std::move(job_tiles);
     b->ray_off = std::move(ray_off);

@@ -64,7 +64,7 @@ struct SceneDesc {
     uint32_t q_base;      // offset of the scene's top-level task queues
     uint32_t top_base;    // offset of the scene's top node list
     uint32_t lvl_base;    // offset of the scene's level table
-    uint32_t pad;
+    uint32_t top_cap;     // entries of the scene's top node list
 };
 
 // Events bracketing the phases of the last build/trace.

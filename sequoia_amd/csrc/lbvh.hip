@@ -186,11 +186,18 @@ struct LbvhArgs {
     uint32_t* scene_levels;    // per scene: number of top levels
     uint32_t* scene_small;     // per scene: most levels of one small range
     uint32_t* depth_out;       // per scene: tree depth (root = 1), may be null
-    uint32_t* counters;        // [0] small-task count, [1] max small-range levels
+    uint32_t* counters;        // [0] small-task count, [1] max small-range levels, [2] error flag
+    uint32_t small_cap;        // entries of `small`
 };
 
 __device__ __forceinline__ EmitArgs scene_emit(const LbvhArgs& a, const SceneDesc& d) {
     return EmitArgs{a.tris, a.idx, a.nodes + d.node_base, a.layout, a.refit, 2 * d.n, d.tri_base};
+}
+
+__device__ __forceinline__ void push_small(const LbvhArgs& a, const Task& t) {
+    const uint32_t k = atomicAdd(&a.counters[0], 1u);
+    if (k < a.small_cap) a.small[k] = t;
+    else atomicOr(&a.counters[2], 1u);
 }
 
 // One workgroup per scene: splits every range larger than T level by level.
@@ -202,6 +209,7 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
     const EmitArgs e = scene_emit(a, d);
     const K* codes = (const K*)a.codes;
     Task* q[2] = {a.qa + d.q_base, a.qb + d.q_base};
+    const uint32_t qcap = d.n / a.threshold + 4;  // ranges > T in one level are disjoint
     uint32_t* top_list = a.top_list + d.top_base;
     uint32_t* level_off = a.level_off + d.lvl_base;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = kTopThreads / 64;
@@ -221,7 +229,7 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
             sqd::store_node_links(e.nodes, 0, 1, 2, 0);
             Task t{d.seg_begin, d.seg_begin + d.n, 1, 0, scene};
             if (d.n > a.threshold) { q[0][0] = t; s_ncur = 1; }
-            else a.small[atomicAdd(&a.counters[0], 1u)] = t;
+            else push_small(a, t);
         }
     }
     __syncthreads();
@@ -240,10 +248,17 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
                 emit_children(e, tk.begin, tk.end, tk.free, split,
                               [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
                                   Task c{b, en, f, self, scene};
-                                  if (en - b > a.threshold) nxt[atomicAdd(&s_nnext, 1u)] = c;
-                                  else a.small[atomicAdd(&a.counters[0], 1u)] = c;
+                                  if (en - b > a.threshold) {
+                                      const uint32_t k = atomicAdd(&s_nnext, 1u);
+                                      if (k < qcap) nxt[k] = c;
+                                      else atomicOr(&a.counters[2], 1u);
+                                  } else {
+                                      push_small(a, c);
+                                  }
                               });
-                top_list[atomicAdd(&s_ntop, 1u)] = tk.self;
+                const uint32_t k = atomicAdd(&s_ntop, 1u);
+                if (k < d.top_cap) top_list[k] = tk.self;
+                else atomicOr(&a.counters[2], 1u);
             }
         }
         __syncthreads();
@@ -261,14 +276,29 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
     }
 }
 
+// A split task inside one small range, in LDS: begin/end relative to the
+// range (16 bits each), first free slot, own slot.
+struct SmallTask {
+    uint32_t be;
+    uint32_t free;
+    uint32_t self;
+};
+
 // Persistent: one workgroup finishes one queued range at a time (codes in
 // LDS, the reference's binary search per thread), then refits it.
+//
+// Capacity: a range of c <= T primitives has at most c - 6 split tasks
+// (tasks are ranges of >= 8; leaves of the task tree are disjoint, and every
+// task with a single task child peels off primitives no other task holds),
+// so T entries always suffice; the level count is at most the number of
+// code bits plus log2(T). Both are still checked, and a violation raises the
+// launch's error flag instead of writing out of bounds.
 template <typename K>
 __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
     __shared__ K s_codes[kMaxThreshold];
-    __shared__ Task s_tasks[kMaxThreshold / 4 + 2];
+    __shared__ SmallTask s_tasks[kMaxThreshold];
     __shared__ uint32_t s_lvl[kMaxLevels + 1];
-    __shared__ uint32_t s_total;
+    __shared__ uint32_t s_total, s_err;
     const K* codes = (const K*)a.codes;
     const uint32_t ntasks = a.counters[0];
     uint32_t max_lev = 0;
@@ -277,30 +307,47 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
         const SceneDesc d = a.scenes[root.scene];
         const EmitArgs e = scene_emit(a, d);
         const uint32_t base = root.begin, count = root.end - root.begin;
+        if (count > kMaxThreshold || count < 8) {  // never queued so; flag and skip
+            if (threadIdx.x == 0) atomicOr(&a.counters[2], 1u);
+            continue;
+        }
         for (uint32_t i = threadIdx.x; i < count; i += kSmallThreads) s_codes[i] = codes[base + i];
         if (threadIdx.x == 0) {
-            s_tasks[0] = root;
+            s_tasks[0] = SmallTask{count << 16, root.free, root.self};
             s_total = 1;
+            s_err = 0;
         }
         __syncthreads();
         uint32_t lb = 0, le = 1, nlev = 0;
         while (lb < le) {
-            if (threadIdx.x == 0 && nlev < (uint32_t)kMaxLevels) s_lvl[nlev] = lb;
+            if (threadIdx.x == 0) {
+                if (nlev < (uint32_t)kMaxLevels) s_lvl[nlev] = lb;
+                else s_err = 1;
+            }
             for (uint32_t i = lb + threadIdx.x; i < le; i += kSmallThreads) {
-                Task tk = s_tasks[i];
-                uint32_t split = find_split_ref<K>(s_codes + (tk.begin - base), tk.end - tk.begin);
-                emit_children(e, tk.begin, tk.end, tk.free, split,
+                const SmallTask tk = s_tasks[i];
+                const uint32_t tb = tk.be & 0xFFFFu, te = tk.be >> 16;
+                uint32_t split = find_split_ref<K>(s_codes + tb, te - tb);
+                emit_children(e, base + tb, base + te, tk.free, split,
                               [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
-                                  s_tasks[atomicAdd(&s_total, 1u)] = Task{b, en, f, self, root.scene};
+                                  const uint32_t k = atomicAdd(&s_total, 1u);
+                                  if (k < kMaxThreshold) s_tasks[k] = SmallTask{(b - base) | ((en - base) << 16), f, self};
+                                  else s_err = 1;
                               });
             }
             __syncthreads();
             lb = le;
-            le = s_total;
+            le = s_total < kMaxThreshold ? s_total : kMaxThreshold;
             nlev++;
+            const bool stop = s_err != 0;
             __syncthreads();
+            if (stop) break;
         }
-        if (nlev > (uint32_t)kMaxLevels) nlev = kMaxLevels;
+        if (s_err) {
+            if (threadIdx.x == 0) atomicOr(&a.counters[2], 1u);
+            __syncthreads();
+            continue;
+        }
         if (threadIdx.x == 0) s_lvl[nlev] = le;
         max_lev = nlev > max_lev ? nlev : max_lev;
         if (threadIdx.x == 0) atomicMax(&a.scene_small[root.scene], nlev);
@@ -323,8 +370,10 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_finish(LbvhArgs a) {
     const uint32_t scene = blockIdx.x;
     const SceneDesc d = a.scenes[scene];
     const uint32_t levels = a.scene_levels[scene];
-    if (threadIdx.x == 0 && a.depth_out) a.depth_out[scene] = levels + a.scene_small[scene] + 2;
-    if (!a.refit || levels >= (uint32_t)kMaxLevels) return;
+    const bool failed = a.counters[2] != 0 || levels >= (uint32_t)kMaxLevels;
+    // depth 0 marks a tree whose emission ran out of workspace (see counters[2])
+    if (threadIdx.x == 0 && a.depth_out) a.depth_out[scene] = failed ? 0u : levels + a.scene_small[scene] + 2;
+    if (!a.refit || failed) return;
     sq_node* nodes = a.nodes + d.node_base;
     const uint32_t cap = 2 * d.n;
     const uint32_t* top_list = a.top_list + d.top_base;
@@ -347,35 +396,44 @@ static uint32_t clamp_threshold(uint32_t threshold) {
     return T;
 }
 
+// Ranges split by the top phase: at most n/T + 1 per level (disjoint, > T
+// each) over at most 3*21 code-bit levels plus log2(n) midpoint levels of
+// all-equal ranges, so 96 levels' worth bounds a scene's top list.
+constexpr uint32_t kTopListLevels = 96;
+static uint32_t top_list_cap(uint32_t n, uint32_t T) { return (n / T + 1) * kTopListLevels; }
+
 // Scene descriptors for a concatenation of scenes: scene s covers triangles
 // (and sorted positions) [tri_base, tri_base + n) and nodes [2 tri_base,
 // 2 (tri_base + n)); its queue, top-list and level-table regions are prefix
-// sums of n/T + 4, 2 n/T + 8 and kMaxLevels + 2 entries.
+// sums of n/T + 4, top_list_cap(n, T) and kMaxLevels + 2 entries.
 void lbvh_fill_descs(SceneDesc* d, const uint32_t* counts, uint32_t nscenes, uint32_t threshold) {
     const uint32_t T = clamp_threshold(threshold);
     uint32_t tri = 0, q = 0, top = 0;
     for (uint32_t s = 0; s < nscenes; s++) {
         const uint32_t n = counts[s];
-        d[s] = SceneDesc{tri, n, tri, 2 * tri, q, top, s * (uint32_t)(kMaxLevels + 2), 0};
+        const uint32_t top_cap = top_list_cap(n, T);
+        d[s] = SceneDesc{tri, n, tri, 2 * tri, q, top, s * (uint32_t)(kMaxLevels + 2), top_cap};
         tri += n;
         q += n / T + 4;
-        top += 2 * (n / T) + 8;
+        top += top_cap;
     }
 }
 
 struct WsLayout {
     size_t qa, qb, small, top, lvl, counters, scene_levels, scene_small, total;
+    size_t small_cap;
 };
 static WsLayout ws_layout(uint32_t total, uint32_t nscenes, uint32_t T) {
     WsLayout w;
     size_t q = (size_t)total / T + 4 * (size_t)nscenes + 4;
-    size_t sm = (size_t)total / 8 + nscenes + 4;
-    size_t tp = 2 * ((size_t)total / T) + 8 * (size_t)nscenes + 8;
+    size_t sm = (size_t)total / 8 + nscenes + 4;  // small ranges are disjoint, >= 8 each
+    size_t tp = ((size_t)total / T + nscenes) * kTopListLevels + 8;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     w.qa = take(q * sizeof(Task));
     w.qb = take(q * sizeof(Task));
     w.small = take(sm * sizeof(Task));
+    w.small_cap = sm;
     w.top = take(tp * 4);
     w.lvl = take((size_t)nscenes * (kMaxLevels + 2) * 4);
     // counters, scene_levels and scene_small are contiguous (zeroed together)
@@ -417,6 +475,7 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     a.scene_small = (uint32_t*)(base + w.scene_small);
     a.depth_out = d_depth;
     a.counters = (uint32_t*)(base + w.counters);
+    a.small_cap = (uint32_t)w.small_cap;
     const uint32_t zero_words = 16 + 2 * nscenes;
     uint32_t zb = (zero_words + 255) / 256;
     zb = zb > 64 ? 64 : zb;
@@ -451,23 +510,24 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
     SQ_TRY(c->aux[2].ensure(sizeof(SceneDesc)));
     SQ_TRY(pinned_ensure(c, 256));
     SceneDesc* hd = (SceneDesc*)((char*)c->pinned + 128);
-    *hd = SceneDesc{0, n, 0, 0, 0, 0, 0, 0};
+    lbvh_fill_descs(hd, &n, 1, T);
     SQ_CHECK_HIP(hipMemcpyAsync(c->aux[2].p, hd, sizeof(SceneDesc), hipMemcpyHostToDevice, c->stream));
     char* base = c->aux[1].as<char>();
     uint32_t* d_depth = (uint32_t*)(base + w.total);
     SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit, T,
                        d_nodes, c->aux[1].p, d_depth, true));
     uint32_t* h = (uint32_t*)c->pinned;
-    SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 8, hipMemcpyDeviceToHost, c->stream));
-    SQ_CHECK_HIP(hipMemcpyAsync(h + 2, base + w.scene_levels, 4, hipMemcpyDeviceToHost, c->stream));
-    SQ_CHECK_HIP(hipMemcpyAsync(h + 3, d_depth, 4, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 12, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipMemcpyAsync(h + 4, base + w.scene_levels, 4, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipMemcpyAsync(h + 5, d_depth, 4, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
-    const uint32_t nsmall = h[0], top_levels = h[2];
+    const uint32_t nsmall = h[0], top_levels = h[4];
+    if (h[2]) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: emission workspace overflow");
     if (top_levels >= (uint32_t)kMaxLevels) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: more than %d top levels", kMaxLevels);
     c->last.top_levels = top_levels;
     c->last.small_tasks = nsmall;
     // levels of split ranges + fan/leaf level + root
-    *depth = h[3];
+    *depth = h[5];
     return SQ_OK;
 }
 

@@ -14,8 +14,10 @@
 //   5. one device-to-host copy of all outputs,
 // then the scheduler copies each request's outputs to its destinations. Two
 // slots alternate, so a wave is formed and uploaded while the previous one
-// is still on the device. All device memory comes from an arena of large
-// chunks (no device frees while serving).
+// is still on the device. Waves never wait on each other on the device: a
+// trace of a scene whose building wave is still in flight joins a later
+// wave. All device memory comes from an arena of large chunks (no device
+// frees while serving).
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -100,7 +102,8 @@ struct Resident {
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t ev_build = nullptr, ev_done = nullptr;
+    hipEvent_t ev_done = nullptr;
+    bool own_stream = false;
     char* pin_in = nullptr;
     size_t pin_in_bytes = 0;
     char* pin_out = nullptr;
@@ -114,6 +117,7 @@ struct Slot {
     uint64_t block = 0;   // arena block of the wave
     size_t out_off = 0, out_bytes = 0;  // output region inside the block
     size_t fail_off = 0;                // per-request fault bits in the output region
+    size_t dep_off = 0;                 // per-new-scene tree depth in the output region
 };
 
 }  // namespace
@@ -213,7 +217,7 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     // device-only workspace
     size_t off = in_bytes;
     auto take = [&](size_t bytes) { size_t o = off; off += Arena::round(bytes); return o; };
-    size_t o_keys[2] = {0, 0}, o_vals[2] = {0, 0}, o_hist = 0, o_ws = 0, o_ord = 0, o_depth = 0, o_nodes = 0;
+    size_t o_keys[2] = {0, 0}, o_vals[2] = {0, 0}, o_hist = 0, o_ws = 0, o_ord = 0, o_nodes = 0;
     const uint32_t digit = 8;
     if (S) {
         o_keys[0] = take(8 * T);
@@ -223,7 +227,6 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
         o_hist = take(sort_hist_bytes((uint32_t)T, digit));
         o_ws = take(lbvh_batch_workspace((uint32_t)T, S, 0));
         o_ord = take(24ull * S);
-        o_depth = take(4ull * S);
         o_nodes = take(64 * T);
     }
     // outputs: work counters first
@@ -232,6 +235,7 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     auto take_out = [&](size_t bytes) { size_t o = oo; oo += Arena::round(bytes); return o; };
     const size_t o_work = take_out(64);
     const size_t o_fail = take_out(4 * rq.size());
+    const size_t o_dep = take_out(4ull * (S ? S : 1));  // per new scene: tree depth, 0 = build failed
     for (Req* q : rq) {
         const bool cam = q->r.nrays == 0;
         q->o_hit_id = q->r.out.hit_id ? take_out(4 * q->rays) : SIZE_MAX;
@@ -337,15 +341,10 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     std::memcpy(h + o_jt, jt.data(), 4ull * jt.size());
 
     // ---- device work on the slot stream ----
+    // Every scene a trace of this wave reads was built by a retired wave or
+    // by this wave (the policy defers the rest), so the wave depends only on
+    // earlier work of its own stream.
     hipStream_t st = sl.stream;
-    // trace-only requests on scenes still being built by the other slot
-    for (Slot& o : s->slots) {
-        if (&o == &sl || !o.busy) continue;
-        bool dep = false;
-        for (Req* q : rq)
-            if (!q->r.tris && std::find(o.keys.begin(), o.keys.end(), q->r.key) != o.keys.end()) dep = true;
-        if (dep) SQ_CHECK_HIP(hipStreamWaitEvent(st, o.ev_build, 0));
-    }
     SQ_CHECK_HIP(hipMemcpyAsync(dev, h, in_bytes, hipMemcpyHostToDevice, st));
     if (S) {
         BatchBuild j;
@@ -361,7 +360,7 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
         j.vals[1] = (uint32_t*)(dev + o_vals[1]);
         j.hist = (uint32_t*)(dev + o_hist);
         j.ws = dev + o_ws;
-        j.d_depth = (uint32_t*)(dev + o_depth);
+        j.d_depth = (uint32_t*)(dev + out_off + o_dep);
         j.nodes = (sq_node*)(dev + o_nodes);
         j.bits = kMortonBits;
         j.digit = digit;
@@ -372,7 +371,6 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
                            (uint32_t)copies.size());
         SQ_CHECK_HIP(hipGetLastError());
     }
-    SQ_CHECK_HIP(hipEventRecord(sl.ev_build, st));
     TraceArgs a{};
     a.slots = s->plan.slots;
     a.spill = sl.spill;
@@ -398,6 +396,7 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     sl.out_off = out_off;
     sl.out_bytes = out_bytes;
     sl.fail_off = o_fail;
+    sl.dep_off = o_dep;
     for (Req* q : rq) {
         q->state = SQ_REQ_RUNNING;
         q->wave = wave;
@@ -408,13 +407,30 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     return SQ_OK;
 }
 
+void release_evicted(sq_server* s, const std::vector<uint64_t>& keys) {
+    for (uint64_t k : keys) {
+        auto it = s->resident.find(k);
+        if (it == s->resident.end()) continue;
+        if (it->second.block) s->arena->release(it->second.block);
+        s->resident.erase(it);
+    }
+}
+
 // Copies a finished wave's outputs to the requests' destinations.
 void complete_wave(sq_server* s, Slot& sl, double t) {
     const uint32_t* fail = (const uint32_t*)(sl.pin_out + sl.fail_off);
+    const uint32_t* depth = (const uint32_t*)(sl.pin_out + sl.dep_off);
+    // a tree whose emission overflowed its workspace fails every request of the wave using it
+    std::vector<uint64_t> bad_keys;
+    for (uint64_t id : sl.reqs) {
+        const Req& q = s->reqs[id];
+        if (q.r.tris && depth[q.scene_idx] == 0) bad_keys.push_back(q.r.key);
+    }
     for (size_t k = 0; k < sl.reqs.size(); k++) {
         Req& q = s->reqs[sl.reqs[k]];
         q.fault = fail[k];
-        const int st = q.fault ? SQ_ERR_OVERFLOW : SQ_OK;
+        const bool bad_tree = std::find(bad_keys.begin(), bad_keys.end(), q.r.key) != bad_keys.end();
+        const int st = (q.fault || bad_tree) ? SQ_ERR_OVERFLOW : SQ_OK;
         const bool cam = q.r.nrays == 0;
         if (st == SQ_OK) {
             if (q.r.out.hit_id) std::memcpy(q.r.out.hit_id, sl.pin_out + q.o_hit_id, 4 * q.rays);
@@ -425,6 +441,8 @@ void complete_wave(sq_server* s, Slot& sl, double t) {
         finish(s, q, st, t);
     }
     s->adm->wave_done(sl.wave, sl.work, sl.keys);
+    for (uint64_t k : bad_keys)
+        if (s->adm->evict(k)) release_evicted(s, {k});
     s->arena->release(sl.block);
     sl.busy = false;
     sl.reqs.clear();
@@ -434,15 +452,6 @@ void complete_wave(sq_server* s, Slot& sl, double t) {
 void fail_decision(sq_server* s, uint64_t id, int status, double t) {
     auto it = s->reqs.find(id);
     if (it != s->reqs.end() && it->second.state == SQ_REQ_QUEUED) finish(s, it->second, status, t);
-}
-
-void release_evicted(sq_server* s, const std::vector<uint64_t>& keys) {
-    for (uint64_t k : keys) {
-        auto it = s->resident.find(k);
-        if (it == s->resident.end()) continue;
-        if (it->second.block) s->arena->release(it->second.block);
-        s->resident.erase(it);
-    }
 }
 
 void scheduler(sq_server* s) {
@@ -556,13 +565,22 @@ int sq_server_create(int device, const sq_server_config* cfg, sq_server** out) {
     tp.semantics = c.semantics;
     int rc = trace_plan(s->ctx, &tp, morton_depth_bound(kMortonBits, 0x7FFFFFFFu), &s->plan);
     s->slots.resize(c.inflight_waves);
-    for (Slot& sl : s->slots) {
+    // slot 0 runs on the context's stream, slot 1 on one more stream
+    for (size_t i = 0; i < s->slots.size(); i++) {
+        Slot& sl = s->slots[i];
         if (rc != SQ_OK) break;
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.ev_build, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming) != hipSuccess ||
+        if (i == 0) {
+            sl.stream = s->ctx->stream;
+        } else if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) == hipSuccess) {
+            sl.own_stream = true;
+        } else {
+            set_error("server: stream setup failed");
+            rc = SQ_ERR_HIP;
+            break;
+        }
+        if (hipEventCreateWithFlags(&sl.ev_done, hipEventDisableTiming) != hipSuccess ||
             hipMalloc((void**)&sl.spill, s->plan.spill_words * 4) != hipSuccess) {
-            set_error("server: stream/event/spill setup failed");
+            set_error("server: event/spill setup failed");
             rc = SQ_ERR_HIP;
         }
     }
@@ -589,9 +607,8 @@ void sq_server_destroy(sq_server* s) {
     if (s->ctx) (void)hipSetDevice(s->ctx->device);
     for (Slot& sl : s->slots) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
-        if (sl.ev_build) (void)hipEventDestroy(sl.ev_build);
         if (sl.ev_done) (void)hipEventDestroy(sl.ev_done);
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.own_stream) (void)hipStreamDestroy(sl.stream);
         if (sl.spill) (void)hipFree(sl.spill);
         if (sl.pin_in) (void)hipHostFree(sl.pin_in);
         if (sl.pin_out) (void)hipHostFree(sl.pin_out);

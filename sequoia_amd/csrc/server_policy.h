@@ -155,6 +155,8 @@ class Admission {
     // Forms one wave from the FIFO `pending` (ids in order). Admitted and
     // finished requests leave `pending`; the first request that must wait
     // for memory stops admission (FIFO, no overtaking of a waiting build).
+    // Trace-only requests on a scene whose building wave is still in flight
+    // stay queued (later requests may pass them).
     // `evicted` receives the keys of scenes dropped to make room.
     struct Decision {
         uint64_t id;
@@ -179,6 +181,15 @@ class Admission {
                 out.push_back({r.id, Verdict::UnknownScene});
                 it = pending.erase(it);
                 continue;
+            }
+            if (!r.build) {
+                // a scene still being built by a wave in flight is traced by a
+                // later wave: waves never wait on each other on the device
+                const Scene& sc = scenes_.at(r.key);
+                if (!sc.ready && sc.wave != wave_id) {
+                    ++it;
+                    continue;
+                }
             }
             if (nreq >= lim_.max_requests || (nreq && (rays + r.rays > lim_.max_rays || tris + r.tris > lim_.max_tris)))
                 break;

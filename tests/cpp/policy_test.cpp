@@ -159,8 +159,30 @@ static void test_cancel_expire_limits() {
     CHECK(d.size() == 1 && d[0].v == Verdict::Admit && adm.scene(3)->bytes == 200 && ev.size() == 1);
 }
 
+static void test_trace_waits_for_build_wave() {
+    Admission adm(1 << 30);
+    std::deque<PolicyRequest> q;
+    q.push_back(build(1, 7, 100, 100));
+    q.push_back(trace(2, 7, 100));  // same wave as the build: admitted
+    std::vector<uint64_t> ev;
+    auto d = adm.form_wave(q, 0, 1, &ev);
+    CHECK(d.size() == 2 && d[1].v == Verdict::Admit);
+    // while wave 1 is in flight, a trace of scene 7 waits for a later wave;
+    // requests behind it still go
+    q.push_back(trace(3, 7, 100));
+    q.push_back(build(4, 8, 100, 100));
+    d = adm.form_wave(q, 0, 2, &ev);
+    CHECK(d.size() == 1 && d[0].id == 4 && d[0].v == Verdict::Admit);
+    CHECK(q.size() == 1 && q.front().id == 3);
+    adm.wave_done(1, 200, {7, 7});
+    CHECK(adm.scene(7)->ready);
+    d = adm.form_wave(q, 0, 3, &ev);
+    CHECK(d.size() == 1 && d[0].id == 3 && d[0].v == Verdict::Admit);
+}
+
 int main() {
     test_arena();
+    test_trace_waits_for_build_wave();
     test_budget_and_queue();
     test_trace_only_and_lru();
     test_cancel_expire_limits();

@@ -194,3 +194,23 @@ def test_batch_rejects_bad_input(ctx):
     with pytest.raises(sq.SqError):
         b.trace()
     b.free()
+
+
+def test_batch_large_scenes(ctx):
+    """Scenes of 0.9M-2M triangles in one batch (two segment digits' worth of
+    triangles, code ranges split by the top phase over many levels) equal
+    their single-scene builds and traces."""
+    scenes = [sq.gen_icosphere(1837637, 6023, 0.1), sq.gen_soup(2_000_000, 31, 0.01),
+              sq.gen_icosphere(918884, 6003, 0.1), sq.gen_soup(3000, 32, 0.02)]
+    cams = [sq.pinhole_camera(t, 320, 180, 60.0, 3.0, spp=2, jitter=1, seed=40 + i) for i, t in enumerate(scenes)]
+    b = sq.Batch(ctx, scenes)
+    say("large batch build")
+    b.build("morton63")
+    b.set_cameras(cams)
+    b.trace()
+    for i, t in enumerate(scenes):
+        want, depth = plain_tree(ctx, t, "morton63")
+        assert len(nodes_equal(b.bvh(i).download(), want)) == 0, i
+        assert b.bvh(i).info()[2] == depth, i
+        assert_same(b.download(i), plain_trace(ctx, t, cams[i], "morton63", sq.TRACE_EXHAUSTIVE), ("large", i))
+    b.free()

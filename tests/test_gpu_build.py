@@ -115,3 +115,20 @@ def test_large_soup_morton63(ctx):
     assert len(nodes_equal(got, want)) == 0
     t = ctx.timings()
     say("timings", {k: round(v, 3) if isinstance(v, float) else v for k, v in t.items()})
+
+
+@pytest.mark.parametrize("kind,bits", [("morton63", 21), ("morton", 10)])
+def test_large_icosphere_many_small_tasks(ctx, kind, bits):
+    """A 1.84M-triangle icosphere whose Morton ranges of <= 2048 primitives
+    split into more than T/4 + 2 tasks (517 in one range): the emission
+    must still equal the oracle's tree for the default and smaller subtree
+    thresholds."""
+    say("large icosphere", kind)
+    tris = sq.gen_icosphere(1837637, 6023, 0.1)
+    want, _, _, _ = ob.build_morton(tris, bits)
+    scene = ctx.upload(tris)
+    for thr in (0, 1024):
+        bvh = ctx.build(scene, kind, subtree_threshold=thr)
+        got = bvh.download()
+        assert len(nodes_equal(got, want)) == 0, (kind, thr, nodes_equal(got, want)[:5])
+        bvh.free()
