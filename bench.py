@@ -10,7 +10,13 @@ One step = build (Morton codes, radix sort, emission, refit) + trace, inputs
 resident in HBM. `value` is Mrays/s of the trace (HIP events on the library
 stream around the traversal kernel, summed over the timed steps).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+The same JSON line carries two more sections: "batched" (256 scenes built and
+traced in one captured graph against 256 sequential plain calls) and
+"served" (a Poisson request stream through the serving loop against one
+plain call per request, throughput and p50/p99 latency at three rates), and
+"cpu_baseline" (the reference's own CPU build + traverse on a bounded sample).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batched] [--no-served]
 """
 import argparse
 import json
@@ -307,7 +313,8 @@ def served_section(sq, ctx):
                        "p50_ms": round(float(np.percentile(sl, 50)), 2),
                        "p99_ms": round(float(np.percentile(sl, 99)), 2),
                        "waves": int(st["waves"]), "mean_wave_requests": round(st["mean_wave_requests"], 2),
-                       "failed": len(st["failed_requests"])},
+                       "failed": len(st["failed_requests"]), "host_stage_in_ms": round(st["stage_in_ms"], 1),
+                       "host_stage_out_ms": round(st["stage_out_ms"], 1), "wave_ms_total": round(st["wave_ms"], 1)},
             "plain": {"req_s": round(len(reqs) / pw, 1), "mrays_s": round(nrays / pw / 1e6, 1),
                       "p50_ms": round(float(np.percentile(pl, 50)), 2),
                       "p99_ms": round(float(np.percentile(pl, 99)), 2)},
@@ -332,8 +339,7 @@ def main():
     ap.add_argument("--semantics", default="exhaustive", choices=["exhaustive", "closest"])
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-scene batched-vs-plain comparison")
     ap.add_argument("--batched-steps", type=int, default=3)
-    ap.add_argument("--served", action="store_true",
-                    help="also run the Poisson-stream served-vs-plain comparison (off by default)")
+    ap.add_argument("--no-served", action="store_true", help="skip the Poisson-stream served-vs-plain comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (byte-counter calibration)")
     a = ap.parse_args()
@@ -424,10 +430,17 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_trace<false>", "algorithmic_bytes_per_launch": alg_bytes},
     }
+    # the secondary sections never take the headline line down with them
     if not a.no_batched:
-        line["batched"] = batched_section(sq, ctx, a.batched_steps, 1)
-    if a.served:
-        line["served"] = served_section(sq, ctx)
+        try:
+            line["batched"] = batched_section(sq, ctx, a.batched_steps, 1)
+        except Exception as e:  # noqa: BLE001
+            line["batched"] = {"error": repr(e)}
+    if not a.no_served:
+        try:
+            line["served"] = served_section(sq, ctx)
+        except Exception as e:  # noqa: BLE001
+            line["served"] = {"error": repr(e)}
     if not a.no_cpu:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(tris, cam, rays_per_s, a.cpu_budget)

@@ -262,8 +262,8 @@ typedef struct sq_server sq_server;
 typedef struct {
     uint64_t budget_bytes;      /* device bytes for resident scenes + in-flight waves; 0 = 200 GB */
     uint32_t max_wave_requests; /* 0 = 256 */
-    uint32_t max_wave_rays;     /* 0 = 64 Mi */
-    uint32_t max_wave_tris;     /* 0 = 32 Mi */
+    uint32_t max_wave_rays;     /* 0 = 16 Mi (a single larger request still forms its own wave) */
+    uint32_t max_wave_tris;     /* 0 = 8 Mi */
     uint32_t semantics;         /* sq_semantics of every trace */
     uint32_t inflight_waves;    /* wave slots on the device at once, 1 or 2; 0 = 2 */
     uint32_t reserved[3];
@@ -305,6 +305,8 @@ typedef struct {
     float p50_ms, p99_ms, mean_ms, max_ms; /* latency over completed requests */
     float span_s;                          /* first submit -> last completion */
     float mean_wave_requests;
+    float stage_in_ms, stage_out_ms;       /* host time copying inputs to / outputs from pinned staging */
+    float wave_ms;                         /* summed enqueue -> retire time of the waves */
 } sq_server_stats;
 
 int sq_server_create(int device, const sq_server_config* cfg, sq_server** out);

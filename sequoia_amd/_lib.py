@@ -94,7 +94,8 @@ class ServerStats(C.Structure):
                 ("evictions", C.c_uint64), ("resident_scenes", C.c_uint64), ("resident_bytes", C.c_uint64),
                 ("inflight_bytes", C.c_uint64), ("pool_reserved_bytes", C.c_uint64), ("rays_traced", C.c_uint64),
                 ("tris_built", C.c_uint64), ("p50_ms", C.c_float), ("p99_ms", C.c_float), ("mean_ms", C.c_float),
-                ("max_ms", C.c_float), ("span_s", C.c_float), ("mean_wave_requests", C.c_float)]
+                ("max_ms", C.c_float), ("span_s", C.c_float), ("mean_wave_requests", C.c_float),
+                ("stage_in_ms", C.c_float), ("stage_out_ms", C.c_float), ("wave_ms", C.c_float)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

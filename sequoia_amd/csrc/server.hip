@@ -118,6 +118,7 @@ struct Slot {
     size_t out_off = 0, out_bytes = 0;  // output region inside the block
     size_t fail_off = 0;                // per-request fault bits in the output region
     size_t dep_off = 0;                 // per-new-scene tree depth in the output region
+    double t_launch = 0;                // host time the wave was enqueued
 };
 
 }  // namespace
@@ -141,6 +142,7 @@ struct sq_server {
     Clock::time_point t0;
     // statistics
     uint64_t submitted = 0, completed = 0, failed = 0, waves = 0, wave_reqs = 0, rays_traced = 0, tris_built = 0;
+    double stage_in_ms = 0, stage_out_ms = 0, wave_ms = 0;  // host copy time, launch -> retire time
     std::vector<float> lat;
     double first_submit = -1, last_done = 0;
     uint64_t outstanding = 0;
@@ -179,7 +181,8 @@ int pinned_grow(char** p, size_t* have, size_t need) {
     if (*p) (void)hipHostFree(*p);
     *p = nullptr;
     *have = 0;
-    size_t sz = need + need / 4;
+    // generous slack: a pinned reallocation stalls the loop
+    size_t sz = need * 2 > ((size_t)64 << 20) ? need * 2 : ((size_t)64 << 20);
     SQ_CHECK_HIP(hipHostMalloc((void**)p, sz, hipHostMallocDefault));
     *have = sz;
     return SQ_OK;
@@ -187,7 +190,8 @@ int pinned_grow(char** p, size_t* have, size_t need) {
 
 // Lays out, stages and enqueues one wave on slot `sl`. Called with the lock
 // held; returns an error only for device/runtime failures.
-int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64_t wave) {
+int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64_t wave,
+                std::unique_lock<std::mutex>& lk) {
     // ---- layout ----
     std::vector<Req*> rq;
     for (uint64_t id : ids) rq.push_back(&s->reqs[id]);
@@ -280,15 +284,22 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     for (uint32_t i = 0; i < S; i++) offs[i + 1] = offs[i] + counts[i];
     size_t list_pos = 0;
     std::vector<size_t> list_off(rq.size(), 0);
-    for (size_t k = 0; k < rq.size(); k++) {
-        Req* q = rq[k];
-        if (q->r.tris) std::memcpy(h + o_tris + 36ull * offs[q->scene_idx], q->r.tris, 36ull * q->r.ntris);
-        if (q->r.nrays) {
+    for (size_t k = 0; k < rq.size(); k++)
+        if (rq[k]->r.nrays) {
             list_off[k] = list_pos;
-            std::memcpy(h + o_rays + 32ull * list_pos, q->r.rays, 32ull * q->r.nrays);
-            list_pos += q->r.nrays;
+            list_pos += rq[k]->r.nrays;
         }
+    // The requests of this wave and their host buffers are stable while the
+    // lock is released (only the scheduler touches admitted requests).
+    const double t_in = now_ms(s->t0);
+    lk.unlock();
+    for (size_t k = 0; k < rq.size(); k++) {
+        const Req* q = rq[k];
+        if (q->r.tris) par_memcpy(h + o_tris + 36ull * offs[q->scene_idx], q->r.tris, 36ull * q->r.ntris);
+        if (q->r.nrays) par_memcpy(h + o_rays + 32ull * list_off[k], q->r.rays, 32ull * q->r.nrays);
     }
+    lk.lock();
+    s->stage_in_ms += now_ms(s->t0) - t_in;
     std::memcpy(h + o_off, offs.data(), 4ull * (S + 1));
     std::vector<SceneDesc> desc(S);
     if (S) lbvh_fill_descs(desc.data(), counts.data(), S, 0);
@@ -387,6 +398,7 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     SQ_CHECK_HIP(hipMemcpyAsync(sl.pin_out, outd, out_bytes, hipMemcpyDeviceToHost, st));
     SQ_CHECK_HIP(hipEventRecord(sl.ev_done, st));
     guard.armed = false;
+    sl.t_launch = now_ms(s->t0);
     sl.busy = true;
     sl.wave = wave;
     sl.reqs = ids;
@@ -417,7 +429,7 @@ void release_evicted(sq_server* s, const std::vector<uint64_t>& keys) {
 }
 
 // Copies a finished wave's outputs to the requests' destinations.
-void complete_wave(sq_server* s, Slot& sl, double t) {
+void complete_wave(sq_server* s, Slot& sl, std::unique_lock<std::mutex>& lk) {
     const uint32_t* fail = (const uint32_t*)(sl.pin_out + sl.fail_off);
     const uint32_t* depth = (const uint32_t*)(sl.pin_out + sl.dep_off);
     // a tree whose emission overflowed its workspace fails every request of the wave using it
@@ -426,20 +438,31 @@ void complete_wave(sq_server* s, Slot& sl, double t) {
         const Req& q = s->reqs[id];
         if (q.r.tris && depth[q.scene_idx] == 0) bad_keys.push_back(q.r.key);
     }
+    std::vector<Req*> rq;
+    std::vector<int> status;
     for (size_t k = 0; k < sl.reqs.size(); k++) {
         Req& q = s->reqs[sl.reqs[k]];
         q.fault = fail[k];
         const bool bad_tree = std::find(bad_keys.begin(), bad_keys.end(), q.r.key) != bad_keys.end();
-        const int st = (q.fault || bad_tree) ? SQ_ERR_OVERFLOW : SQ_OK;
-        const bool cam = q.r.nrays == 0;
-        if (st == SQ_OK) {
-            if (q.r.out.hit_id) std::memcpy(q.r.out.hit_id, sl.pin_out + q.o_hit_id, 4 * q.rays);
-            if (q.r.out.hit_t) std::memcpy(q.r.out.hit_t, sl.pin_out + q.o_hit_t, 4 * q.rays);
-            if (q.r.out.counts) std::memcpy(q.r.out.counts, sl.pin_out + q.o_counts, 4 * (cam ? q.pix : q.rays));
-            if (cam && q.r.out.rgb) std::memcpy(q.r.out.rgb, sl.pin_out + q.o_rgb, 3 * q.pix);
-        }
-        finish(s, q, st, t);
+        rq.push_back(&q);
+        status.push_back((q.fault || bad_tree) ? SQ_ERR_OVERFLOW : SQ_OK);
     }
+    const double t_out = now_ms(s->t0);
+    s->wave_ms += t_out - sl.t_launch;
+    lk.unlock();
+    for (size_t k = 0; k < rq.size(); k++) {
+        const Req& q = *rq[k];
+        const bool cam = q.r.nrays == 0;
+        if (status[k] != SQ_OK) continue;
+        if (q.r.out.hit_id) par_memcpy(q.r.out.hit_id, sl.pin_out + q.o_hit_id, 4 * q.rays);
+        if (q.r.out.hit_t) par_memcpy(q.r.out.hit_t, sl.pin_out + q.o_hit_t, 4 * q.rays);
+        if (q.r.out.counts) par_memcpy(q.r.out.counts, sl.pin_out + q.o_counts, 4 * (cam ? q.pix : q.rays));
+        if (cam && q.r.out.rgb) par_memcpy(q.r.out.rgb, sl.pin_out + q.o_rgb, 3 * q.pix);
+    }
+    lk.lock();
+    const double t = now_ms(s->t0);
+    s->stage_out_ms += t - t_out;
+    for (size_t k = 0; k < rq.size(); k++) finish(s, *rq[k], status[k], t);
     s->adm->wave_done(sl.wave, sl.work, sl.keys);
     for (uint64_t k : bad_keys)
         if (s->adm->evict(k)) release_evicted(s, {k});
@@ -464,7 +487,7 @@ void scheduler(sq_server* s) {
             if (!sl.busy) continue;
             hipError_t q = hipEventQuery(sl.ev_done);
             if (q == hipSuccess) {
-                complete_wave(s, sl, now_ms(s->t0));
+                complete_wave(s, sl, lk);
                 progressed = true;
             } else if (q != hipErrorNotReady) {
                 s->fatal = SQ_ERR_HIP;
@@ -498,7 +521,7 @@ void scheduler(sq_server* s) {
             }
             if (!admit.empty()) {
                 free_slot->work = work;
-                int rc = launch_wave(s, *free_slot, admit, wave);
+                int rc = launch_wave(s, *free_slot, admit, wave, lk);
                 if (rc != SQ_OK) {
                     // nothing of the wave was enqueued: fail its requests and
                     // drop the scenes it would have built
@@ -539,8 +562,8 @@ int sq_server_create(int device, const sq_server_config* cfg, sq_server** out) {
     if (cfg) c = *cfg;
     if (!c.budget_bytes) c.budget_bytes = 200ull * 1000 * 1000 * 1000;
     if (!c.max_wave_requests) c.max_wave_requests = 256;
-    if (!c.max_wave_rays) c.max_wave_rays = 64u << 20;
-    if (!c.max_wave_tris) c.max_wave_tris = 32u << 20;
+    if (!c.max_wave_rays) c.max_wave_rays = 16u << 20;
+    if (!c.max_wave_tris) c.max_wave_tris = 8u << 20;
     if (!c.inflight_waves) c.inflight_waves = 2;
     if (c.inflight_waves > 2) SQ_RET(SQ_ERR_INVALID, "sq_server_create: at most 2 in-flight waves");
     if (c.semantics > SQ_TRACE_CLOSEST) SQ_RET(SQ_ERR_INVALID, "sq_server_create: semantics %u", c.semantics);
@@ -755,6 +778,9 @@ int sq_server_get_stats(sq_server* s, sq_server_stats* o) {
     }
     o->span_s = s->first_submit >= 0 ? (float)((s->last_done - s->first_submit) / 1000.0) : 0.f;
     o->mean_wave_requests = s->waves ? (float)s->wave_reqs / s->waves : 0.f;
+    o->stage_in_ms = (float)s->stage_in_ms;
+    o->stage_out_ms = (float)s->stage_out_ms;
+    o->wave_ms = (float)s->wave_ms;
     return SQ_OK;
 }
 
@@ -764,6 +790,7 @@ int sq_server_reset_stats(sq_server* s) {
     s->completed = s->failed = s->waves = s->wave_reqs = s->rays_traced = s->tris_built = 0;
     s->submitted = s->outstanding;
     s->lat.clear();
+    s->stage_in_ms = s->stage_out_ms = s->wave_ms = 0;
     s->first_submit = -1;
     s->last_done = 0;
     return SQ_OK;
