@@ -159,6 +159,9 @@ typedef enum {
     SQ_HEATMAP_BUNDLE = 2  /* trace_bundle<K>, K in {16,64} */
 } sq_heatmap;
 
+#define SQ_KERNEL_AUTO 0u    /* per-lane rays (the measured faster one on primary rays) */
+#define SQ_KERNEL_RAYS 1u    /* one ray per lane, per-lane stacks, dynamic refill */
+#define SQ_KERNEL_PACKETS 2u /* one 64-ray packet per wave, union walk with lane masks */
 typedef struct {
     uint32_t semantics;   /* sq_semantics */
     uint32_t heatmap;     /* sq_heatmap */
@@ -166,7 +169,8 @@ typedef struct {
     uint32_t block_size;  /* 0 = default; multiple of 64 */
     uint32_t stack_slots; /* 0 = default; LDS stack entries per lane */
     uint32_t count_work;  /* 1 = accumulate total node/triangle tests into sq_timings */
-    uint32_t reserved[2];
+    uint32_t kernel;      /* camera rays: SQ_KERNEL_* (packets: exhaustive semantics only) */
+    uint32_t reserved;
 } sq_trace_params;
 
 /* Outputs (any may be NULL). hit_id/hit_t: one per ray (SQ_MISS / +inf on
@@ -232,15 +236,17 @@ int sq_batch_download(sq_ctx* ctx, const sq_batch* b, uint32_t scene, const sq_t
 int sq_batch_times(const sq_batch* b, float* build_ms, float* trace_ms, float* run_ms);
 
 /* Autotuner: times candidate radix digit widths (4..8) and subtree
- * thresholds (512..2048) for sq_batch_build, and stack slots (8..32) and
- * workgroup sizes (64, 128) for sq_batch_trace, on this batch (which must
- * have cameras set), keeps the fastest in the batch and reports them. None
- * of them changes any result. */
+ * thresholds (512..2048) for sq_batch_build, and the traversal kernel
+ * (per-lane rays with stack slots 8..32 and workgroup sizes 64, 128, or wave
+ * packets for exhaustive semantics) for sq_batch_trace, on this batch (which
+ * must have cameras set), keeps the fastest in the batch and reports them.
+ * None of them changes any result. */
 typedef struct {
     uint32_t radix_digit_bits, subtree_threshold, stack_slots, block_size;
     float build_ms, trace_ms;           /* best times */
     float default_build_ms, default_trace_ms;
     uint32_t candidates_tried;
+    uint32_t kernel;                    /* SQ_KERNEL_RAYS or SQ_KERNEL_PACKETS */
 } sq_tune_result;
 int sq_batch_autotune(sq_ctx* ctx, sq_batch* b, sq_builder kind, uint32_t repeats, sq_tune_result* out);
 

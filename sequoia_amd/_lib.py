@@ -24,6 +24,7 @@ BUILDERS = {
 LAYOUT_REFERENCE, LAYOUT_FIXED = 0, 1
 TRACE_EXHAUSTIVE, TRACE_CLOSEST = 0, 1
 HEATMAP_RAY, HEATMAP_PACKET, HEATMAP_BUNDLE = 0, 1, 2
+KERNEL_AUTO, KERNEL_RAYS, KERNEL_PACKETS = 0, 1, 2
 MISS = 0xFFFFFFFF
 
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
@@ -56,7 +57,7 @@ class BuildParams(C.Structure):
 class TraceParams(C.Structure):
     _fields_ = [("semantics", C.c_uint32), ("heatmap", C.c_uint32), ("width", C.c_uint32),
                 ("block_size", C.c_uint32), ("stack_slots", C.c_uint32), ("count_work", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("kernel", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class TraceOut(C.Structure):
@@ -66,7 +67,8 @@ class TraceOut(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("radix_digit_bits", C.c_uint32), ("subtree_threshold", C.c_uint32), ("stack_slots", C.c_uint32),
                 ("block_size", C.c_uint32), ("build_ms", C.c_float), ("trace_ms", C.c_float),
-                ("default_build_ms", C.c_float), ("default_trace_ms", C.c_float), ("candidates_tried", C.c_uint32)]
+                ("default_build_ms", C.c_float), ("default_trace_ms", C.c_float), ("candidates_tried", C.c_uint32),
+                ("kernel", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -359,10 +361,10 @@ class Batch:
         return BatchBvh(self.ctx, h, None)
 
     def set_cameras(self, cams, semantics=TRACE_EXHAUSTIVE, outputs=OUT_ALL, stack_slots=0, block_size=0,
-                    count_work=False):
+                    count_work=False, kernel=0):
         arr = (Camera * len(cams))(*cams)
         p = TraceParams(semantics, HEATMAP_RAY, 0, block_size, stack_slots, 1 if count_work else 0,
-                        (C.c_uint32 * 2)())
+                        kernel, 0)
         _check(lib().sq_batch_set_cameras(self.ctx.h, self.h, arr, C.byref(p), outputs))
         self.cams = list(cams)
         self._cam_arr = arr
@@ -533,10 +535,10 @@ class Context:
         return out
 
     def trace_camera_device(self, bvh, cam, hit_id=None, hit_t=None, counts=None, rgb=None,
-                            semantics=TRACE_EXHAUSTIVE, stack_slots=0, count_work=False):
+                            semantics=TRACE_EXHAUSTIVE, stack_slots=0, count_work=False, kernel=0):
         """Trace with outputs already in device memory (no host copies)."""
         out = TraceOut(hit_id, hit_t, counts, rgb)
-        p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work)
+        p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work, kernel)
         _check(lib().sq_trace_camera(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 1))
 
     def upload(self, tris):
@@ -574,11 +576,11 @@ class Context:
         _check(lib().sq_refit(self.h, bvh.h))
 
     @staticmethod
-    def _params(semantics, heatmap, width, stack_slots, count_work):
-        return TraceParams(semantics, heatmap, width, 0, stack_slots, 1 if count_work else 0, (C.c_uint32 * 2)())
+    def _params(semantics, heatmap, width, stack_slots, count_work, kernel=0):
+        return TraceParams(semantics, heatmap, width, 0, stack_slots, 1 if count_work else 0, kernel, 0)
 
     def trace_camera(self, bvh, cam, semantics=TRACE_EXHAUSTIVE, heatmap=HEATMAP_RAY, width=0, want_hits=True,
-                     want_counts=True, want_rgb=True, stack_slots=0, count_work=False):
+                     want_counts=True, want_rgb=True, stack_slots=0, count_work=False, kernel=0):
         npix = cam.width * cam.height
         nray = npix * cam.spp
         res = {}
@@ -587,7 +589,7 @@ class Context:
         cnt = np.empty(npix, np.uint32) if want_counts else None
         rgb = np.empty((cam.height, cam.width, 3), np.uint8) if want_rgb else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), _ptr(rgb))
-        p = self._params(semantics, heatmap, width, stack_slots, count_work)
+        p = self._params(semantics, heatmap, width, stack_slots, count_work, kernel)
         _check(lib().sq_trace_camera(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 0))
         res.update(hit_id=hid, hit_t=ht, counts=cnt, rgb=rgb)
         return res

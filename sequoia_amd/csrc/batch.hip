@@ -198,6 +198,7 @@ struct sq_batch {
     uint32_t* d_work = nullptr;
     unsigned long long* d_stats = nullptr;
     uint32_t* d_job_fail = nullptr;  // fault bits per scene of the last trace
+    uint32_t pk_grid = 0;            // workgroups of a wave-packet trace launch
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -332,12 +333,24 @@ int enqueue_build(sq_ctx* c, sq_batch* b) {
 }
 
 int plan_trace(sq_ctx* c, sq_batch* b, TracePlan* L) {
+    if (!b->pk_grid) b->pk_grid = trace_packet_grid(c, true);  // outside any graph capture
     return trace_plan(c, &b->tparams, morton_depth_bound(b->bits, b->max_n), L);
 }
 
 // Enqueues the batched trace (cameras, outputs and spill already set up).
 int enqueue_trace(sq_ctx* c, sq_batch* b, const TracePlan& L) {
     TraceArgs a{};
+    if (use_packets(&b->tparams)) {
+        a.work = b->d_work;
+        a.stats = b->tparams.count_work ? b->d_stats : nullptr;
+        a.num_tiles = b->job_tiles.back();
+        a.jobs = b->d_jobs;
+        a.job_tiles = b->d_job_tiles;
+        a.njobs = b->nscenes;
+        a.job_fail = b->d_job_fail;
+        SQ_TRY(launch_fill_u32_on(c->stream, c->num_cus, b->d_job_fail, 0u, b->nscenes));
+        return trace_launch_packets(c->stream, b->pk_grid, a, true);
+    }
     a.slots = L.slots;
     a.spill = b->d_spill;
     a.spill_cap = L.spill_cap;
@@ -508,6 +521,7 @@ int sq_batch_bvh(sq_batch* b, uint32_t scene, const sq_bvh** out) {
 int sq_batch_set_cameras(sq_ctx* c, sq_batch* b, const sq_camera* cams, const sq_trace_params* p, uint32_t outputs) {
     if (!c || !b || !cams || b->ctx != c) SQ_RET(SQ_ERR_INVALID, "sq_batch_set_cameras: bad argument");
     if (p && p->heatmap != SQ_HEATMAP_RAY) SQ_RET(SQ_ERR_UNSUPPORTED, "batches trace per-ray heatmaps only");
+    if (p && p->kernel > 2) SQ_RET(SQ_ERR_INVALID, "sq_batch_set_cameras: kernel %u", p->kernel);
     if (outputs & ~0xFu) SQ_RET(SQ_ERR_INVALID, "sq_batch_set_cameras: output mask %#x", outputs);
     SQ_CHECK_HIP(hipSetDevice(c->device));
     sq_trace_params tp{};
@@ -711,26 +725,40 @@ int sq_batch_autotune(sq_ctx* c, sq_batch* b, sq_builder kind, uint32_t repeats,
     const uint32_t slots[] = {16, 8, 32};
     const uint32_t blocks[] = {128, 64};
     float best_t = 1e30f;
-    uint32_t bs = 16, bb = 128;
+    uint32_t bs = 16, bb = 128, bk = SQ_KERNEL_RAYS;
+    auto time_trace = [&](uint32_t kernel, uint32_t sl, uint32_t bl, float* best) -> int {
+        sq_trace_params tp = base;
+        tp.kernel = kernel;
+        tp.stack_slots = sl;
+        tp.block_size = bl;
+        SQ_TRY(sq_batch_set_cameras(c, b, cams.data(), &tp, outputs));
+        float m = 1e30f;
+        for (uint32_t r = 0; r < repeats + 1; r++) {
+            SQ_TRY(sq_batch_trace(c, b));
+            if (r > 0) m = std::min(m, b->trace_ms);
+        }
+        out->candidates_tried++;
+        *best = m;
+        return SQ_OK;
+    };
     for (uint32_t sl : slots)
         for (uint32_t bl : blocks) {
-            sq_trace_params tp = base;
-            tp.stack_slots = sl;
-            tp.block_size = bl;
-            SQ_TRY(sq_batch_set_cameras(c, b, cams.data(), &tp, outputs));
-            float m = 1e30f;
-            for (uint32_t r = 0; r < repeats + 1; r++) {
-                SQ_TRY(sq_batch_trace(c, b));
-                if (r > 0) m = std::min(m, b->trace_ms);
-            }
-            out->candidates_tried++;
+            float m = 0;
+            SQ_TRY(time_trace(SQ_KERNEL_RAYS, sl, bl, &m));
             if (sl == 16 && bl == 128) out->default_trace_ms = m;
-            if (m < best_t) { best_t = m; bs = sl; bb = bl; }
+            if (m < best_t) { best_t = m; bs = sl; bb = bl; bk = SQ_KERNEL_RAYS; }
         }
+    if (base.semantics == SQ_TRACE_EXHAUSTIVE) {  // wave packets: exhaustive semantics only
+        float m = 0;
+        SQ_TRY(time_trace(SQ_KERNEL_PACKETS, 0, 0, &m));
+        if (m < best_t) { best_t = m; bk = SQ_KERNEL_PACKETS; }
+    }
     sq_trace_params tp = base;
+    tp.kernel = bk;
     tp.stack_slots = bs;
     tp.block_size = bb;
     SQ_TRY(sq_batch_set_cameras(c, b, cams.data(), &tp, outputs));
+    out->kernel = bk;
     out->radix_digit_bits = bd;
     out->subtree_threshold = bt;
     out->stack_slots = bs;

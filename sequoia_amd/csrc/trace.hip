@@ -336,7 +336,24 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
     return SQ_OK;
 }
 
+bool use_packets(const sq_trace_params* p) {
+    const bool exhaustive = !p || p->semantics == SQ_TRACE_EXHAUSTIVE;
+    const uint32_t k = p ? p->kernel : 0u;
+    return exhaustive && k == 2u;
+}
+
 static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace_params* p) {
+    if (a.camera && use_packets(p)) {
+        SQ_TRY(c->counters.ensure(64));
+        SQ_TRY(c->stats.ensure(64));
+        a.work = c->counters.as<uint32_t>();
+        a.stats = (p && p->count_work) ? c->stats.as<unsigned long long>() : nullptr;
+        const uint32_t grid = trace_packet_grid(c, false);
+        phase_begin(c, PH_TRACE);
+        int rc = trace_launch_packets(c->stream, grid, a, false);
+        phase_end(c, PH_TRACE);
+        return rc;
+    }
     TracePlan L;
     SQ_TRY(trace_plan(c, p, b->depth, &L));
     SQ_TRY(c->spill.ensure(L.spill_words * 4));
@@ -380,6 +397,7 @@ int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_t
     if (!c || !b || !cam || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: NULL argument");
     if (cam->width == 0 || cam->height == 0 || cam->spp == 0) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: empty image");
     if (p && p->heatmap > SQ_HEATMAP_BUNDLE) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: heatmap mode %u", p->heatmap);
+    if (p && p->kernel > 2) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: kernel %u", p->kernel);
     const bool heat = p && p->heatmap != SQ_HEATMAP_RAY;
     if (heat && (cam->spp != 1 || out->hit_id || out->hit_t))
         SQ_RET(SQ_ERR_UNSUPPORTED, "packet/bundle heatmaps take one sample per pixel and produce counts/rgb only");

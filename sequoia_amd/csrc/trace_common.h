@@ -210,6 +210,13 @@ int trace_fault(uint32_t bits, const char* what);
 // Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).
 int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi);
 
+// Wave-packet traversal of camera rays, exhaustive semantics (trace_packet.hip):
+// grid size for a launch, and the launch (zeroes a.work / a.stats first).
+uint32_t trace_packet_grid(sq_ctx* c, bool multi);
+int trace_launch_packets(hipStream_t stream, uint32_t grid, const TraceArgs& a, bool multi);
+// Kernel choice for camera rays: wave packets when the semantics allow.
+bool use_packets(const sq_trace_params* p);
+
 // Packet / bundle heatmap launch (heatmap.hip).
 int heat_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& t, const sq_trace_params* p);
 

@@ -110,13 +110,15 @@ def assert_same(got, want, tag):
         assert np.array_equal(np.ascontiguousarray(g).view(np.uint8), np.ascontiguousarray(w).view(np.uint8)), (tag, k)
 
 
-@pytest.mark.parametrize("semantics", [sq.TRACE_EXHAUSTIVE, sq.TRACE_CLOSEST])
-def test_batch_trace_equals_plain(ctx, mixed, semantics):
+@pytest.mark.parametrize("semantics,kernel", [(sq.TRACE_EXHAUSTIVE, sq.KERNEL_RAYS),
+                                              (sq.TRACE_EXHAUSTIVE, sq.KERNEL_PACKETS),
+                                              (sq.TRACE_CLOSEST, sq.KERNEL_AUTO)])
+def test_batch_trace_equals_plain(ctx, mixed, semantics, kernel):
     scenes, b = mixed
     cams = cameras_for(scenes)
-    say("batch trace", semantics)
+    say("batch trace", semantics, kernel)
     b.build("morton63")
-    b.set_cameras(cams, semantics=semantics)
+    b.set_cameras(cams, semantics=semantics, kernel=kernel)
     b.trace()
     for i, t in enumerate(scenes):
         assert_same(b.download(i), plain_trace(ctx, t, cams[i], "morton63", semantics), (semantics, i))
@@ -163,7 +165,7 @@ def test_batch_autotune_keeps_results(ctx, mixed):
     trees = [b.bvh(i).download() for i in range(len(scenes))]
     r = b.autotune("morton63", repeats=1)
     say("autotune", r)
-    assert r["candidates_tried"] == 21
+    assert r["candidates_tried"] == 22 and r["kernel"] in (sq.KERNEL_RAYS, sq.KERNEL_PACKETS)
     assert 4 <= r["radix_digit_bits"] <= 8 and r["stack_slots"] in (8, 16, 32) and r["block_size"] in (64, 128)
     b.run("morton63")
     for i in range(len(scenes)):

@@ -33,32 +33,34 @@ def check_hits(got_id, got_t, want_id, want_t):
     assert np.array_equal(got_t[hit].view(np.uint32), want_t[hit].view(np.uint32))
 
 
+@pytest.mark.parametrize("kernel", [sq.KERNEL_RAYS, sq.KERNEL_PACKETS])
 @pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000", "ico69632"])
-def test_reference_camera_on_reference_tree(ctx, name, manifest):
-    say("reference camera", name)
+def test_reference_camera_on_reference_tree(ctx, name, manifest, kernel):
+    say("reference camera", name, kernel)
     tris = GOLDEN_SCENES[name]()
     tr = manifest["traces"][name]
     nodes, root = ob.build_sah(tris, 8)       # equals the reference's recursive tree (pinned)
     scene = ctx.upload(tris)
     bvh = ctx.bvh_from_host(scene, nodes, root)
     cam = sq.reference_camera(tris, tr["w"], tr["h"])
-    got = ctx.trace_camera(bvh, cam)
+    got = ctx.trace_camera(bvh, cam, kernel=kernel)
     assert sha(got["counts"]) == tr["counts_sha256"]
     assert sha(got["rgb"]) == tr["basic_rgb_sha256"]
     want = ob.trace_camera(nodes, root, tris, cam)
     check_hits(got["hit_id"], got["hit_t"], want["hit_id"], want["hit_t"])
 
 
+@pytest.mark.parametrize("kernel", [sq.KERNEL_RAYS, sq.KERNEL_PACKETS])
 @pytest.mark.parametrize("kind", ["morton63", "morton"])
 @pytest.mark.parametrize("name", ["ico1280", "clus3000", "ico69632"])
-def test_jittered_pinhole_on_gpu_tree(ctx, name, kind):
-    say("jittered pinhole", name, kind)
+def test_jittered_pinhole_on_gpu_tree(ctx, name, kind, kernel):
+    say("jittered pinhole", name, kind, kernel)
     tris = GOLDEN_SCENES[name]()
     scene = ctx.upload(tris)
     bvh = ctx.build(scene, kind)
     nodes = bvh.download()
     cam = sq.pinhole_camera(tris, 200, 120, 60.0, 3.0, spp=4, jitter=1, seed=9)
-    got = ctx.trace_camera(bvh, cam)
+    got = ctx.trace_camera(bvh, cam, kernel=kernel)
     want = ob.trace_camera(nodes, bvh.root, tris, cam)
     assert np.array_equal(got["counts"], want["counts"])
     assert np.array_equal(got["rgb"], want["rgb"])
@@ -146,3 +148,19 @@ def test_packet_and_bundle_on_gpu_morton_tree(ctx):
     for K in (16, 64):
         got = ctx.trace_camera(bvh, cam, heatmap=sq.HEATMAP_BUNDLE, width=K, want_hits=False)
         assert np.array_equal(got["rgb"], ob.trace_bundle(nodes, bvh.root, tris, cam, K))
+
+
+@pytest.mark.parametrize("kind", ["morton63", "recursive"])
+def test_packets_equal_rays_at_full_size(ctx, kind):
+    """The bench frame (1M soup, 1920x1080x4 jittered): the wave-packet
+    kernel's per-pixel counts, heatmap bytes and per-ray hits equal the
+    per-lane kernel's bit for bit, on a Morton and on a SAH tree."""
+    say("packets vs rays", kind)
+    tris = sq.gen_soup(1_000_000, 2, 0.01)
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, kind)
+    cam = sq.pinhole_camera(tris, 1920, 1080, 60.0, 3.0, spp=4, jitter=1, seed=2)
+    a = ctx.trace_camera(bvh, cam, kernel=sq.KERNEL_RAYS)
+    b = ctx.trace_camera(bvh, cam, kernel=sq.KERNEL_PACKETS)
+    for k in ("hit_id", "hit_t", "counts", "rgb"):
+        assert np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8), np.ascontiguousarray(b[k]).view(np.uint8)), k
