@@ -141,7 +141,7 @@ void sq_ctx_destroy(sq_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto* b : {&c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->scratch, &c->tasks[0],
                     &c->tasks[1], &c->tasks[2], &c->counters, &c->spill, &c->stats, &c->aux[0], &c->aux[1],
-                    &c->aux[2], &c->aux[3]})
+                    &c->aux[2], &c->aux[3], &c->sah[0], &c->sah[1], &c->sah[2], &c->sah[3], &c->sah[4], &c->sah[5]})
         b->release();
     for (int i = 0; i < PH_COUNT; i++) {
         (void)hipEventDestroy(c->ev_begin[i]);

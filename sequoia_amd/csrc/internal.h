@@ -78,6 +78,7 @@ struct sq_ctx {
     int num_cus = 256;
     // Workspace reused across calls (grow-only).
     sq::DevBuf keys[2], vals[2], hist, scratch, tasks[3], counters, spill, stats, aux[4];
+    sq::DevBuf sah[6];  // binned-SAH large-task queues, plans, partial bins, counts, bin ids
     hipEvent_t ev_begin[sq::PH_COUNT] = {};
     hipEvent_t ev_end[sq::PH_COUNT] = {};
     bool phase_used[sq::PH_COUNT] = {};

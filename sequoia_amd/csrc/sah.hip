@@ -15,10 +15,15 @@
 //   * leaves when count <= 8 or sa(centroid box) <= 0; the slot allocator of
 //     recurse(), right-singleton quirk included (or fixed).
 //
-// GPU structure: tasks are processed level by level, one workgroup per task.
-// Bins are reduced with per-wave LDS atomics on order-preserving integer
-// images of the floats (min/max are exact and order-free), the plane is chosen
-// by one lane, and the partition is done out of place between two id buffers.
+// GPU structure: tasks are processed level by level. A task of up to
+// kBigTask primitives gets one wave; a larger one is cut into chunks, one
+// workgroup each, whose bins are folded by a second kernel before the plane is
+// chosen, and whose partition ranks start at the counts of earlier chunks.
+// Bins are reduced with LDS atomics on order-preserving integer images of the
+// floats (min/max are exact and order-free; of +0 and -0, -0 is the smaller),
+// the plane is chosen by lane-parallel prefix/suffix unions over the bins, and
+// the partition is done out of place between two id buffers, so both paths
+// give the same tree.
 // The Hoare result is produced without the sequential swap loop: the r-th
 // right-side element found left of the final midpoint is exchanged with the
 // r-th left-side element counted from the right end, which is exactly the
@@ -31,11 +36,23 @@ constexpr int kSahThreads = 256;
 constexpr int kSahWaves = kSahThreads / 64;
 constexpr int kMaxBins = 64;
 constexpr int kBinWords = 13;  // count, p.min[3], p.max[3], c.min[3], c.max[3]
+// Tasks of more than kBigTask primitives are split by many workgroups, one per
+// chunk of at least kChunkMin primitives (at most kMaxChunks chunks a task).
+constexpr uint32_t kBigTask = 4096;
+constexpr uint32_t kChunkMin = 2048;
+constexpr uint32_t kMaxChunks = 512;
+constexpr int kBigShift = 40;  // large-queue counter: (tasks << 40) | chunks
 
 struct SahTask {
     uint32_t b, e, free, self;
-    float cb[6];  // centroid box: min xyz, max xyz
-    uint32_t pad[2];
+    float cb[6];               // centroid box: min xyz, max xyz
+    uint32_t chunk0, nchunks;  // large tasks: first chunk of the level, chunks
+};
+
+// Split chosen for a task: child bounds and centroid boxes, plane, left count.
+struct SahPlan {
+    float lp[6], rp[6], lc[6], rc[6];
+    uint32_t plane, m, pad[2];
 };
 
 struct SahArgs {
@@ -46,13 +63,23 @@ struct SahArgs {
     uint32_t part;     // 0 Hoare, 1 stable
     uint32_t sa_x2;    // 1: AABB1::sa (x2)
     uint32_t layout;
-    const SahTask* tasks;
-    SahTask* next;
-    uint32_t* counters;  // [0] next count, [1] max level
+    uint32_t small_cap, big_cap;
+    uint32_t nbig;          // large tasks of this level
+    const SahTask* tasks;   // small tasks of this level
+    SahTask* next;          // small tasks of the next level
+    const SahTask* big;     // large tasks of this level
+    SahTask* next_big;      // large tasks of the next level
+    uint32_t* counters;     // [0] next small count, [2..3] next large (packed), [4] error flag
     const uint32_t* ids_in;
     uint32_t* ids_out;
-    uint32_t* scratch;   // Hoare misplaced positions
+    uint32_t* scratch;      // Hoare misplaced positions
+    SahPlan* plans;         // per large task
+    uint32_t* part_bins;    // per chunk: nb * kBinWords words
+    uint32_t* chunk_cnt;    // per chunk: 2 words
+    uint8_t* bin_of;        // bin of the primitive at each position of a large task
 };
+
+typedef uint32_t BinsLds[kSahWaves][kMaxBins][kBinWords];
 
 __device__ __forceinline__ float box_sa(const float* mn, const float* mx, uint32_t x2) {
     float lx = mx[0] - mn[0], ly = mx[1] - mn[1], lz = mx[2] - mn[2];
@@ -116,30 +143,44 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* s_tmp, uint
     return wbase + x - v;
 }
 
-// One workgroup per task: bin, choose the plane, partition, emit children.
-__global__ __launch_bounds__(kSahThreads) void k_sah_level(SahArgs a) {
-    __shared__ uint32_t s_bins[kSahWaves][kMaxBins][kBinWords];
-    __shared__ float s_lp[6], s_rp[6], s_lc[6], s_rc[6];
-    __shared__ int s_plane;
-    __shared__ uint32_t s_m, s_tmp[kSahWaves];
-
-    const SahTask t = a.tasks[blockIdx.x];
-    const uint32_t b = t.b, e = t.e, count = e - b;
-    const int wave = threadIdx.x >> 6;
-    const int axis = choose_axis(t.cb);
-    const float k1 = bin_k1(a.nb, t.cb, axis);
-    const float cmin = t.cb[axis];
-    const uint32_t nb = a.nb;
-
-    // ---- binning ----
-    const uint32_t omax = sqd::f2ord(3.402823466e+38f), omin = sqd::f2ord(-3.402823466e+38f);
-    for (uint32_t i = threadIdx.x; i < (uint32_t)kSahWaves * nb * kBinWords; i += kSahThreads) {
-        uint32_t w = i / (nb * kBinWords), r = i % (nb * kBinWords), bi = r / kBinWords, f = r % kBinWords;
-        uint32_t v = f == 0 ? 0u : ((f - 1) % 6 < 3 ? omax : omin);
-        s_bins[w][bi][f] = v;
-    }
+// Sum over the block (every thread gets the total).
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* s_tmp) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     __syncthreads();
-    for (uint32_t i = b + threadIdx.x; i < e; i += kSahThreads) {
+    if ((threadIdx.x & 63) == 0) s_tmp[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int w = 0; w < kSahWaves; w++) t += s_tmp[w];
+    __syncthreads();
+    return t;
+}
+
+// ---- bins: ordered-integer images of the floats, so min/max/sum are exact
+// and independent of the order primitives are added in ----
+__device__ __forceinline__ uint32_t bin_identity(uint32_t f) {
+    return f == 0 ? 0u : ((f - 1) % 6 < 3 ? sqd::f2ord(3.402823466e+38f) : sqd::f2ord(-3.402823466e+38f));
+}
+
+__device__ __forceinline__ uint32_t bin_combine(uint32_t f, uint32_t v, uint32_t x) {
+    if (f == 0) return v + x;
+    if ((f - 1) % 6 < 3) return x < v ? x : v;
+    return x > v ? x : v;
+}
+
+__device__ __forceinline__ void bins_clear(BinsLds& s, uint32_t nb) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kSahWaves * nb * kBinWords; i += kSahThreads) {
+        uint32_t w = i / (nb * kBinWords), r = i % (nb * kBinWords);
+        s[w][r / kBinWords][r % kBinWords] = bin_identity(r % kBinWords);
+    }
+}
+
+// Bins the positions [lo, hi) of ids_in into the block's per-wave bins;
+// with `keep`, records each position's bin in bin_of.
+__device__ __forceinline__ void bin_range(const SahArgs& a, uint32_t lo, uint32_t hi, int axis, float k1, float cmin,
+                                          BinsLds& s, bool keep) {
+    const int wave = threadIdx.x >> 6;
+    const uint32_t nb = a.nb;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSahThreads) {
         uint32_t id = a.ids_in[i];
         float mn[3], mx[3], c[3];
         prim_box(a.boxes, id, mn, mx);
@@ -147,7 +188,8 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_level(SahArgs a) {
         for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
         int bi = (int)(k1 * (c[axis] - cmin));
         bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
-        uint32_t* B = s_bins[wave][bi];
+        if (keep) a.bin_of[i] = (uint8_t)bi;
+        uint32_t* B = s[wave][bi];
         atomicAdd(&B[0], 1u);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -157,150 +199,55 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_level(SahArgs a) {
             atomicMax(&B[10 + k], sqd::f2ord(c[k]));
         }
     }
-    __syncthreads();
+}
+
+// Folds the per-wave bins into s[0].
+__device__ __forceinline__ void bins_merge_waves(BinsLds& s, uint32_t nb) {
     for (uint32_t i = threadIdx.x; i < nb * kBinWords; i += kSahThreads) {
         uint32_t bi = i / kBinWords, f = i % kBinWords;
-        uint32_t v = s_bins[0][bi][f];
-        for (int w = 1; w < kSahWaves; w++) {
-            uint32_t x = s_bins[w][bi][f];
-            if (f == 0) v += x;
-            else if ((f - 1) % 6 < 3) v = x < v ? x : v;
-            else v = x > v ? x : v;
-        }
-        s_bins[0][bi][f] = v;
+        uint32_t v = s[0][bi][f];
+        for (int w = 1; w < kSahWaves; w++) v = bin_combine(f, v, s[w][bi][f]);
+        s[0][bi][f] = v;
     }
-    __syncthreads();
+}
 
-    // ---- choose_plane (one lane) ----
-    if (threadIdx.x == 0) {
-        // decoded bins: pmin pmax cmin cmax, count
-        auto bin_box = [&](uint32_t bi, float* p, float* c, uint32_t& n) {
-            const uint32_t* B = s_bins[0][bi];
-            n = B[0];
-            for (int k = 0; k < 3; k++) {
-                p[k] = sqd::ord2f(B[1 + k]); p[3 + k] = sqd::ord2f(B[4 + k]);
-                c[k] = sqd::ord2f(B[7 + k]); c[3 + k] = sqd::ord2f(B[10 + k]);
-            }
-        };
-        auto grow = [](float* dst, const float* src) {
-            for (int k = 0; k < 3; k++) {
-                dst[k] = sqd::std_min(dst[k], src[k]);
-                dst[3 + k] = sqd::std_max(dst[3 + k], src[3 + k]);
-            }
-        };
-        // l2r prefix unions are rebuilt on demand: keep them in registers per plane via a small array
-        float l2r_p[kMaxBins][6], l2r_c[kMaxBins][6];
-        uint32_t l2r_n[kMaxBins];
-        {
-            float p[6], c[6];
-            uint32_t n;
-            bin_box(0, p, c, n);
-            for (int k = 0; k < 6; k++) { l2r_p[0][k] = p[k]; l2r_c[0][k] = c[k]; }
-            l2r_n[0] = n;
-            for (uint32_t i = 1; i + 1 < nb; i++) {
-                bin_box(i, p, c, n);
-                for (int k = 0; k < 6; k++) { l2r_p[i][k] = l2r_p[i - 1][k]; l2r_c[i][k] = l2r_c[i - 1][k]; }
-                grow(l2r_p[i], p);
-                grow(l2r_c[i], c);
-                l2r_n[i] = l2r_n[i - 1] + n;
-            }
+// Queues a child task: above kBigTask primitives on the large queue with its
+// chunk range (one packed atomic hands out the queue slot and the chunks, so
+// chunk ranges ascend with queue slots), else on the small queue.
+__device__ void push_task(const SahArgs& a, SahTask c) {
+    const uint32_t cnt = c.e - c.b;
+    if (cnt > kBigTask) {
+        uint32_t nch = (cnt + kChunkMin - 1) / kChunkMin;
+        nch = nch < kMaxChunks ? nch : kMaxChunks;
+        const unsigned long long old =
+            atomicAdd((unsigned long long*)(a.counters + 2), (1ull << kBigShift) | (unsigned long long)nch);
+        const uint32_t idx = (uint32_t)(old >> kBigShift);
+        if (idx >= a.big_cap) {
+            atomicOr(&a.counters[4], 1u);
+            return;
         }
-        float rp[6], rc[6], p[6], c[6];
-        uint32_t rn, n;
-        bin_box(nb - 1, rp, rc, rn);
-        int result = 0;
-        float best = 3.402823466e+38f;
-        bool set = false;
-        for (int i = (int)nb - 2; i >= 0; i--) {
-            float score = __fmaf_rn(box_sa(l2r_p[i], l2r_p[i] + 3, a.sa_x2), (float)(int)l2r_n[i],
-                                    box_sa(rp, rp + 3, a.sa_x2) * (float)(int)rn);
-            if (score < best) {
-                best = score; result = i; set = true;
-                for (int k = 0; k < 6; k++) { s_lp[k] = l2r_p[i][k]; s_lc[k] = l2r_c[i][k]; s_rp[k] = rp[k]; s_rc[k] = rc[k]; }
-            }
-            bin_box(i, p, c, n);
-            grow(rp, p);
-            grow(rc, c);
-            rn += n;
-        }
-        if (!set) {
-            float qp[6], qc[6];
-            uint32_t qn;
-            bin_box(nb - 1, qp, qc, qn);
-            for (uint32_t i = nb - 2; i >= 1; i--) {
-                bin_box(i, p, c, n);
-                grow(qp, p);
-                grow(qc, c);
-            }
-            for (int k = 0; k < 6; k++) { s_lp[k] = l2r_p[0][k]; s_lc[k] = l2r_c[0][k]; s_rp[k] = qp[k]; s_rc[k] = qc[k]; }
-            result = 0;
-        }
-        s_plane = result;
-        s_m = l2r_n[result];
-    }
-    __syncthreads();
-    const int plane = s_plane;
-    const uint32_t m = s_m;
-
-    // ---- partition ids_in[b,e) -> ids_out[b,e) ----
-    auto is_right = [&](uint32_t id) {
-        float mn[3], mx[3];
-        prim_box(a.boxes, id, mn, mx);
-        float c = (mn[axis] + mx[axis]) * 0.5f;
-        return (int)(k1 * (c - cmin)) > plane;
-    };
-    if (a.part == 1) {
-        uint32_t run_l = 0;
-        for (uint32_t base = b; base < e; base += kSahThreads) {
-            uint32_t i = base + threadIdx.x;
-            bool valid = i < e;
-            uint32_t id = valid ? a.ids_in[i] : 0u;
-            bool right = valid && is_right(id);
-            uint32_t isl = (valid && !right) ? 1u : 0u, tot;
-            uint32_t pre = block_scan(isl, s_tmp, &tot);
-            if (valid) {
-                uint32_t rank_l = run_l + pre;
-                uint32_t pos = isl ? b + rank_l : b + m + ((i - b) - rank_l);
-                if (pos < e) a.ids_out[pos] = id;
-            }
-            run_l += tot;
-        }
+        c.chunk0 = (uint32_t)(old & ((1ull << kBigShift) - 1));
+        c.nchunks = nch;
+        a.next_big[idx] = c;
     } else {
-        uint32_t run_r = 0, run_l = 0;
-        for (uint32_t base = b; base < e; base += kSahThreads) {
-            uint32_t i = base + threadIdx.x;
-            bool valid = i < e;
-            uint32_t id = valid ? a.ids_in[i] : 0u;
-            bool right = valid && is_right(id);
-            bool mis_r = valid && right && i < b + m;
-            bool mis_l = valid && !right && i >= b + m;
-            uint32_t packed = (mis_r ? 1u : 0u) | (mis_l ? 0x10000u : 0u), tot;
-            uint32_t pre = block_scan(packed, s_tmp, &tot);
-            if (mis_r) {
-                uint32_t r = run_r + (pre & 0xFFFFu);
-                if (b + r < e) a.scratch[b + r] = i;
-            } else if (mis_l) {
-                uint32_t r = run_l + (pre >> 16);
-                if (b + m + r < e) a.scratch[b + m + r] = i;
-            } else if (valid) {
-                a.ids_out[i] = id;
-            }
-            run_r += tot & 0xFFFFu;
-            run_l += tot >> 16;
+        const uint32_t idx = atomicAdd(&a.counters[0], 1u);
+        if (idx >= a.small_cap) {
+            atomicOr(&a.counters[4], 1u);
+            return;
         }
-        __syncthreads();
-        const uint32_t k = run_r;  // == run_l
-        for (uint32_t r = threadIdx.x; r < k; r += kSahThreads) {
-            uint32_t pr = a.scratch[b + r], pl = a.scratch[b + m + (k - 1 - r)];
-            a.ids_out[pr] = a.ids_in[pl];
-            a.ids_out[pl] = a.ids_in[pr];
-        }
+        c.chunk0 = 0;
+        c.nchunks = 0;
+        a.next[idx] = c;
     }
-    __threadfence_block();
-    __syncthreads();
+}
 
-    // ---- emit children (recurse(), builder.cpp:1112-1155) ----
-    const uint32_t L = m, R = count - m, f = t.free;
+// Children of a split task (recurse(), builder.cpp:1112-1155): slots, leaf
+// boxes from ids_out, interior nodes and child tasks, by `nthr` threads of
+// which `tid` is this one; ids_out of the task's range must be complete and
+// visible.
+__device__ void emit_children(const SahArgs& a, const SahTask& t, const SahPlan& P, uint32_t tid, uint32_t nthr) {
+    const uint32_t b = t.b, e = t.e, count = e - b;
+    const uint32_t L = P.m, R = count - P.m, f = t.free;
     uint32_t left_index = f, fi = f + (L != 1);
     uint32_t right_index = fi;
     fi += (R != 1);
@@ -310,28 +257,364 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_level(SahArgs a) {
         left_free = f + 2;
         right_free = f + 1;
     }
-    const bool left_leaf = L <= 8 || box_sa(s_lc, s_lc + 3, a.sa_x2) <= 0.0f;
-    const bool right_leaf = R <= 8 || box_sa(s_rc, s_rc + 3, a.sa_x2) <= 0.0f;
+    const bool left_leaf = L <= 8 || box_sa(P.lc, P.lc + 3, a.sa_x2) <= 0.0f;
+    const bool right_leaf = R <= 8 || box_sa(P.rc, P.rc + 3, a.sa_x2) <= 0.0f;
     if (left_leaf)
-        for (uint32_t i = threadIdx.x; i < L; i += kSahThreads) write_leaf_box(a, left_free + i, a.ids_out[b + i]);
+        for (uint32_t i = tid; i < L; i += nthr) write_leaf_box(a, left_free + i, a.ids_out[b + i]);
     if (right_leaf)
-        for (uint32_t i = threadIdx.x; i < R; i += kSahThreads) write_leaf_box(a, right_free + i, a.ids_out[b + L + i]);
-    if (threadIdx.x == 0) {
-        if (L != 1) write_inner_box(a, left_index, s_lp, s_lp + 3, left_free, left_leaf ? L : 2);
-        if (R != 1) write_inner_box(a, right_index, s_rp, s_rp + 3, right_free, right_leaf ? R : 2);
+        for (uint32_t i = tid; i < R; i += nthr) write_leaf_box(a, right_free + i, a.ids_out[b + L + i]);
+    if (tid == 0) {
+        if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2);
+        if (R != 1) write_inner_box(a, right_index, P.rp, P.rp + 3, right_free, right_leaf ? R : 2);
         if (!left_leaf) {
             SahTask c;
             c.b = b; c.e = b + L; c.free = left_free; c.self = left_index;
-            for (int k = 0; k < 6; k++) c.cb[k] = s_lc[k];
-            a.next[atomicAdd(&a.counters[0], 1u)] = c;
+            for (int k = 0; k < 6; k++) c.cb[k] = P.lc[k];
+            push_task(a, c);
         }
         if (!right_leaf) {
             SahTask c;
             c.b = b + L; c.e = e; c.free = right_free; c.self = right_index;
-            for (int k = 0; k < 6; k++) c.cb[k] = s_rc[k];
-            a.next[atomicAdd(&a.counters[0], 1u)] = c;
+            for (int k = 0; k < 6; k++) c.cb[k] = P.rc[k];
+            push_task(a, c);
         }
     }
+}
+
+// ---- small tasks: one wave per task ----
+// Lane-parallel choose_plane: lane i holds bin i; inclusive prefix unions give
+// the left side of plane i, suffix unions from lane i+1 its right side (exact
+// on the ordered-integer images). The sequential scan from plane nb-2 down to
+// 0 with a strict < keeps the smallest score and, among equal scores, the
+// highest plane; scores that are NaN or not below FLT_MAX never win, and when
+// none wins plane 0 is taken with the same boxes as the scan's fallback.
+__device__ __forceinline__ uint32_t wave_scan_up(uint32_t v, uint32_t f, uint32_t lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(v, off, 64);
+        if (lane >= (uint32_t)off) v = bin_combine(f, v, y);
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_scan_down(uint32_t v, uint32_t f, uint32_t lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_down(v, off, 64);
+        if (lane + off < 64) v = bin_combine(f, v, y);
+    }
+    return v;
+}
+
+__device__ __forceinline__ void choose_plane_wave(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
+                                                  uint32_t lane, SahPlan& P) {
+    uint32_t pre[kBinWords], rgt[kBinWords];
+#pragma unroll
+    for (uint32_t f = 0; f < (uint32_t)kBinWords; f++) {
+        const uint32_t v = lane < nb ? bins[lane][f] : bin_identity(f);
+        pre[f] = wave_scan_up(v, f, lane);
+        rgt[f] = __shfl_down(wave_scan_down(v, f, lane), 1, 64);
+    }
+    float lp[6], rp[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        lp[k] = sqd::ord2f(pre[1 + k]);
+        rp[k] = sqd::ord2f(rgt[1 + k]);
+    }
+    const float score = __fmaf_rn(box_sa(lp, lp + 3, sa_x2), (float)(int)pre[0],
+                                  box_sa(rp, rp + 3, sa_x2) * (float)(int)rgt[0]);
+    const bool valid = lane + 2 <= nb && score < 3.402823466e+38f;
+    float best = valid ? score : 3.402823466e+38f;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, off, 64));
+    const uint64_t win = __ballot(valid && score == best);
+    const uint32_t plane = win ? 63u - (uint32_t)__clzll((long long)win) : 0u;
+    P.plane = plane;
+    P.m = __shfl(pre[0], plane, 64);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        P.lp[k] = sqd::ord2f(__shfl(pre[1 + k], plane, 64));
+        P.lc[k] = sqd::ord2f(__shfl(pre[7 + k], plane, 64));
+        P.rp[k] = sqd::ord2f(__shfl(rgt[1 + k], plane, 64));
+        P.rc[k] = sqd::ord2f(__shfl(rgt[7 + k], plane, 64));
+    }
+}
+
+// Makes a wave's LDS and global writes visible to its other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t ntasks) {
+    __shared__ uint32_t s_bins[kSahWaves][kMaxBins][kBinWords];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t ti = blockIdx.x * kSahWaves + wave;
+    if (ti >= ntasks) return;  // whole wave; the kernel has no workgroup barrier
+    const SahTask t = a.tasks[ti];
+    const uint32_t b = t.b, e = t.e, nb = a.nb;
+    const int axis = choose_axis(t.cb);
+    const float k1 = bin_k1(nb, t.cb, axis);
+    const float cmin = t.cb[axis];
+    uint32_t(*B)[kBinWords] = s_bins[wave];
+
+    for (uint32_t i = lane; i < nb * kBinWords; i += 64) B[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
+    wave_sync();
+    for (uint32_t i = b + lane; i < e; i += 64) {
+        const uint32_t id = a.ids_in[i];
+        float mn[3], mx[3], c[3];
+        prim_box(a.boxes, id, mn, mx);
+#pragma unroll
+        for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
+        int bi = (int)(k1 * (c[axis] - cmin));
+        bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
+        a.bin_of[i] = (uint8_t)bi;
+        uint32_t* W = B[bi];
+        atomicAdd(&W[0], 1u);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&W[1 + k], sqd::f2ord(mn[k]));
+            atomicMax(&W[4 + k], sqd::f2ord(mx[k]));
+            atomicMin(&W[7 + k], sqd::f2ord(c[k]));
+            atomicMax(&W[10 + k], sqd::f2ord(c[k]));
+        }
+    }
+    wave_sync();
+    SahPlan P;
+    choose_plane_wave(B, nb, a.sa_x2, lane, P);
+    const uint32_t plane = P.plane, m = P.m;
+
+    // partition: each lane reads back the bins it wrote (same lane mapping)
+    const uint64_t lt = (1ull << lane) - 1;
+    if (a.part == 1) {
+        uint32_t run_l = 0;
+        for (uint32_t base = b; base < e; base += 64) {
+            const uint32_t i = base + lane;
+            const bool valid = i < e;
+            const uint32_t id = valid ? a.ids_in[i] : 0u;
+            const bool isl = valid && a.bin_of[i] <= plane;
+            const uint64_t ml = __ballot(isl);
+            if (valid) {
+                const uint32_t rank_l = run_l + (uint32_t)__popcll(ml & lt);
+                const uint32_t pos = isl ? b + rank_l : b + m + ((i - b) - rank_l);
+                if (pos < e) a.ids_out[pos] = id;
+            }
+            run_l += (uint32_t)__popcll(ml);
+        }
+    } else {
+        uint32_t run_r = 0, run_l = 0;
+        for (uint32_t base = b; base < e; base += 64) {
+            const uint32_t i = base + lane;
+            const bool valid = i < e;
+            const uint32_t id = valid ? a.ids_in[i] : 0u;
+            const bool right = valid && a.bin_of[i] > plane;
+            const bool mis_r = valid && right && i < b + m;
+            const bool mis_l = valid && !right && i >= b + m;
+            const uint64_t mr = __ballot(mis_r), ml = __ballot(mis_l);
+            if (mis_r) {
+                const uint32_t r = run_r + (uint32_t)__popcll(mr & lt);
+                if (b + r < e) a.scratch[b + r] = i;
+            } else if (mis_l) {
+                const uint32_t r = run_l + (uint32_t)__popcll(ml & lt);
+                if (b + m + r < e) a.scratch[b + m + r] = i;
+            } else if (valid) {
+                a.ids_out[i] = id;
+            }
+            run_r += (uint32_t)__popcll(mr);
+            run_l += (uint32_t)__popcll(ml);
+        }
+        wave_sync();
+        const uint32_t k = run_r;  // == run_l
+        for (uint32_t r = lane; r < k; r += 64) {
+            const uint32_t pr = a.scratch[b + r], pl = a.scratch[b + m + (k - 1 - r)];
+            if (pr - b >= e - b || pl - b >= e - b) continue;
+            a.ids_out[pr] = a.ids_in[pl];
+            a.ids_out[pl] = a.ids_in[pr];
+        }
+    }
+    wave_sync();
+    emit_children(a, t, P, lane, 64);
+}
+
+// ---- large tasks: the same steps spread over one workgroup per chunk ----
+// Chunks of a level are numbered task after task in queue order; a block
+// finds its task by binary search over the tasks' first chunks.
+__device__ __forceinline__ uint32_t big_task_of(const SahArgs& a, uint32_t chunk) {
+    uint32_t lo = 0, hi = a.nbig;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.big[mid].chunk0 <= chunk) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void chunk_range(const SahTask& t, uint32_t g, uint32_t& lo, uint32_t& hi) {
+    const uint64_t cnt = t.e - t.b, per = (cnt + t.nchunks - 1) / t.nchunks;
+    const uint64_t l = (uint64_t)g * per, h = l + per;
+    lo = t.b + (uint32_t)(l < cnt ? l : cnt);
+    hi = t.b + (uint32_t)(h < cnt ? h : cnt);
+}
+
+// Bins one chunk; writes the chunk's bins and each position's bin.
+__global__ __launch_bounds__(kSahThreads) void k_big_bin(SahArgs a) {
+    __shared__ BinsLds s_bins;
+    const uint32_t chunk = blockIdx.x;
+    const SahTask t = a.big[big_task_of(a, chunk)];
+    uint32_t lo, hi;
+    chunk_range(t, chunk - t.chunk0, lo, hi);
+    const int axis = choose_axis(t.cb);
+    const float k1 = bin_k1(a.nb, t.cb, axis);
+    bins_clear(s_bins, a.nb);
+    __syncthreads();
+    bin_range(a, lo, hi, axis, k1, t.cb[axis], s_bins, true);
+    __syncthreads();
+    bins_merge_waves(s_bins, a.nb);
+    __syncthreads();
+    const uint32_t words = a.nb * kBinWords;
+    for (uint32_t w = threadIdx.x; w < words; w += kSahThreads)
+        a.part_bins[(size_t)chunk * words + w] = s_bins[0][w / kBinWords][w % kBinWords];
+}
+
+// Folds a task's chunk bins and chooses its plane.
+__global__ __launch_bounds__(kSahThreads) void k_big_plane(SahArgs a) {
+    __shared__ uint32_t s_red[kMaxBins][kBinWords];
+    const SahTask t = a.big[blockIdx.x];
+    const uint32_t words = a.nb * kBinWords;
+    for (uint32_t w = threadIdx.x; w < words; w += kSahThreads) {
+        const uint32_t f = w % kBinWords;
+        uint32_t v = bin_identity(f);
+        const uint32_t* src = a.part_bins + (size_t)t.chunk0 * words + w;
+#pragma unroll 8
+        for (uint32_t g = 0; g < t.nchunks; g++) v = bin_combine(f, v, src[(size_t)g * words]);
+        s_red[w / kBinWords][f] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        SahPlan P;
+        choose_plane_wave(s_red, a.nb, a.sa_x2, threadIdx.x, P);
+        if (threadIdx.x == 0) a.plans[blockIdx.x] = P;
+    }
+}
+
+// Per chunk: left count (stable) or misplaced right/left counts (Hoare).
+__global__ __launch_bounds__(kSahThreads) void k_big_count(SahArgs a) {
+    __shared__ uint32_t s_tmp[kSahWaves];
+    const uint32_t chunk = blockIdx.x, ti = big_task_of(a, chunk);
+    const SahTask t = a.big[ti];
+    const uint32_t plane = a.plans[ti].plane, mid = t.b + a.plans[ti].m;
+    uint32_t lo, hi;
+    chunk_range(t, chunk - t.chunk0, lo, hi);
+    uint32_t c0 = 0, c1 = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSahThreads) {
+        const bool right = a.bin_of[i] > plane;
+        if (a.part == 1) {
+            c0 += right ? 0u : 1u;
+        } else {
+            c0 += (right && i < mid) ? 1u : 0u;
+            c1 += (!right && i >= mid) ? 1u : 0u;
+        }
+    }
+    c0 = block_sum(c0, s_tmp);
+    c1 = block_sum(c1, s_tmp);
+    if (threadIdx.x == 0) {
+        a.chunk_cnt[2 * chunk] = c0;
+        a.chunk_cnt[2 * chunk + 1] = c1;
+    }
+}
+
+// Per chunk: the scatter of the single-workgroup partition, its running
+// ranks starting at the counts of the task's earlier chunks.
+__global__ __launch_bounds__(kSahThreads) void k_big_scatter(SahArgs a) {
+    __shared__ uint32_t s_tmp[kSahWaves];
+    const uint32_t chunk = blockIdx.x, ti = big_task_of(a, chunk);
+    const SahTask t = a.big[ti];
+    const uint32_t g = chunk - t.chunk0;
+    const uint32_t plane = a.plans[ti].plane, m = a.plans[ti].m, b = t.b, e = t.e;
+    uint32_t lo, hi;
+    chunk_range(t, g, lo, hi);
+    uint32_t p0 = 0, p1 = 0;
+    for (uint32_t j = threadIdx.x; j < g; j += kSahThreads) {
+        p0 += a.chunk_cnt[2 * (t.chunk0 + j)];
+        p1 += a.chunk_cnt[2 * (t.chunk0 + j) + 1];
+    }
+    p0 = block_sum(p0, s_tmp);
+    p1 = block_sum(p1, s_tmp);
+    if (a.part == 1) {
+        uint32_t run_l = p0;
+        for (uint32_t base = lo; base < hi; base += kSahThreads) {
+            const uint32_t i = base + threadIdx.x;
+            const bool valid = i < hi;
+            const uint32_t id = valid ? a.ids_in[i] : 0u;
+            const bool right = valid && a.bin_of[i] > plane;
+            uint32_t isl = (valid && !right) ? 1u : 0u, tot;
+            const uint32_t pre = block_scan(isl, s_tmp, &tot);
+            if (valid) {
+                const uint32_t rank_l = run_l + pre;
+                const uint32_t pos = isl ? b + rank_l : b + m + ((i - b) - rank_l);
+                if (pos < e) a.ids_out[pos] = id;
+            }
+            run_l += tot;
+        }
+    } else {
+        uint32_t run_r = p0, run_l = p1;
+        for (uint32_t base = lo; base < hi; base += kSahThreads) {
+            const uint32_t i = base + threadIdx.x;
+            const bool valid = i < hi;
+            const uint32_t id = valid ? a.ids_in[i] : 0u;
+            const bool right = valid && a.bin_of[i] > plane;
+            const bool mis_r = valid && right && i < b + m;
+            const bool mis_l = valid && !right && i >= b + m;
+            const uint32_t packed = (mis_r ? 1u : 0u) | (mis_l ? 0x10000u : 0u);
+            uint32_t tot;
+            const uint32_t pre = block_scan(packed, s_tmp, &tot);
+            if (mis_r) {
+                const uint32_t r = run_r + (pre & 0xFFFFu);
+                if (b + r < e) a.scratch[b + r] = i;
+            } else if (mis_l) {
+                const uint32_t r = run_l + (pre >> 16);
+                if (b + m + r < e) a.scratch[b + m + r] = i;
+            } else if (valid) {
+                a.ids_out[i] = id;
+            }
+            run_r += tot & 0xFFFFu;
+            run_l += tot >> 16;
+        }
+    }
+}
+
+// Hoare: the r-th misplaced right element and the r-th misplaced left element
+// from the end exchange places; chunk g takes pairs [g*per, (g+1)*per).
+__global__ __launch_bounds__(kSahThreads) void k_big_swap(SahArgs a) {
+    __shared__ uint32_t s_tmp[kSahWaves];
+    const uint32_t chunk = blockIdx.x, ti = big_task_of(a, chunk);
+    const SahTask t = a.big[ti];
+    const uint32_t g = chunk - t.chunk0, b = t.b, m = a.plans[ti].m;
+    uint32_t k = 0;
+    for (uint32_t j = threadIdx.x; j < t.nchunks; j += kSahThreads) k += a.chunk_cnt[2 * (t.chunk0 + j)];
+    k = block_sum(k, s_tmp);
+    const uint32_t cnt = t.e - t.b, per = (uint32_t)(((uint64_t)cnt + t.nchunks - 1) / t.nchunks);
+    const uint64_t r0 = (uint64_t)g * per, r1 = r0 + per < k ? r0 + per : k;
+    for (uint64_t r = r0 + threadIdx.x; r < r1; r += kSahThreads) {
+        const uint32_t pr = a.scratch[b + r], pl = a.scratch[b + m + (k - 1 - r)];
+        if (pr - b >= cnt || pl - b >= cnt) continue;
+        a.ids_out[pr] = a.ids_in[pl];
+        a.ids_out[pl] = a.ids_in[pr];
+    }
+}
+
+__global__ __launch_bounds__(kSahThreads) void k_big_emit(SahArgs a) {
+    __shared__ SahPlan s_plan;
+    const uint32_t* s = (const uint32_t*)&a.plans[blockIdx.x];
+    uint32_t* d = (uint32_t*)&s_plan;
+    for (uint32_t i = threadIdx.x; i < sizeof(SahPlan) / 4; i += kSahThreads) d[i] = s[i];
+    __syncthreads();
+    emit_children(a, a.big[blockIdx.x], s_plan, threadIdx.x, kSahThreads);
+}
+
+__global__ void k_sah_counters_reset(uint32_t* counters) {
+    if (threadIdx.x == 0) counters[0] = 0;
+    if (threadIdx.x == 2 || threadIdx.x == 3) counters[threadIdx.x] = 0;
 }
 
 // Per-primitive boxes, identity ids and the scene/centroid boxes (setup_reduction,
@@ -385,7 +668,7 @@ __device__ __forceinline__ void decode_box(const uint32_t* r, float* mn, float* 
 
 // Root of build_hierarchy (builder.cpp:1587-1612): recursion from slot 0,
 // root node at 2n-1 holding the scene box.
-__global__ void k_sah_root(SahArgs a, uint32_t n, const uint32_t* red, SahTask* q0) {
+__global__ void k_sah_root(SahArgs a, uint32_t n, const uint32_t* red) {
     float pmn[3], pmx[3], cb[6];
     decode_box(red, pmn, pmx);
     decode_box(red + 6, cb, cb + 3);
@@ -400,8 +683,7 @@ __global__ void k_sah_root(SahArgs a, uint32_t n, const uint32_t* red, SahTask* 
         SahTask t;
         t.b = 0; t.e = n; t.free = 0; t.self = ri;
         for (int k = 0; k < 6; k++) t.cb[k] = cb[k];
-        q0[0] = t;
-        a.counters[0] = 1;
+        push_task(a, t);
     }
 }
 
@@ -413,10 +695,17 @@ __device__ __forceinline__ uint32_t spread3(uint32_t x) {
 }
 
 // Cell of each primitive in Morton cell order (compact_1_by_2 inverse) and the
-// per-cell bins (bin_centroids_to_grid_sse, builder.cpp:1755-1839).
+// per-cell bins (bin_centroids_to_grid_sse, builder.cpp:1755-1839). Bins are
+// gathered in LDS per workgroup and merged into the global cell bins once per
+// workgroup (exact: counts add, ordered-integer min/max commute).
+constexpr uint32_t kMaxCells = 512;
 __global__ __launch_bounds__(256) void k_grid_cells(const float4* __restrict__ boxes, uint32_t n, uint32_t dim,
                                                     const uint32_t* __restrict__ red, uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ vals, uint32_t* __restrict__ cellbins) {
+    __shared__ uint32_t s_cells[kMaxCells * kBinWords];
+    const uint32_t cells = dim * dim * dim;
+    for (uint32_t i = threadIdx.x; i < cells * kBinWords; i += blockDim.x) s_cells[i] = bin_identity(i % kBinWords);
+    __syncthreads();
     float cb[6];
     decode_box(red + 6, cb, cb + 3);
     float k1[3];
@@ -435,7 +724,7 @@ __global__ __launch_bounds__(256) void k_grid_cells(const float4* __restrict__ b
         uint32_t cell = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
         keys[i] = cell;
         vals[i] = i;
-        uint32_t* C = cellbins + cell * kBinWords;
+        uint32_t* C = s_cells + cell * kBinWords;
         atomicAdd(&C[0], 1u);
         for (int k = 0; k < 3; k++) {
             atomicMin(&C[1 + k], sqd::f2ord(mn[k]));
@@ -443,6 +732,14 @@ __global__ __launch_bounds__(256) void k_grid_cells(const float4* __restrict__ b
             atomicMin(&C[7 + k], sqd::f2ord(c[k]));
             atomicMax(&C[10 + k], sqd::f2ord(c[k]));
         }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cells * kBinWords; i += blockDim.x) {
+        const uint32_t f = i % kBinWords, v = s_cells[i];
+        if (s_cells[i - f] == 0) continue;  // empty in this workgroup
+        if (f == 0) atomicAdd(&cellbins[i], v);
+        else if ((f - 1) % 6 < 3) atomicMin(&cellbins[i], v);
+        else atomicMax(&cellbins[i], v);
     }
 }
 
@@ -456,7 +753,7 @@ __global__ void k_grid_bins_init(uint32_t* cellbins, uint32_t cells) {
 
 // Cell nodes at cells-1+i, per-cell subtrees from 2*cells + 2*begin, heap top.
 __global__ __launch_bounds__(256) void k_grid_init(SahArgs a, uint32_t cells, uint32_t cell_tight,
-                                                   const uint32_t* cellbins, const uint32_t* red, SahTask* q0) {
+                                                   const uint32_t* cellbins, const uint32_t* red) {
     __shared__ uint32_t s_begin[513];
     if (threadIdx.x == 0) {
         uint32_t run = 0;
@@ -486,7 +783,7 @@ __global__ __launch_bounds__(256) void k_grid_init(SahArgs a, uint32_t cells, ui
                 SahTask t;
                 t.b = begin; t.e = begin + cnt; t.free = free; t.self = cells - 1 + i;
                 for (int k = 0; k < 6; k++) t.cb[k] = cb[k];
-                q0[a.counters[0]++] = t;
+                push_task(a, t);
             }
         }
         __syncthreads();
@@ -517,7 +814,12 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     a.sa_x2 = (kind == SQ_BUILDER_GRID_SSE || kind == SQ_BUILDER_GRID_AVX) ? 0u : 1u;
     a.layout = p ? p->layout : SQ_LAYOUT_REFERENCE;
 
-    const size_t qcap = n / 9 + 2 + cells;
+    // A task that is split has at least 9 primitives (a large one more than
+    // kBigTask), and the tasks of one level own disjoint ranges.
+    const size_t qcap = n / 9 + 2 + cells, bigcap = n / kBigTask + 2 + cells;
+    const size_t chunkcap = n / kChunkMin + bigcap + 1;  // sum of ceil(count / kChunkMin) over a level
+    a.small_cap = (uint32_t)qcap;
+    a.big_cap = (uint32_t)bigcap;
     SQ_TRY(c->aux[3].ensure((size_t)n * 32));             // boxes
     SQ_TRY(c->vals[0].ensure((size_t)n * 4));              // ids ping
     SQ_TRY(c->vals[1].ensure((size_t)n * 4));              // ids pong
@@ -526,6 +828,12 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     SQ_TRY(c->scratch.ensure((size_t)n * 4 + 256));
     SQ_TRY(c->tasks[0].ensure(qcap * sizeof(SahTask)));
     SQ_TRY(c->tasks[1].ensure(qcap * sizeof(SahTask)));
+    SQ_TRY(c->sah[0].ensure(bigcap * sizeof(SahTask)));
+    SQ_TRY(c->sah[1].ensure(bigcap * sizeof(SahTask)));
+    SQ_TRY(c->sah[2].ensure(bigcap * sizeof(SahPlan)));
+    SQ_TRY(c->sah[3].ensure(chunkcap * nb * kBinWords * 4));
+    SQ_TRY(c->sah[4].ensure(chunkcap * 8));
+    SQ_TRY(c->sah[5].ensure((size_t)n + 256));
     SQ_TRY(c->aux[0].ensure(64 * 4));
     SQ_TRY(c->counters.ensure(64));
     SQ_TRY(c->aux[2].ensure((size_t)cells * kBinWords * 4 + 64));
@@ -535,6 +843,12 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     a.boxes = boxes;
     a.counters = counters;
     a.scratch = c->scratch.as<uint32_t>();
+    a.plans = c->sah[2].as<SahPlan>();
+    a.part_bins = c->sah[3].as<uint32_t>();
+    a.chunk_cnt = c->sah[4].as<uint32_t>();
+    a.bin_of = c->sah[5].as<uint8_t>();
+    a.next = c->tasks[0].as<SahTask>();
+    a.next_big = c->sah[0].as<SahTask>();
 
     phase_begin(c, grid ? PH_GRID : PH_SAH);
     hipLaunchKernelGGL(k_sah_red_init, dim3(1), dim3(64), 0, c->stream, red, counters);
@@ -545,7 +859,8 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     if (grid) {
         uint32_t* cellbins = c->aux[2].as<uint32_t>();
         hipLaunchKernelGGL(k_grid_bins_init, dim3(1), dim3(256), 0, c->stream, cellbins, cells);
-        hipLaunchKernelGGL(k_grid_cells, dim3(blocks), dim3(256), 0, c->stream, boxes, n, dim, red,
+        const uint32_t cblocks = blocks < (uint32_t)c->num_cus * 2 ? blocks : (uint32_t)c->num_cus * 2;
+        hipLaunchKernelGGL(k_grid_cells, dim3(cblocks), dim3(256), 0, c->stream, boxes, n, dim, red,
                            c->keys[0].as<uint32_t>(), c->vals[0].as<uint32_t>(), cellbins);
         int slot = 0;
         uint32_t kbits = 0;
@@ -554,32 +869,51 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
         in = slot;
         a.ids_in = c->vals[in].as<uint32_t>();
         hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(256), 0, c->stream, a, cells,
-                           kind == SQ_BUILDER_GRID ? 0u : 1u, cellbins, red, c->tasks[0].as<SahTask>());
+                           kind == SQ_BUILDER_GRID ? 0u : 1u, cellbins, red);
         *root = 0;
     } else {
         a.ids_in = c->vals[0].as<uint32_t>();
-        hipLaunchKernelGGL(k_sah_root, dim3(1), dim3(256), 0, c->stream, a, n, red, c->tasks[0].as<SahTask>());
+        hipLaunchKernelGGL(k_sah_root, dim3(1), dim3(256), 0, c->stream, a, n, red);
         *root = 2 * n - 1;
     }
     SQ_CHECK_HIP(hipGetLastError());
     SQ_TRY(pinned_ensure(c, 64));
     uint32_t* h = (uint32_t*)c->pinned;
-    SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 4, hipMemcpyDeviceToHost, c->stream));
-    SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
-    uint32_t ncur = h[0], level = 0;
+    uint32_t level = 0;
     int q = 0;
-    while (ncur > 0) {
-        if (ncur > qcap) SQ_RET(SQ_ERR_OVERFLOW, "sah: task queue overflow (%u > %zu)", ncur, qcap);
-        hipLaunchKernelGGL(k_sah_red_init, dim3(1), dim3(1), 0, c->stream, red + 32, counters);  // counters[0] = 0
+    for (;;) {
+        SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 20, hipMemcpyDeviceToHost, c->stream));
+        SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+        if (h[4]) SQ_RET(SQ_ERR_OVERFLOW, "sah: task queue overflow at level %u", level);
+        const uint32_t nsmall = h[0];
+        const uint64_t packed = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
+        const uint32_t nbig = (uint32_t)(packed >> kBigShift);
+        const uint64_t nchunks = packed & ((1ull << kBigShift) - 1);
+        if (!nsmall && !nbig) break;
+        if (nsmall > qcap || nbig > bigcap || nchunks > chunkcap)
+            SQ_RET(SQ_ERR_OVERFLOW, "sah: level %u holds %u + %u tasks, %llu chunks", level, nsmall, nbig,
+                   (unsigned long long)nchunks);
+        hipLaunchKernelGGL(k_sah_counters_reset, dim3(1), dim3(64), 0, c->stream, counters);
         a.tasks = c->tasks[q].as<SahTask>();
         a.next = c->tasks[q ^ 1].as<SahTask>();
+        a.big = c->sah[q].as<SahTask>();
+        a.next_big = c->sah[q ^ 1].as<SahTask>();
+        a.nbig = nbig;
         a.ids_in = c->vals[in].as<uint32_t>();
         a.ids_out = c->vals[in ^ 1].as<uint32_t>();
-        hipLaunchKernelGGL(k_sah_level, dim3(ncur), dim3(kSahThreads), 0, c->stream, a);
+        if (nsmall)
+            hipLaunchKernelGGL(k_sah_wave, dim3((nsmall + kSahWaves - 1) / kSahWaves), dim3(kSahThreads), 0, c->stream, a,
+                               nsmall);
+        if (nbig) {
+            const dim3 gc((uint32_t)nchunks), gt(nbig), bt(kSahThreads);
+            hipLaunchKernelGGL(k_big_bin, gc, bt, 0, c->stream, a);
+            hipLaunchKernelGGL(k_big_plane, gt, bt, 0, c->stream, a);
+            hipLaunchKernelGGL(k_big_count, gc, bt, 0, c->stream, a);
+            hipLaunchKernelGGL(k_big_scatter, gc, bt, 0, c->stream, a);
+            if (a.part == 0) hipLaunchKernelGGL(k_big_swap, gc, bt, 0, c->stream, a);
+            hipLaunchKernelGGL(k_big_emit, gt, bt, 0, c->stream, a);
+        }
         SQ_CHECK_HIP(hipGetLastError());
-        SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 4, hipMemcpyDeviceToHost, c->stream));
-        SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
-        ncur = h[0];
         q ^= 1;
         in ^= 1;
         level++;
