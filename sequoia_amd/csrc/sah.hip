@@ -38,15 +38,20 @@ constexpr int kMaxBins = 64;
 constexpr int kBinWords = 13;  // count, p.min[3], p.max[3], c.min[3], c.max[3]
 // Tasks of more than kBigTask primitives are split by many workgroups, one per
 // chunk of at least kChunkMin primitives (at most kMaxChunks chunks a task).
-constexpr uint32_t kBigTask = 4096;
+constexpr uint32_t kBigTask = 2048;
 constexpr uint32_t kChunkMin = 2048;
 constexpr uint32_t kMaxChunks = 512;
 constexpr int kBigShift = 40;  // large-queue counter: (tasks << 40) | chunks
+// Tasks of at most kSubtree primitives are queued once and finished, their
+// whole subtree, by one wave in LDS after the level loop.
+constexpr uint32_t kSubtree = 256;
+constexpr int kSubStack = 16;
 
 struct SahTask {
     uint32_t b, e, free, self;
     float cb[6];               // centroid box: min xyz, max xyz
-    uint32_t chunk0, nchunks;  // large tasks: first chunk of the level, chunks
+    uint32_t chunk0, nchunks;  // large tasks: first chunk of the level, chunks;
+                               // subtree tasks: id buffer holding the range, level
 };
 
 // Split chosen for a task: child bounds and centroid boxes, plane, left count.
@@ -69,7 +74,13 @@ struct SahArgs {
     SahTask* next;          // small tasks of the next level
     const SahTask* big;     // large tasks of this level
     SahTask* next_big;      // large tasks of the next level
-    uint32_t* counters;     // [0] next small count, [2..3] next large (packed), [4] error flag
+    SahTask* sub;           // subtree tasks (all levels)
+    uint32_t sub_cap;
+    uint32_t out_buf;       // id buffer (0/1) that holds the ranges of queued children
+    uint32_t child_level;   // level of queued children
+    const uint32_t* ids_buf[2];
+    uint32_t* counters;     // [0] next small count, [2..3] next large (packed), [4] error flag,
+                            // [5] 1 + deepest subtree level, [6] subtree tasks
     const uint32_t* ids_in;
     uint32_t* ids_out;
     uint32_t* scratch;      // Hoare misplaced positions
@@ -213,7 +224,8 @@ __device__ __forceinline__ void bins_merge_waves(BinsLds& s, uint32_t nb) {
 
 // Queues a child task: above kBigTask primitives on the large queue with its
 // chunk range (one packed atomic hands out the queue slot and the chunks, so
-// chunk ranges ascend with queue slots), else on the small queue.
+// chunk ranges ascend with queue slots), above kSubtree on the small queue,
+// else on the subtree queue with the buffer holding its ids and its level.
 __device__ void push_task(const SahArgs& a, SahTask c) {
     const uint32_t cnt = c.e - c.b;
     if (cnt > kBigTask) {
@@ -229,7 +241,7 @@ __device__ void push_task(const SahArgs& a, SahTask c) {
         c.chunk0 = (uint32_t)(old & ((1ull << kBigShift) - 1));
         c.nchunks = nch;
         a.next_big[idx] = c;
-    } else {
+    } else if (cnt > kSubtree) {
         const uint32_t idx = atomicAdd(&a.counters[0], 1u);
         if (idx >= a.small_cap) {
             atomicOr(&a.counters[4], 1u);
@@ -238,6 +250,15 @@ __device__ void push_task(const SahArgs& a, SahTask c) {
         c.chunk0 = 0;
         c.nchunks = 0;
         a.next[idx] = c;
+    } else {
+        const uint32_t idx = atomicAdd(&a.counters[6], 1u);
+        if (idx >= a.sub_cap) {
+            atomicOr(&a.counters[4], 1u);
+            return;
+        }
+        c.chunk0 = a.out_buf;
+        c.nchunks = a.child_level;
+        a.sub[idx] = c;
     }
 }
 
@@ -436,6 +457,190 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
     emit_children(a, t, P, lane, 64);
 }
 
+// ---- subtree tasks: one wave builds the whole subtree of a range of at
+// most kSubtree primitives in LDS ----
+// The range's ids and boxes are loaded once; the splits permute a list of
+// local slots (Hoare in place by the same pairing, stable through a staging
+// list), so every split sees the order the level kernels would give it, and
+// node slots follow from counts alone. Pending ranges sit on a small LDS stack
+// (the larger child is pushed first, so the smaller is split next and the
+// stack never holds more than log2(kSubtree / 9) + 2 ranges).
+struct SubEntry {
+    uint32_t lo, hi, free, self, level;
+    float cb[6];
+};
+
+__global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
+    __shared__ float s_box[kSubtree][6];
+    __shared__ uint32_t s_id[kSubtree];
+    __shared__ uint16_t s_ord[kSubtree];
+    __shared__ uint16_t s_tmp[kSubtree];
+    __shared__ uint8_t s_bin[kSubtree];
+    __shared__ uint32_t s_bins[kMaxBins][kBinWords];
+    __shared__ SubEntry s_stack[kSubStack];
+    const uint32_t lane = threadIdx.x;
+    if (blockIdx.x >= nsub) return;
+    const SahTask T = a.sub[blockIdx.x];
+    const uint32_t n = T.e - T.b, nb = a.nb;
+    if (n > kSubtree) return;  // never queued (push_task), kept as a guard of the LDS arrays
+    const uint32_t* ids = a.ids_buf[T.chunk0 & 1];
+    for (uint32_t j = lane; j < n; j += 64) {
+        const uint32_t id = ids[T.b + j];
+        float mn[3], mx[3];
+        prim_box(a.boxes, id, mn, mx);
+        s_id[j] = id;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            s_box[j][k] = mn[k];
+            s_box[j][3 + k] = mx[k];
+        }
+        s_ord[j] = (uint16_t)j;
+    }
+    if (lane == 0) {
+        SubEntry E;
+        E.lo = 0; E.hi = n; E.free = T.free; E.self = T.self; E.level = T.nchunks;
+        for (int k = 0; k < 6; k++) E.cb[k] = T.cb[k];
+        s_stack[0] = E;
+    }
+    uint32_t sp = 1, deepest = 0;
+    const uint64_t lt = (1ull << lane) - 1;
+    wave_sync();
+    while (sp > 0) {
+        sp--;
+        const SubEntry E = s_stack[sp];
+        deepest = E.level + 1 > deepest ? E.level + 1 : deepest;
+        const uint32_t lo = E.lo, hi = E.hi, cnt = hi - lo;
+        const int axis = choose_axis(E.cb);
+        const float k1 = bin_k1(nb, E.cb, axis);
+        const float cmin = E.cb[axis];
+        for (uint32_t i = lane; i < nb * kBinWords; i += 64) s_bins[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
+        wave_sync();
+        for (uint32_t j = lo + lane; j < hi; j += 64) {
+            const uint32_t sl = s_ord[j];
+            float mn[3], mx[3], c[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                mn[k] = s_box[sl][k];
+                mx[k] = s_box[sl][3 + k];
+                c[k] = (mn[k] + mx[k]) * 0.5f;
+            }
+            int bi = (int)(k1 * (c[axis] - cmin));
+            bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
+            s_bin[j] = (uint8_t)bi;
+            uint32_t* W = s_bins[bi];
+            atomicAdd(&W[0], 1u);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                atomicMin(&W[1 + k], sqd::f2ord(mn[k]));
+                atomicMax(&W[4 + k], sqd::f2ord(mx[k]));
+                atomicMin(&W[7 + k], sqd::f2ord(c[k]));
+                atomicMax(&W[10 + k], sqd::f2ord(c[k]));
+            }
+        }
+        wave_sync();
+        SahPlan P;
+        choose_plane_wave(s_bins, nb, a.sa_x2, lane, P);
+        const uint32_t plane = P.plane, m = P.m;
+        if (a.part == 1) {
+            uint32_t run_l = 0;
+            for (uint32_t base = lo; base < hi; base += 64) {
+                const uint32_t j = base + lane;
+                const bool valid = j < hi;
+                const bool isl = valid && s_bin[j] <= plane;
+                const uint64_t ml = __ballot(isl);
+                if (valid) {
+                    const uint32_t rank_l = run_l + (uint32_t)__popcll(ml & lt);
+                    const uint32_t pos = isl ? rank_l : m + ((j - lo) - rank_l);
+                    if (pos < cnt) s_tmp[lo + pos] = s_ord[j];
+                }
+                run_l += (uint32_t)__popcll(ml);
+            }
+            wave_sync();
+            for (uint32_t j = lo + lane; j < hi; j += 64) s_ord[j] = s_tmp[j];
+        } else {
+            uint32_t run_r = 0, run_l = 0;
+            for (uint32_t base = lo; base < hi; base += 64) {
+                const uint32_t j = base + lane;
+                const bool valid = j < hi;
+                const bool right = valid && s_bin[j] > plane;
+                const bool mis_r = valid && right && j < lo + m;
+                const bool mis_l = valid && !right && j >= lo + m;
+                const uint64_t mr = __ballot(mis_r), ml = __ballot(mis_l);
+                if (mis_r) {
+                    const uint32_t r = run_r + (uint32_t)__popcll(mr & lt);
+                    if (r < cnt) s_tmp[lo + r] = (uint16_t)j;
+                } else if (mis_l) {
+                    const uint32_t r = run_l + (uint32_t)__popcll(ml & lt);
+                    if (m + r < cnt) s_tmp[lo + m + r] = (uint16_t)j;
+                }
+                run_r += (uint32_t)__popcll(mr);
+                run_l += (uint32_t)__popcll(ml);
+            }
+            wave_sync();
+            const uint32_t k = run_r;  // == run_l
+            for (uint32_t r = lane; r < k; r += 64) {
+                const uint32_t pr = s_tmp[lo + r], pl = s_tmp[lo + m + (k - 1 - r)];
+                if (pr - lo >= cnt || pl - lo >= cnt) continue;
+                const uint16_t x = s_ord[pr], y = s_ord[pl];
+                s_ord[pr] = y;
+                s_ord[pl] = x;
+            }
+        }
+        wave_sync();
+
+        // children: the slots of emit_children
+        const uint32_t L = m, R = cnt - m, f = E.free;
+        uint32_t left_index = f, fi = f + (L != 1);
+        uint32_t right_index = fi;
+        fi += (R != 1);
+        uint32_t left_free = fi, right_free = fi + (L - 1) * 2 + (L == 1);
+        if (a.layout == SQ_LAYOUT_FIXED && R == 1 && L != 1) {
+            right_index = f + 1;
+            left_free = f + 2;
+            right_free = f + 1;
+        }
+        const bool left_leaf = L <= 8 || box_sa(P.lc, P.lc + 3, a.sa_x2) <= 0.0f;
+        const bool right_leaf = R <= 8 || box_sa(P.rc, P.rc + 3, a.sa_x2) <= 0.0f;
+        for (uint32_t i = lane; i < cnt; i += 64) {
+            const bool is_left = i < L;
+            if (is_left ? !left_leaf : !right_leaf) continue;
+            const uint32_t slot = is_left ? left_free + i : right_free + (i - L);
+            if (slot >= a.cap) continue;
+            const uint32_t sl = s_ord[lo + i];
+            sqd::Box3 bx{{s_box[sl][0], s_box[sl][1], s_box[sl][2]}, {s_box[sl][3], s_box[sl][4], s_box[sl][5]}};
+            sqd::store_node(a.nodes, slot, bx, s_id[sl], 1, 1);
+        }
+        if (lane == 0) {
+            if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2);
+            if (R != 1) write_inner_box(a, right_index, P.rp, P.rp + 3, right_free, right_leaf ? R : 2);
+        }
+        SubEntry C[2];
+        uint32_t nc = 0;
+        // larger child first: the smaller one is popped next
+        const bool left_first = L >= R;
+        for (int side = 0; side < 2; side++) {
+            const bool left = (side == 0) == left_first;
+            if (left ? left_leaf : right_leaf) continue;
+            SubEntry& c = C[nc++];
+            c.lo = left ? lo : lo + L;
+            c.hi = left ? lo + L : hi;
+            c.free = left ? left_free : right_free;
+            c.self = left ? left_index : right_index;
+            c.level = E.level + 1;
+            for (int k = 0; k < 6; k++) c.cb[k] = left ? P.lc[k] : P.rc[k];
+        }
+        if (sp + nc > (uint32_t)kSubStack) {
+            if (lane == 0) atomicOr(&a.counters[4], 2u);
+            break;
+        }
+        if (lane == 0)
+            for (uint32_t i = 0; i < nc; i++) s_stack[sp + i] = C[i];
+        sp += nc;
+        wave_sync();
+    }
+    if (lane == 0) atomicMax(&a.counters[5], deepest);
+}
+
 // ---- large tasks: the same steps spread over one workgroup per chunk ----
 // Chunks of a level are numbered task after task in queue order; a block
 // finds its task by binary search over the tasks' first chunks.
@@ -476,18 +681,22 @@ __global__ __launch_bounds__(kSahThreads) void k_big_bin(SahArgs a) {
         a.part_bins[(size_t)chunk * words + w] = s_bins[0][w / kBinWords][w % kBinWords];
 }
 
-// Folds a task's chunk bins and chooses its plane.
+// Folds a task's chunk bins (all threads over (chunk, word) pairs, LDS
+// atomics on the ordered-integer images) and chooses its plane.
 __global__ __launch_bounds__(kSahThreads) void k_big_plane(SahArgs a) {
     __shared__ uint32_t s_red[kMaxBins][kBinWords];
     const SahTask t = a.big[blockIdx.x];
     const uint32_t words = a.nb * kBinWords;
-    for (uint32_t w = threadIdx.x; w < words; w += kSahThreads) {
-        const uint32_t f = w % kBinWords;
-        uint32_t v = bin_identity(f);
-        const uint32_t* src = a.part_bins + (size_t)t.chunk0 * words + w;
-#pragma unroll 8
-        for (uint32_t g = 0; g < t.nchunks; g++) v = bin_combine(f, v, src[(size_t)g * words]);
-        s_red[w / kBinWords][f] = v;
+    for (uint32_t w = threadIdx.x; w < words; w += kSahThreads) s_red[w / kBinWords][w % kBinWords] = bin_identity(w % kBinWords);
+    __syncthreads();
+    const uint32_t* src = a.part_bins + (size_t)t.chunk0 * words;
+    const uint32_t total = t.nchunks * words;
+    for (uint32_t i = threadIdx.x; i < total; i += kSahThreads) {
+        const uint32_t w = i % words, f = w % kBinWords, v = src[i];
+        uint32_t* d = &s_red[w / kBinWords][f];
+        if (f == 0) atomicAdd(d, v);
+        else if ((f - 1) % 6 < 3) atomicMin(d, v);
+        else atomicMax(d, v);
     }
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -646,13 +855,27 @@ __global__ __launch_bounds__(256) void k_sah_setup(const sq_triangle* __restrict
             lo[k] = x < lo[k] ? x : lo[k];
             hi[k] = y > hi[k] ? y : hi[k];
         }
+    __shared__ uint32_t s_red[12][4];
+    const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         for (int k = 0; k < 3; k++) {
-            atomicMin(&red[k], lo[k]);
-            atomicMax(&red[3 + k], hi[k]);
-            atomicMin(&red[6 + k], lo[3 + k]);
-            atomicMax(&red[9 + k], hi[3 + k]);
+            s_red[k][wave] = lo[k];
+            s_red[3 + k][wave] = hi[k];
+            s_red[6 + k][wave] = lo[3 + k];
+            s_red[9 + k][wave] = hi[3 + k];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        const bool is_min = (k / 3) % 2 == 0;
+        uint32_t r = s_red[k][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            const uint32_t x = s_red[k][w];
+            r = is_min ? (x < r ? x : r) : (x > r ? x : r);
+        }
+        if (is_min) atomicMin(&red[k], r);
+        else atomicMax(&red[k], r);
     }
 }
 
@@ -820,6 +1043,7 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     const size_t chunkcap = n / kChunkMin + bigcap + 1;  // sum of ceil(count / kChunkMin) over a level
     a.small_cap = (uint32_t)qcap;
     a.big_cap = (uint32_t)bigcap;
+    a.sub_cap = (uint32_t)qcap;
     SQ_TRY(c->aux[3].ensure((size_t)n * 32));             // boxes
     SQ_TRY(c->vals[0].ensure((size_t)n * 4));              // ids ping
     SQ_TRY(c->vals[1].ensure((size_t)n * 4));              // ids pong
@@ -828,6 +1052,7 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     SQ_TRY(c->scratch.ensure((size_t)n * 4 + 256));
     SQ_TRY(c->tasks[0].ensure(qcap * sizeof(SahTask)));
     SQ_TRY(c->tasks[1].ensure(qcap * sizeof(SahTask)));
+    SQ_TRY(c->tasks[2].ensure(qcap * sizeof(SahTask)));
     SQ_TRY(c->sah[0].ensure(bigcap * sizeof(SahTask)));
     SQ_TRY(c->sah[1].ensure(bigcap * sizeof(SahTask)));
     SQ_TRY(c->sah[2].ensure(bigcap * sizeof(SahPlan)));
@@ -849,6 +1074,10 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     a.bin_of = c->sah[5].as<uint8_t>();
     a.next = c->tasks[0].as<SahTask>();
     a.next_big = c->sah[0].as<SahTask>();
+    a.sub = c->tasks[2].as<SahTask>();
+    a.ids_buf[0] = c->vals[0].as<uint32_t>();
+    a.ids_buf[1] = c->vals[1].as<uint32_t>();
+    a.child_level = 0;
 
     phase_begin(c, grid ? PH_GRID : PH_SAH);
     hipLaunchKernelGGL(k_sah_red_init, dim3(1), dim3(64), 0, c->stream, red, counters);
@@ -868,11 +1097,13 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
         SQ_TRY(radix_sort(c, n, kbits ? kbits : 1, 8, false, 0, &slot));
         in = slot;
         a.ids_in = c->vals[in].as<uint32_t>();
+        a.out_buf = (uint32_t)in;
         hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(256), 0, c->stream, a, cells,
                            kind == SQ_BUILDER_GRID ? 0u : 1u, cellbins, red);
         *root = 0;
     } else {
         a.ids_in = c->vals[0].as<uint32_t>();
+        a.out_buf = 0;
         hipLaunchKernelGGL(k_sah_root, dim3(1), dim3(256), 0, c->stream, a, n, red);
         *root = 2 * n - 1;
     }
@@ -882,7 +1113,7 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     uint32_t level = 0;
     int q = 0;
     for (;;) {
-        SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 20, hipMemcpyDeviceToHost, c->stream));
+        SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 32, hipMemcpyDeviceToHost, c->stream));
         SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
         if (h[4]) SQ_RET(SQ_ERR_OVERFLOW, "sah: task queue overflow at level %u", level);
         const uint32_t nsmall = h[0];
@@ -901,6 +1132,8 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
         a.nbig = nbig;
         a.ids_in = c->vals[in].as<uint32_t>();
         a.ids_out = c->vals[in ^ 1].as<uint32_t>();
+        a.out_buf = (uint32_t)(in ^ 1);
+        a.child_level = level + 1;
         if (nsmall)
             hipLaunchKernelGGL(k_sah_wave, dim3((nsmall + kSahWaves - 1) / kSahWaves), dim3(kSahThreads), 0, c->stream, a,
                                nsmall);
@@ -919,8 +1152,19 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
         level++;
         if (level > 4 * 1024 * 1024) SQ_RET(SQ_ERR_OVERFLOW, "sah: runaway recursion");
     }
+    const uint32_t nsub = h[6];
+    uint32_t levels = level;
+    if (nsub) {
+        if (nsub > qcap) SQ_RET(SQ_ERR_OVERFLOW, "sah: %u subtree tasks", nsub);
+        hipLaunchKernelGGL(k_sah_subtree, dim3(nsub), dim3(64), 0, c->stream, a, nsub);
+        SQ_CHECK_HIP(hipGetLastError());
+        SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 32, hipMemcpyDeviceToHost, c->stream));
+        SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+        if (h[4]) SQ_RET(SQ_ERR_OVERFLOW, "sah: subtree stack overflow");
+        levels = h[5] > levels ? h[5] : levels;
+    }
     phase_end(c, grid ? PH_GRID : PH_SAH);
-    *depth = level + 2 + (grid ? 7 : 0);
+    *depth = levels + 2 + (grid ? 7 : 0);
     return SQ_OK;
 }
 

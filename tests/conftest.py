@@ -32,6 +32,20 @@ def ctx():
     c.close()
 
 
+def tree_depth(nodes, root):
+    """Levels from the root (1) to the deepest node, by breadth-first expansion
+    of the interior nodes' child ranges."""
+    frontier = np.array([root], np.int64)
+    depth = 0
+    while len(frontier):
+        depth += 1
+        inner = frontier[nodes["flags"][frontier] == 0]
+        cnt = nodes["count"][inner].astype(np.int64)
+        first = nodes["child"][inner].astype(np.int64)
+        frontier = np.repeat(first, cnt) + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+    return depth
+
+
 def nodes_equal(a, b, fields=("min", "count", "flags", "max", "child")):
     """Indices where two Node1 arrays differ bitwise in the given fields."""
     bad = np.zeros(len(a), bool)

@@ -6,7 +6,7 @@ exceed the single-workgroup task size, for every partition kind."""
 import numpy as np
 import pytest
 
-from conftest import nodes_equal
+from conftest import nodes_equal, tree_depth
 from oracle import binding as ob
 from test_gpu_trace import check_hits
 
@@ -26,14 +26,17 @@ def test_large_tasks_bit_exact(ctx, kind):
     say("large tasks", kind)
     tris = sq.gen_clusters(3_000_000, 64, 17)
     scene = ctx.upload(tris)
-    got = ctx.build(scene, kind, num_bins=16).download()
+    bvh = ctx.build(scene, kind, num_bins=16)
+    got = bvh.download()
     if kind == "recursive":
-        want, _ = ob.build_sah(tris, 16, ob.PART_HOARE, ob.SA_SCALAR)
+        want, root = ob.build_sah(tris, 16, ob.PART_HOARE, ob.SA_SCALAR)
     elif kind == "grid":
-        want, _ = ob.build_grid(tris, 4, 16, ob.PART_HOARE, ob.SA_SCALAR, ob.CELLBOX_SCENE)
+        want, root = ob.build_grid(tris, 4, 16, ob.PART_HOARE, ob.SA_SCALAR, ob.CELLBOX_SCENE)
     else:
-        want, _ = ob.build_grid(tris, 4, 16, ob.PART_STABLE, ob.SA_SSE, ob.CELLBOX_TIGHT)
+        want, root = ob.build_grid(tris, 4, 16, ob.PART_STABLE, ob.SA_SSE, ob.CELLBOX_TIGHT)
     assert len(nodes_equal(got, want)) == 0
+    # the reported depth bounds the traversal stack: never below the tree's
+    assert bvh.info()[2] >= tree_depth(got, root)
 
 
 @pytest.fixture(scope="module")
@@ -57,6 +60,7 @@ def test_clusters_10m_tree_and_secondary_rays(ctx, clusters10m, kind, nb):
         want, root = ob.build_sah(tris, nb, ob.PART_HOARE, ob.SA_SCALAR)
     assert bvh.root == root
     assert len(nodes_equal(got, want)) == 0
+    assert bvh.info()[2] >= tree_depth(got, root)
     del want
     say("10M trees equal; secondary rays")
     rays = sq.gen_secondary_rays(tris, 1_000_000, seed=3, tmin=1e-4)

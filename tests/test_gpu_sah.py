@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from conftest import golden, nodes_equal
+from conftest import golden, nodes_equal, tree_depth
 from oracle import binding as ob
 from scenes import GOLDEN_SCENES
 
@@ -84,11 +84,13 @@ def test_sah_one_million(ctx):
     say("sah 1M soup, 32 bins")
     tris = sq.gen_soup(1_000_000, 2, 0.01)
     scene = ctx.upload(tris)
-    got = ctx.build(scene, "recursive", num_bins=32).download()
+    bvh = ctx.build(scene, "recursive", num_bins=32)
+    got = bvh.download()
     t = ctx.timings()
     say("sah timings", round(t["sah_ms"], 3), "depth", t["tree_depth"])
-    want, _ = ob.build_sah(tris, 32)
+    want, root = ob.build_sah(tris, 32)
     assert len(nodes_equal(got, want)) == 0
+    assert bvh.info()[2] >= tree_depth(got, root)
 
 
 def test_trace_on_gpu_sah_tree(ctx, manifest):
