@@ -342,6 +342,25 @@ int sq_reference_camera(const sq_triangle* tris, uint32_t n, uint32_t w, uint32_
 int sq_pinhole_camera(const sq_triangle* tris, uint32_t n, uint32_t w, uint32_t h, float fov_deg, float distance,
                       uint32_t spp, uint32_t jitter, uint32_t seed, sq_camera* out);
 
+/* ---- scene files (host) ----
+ * Replaces loadOBJFromFile (main.cpp:108-179): the same triangles for every
+ * file that loader reads ('v' and 'f' records, quads fanned, 1-based or
+ * negative indices, 255-byte line pieces, the reference's tokenizer);
+ * inputs it cannot read get defined results (a last token without a trailing
+ * delimiter is read, an empty fourth index is ignored) or SQ_ERR_INVALID (an
+ * index that names no vertex, a record with too few fields, a missing file).
+ * *out is allocated with malloc: release it with sq_host_free. nverts may be
+ * NULL. Nothing is printed. */
+int sq_load_obj(const char* path, sq_triangle** out, uint32_t* ntris, uint32_t* nverts);
+void sq_host_free(void* p);
+/* Replaces stbi_flip_vertically_on_write(true) + stbi_write_bmp(name, w, h,
+ * 3, img) (main.cpp:335-336): a 24-bit BMP whose first stored row is memory
+ * row 0 (the bottom of a heatmap), BGR, rows padded to 4 bytes; the same
+ * bytes as the reference's file. rgb is h rows of w RGB8 pixels. */
+size_t sq_bmp_size(uint32_t w, uint32_t h);
+int sq_encode_bmp(const uint8_t* rgb, uint32_t w, uint32_t h, uint8_t* out, size_t cap);
+int sq_write_bmp(const char* path, const uint8_t* rgb, uint32_t w, uint32_t h);
+
 #ifdef __cplusplus
 }
 #endif

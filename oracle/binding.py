@@ -103,6 +103,9 @@ def reference():
             "ref_radix_sort32": (None, [V, U, V, V]),
             "ref_trace": (C.c_int, [C.c_int, C.c_int, V, U, V, U, U, U, V, V]),
             "ref_traverse_rays": (None, [V, U, V, U, V, U, V, V]),
+            "ref_load_obj": (U, [C.c_char_p, V, U]),
+            "ref_write_bmp": (C.c_int, [C.c_char_p, C.c_int, C.c_int, V]),
+            "ref_cli_main": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
         }
         for k, (res, args) in sig.items():
             f = getattr(_r, k)
@@ -268,3 +271,20 @@ def ref_traverse_rays(nodes, root, tris, rays):
     counts, hit = np.empty(n, np.uint32), np.empty(n, np.uint32)
     reference().ref_traverse_rays(_p(nodes), root, _p(tris), len(tris), _p(rays), n, _p(counts), _p(hit))
     return counts, hit
+
+
+def ref_load_obj(path):
+    """The reference's loadOBJFromFile (main.cpp:108-179) on `path`."""
+    p = os.fsencode(path)
+    n = reference().ref_load_obj(p, None, 0)
+    out = np.empty((n, 9), np.float32)
+    if n:
+        reference().ref_load_obj(p, _p(out), n)
+    return out
+
+
+def ref_write_bmp(path, rgb):
+    """The reference's heatmap file (stb BMP writer, flipped; main.cpp:335-336)."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    h, w = rgb.shape[:2]
+    assert reference().ref_write_bmp(os.fsencode(path), w, h, _p(rgb)) == 1

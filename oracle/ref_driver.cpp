@@ -43,6 +43,13 @@ typedef struct alignas(16) RadixInput64 {
     __forceinline bool operator<(const RadixInput64& m) const { return code < m.code; }
 } RadixInput64;
 
+// main.cpp:108 and the stb writer compiled into main.o (main.cpp:25-26);
+// main.cpp's main() is compiled as seqref_cli_main (oracle/Makefile).
+std::vector<Triangle> loadOBJFromFile(std::string filename);
+int seqref_cli_main(int argc, char** argv);
+extern "C" void stbi_flip_vertically_on_write(int flag);
+extern "C" int stbi_write_bmp(char const* filename, int w, int h, int comp, const void* data);
+
 void setup_morton(std::vector<Triangle>& primitives, std::vector<RadixInput32>& morton_codes, AABB1& scene_aabb);
 void setup_morton2(std::vector<Triangle>& primitives, std::vector<RadixInput64>& morton_codes, AABB1& scene_aabb);
 
@@ -168,6 +175,25 @@ void ref_traverse_rays(void* nodes, uint32_t root, const float* tris, uint32_t n
         counts[i] = (uint32_t)a.num_tests[0];
         hit[i] = a.hit[0] ? 1u : 0u;
     }
+}
+
+// loadOBJFromFile (main.cpp:108-179): copies up to `cap` triangles, returns
+// how many the file holds.
+uint32_t ref_load_obj(const char* path, float* out, uint32_t cap) {
+    std::vector<Triangle> t = loadOBJFromFile(std::string(path));
+    const size_t n = t.size() < cap ? t.size() : cap;
+    std::memcpy(out, t.data(), sizeof(Triangle) * n);
+    return (uint32_t)t.size();
+}
+
+// The reference command line (main.cpp:339-538). It ends with exit(0), so
+// callers run it in a child process; files go to the working directory.
+int ref_cli_main(int argc, char** argv) { return seqref_cli_main(argc, argv); }
+
+// The heatmap file as main.cpp:335-336 writes it.
+int ref_write_bmp(const char* path, int w, int h, const uint8_t* rgb) {
+    stbi_flip_vertically_on_write(1);
+    return stbi_write_bmp(path, w, h, 3, rgb);
 }
 
 }  // extern "C"

@@ -175,6 +175,11 @@ _SIGS = {
     "sq_reference_camera": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Camera)]),
     "sq_pinhole_camera": (C.c_int, [_P, _U32, _U32, _U32, C.c_float, C.c_float, _U32, _U32, _U32,
                                     C.POINTER(Camera)]),
+    "sq_load_obj": (C.c_int, [C.c_char_p, C.POINTER(_P), C.POINTER(_U32), C.POINTER(_U32)]),
+    "sq_host_free": (None, [_P]),
+    "sq_bmp_size": (C.c_size_t, [_U32, _U32]),
+    "sq_encode_bmp": (C.c_int, [_P, _U32, _U32, _P, C.c_size_t]),
+    "sq_write_bmp": (C.c_int, [C.c_char_p, _P, _U32, _U32]),
 }
 
 
@@ -259,6 +264,49 @@ def pinhole_camera(tris, w, h, fov_deg=60.0, distance=3.0, spp=1, jitter=0, seed
     cam = Camera()
     _check(lib().sq_pinhole_camera(_ptr(tris), len(tris), w, h, fov_deg, distance, spp, jitter, seed, C.byref(cam)))
     return cam
+
+
+# ---- scene files (host) ----
+def load_obj(path):
+    """Triangles of a Wavefront OBJ, read as the reference's loader reads it;
+    returns (tris (n, 9) float32, vertex count)."""
+    p, n, nv = _P(), _U32(), _U32()
+    _check(lib().sq_load_obj(os.fsencode(path), C.byref(p), C.byref(n), C.byref(nv)))
+    try:
+        out = np.empty((n.value, 9), np.float32)
+        if n.value:
+            C.memmove(out.ctypes.data, p, out.nbytes)
+    finally:
+        lib().sq_host_free(p)
+    return out, nv.value
+
+
+def write_obj(path, tris):
+    """Writes triangles as OBJ text: three 'v' records per triangle with 9
+    significant digits (which read back as the same float32 values), then one
+    'f' record per triangle."""
+    v = as_tris(tris).reshape(-1, 3)
+    with open(path, "w", newline="\n") as f:
+        f.write("# %d triangles\n" % (len(v) // 3))
+        f.writelines("v %.9g %.9g %.9g\n" % (float(a), float(b), float(c)) for a, b, c in v)
+        f.writelines("f %d %d %d\n" % (3 * i + 1, 3 * i + 2, 3 * i + 3) for i in range(len(v) // 3))
+
+
+def encode_bmp(rgb):
+    """The reference's heatmap BMP bytes for an (h, w, 3) uint8 image."""
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    assert rgb.ndim == 3 and rgb.shape[2] == 3, rgb.shape
+    h, w = rgb.shape[:2]
+    out = np.empty(lib().sq_bmp_size(w, h), np.uint8)
+    _check(lib().sq_encode_bmp(_ptr(rgb), w, h, _ptr(out), out.nbytes))
+    return out.tobytes()
+
+
+def write_bmp(path, rgb):
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    assert rgb.ndim == 3 and rgb.shape[2] == 3, rgb.shape
+    h, w = rgb.shape[:2]
+    _check(lib().sq_write_bmp(os.fsencode(path), _ptr(rgb), w, h))
 
 
 # ---- device objects ----

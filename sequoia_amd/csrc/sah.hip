@@ -309,6 +309,7 @@ __device__ void emit_children(const SahArgs& a, const SahTask& t, const SahPlan&
 // 0 with a strict < keeps the smallest score and, among equal scores, the
 // highest plane; scores that are NaN or not below FLT_MAX never win, and when
 // none wins plane 0 is taken with the same boxes as the scan's fallback.
+// Inclusive scans over the wave (fixed distances: the shuffles unroll).
 __device__ __forceinline__ uint32_t wave_scan_up(uint32_t v, uint32_t f, uint32_t lane) {
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
