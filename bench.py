@@ -10,13 +10,18 @@ One step = build (Morton codes, radix sort, emission, refit) + trace, inputs
 resident in HBM. `value` is Mrays/s of the trace (HIP events on the library
 stream around the traversal kernel, summed over the timed steps).
 
-The same JSON line carries two more sections: "batched" (256 scenes built and
-traced in one captured graph against 256 sequential plain calls) and
-"served" (a Poisson request stream through the serving loop against one
-plain call per request, throughput and p50/p99 latency at three rates), and
-"cpu_baseline" (the reference's own CPU build + traverse on a bounded sample).
+The same JSON line carries more sections: "sah32" (the same soup built with
+32-bin binned SAH and the same rays traced on that tree), "incoherent" (10M
+clustered triangles built with the grid and binned-SAH builders, 8M
+cosine-weighted secondary rays traced to their closest hits), "batched" (256
+scenes built and traced in one captured graph against 256 sequential plain
+calls), "served" (a Poisson request stream through the serving loop against
+one plain call per request, throughput and p50/p99 latency at three rates),
+and "cpu_baseline" (the reference's own CPU build + traverse on a bounded
+sample).
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batched] [--no-served]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-sah] [--no-incoherent]
+                    [--no-batched] [--no-served]
 """
 import argparse
 import json
@@ -81,6 +86,72 @@ def cpu_baseline(tris, cam, gpu_rays_per_s, budget_s):
 
 
 BATCH_SCENES = 256
+
+
+def sah_section(sq, ctx, scene, cam, nrays, reps=3):
+    """The headline soup built with 32-bin binned SAH (median of `reps`
+    builds) and the headline rays traced on that tree."""
+    log("sah32 build + trace")
+    ms = []
+    for _ in range(reps):
+        b = ctx.build(scene, "recursive", num_bins=32)
+        ms.append(ctx.timings()["total_build_ms"])
+        b.free()
+    b = ctx.build(scene, "recursive", num_bins=32)
+    depth = b.info()[2]
+    ctx.trace_camera(b, cam, want_hits=False)  # warm
+    tms = []
+    for _ in range(reps):
+        ctx.trace_camera(b, cam, want_hits=False)
+        tms.append(ctx.timings()["trace_ms"])
+    b.free()
+    bms, tm = float(np.median(ms)), float(np.median(tms))
+    return {"build_ms": round(bms, 3), "build_mtri_s": round(N_TRI / (bms / 1e3) / 1e6, 1), "tree_depth": depth,
+            "trace_ms": round(tm, 3), "mrays_s": round(nrays / (tm / 1e3) / 1e6, 1)}
+
+
+INC_TRI, INC_RAYS, INC_SUB = 10_000_000, 8_000_000, 100_000
+
+
+def incoherent_section(sq, ctx, reps=3):
+    """10M triangles in 64 Gaussian clusters (seed 3): grid and 8-bin binned
+    SAH builds (median of `reps`), then 8M secondary rays (origins on random
+    triangles, cosine-weighted about the normal, tmin 1e-4) traced to their
+    closest hits on the SAH tree, and a 100k-ray subset with the exhaustive
+    reference semantics."""
+    log("incoherent: generating %d clustered triangles, %d secondary rays" % (INC_TRI, INC_RAYS))
+    tris = sq.gen_clusters(INC_TRI, 64, 3)
+    rays = sq.gen_secondary_rays(tris, INC_RAYS, seed=3, tmin=1e-4)
+    sc = ctx.upload(tris)
+    out = {"workload": "10M clustered triangles (64 clusters, seed 3), 8M secondary rays, tmin 1e-4"}
+    try:
+        for kind in ("grid", "recursive"):
+            ms = []
+            for _ in range(reps):
+                b = ctx.build(sc, kind, num_bins=8)
+                ms.append(ctx.timings()["total_build_ms"])
+                b.free()
+            bms = float(np.median(ms))
+            out["build_%s_ms" % kind] = round(bms, 2)
+            out["build_%s_mtri_s" % kind] = round(INC_TRI / (bms / 1e3) / 1e6, 1)
+        b = ctx.build(sc, "recursive", num_bins=8)
+        log("incoherent: closest-hit trace")
+        r = ctx.trace_rays(b, rays, semantics=sq.TRACE_CLOSEST, count_work=True)
+        t = ctx.timings()
+        out["closest_trace_ms"] = round(t["trace_ms"], 1)
+        out["closest_mrays_s"] = round(INC_RAYS / (t["trace_ms"] / 1e3) / 1e6, 3)
+        out["closest_node_tests_per_ray"] = round(t["node_tests"] / INC_RAYS, 1)
+        out["hit_fraction"] = round(float((r["hit_id"] != sq.MISS).mean()), 4)
+        sub = rays[:INC_SUB]
+        ctx.trace_rays(b, sub, count_work=True)
+        t = ctx.timings()
+        out["exhaustive_subset_rays"] = INC_SUB
+        out["exhaustive_mrays_s"] = round(INC_SUB / (t["trace_ms"] / 1e3) / 1e6, 3)
+        out["exhaustive_node_tests_per_ray"] = round(t["node_tests"] / INC_SUB, 1)
+        b.free()
+    finally:
+        sc.free()
+    return out
 
 
 def batched_workload(sq):
@@ -337,6 +408,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--semantics", default="exhaustive", choices=["exhaustive", "closest"])
+    ap.add_argument("--no-sah", action="store_true", help="skip the 32-bin SAH build + trace of the headline soup")
+    ap.add_argument("--no-incoherent", action="store_true", help="skip the 10M-triangle secondary-ray section")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-scene batched-vs-plain comparison")
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--no-served", action="store_true", help="skip the Poisson-stream served-vs-plain comparison")
@@ -431,6 +504,16 @@ def main():
                      "kernel": "k_trace<false, false>", "algorithmic_bytes_per_launch": alg_bytes},
     }
     # the secondary sections never take the headline line down with them
+    if not a.no_sah:
+        try:
+            line["sah32"] = sah_section(sq, ctx, scene, cam, nrays)
+        except Exception as e:  # noqa: BLE001
+            line["sah32"] = {"error": repr(e)}
+    if not a.no_incoherent:
+        try:
+            line["incoherent"] = incoherent_section(sq, ctx)
+        except Exception as e:  # noqa: BLE001
+            line["incoherent"] = {"error": repr(e)}
     if not a.no_batched:
         try:
             line["batched"] = batched_section(sq, ctx, a.batched_steps, 1)

@@ -170,7 +170,8 @@ typedef struct {
     uint32_t stack_slots; /* 0 = default; LDS stack entries per lane */
     uint32_t count_work;  /* 1 = accumulate total node/triangle tests into sq_timings */
     uint32_t kernel;      /* camera rays: SQ_KERNEL_* (packets: exhaustive semantics only) */
-    uint32_t reserved;
+    uint32_t blocks_per_cu; /* per-ray kernel: 0 = default (8 waves per CU for one camera's rays, 16
+                               otherwise, at most what fits); else 1..64 */
 } sq_trace_params;
 
 /* Outputs (any may be NULL). hit_id/hit_t: one per ray (SQ_MISS / +inf on
@@ -237,16 +238,18 @@ int sq_batch_times(const sq_batch* b, float* build_ms, float* trace_ms, float* r
 
 /* Autotuner: times candidate radix digit widths (4..8) and subtree
  * thresholds (512..2048) for sq_batch_build, and the traversal kernel
- * (per-lane rays with stack slots 8..32 and workgroup sizes 64, 128, or wave
- * packets for exhaustive semantics) for sq_batch_trace, on this batch (which
- * must have cameras set), keeps the fastest in the batch and reports them.
- * None of them changes any result. */
+ * (per-lane rays with stack slots 8..32 and launch shapes of 8 to 24 waves
+ * per CU in workgroups of 64 or 128, or wave packets for exhaustive
+ * semantics) for sq_batch_trace, on this batch (which must have cameras
+ * set), keeps the fastest in the batch and reports them. None of them
+ * changes any result. */
 typedef struct {
     uint32_t radix_digit_bits, subtree_threshold, stack_slots, block_size;
     float build_ms, trace_ms;           /* best times */
     float default_build_ms, default_trace_ms;
     uint32_t candidates_tried;
     uint32_t kernel;                    /* SQ_KERNEL_RAYS or SQ_KERNEL_PACKETS */
+    uint32_t blocks_per_cu;             /* 0 = the default launch shape */
 } sq_tune_result;
 int sq_batch_autotune(sq_ctx* ctx, sq_batch* b, sq_builder kind, uint32_t repeats, sq_tune_result* out);
 

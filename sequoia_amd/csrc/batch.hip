@@ -334,7 +334,7 @@ int enqueue_build(sq_ctx* c, sq_batch* b) {
 
 int plan_trace(sq_ctx* c, sq_batch* b, TracePlan* L) {
     if (!b->pk_grid) b->pk_grid = trace_packet_grid(c, true);  // outside any graph capture
-    return trace_plan(c, &b->tparams, morton_depth_bound(b->bits, b->max_n), L);
+    return trace_plan(c, &b->tparams, morton_depth_bound(b->bits, b->max_n), false, L);
 }
 
 // Enqueues the batched trace (cameras, outputs and spill already set up).
@@ -527,7 +527,7 @@ int sq_batch_set_cameras(sq_ctx* c, sq_batch* b, const sq_camera* cams, const sq
     sq_trace_params tp{};
     if (p) tp = *p;
     TracePlan probe;
-    SQ_TRY(trace_plan(c, &tp, 64, &probe));  // validates block size
+    SQ_TRY(trace_plan(c, &tp, 64, false, &probe));  // validates block size
     size_t rays = 0, pix = 0;
     std::vector<size_t> ray_off(b->nscenes + 1), pix_off(b->nscenes + 1);
     std::vector<uint32_t> job_tiles(b->nscenes + 1);
@@ -723,14 +723,16 @@ int sq_batch_autotune(sq_ctx* c, sq_batch* b, sq_builder kind, uint32_t repeats,
     const std::vector<sq_camera> cams = b->cams;
     const uint32_t outputs = b->outputs;
     const uint32_t slots[] = {16, 8, 32};
-    const uint32_t blocks[] = {128, 64};
+    // (workgroup size, workgroups per CU; 0 = the default, 16 waves per CU)
+    const uint32_t shapes[][2] = {{64, 0}, {64, 8}, {64, 12}, {64, 24}, {128, 0}};
     float best_t = 1e30f;
-    uint32_t bs = 16, bb = 128, bk = SQ_KERNEL_RAYS;
-    auto time_trace = [&](uint32_t kernel, uint32_t sl, uint32_t bl, float* best) -> int {
+    uint32_t bs = 16, bb = 64, bw = 0, bk = SQ_KERNEL_RAYS;
+    auto time_trace = [&](uint32_t kernel, uint32_t sl, uint32_t bl, uint32_t bpc, float* best) -> int {
         sq_trace_params tp = base;
         tp.kernel = kernel;
         tp.stack_slots = sl;
         tp.block_size = bl;
+        tp.blocks_per_cu = bpc;
         SQ_TRY(sq_batch_set_cameras(c, b, cams.data(), &tp, outputs));
         float m = 1e30f;
         for (uint32_t r = 0; r < repeats + 1; r++) {
@@ -742,27 +744,29 @@ int sq_batch_autotune(sq_ctx* c, sq_batch* b, sq_builder kind, uint32_t repeats,
         return SQ_OK;
     };
     for (uint32_t sl : slots)
-        for (uint32_t bl : blocks) {
+        for (const auto& sh : shapes) {
             float m = 0;
-            SQ_TRY(time_trace(SQ_KERNEL_RAYS, sl, bl, &m));
-            if (sl == 16 && bl == 128) out->default_trace_ms = m;
-            if (m < best_t) { best_t = m; bs = sl; bb = bl; bk = SQ_KERNEL_RAYS; }
+            SQ_TRY(time_trace(SQ_KERNEL_RAYS, sl, sh[0], sh[1], &m));
+            if (sl == 16 && sh[0] == 64 && sh[1] == 0) out->default_trace_ms = m;
+            if (m < best_t) { best_t = m; bs = sl; bb = sh[0]; bw = sh[1]; bk = SQ_KERNEL_RAYS; }
         }
     if (base.semantics == SQ_TRACE_EXHAUSTIVE) {  // wave packets: exhaustive semantics only
         float m = 0;
-        SQ_TRY(time_trace(SQ_KERNEL_PACKETS, 0, 0, &m));
+        SQ_TRY(time_trace(SQ_KERNEL_PACKETS, 0, 0, 0, &m));
         if (m < best_t) { best_t = m; bk = SQ_KERNEL_PACKETS; }
     }
     sq_trace_params tp = base;
     tp.kernel = bk;
     tp.stack_slots = bs;
     tp.block_size = bb;
+    tp.blocks_per_cu = bw;
     SQ_TRY(sq_batch_set_cameras(c, b, cams.data(), &tp, outputs));
     out->kernel = bk;
     out->radix_digit_bits = bd;
     out->subtree_threshold = bt;
     out->stack_slots = bs;
     out->block_size = bb;
+    out->blocks_per_cu = bw;
     out->build_ms = best_b;
     out->trace_ms = best_t;
     return SQ_OK;

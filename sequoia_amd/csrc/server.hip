@@ -586,7 +586,7 @@ int sq_server_create(int device, const sq_server_config* cfg, sq_server** out) {
     s->adm = std::make_unique<Admission>(c.budget_bytes, lim);
     sq_trace_params tp{};
     tp.semantics = c.semantics;
-    int rc = trace_plan(s->ctx, &tp, morton_depth_bound(kMortonBits, 0x7FFFFFFFu), &s->plan);
+    int rc = trace_plan(s->ctx, &tp, morton_depth_bound(kMortonBits, 0x7FFFFFFFu), false, &s->plan);
     s->slots.resize(c.inflight_waves);
     // slot 0 runs on the context's stream, slot 1 on one more stream
     for (size_t i = 0; i < s->slots.size(); i++) {

@@ -204,7 +204,8 @@ struct TracePlan {
     uint32_t slots, block, grid, spill_cap;
     size_t lds_bytes, spill_words;
 };
-int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, TracePlan* out);
+// coherent: one camera's primary rays (a smaller default launch).
+int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out);
 // SQ_ERR_OVERFLOW with a message naming the fault bits.
 int trace_fault(uint32_t bits, const char* what);
 // Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).
