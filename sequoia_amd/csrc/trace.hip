@@ -126,6 +126,68 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
     return true;
 }
 
+// Exhaustive semantics: one step tests up to four nodes, the next two of the
+// current list and the next two after them (from the same list or, once it
+// is used up, from the list on top of the stack when that names its nodes
+// directly), so a lane has two independent node fetches in flight instead of
+// one. Counts and the closest hit do not depend on the visiting order.
+__device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+    while (L.cnt == 0) {
+        if (st.sp == 0) return false;
+        float te = 0.f;
+        const uint32_t e = pop<false>(st, te);
+        if ((e & 15u) != 15u) {
+            L.first = e >> 4;
+            L.cnt = (e & 15u) + 1;
+        } else {
+            if ((e >> 4) >= a.node_count) { st.fault |= kFaultNode; return false; }
+            const NodeV p = load_node(a.nodes, e >> 4);
+            L.first = p.child;
+            L.cnt = p.count;
+        }
+    }
+    const uint32_t ia = L.first, na = L.cnt >= 2 ? 2u : 1u;
+    L.first += na;
+    L.cnt -= na;
+    if (L.cnt == 0 && st.sp > 0) {
+        const uint32_t top = st.lds[((st.sp - 1) & st.mask) * st.stride];
+        if ((top & 15u) != 15u) {
+            float te = 0.f;
+            (void)pop<false>(st, te);
+            L.first = top >> 4;
+            L.cnt = (top & 15u) + 1;
+        }
+    }
+    const uint32_t ib = L.first, nb = L.cnt >= 2 ? 2u : L.cnt;
+    L.first += nb;
+    L.cnt -= nb;
+    if (ia >= a.node_count || na > a.node_count - ia || (nb && (ib >= a.node_count || nb > a.node_count - ib))) {
+        st.fault |= kFaultNode;
+        return false;
+    }
+    NodeV n0 = load_node(a.nodes, ia), n1, n2, n3;
+    if (na == 2) n1 = load_node(a.nodes, ia + 1);
+    if (nb >= 1) n2 = load_node(a.nodes, ib);
+    if (nb == 2) n3 = load_node(a.nodes, ib + 1);
+    L.tests += na + nb;
+    float f;
+    const bool h0 = slab(n0, L.r, L.ix, L.iy, L.iz, L.fast, f);
+    const bool h1 = na == 2 && slab(n1, L.r, L.ix, L.iy, L.iz, L.fast, f);
+    const bool h2 = nb >= 1 && slab(n2, L.r, L.ix, L.iy, L.iz, L.fast, f);
+    const bool h3 = nb == 2 && slab(n3, L.r, L.ix, L.iy, L.iz, L.fast, f);
+    bool bad = false;
+    if (h0 && n0.leaf) leaf_tris(a, n0, L, tri_tests, bad);
+    if (h1 && n1.leaf) leaf_tris(a, n1, L, tri_tests, bad);
+    if (h2 && n2.leaf) leaf_tris(a, n2, L, tri_tests, bad);
+    if (h3 && n3.leaf) leaf_tris(a, n3, L, tri_tests, bad);
+    if (bad) { st.fault |= kFaultTri; return false; }
+    if (h0 && !n0.leaf) push<false>(st, encode_list(ia, n0), 0.f);
+    if (h1 && !n1.leaf) push<false>(st, encode_list(ia + 1, n1), 0.f);
+    if (h2 && !n2.leaf) push<false>(st, encode_list(ib, n2), 0.f);
+    if (h3 && !n3.leaf) push<false>(st, encode_list(ib + 1, n3), 0.f);
+    return st.fault == 0;
+}
+
 constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode
 
 // One tile of job J: the wave's lanes take the tile's rays one after
@@ -183,7 +245,7 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
             continue;
         }
         if (active) {
-            if (!lane_step<CLOSEST>(a, L, st, tris_total)) {
+            if (!(CLOSEST ? lane_step<CLOSEST>(a, L, st, tris_total) : lane_step_wide(a, L, st, tris_total))) {
                 // ray finished
                 if (st.fault) {
                     faults |= st.fault;
