@@ -192,6 +192,43 @@ int sq_trace_camera(sq_ctx* ctx, const sq_bvh* bvh, const sq_camera* cam, const 
 int sq_trace_rays(sq_ctx* ctx, const sq_bvh* bvh, const sq_ray* rays, uint32_t n, const sq_trace_params* params,
                   const sq_trace_out* out, int io_on_device);
 
+/* ---- quantised two-child nodes (Node2, aabb.h:44-74) ----
+ * The reference declares Node2 (a float centre, 24-bit child fields, an 8-bit
+ * size, 3-bit counts, leaf bits, twelve 8-bit bounds) and never uses it; here
+ * it is a traversal format converted from any Node1 tree. Each Node2 holds
+ * the two children of one node: a leaf group of 1..7 triangles (contiguous in
+ * a reordered triangle array) or another Node2. Bound q of a child decodes to
+ * c + (q - 128) * 2^(size - 127) and rounds outwards, so the decoded boxes
+ * contain the Node1 boxes: closest hits equal the Node1 tree's, while a child
+ * pair costs one 32-byte fetch. Counts are box tests of this tree (the root
+ * box, then two per visited Node2), not the reference's Node1 counts. */
+typedef struct {
+    float c[3];             /* centre */
+    uint32_t a_child_size;  /* a_child (24) | size (8) */
+    uint32_t b_child_bits;  /* b_child (24) | a_count (3) | a_leaf (1) | b_count (3) | b_leaf (1) */
+    uint8_t q[12];          /* a_min_x a_max_x b_min_x b_max_x, then y, then z */
+} sq_node2;
+typedef struct {
+    float min[3], max[3];   /* root box, tested unquantised */
+    uint32_t kind;          /* 1: leaf group (child = first triangle, count), 2: Node2 `child` */
+    uint32_t child, count, reserved;
+} sq_node2_root;
+typedef struct sq_bvh2 sq_bvh2;
+/* Converts a device Node1 tree (needs leaves of at most 7 triangles, as every
+ * builder here makes; breadth-first, deterministic). */
+int sq_bvh2_create(sq_ctx* ctx, const sq_bvh* bvh, sq_bvh2** out);
+int sq_bvh2_info(const sq_bvh2* t, uint32_t* nodes, uint32_t* tris, uint32_t* levels, sq_node2_root* root);
+/* Host copies of the nodes, the reordered triangles and their original indices (any may be NULL). */
+int sq_bvh2_download(const sq_bvh2* t, sq_node2* nodes, uint32_t node_cap, sq_triangle* tris, uint32_t* perm,
+                     uint32_t tri_cap);
+void sq_bvh2_free(sq_bvh2* t);
+/* As sq_trace_camera / sq_trace_rays (ray heatmaps, per-ray kernel); hit ids
+ * are original triangle indices. */
+int sq_trace2_camera(sq_ctx* ctx, const sq_bvh2* t, const sq_camera* cam, const sq_trace_params* params,
+                     const sq_trace_out* out, int out_on_device);
+int sq_trace2_rays(sq_ctx* ctx, const sq_bvh2* t, const sq_ray* rays, uint32_t n, const sq_trace_params* params,
+                   const sq_trace_out* out, int io_on_device);
+
 /* ---- batches: many scenes built and traced together ----
  * No counterpart in the reference, which builds and traces one Hierarchy per
  * call (Builder::build_hierarchy_morton, builder.cpp:2474-2508; trace,

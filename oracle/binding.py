@@ -16,6 +16,11 @@ REF_SO = os.path.join(HERE, "_ref", "libseqref.so")
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 
+NODE2_DTYPE = np.dtype([("c", "<f4", 3), ("w0", "<u4"), ("w1", "<u4"), ("q", "u1", 12)])
+ROOT2_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("kind", "<u4"), ("child", "<u4"), ("count", "<u4"),
+                        ("pad", "<u4")])
+assert NODE2_DTYPE.itemsize == 32 and ROOT2_DTYPE.itemsize == 40
+
 LAYOUT_REFERENCE, LAYOUT_FIXED = 0, 1
 PART_HOARE, PART_STABLE = 0, 1
 SA_SCALAR, SA_SSE, SA_M128 = 0, 1, 2
@@ -75,6 +80,9 @@ def oracle():
             "sqo_trace_packet": (None, [V, U, V, C.POINTER(Camera), U, V, V]),
             "sqo_trace_bundle": (None, [V, U, V, C.POINTER(Camera), U, V]),
             "sqo_heat_byte": (C.c_uint8, [U, U]),
+            "sqo_node2_convert": (C.c_int, [V, U, U, V, U, V, U, V, V, V, U, V, V]),
+            "sqo_node2_decode": (None, [V, V, V, V, V]),
+            "sqo_node2_trace": (None, [V, V, V, V, V, U, U, V, V, V]),
         }
         for k, (res, args) in sig.items():
             f = getattr(_o, k)
@@ -288,3 +296,51 @@ def ref_write_bmp(path, rgb):
     rgb = np.ascontiguousarray(rgb, np.uint8)
     h, w = rgb.shape[:2]
     assert reference().ref_write_bmp(os.fsencode(path), w, h, _p(rgb)) == 1
+
+
+def node2_convert(nodes, root, tris):
+    """Node1 tree -> (Node2 array, root record, reordered triangles, perm)."""
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    tris = tris32(tris)
+    # reference-layout trees can reach a subtree twice (right-singleton quirk),
+    # and the conversion then holds it twice: capacities grow until they fit
+    cap, tcap = max(1, len(nodes)), max(1, len(tris))
+    while True:
+        out = np.zeros(cap, NODE2_DTYPE)
+        tris2 = np.zeros((tcap, 9), np.float32)
+        perm = np.zeros(tcap, np.uint32)
+        nn, nt = C.c_uint32(), C.c_uint32()
+        r = np.zeros(1, ROOT2_DTYPE)
+        rc = oracle().sqo_node2_convert(_p(nodes), len(nodes), root, _p(tris), len(tris), _p(out), cap, C.byref(nn),
+                                        _p(tris2), _p(perm), tcap, C.byref(nt), _p(r))
+        if rc == -5:
+            cap *= 2
+        elif rc == -4 and tcap < 64 * max(1, len(tris)):
+            tcap *= 2
+        elif rc != 0:
+            raise ValueError("node2 conversion failed: %d" % rc)
+        else:
+            return out[:nn.value].copy(), r[0], tris2[:nt.value].copy(), perm[:nt.value].copy()
+
+
+def node2_decode(n2):
+    """Decoded child boxes: (a_min, a_max, b_min, b_max), each (len, 3)."""
+    n2 = np.ascontiguousarray(n2, NODE2_DTYPE)
+    out = [np.empty((len(n2), 3), np.float32) for _ in range(4)]
+    for i in range(len(n2)):
+        oracle().sqo_node2_decode(n2[i:i + 1].ctypes.data, out[0][i].ctypes.data, out[1][i].ctypes.data,
+                                  out[2][i].ctypes.data, out[3][i].ctypes.data)
+    return out
+
+
+def node2_trace(n2, root, tris2, perm, rays, closest=False):
+    n2 = np.ascontiguousarray(n2, NODE2_DTYPE)
+    r = np.array([root], ROOT2_DTYPE)
+    tris2 = tris32(tris2)
+    perm = np.ascontiguousarray(perm, np.uint32)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = len(rays)
+    counts, hid, ht = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.float32)
+    oracle().sqo_node2_trace(_p(n2), _p(r), _p(tris2), _p(perm), _p(rays), n, 1 if closest else 0, _p(counts),
+                             _p(hid), _p(ht))
+    return {"counts": counts, "hit_id": hid, "hit_t": ht}

@@ -101,6 +101,21 @@ void sqo_trace_bundle(const sqo_node* nodes, uint32_t root, const sqo_tri* tris,
                       uint32_t K, uint8_t* rgb);
 uint8_t sqo_heat_byte(uint32_t count, uint32_t spp);
 
+/* ---- Quantised two-child nodes (Node2, aabb.h:44-74; encoding defined in node2.cpp) ---- */
+typedef struct { float c[3]; uint32_t w0, w1; uint8_t q[12]; } sqo_node2;          /* 32 B */
+typedef struct { float min[3], max[3]; uint32_t kind, child, count, pad; } sqo_node2_root;
+/* kind: 1 = the root is a leaf group (child = first triangle, count), 2 = Node2 `child`.
+ * Returns 0, or < 0 for an unsupported tree (leaf of > 7 triangles, index out
+ * of range, non-finite box, capacity). */
+int sqo_node2_convert(const sqo_node* nodes, uint32_t node_count, uint32_t root, const sqo_tri* tris,
+                      uint32_t ntris, sqo_node2* out, uint32_t node_cap, uint32_t* out_nodes, sqo_tri* tris2,
+                      uint32_t* perm, uint32_t tri_cap, uint32_t* out_tris, sqo_node2_root* root_out);
+void sqo_node2_decode(const sqo_node2* n, float a_min[3], float a_max[3], float b_min[3], float b_max[3]);
+/* counts: box tests per ray (root + two per visited Node2); hits report original triangle ids. */
+void sqo_node2_trace(const sqo_node2* nodes, const sqo_node2_root* root, const sqo_tri* tris2, const uint32_t* perm,
+                     const sqo_ray* rays, uint32_t nrays, uint32_t closest, uint32_t* counts, uint32_t* hit_id,
+                     float* hit_t);
+
 /* Sizing helpers. */
 uint32_t sqo_sah_node_capacity(uint32_t n);
 uint32_t sqo_grid_node_capacity(uint32_t n, uint32_t grid_dim);

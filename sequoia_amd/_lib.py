@@ -29,6 +29,7 @@ MISS = 0xFFFFFFFF
 
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
+NODE2_DTYPE = np.dtype([("c", "<f4", 3), ("w0", "<u4"), ("w1", "<u4"), ("q", "u1", 12)])
 assert NODE_DTYPE.itemsize == 32 and RAY_DTYPE.itemsize == 32
 
 
@@ -175,6 +176,12 @@ _SIGS = {
     "sq_reference_camera": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Camera)]),
     "sq_pinhole_camera": (C.c_int, [_P, _U32, _U32, _U32, C.c_float, C.c_float, _U32, _U32, _U32,
                                     C.POINTER(Camera)]),
+    "sq_bvh2_create": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "sq_bvh2_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32), _P]),
+    "sq_bvh2_download": (C.c_int, [_P, _P, _U32, _P, _P, _U32]),
+    "sq_bvh2_free": (None, [_P]),
+    "sq_trace2_camera": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
+    "sq_trace2_rays": (C.c_int, [_P, _P, _P, _U32, C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
     "sq_load_obj": (C.c_int, [C.c_char_p, C.POINTER(_P), C.POINTER(_U32), C.POINTER(_U32)]),
     "sq_host_free": (None, [_P]),
     "sq_bmp_size": (C.c_size_t, [_U32, _U32]),
@@ -348,6 +355,43 @@ class Bvh:
     def free(self):
         if self.h:
             lib().sq_bvh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Node2Root(C.Structure):
+    _fields_ = [("min", C.c_float * 3), ("max", C.c_float * 3), ("kind", C.c_uint32), ("child", C.c_uint32),
+                ("count", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Bvh2:
+    """Quantised two-child (Node2) tree converted from a Node1 tree."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+
+    def info(self):
+        n, t, lv, r = _U32(), _U32(), _U32(), Node2Root()
+        _check(lib().sq_bvh2_info(self.h, C.byref(n), C.byref(t), C.byref(lv), C.byref(r)))
+        return {"nodes": n.value, "tris": t.value, "levels": lv.value, "root": r}
+
+    def download(self):
+        """(nodes, root record, reordered triangles, original indices)."""
+        i = self.info()
+        nodes = np.empty(i["nodes"], NODE2_DTYPE)
+        tris = np.empty((i["tris"], 9), np.float32)
+        perm = np.empty(i["tris"], np.uint32)
+        _check(lib().sq_bvh2_download(self.h, _ptr(nodes), len(nodes), _ptr(tris), _ptr(perm), len(perm)))
+        return nodes, i["root"], tris, perm
+
+    def free(self):
+        if self.h:
+            lib().sq_bvh2_free(self.h)
             self.h = None
 
     def __del__(self):
@@ -638,11 +682,13 @@ class Context:
         rgb = np.empty((cam.height, cam.width, 3), np.uint8) if want_rgb else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), _ptr(rgb))
         p = self._params(semantics, heatmap, width, stack_slots, count_work, kernel)
-        _check(lib().sq_trace_camera(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 0))
+        f = lib().sq_trace2_camera if isinstance(bvh, Bvh2) else lib().sq_trace_camera
+        _check(f(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 0))
         res.update(hit_id=hid, hit_t=ht, counts=cnt, rgb=rgb)
         return res
 
     def trace_rays(self, bvh, rays, semantics=TRACE_EXHAUSTIVE, want_counts=True, stack_slots=0, count_work=False):
+        """Ray list through a Node1 tree (Bvh) or a Node2 tree (Bvh2)."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         n = len(rays)
         hid = np.empty(n, np.uint32)
@@ -650,5 +696,12 @@ class Context:
         cnt = np.empty(n, np.uint32) if want_counts else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), None)
         p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work)
-        _check(lib().sq_trace_rays(self.h, bvh.h, _ptr(rays), n, C.byref(p), C.byref(out), 0))
+        f = lib().sq_trace2_rays if isinstance(bvh, Bvh2) else lib().sq_trace_rays
+        _check(f(self.h, bvh.h, _ptr(rays), n, C.byref(p), C.byref(out), 0))
         return {"hit_id": hid, "hit_t": ht, "counts": cnt}
+
+    def node2(self, bvh):
+        """Converts a Node1 tree into a Node2 tree on the device."""
+        h = C.c_void_p()
+        _check(lib().sq_bvh2_create(self.h, bvh.h, C.byref(h)))
+        return Bvh2(self, h)

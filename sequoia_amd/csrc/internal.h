@@ -104,6 +104,17 @@ struct sq_bvh {
     bool view = false;   // part of a batch: storage owned by the batch
 };
 
+// A Node2 tree (node2.hip): nodes, triangles in leaf-group order with their
+// original indices, root record, breadth-first levels.
+struct sq_bvh2 {
+    sq_ctx* ctx = nullptr;
+    sq_node2* d_nodes = nullptr;
+    sq_triangle* d_tris = nullptr;
+    uint32_t* d_perm = nullptr;
+    uint32_t nodes = 0, tris = 0, levels = 0;
+    sq_node2_root root{};
+};
+
 namespace sq {
 
 // Phase timing helpers (events recorded on the context stream).

@@ -188,11 +188,104 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
     return st.fault == 0;
 }
 
+// Node2 trees: triangles of a leaf group; hit ids are original indices.
+__device__ __forceinline__ void leaf_tris2(const TraceArgs& a, uint32_t first, uint32_t count, LaneRay& L,
+                                           uint32_t& tri_tests, bool& bad) {
+    for (uint32_t j = 0; j < count; j++) {
+        const uint32_t k = first + j;
+        if (k >= a.tri_count) { bad = true; break; }
+        float v[9];
+        sqd::load_tri(a.tris, k, v);
+        float t;
+        tri_tests++;
+        if (tri_test(v, L.r, t) && t > L.r.tmin && t < L.r.tmax && t <= L.best_t) {
+            const uint32_t id = a.perm[k];
+            if (t < L.best_t || id < L.best_id) {
+                L.best_t = t;
+                L.best_id = id;
+            }
+        }
+    }
+}
+
+// Node2 step: the root box on the first step, then one Node2 (both children
+// from one 32-byte fetch) per step.
+template <bool CLOSEST>
+__device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+    if (L.cnt) {
+        L.cnt = 0;
+        L.tests += 1;
+        NodeV rb;
+        for (int k = 0; k < 3; k++) { rb.mn[k] = a.rb[k]; rb.mx[k] = a.rb[3 + k]; }
+        float f;
+        if (!slab(rb, L.r, L.ix, L.iy, L.iz, L.fast, f)) return false;
+        if (a.root_kind == 1) {
+            bool bad = false;
+            leaf_tris2(a, a.root, a.root_count, L, tri_tests, bad);
+            if (bad) st.fault |= kFaultTri;
+            return false;
+        }
+        if (a.root_kind != 2) return false;
+        push<CLOSEST>(st, a.root, f);
+        return st.fault == 0;
+    }
+    const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
+    uint32_t e = 0;
+    while (true) {
+        if (st.sp == 0) return false;
+        float te = 0.f;
+        e = pop<CLOSEST>(st, te);
+        if (CLOSEST && te > cull) continue;
+        break;
+    }
+    if (e >= a.node_count) { st.fault |= kFaultNode; return false; }
+    const float4* p = (const float4*)(a.nodes2 + e);
+    const float4 A = p[0], B = p[1];
+    const uint32_t w0 = __float_as_uint(A.w), w1 = __float_as_uint(B.x);
+    const uint32_t qw[3] = {__float_as_uint(B.y), __float_as_uint(B.z), __float_as_uint(B.w)};
+    const float c[3] = {A.x, A.y, A.z};
+    const float step = __uint_as_float((w0 >> 24) << 23);
+    NodeV na, nb;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        na.mn[k] = c[k] + (float)((int)(qw[k] & 0xFFu) - 128) * step;
+        na.mx[k] = c[k] + (float)((int)((qw[k] >> 8) & 0xFFu) - 128) * step;
+        nb.mn[k] = c[k] + (float)((int)((qw[k] >> 16) & 0xFFu) - 128) * step;
+        nb.mx[k] = c[k] + (float)((int)(qw[k] >> 24) - 128) * step;
+    }
+    L.tests += 2;
+    float fa, fb;
+    bool ha = slab(na, L.r, L.ix, L.iy, L.iz, L.fast, fa);
+    bool hb = slab(nb, L.r, L.ix, L.iy, L.iz, L.fast, fb);
+    if (CLOSEST) {
+        ha = ha && !(fa > cull);
+        hb = hb && !(fb > cull);
+    }
+    const uint32_t ca = w0 & 0xFFFFFFu, cb = w1 & 0xFFFFFFu;
+    const uint32_t cnta = (w1 >> 24) & 7u, cntb = (w1 >> 28) & 7u;
+    const bool la = (w1 >> 27) & 1u, lb = (w1 >> 31) & 1u;
+    bool bad = false;
+    if (ha && la) leaf_tris2(a, ca, cnta, L, tri_tests, bad);
+    if (hb && lb) leaf_tris2(a, cb, cntb, L, tri_tests, bad);
+    if (bad) { st.fault |= kFaultTri; return false; }
+    const bool pa = ha && !la, pb = hb && !lb;
+    if (pa && pb) {
+        const bool swap = CLOSEST && fb < fa;  // the nearer child is pushed last
+        push<CLOSEST>(st, swap ? ca : cb, swap ? fa : fb);
+        push<CLOSEST>(st, swap ? cb : ca, swap ? fb : fa);
+    } else if (pa) {
+        push<CLOSEST>(st, ca, fa);
+    } else if (pb) {
+        push<CLOSEST>(st, cb, fb);
+    }
+    return st.fault == 0;
+}
+
 constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode
 
 // One tile of job J: the wave's lanes take the tile's rays one after
 // another (per-lane refill) until all are traced.
-template <bool CLOSEST>
+template <bool CLOSEST, int FMT>
 __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, uint32_t lane, Stack& st,
                                            uint32_t* s_cnt, uint32_t& tests_total, uint32_t& tris_total,
                                            uint32_t& faults, uint32_t* job_fail) {
@@ -245,7 +338,10 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
             continue;
         }
         if (active) {
-            if (!(CLOSEST ? lane_step<CLOSEST>(a, L, st, tris_total) : lane_step_wide(a, L, st, tris_total))) {
+            const bool more = FMT == 2 ? lane_step2<CLOSEST>(a, L, st, tris_total)
+                                       : (CLOSEST ? lane_step<CLOSEST>(a, L, st, tris_total)
+                                                  : lane_step_wide(a, L, st, tris_total));
+            if (!more) {
                 // ray finished
                 if (st.fault) {
                     faults |= st.fault;
@@ -288,7 +384,7 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
 // MULTI: the launch covers a.njobs jobs (tree + rays + outputs each); tile t
 // of the launch is tile t - job_tiles[j] of the job j with job_tiles[j] <= t
 // < job_tiles[j + 1]. Waves steal tiles from one counter across all jobs.
-template <bool CLOSEST, bool MULTI>
+template <bool CLOSEST, bool MULTI, int FMT = 1>
 __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
     extern __shared__ uint32_t s_dyn[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -329,10 +425,10 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
                 else hi = mid;
             }
             const TraceArgs J = jobs[lo];
-            trace_tile<CLOSEST>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, faults,
-                                a.job_fail ? a.job_fail + lo : nullptr);
+            trace_tile<CLOSEST, FMT>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, faults,
+                                     a.job_fail ? a.job_fail + lo : nullptr);
         } else {
-            trace_tile<CLOSEST>(a, tile, lane, st, s_cnt, tests_total, tris_total, faults, nullptr);
+            trace_tile<CLOSEST, FMT>(a, tile, lane, st, s_cnt, tests_total, tris_total, faults, nullptr);
         }
     }
     if (faults) atomicOr(&a.work[1], faults);
@@ -413,14 +509,50 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
     return SQ_OK;
 }
 
+int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a) {
+    hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
+    if (L.closest) hipLaunchKernelGGL((k_trace<true, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+    else hipLaunchKernelGGL((k_trace<false, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+    SQ_CHECK_HIP(hipGetLastError());
+    return SQ_OK;
+}
+
 bool use_packets(const sq_trace_params* p) {
     const bool exhaustive = !p || p->semantics == SQ_TRACE_EXHAUSTIVE;
     const uint32_t k = p ? p->kernel : 0u;
     return exhaustive && k == 2u;
 }
 
-static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace_params* p) {
-    if (a.camera && use_packets(p)) {
+// Tree fields of the launch arguments (Node1 tree or Node2 tree).
+static int target_args(const sq_bvh* b, const sq_bvh2* t2, TraceArgs& a) {
+    if (b) {
+        if (!b->scene) SQ_RET(SQ_ERR_INVALID, "trace: the tree has no scene");
+        a.nodes = b->d_nodes;
+        a.tris = b->scene->d_tris;
+        a.root = b->root;
+        a.node_count = b->capacity;
+        a.tri_count = b->scene->n;
+        return SQ_OK;
+    }
+    a.nodes2 = t2->d_nodes;
+    a.perm = t2->d_perm;
+    a.tris = t2->d_tris;
+    a.root = t2->root.child;
+    a.root_kind = t2->root.kind;
+    a.root_count = t2->root.count;
+    for (int k = 0; k < 3; k++) {
+        a.rb[k] = t2->root.min[k];
+        a.rb[3 + k] = t2->root.max[k];
+    }
+    a.node_count = t2->nodes;
+    a.tri_count = t2->tris;
+    return SQ_OK;
+}
+
+static uint32_t target_depth(const sq_bvh* b, const sq_bvh2* t2) { return b ? b->depth : t2->levels + 2; }
+
+static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, const sq_trace_params* p) {
+    if (!node2 && a.camera && use_packets(p)) {
         SQ_TRY(c->counters.ensure(64));
         SQ_TRY(c->stats.ensure(64));
         a.work = c->counters.as<uint32_t>();
@@ -432,7 +564,7 @@ static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace
         return rc;
     }
     TracePlan L;
-    SQ_TRY(trace_plan(c, p, b->depth, a.camera != 0u, &L));
+    SQ_TRY(trace_plan(c, p, depth, a.camera != 0u, &L));
     SQ_TRY(c->spill.ensure(L.spill_words * 4));
     SQ_TRY(c->counters.ensure(64));
     SQ_TRY(c->stats.ensure(64));
@@ -442,7 +574,7 @@ static int trace_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& a, const sq_trace
     a.work = c->counters.as<uint32_t>();
     a.stats = (p && p->count_work) ? c->stats.as<unsigned long long>() : nullptr;
     phase_begin(c, PH_TRACE);
-    int rc = trace_launch_plan(c->stream, L, a, false);
+    int rc = node2 ? trace_launch_plan2(c->stream, L, a) : trace_launch_plan(c->stream, L, a, false);
     phase_end(c, PH_TRACE);
     return rc;
 }
@@ -469,24 +601,22 @@ using namespace sq;
 
 extern "C" {
 
-int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_trace_params* p,
-                    const sq_trace_out* out, int out_on_device) {
-    if (!c || !b || !cam || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: NULL argument");
+static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_camera* cam,
+                       const sq_trace_params* p, const sq_trace_out* out, int out_on_device) {
+    if (!c || !(b || t2) || !cam || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: NULL argument");
     if (cam->width == 0 || cam->height == 0 || cam->spp == 0) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: empty image");
     if (p && p->heatmap > SQ_HEATMAP_BUNDLE) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: heatmap mode %u", p->heatmap);
     if (p && p->kernel > 2) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: kernel %u", p->kernel);
     const bool heat = p && p->heatmap != SQ_HEATMAP_RAY;
     if (heat && (cam->spp != 1 || out->hit_id || out->hit_t))
         SQ_RET(SQ_ERR_UNSUPPORTED, "packet/bundle heatmaps take one sample per pixel and produce counts/rgb only");
+    if (t2 && (heat || (p && p->kernel == SQ_KERNEL_PACKETS)))
+        SQ_RET(SQ_ERR_UNSUPPORTED, "Node2 trees are traced by the per-ray kernel with ray heatmaps only");
     SQ_CHECK_HIP(hipSetDevice(c->device));
     phases_reset(c);
     const size_t npix = (size_t)cam->width * cam->height, nray = npix * cam->spp;
     TraceArgs a{};
-    a.nodes = b->d_nodes;
-    a.tris = b->scene->d_tris;
-    a.root = b->root;
-    a.node_count = b->capacity;
-    a.tri_count = b->scene->n;
+    SQ_TRY(target_args(b, t2, a));
     a.camera = 1;
     a.ox = cam->origin.x; a.oy = cam->origin.y; a.oz = cam->origin.z;
     a.tanh = cam->tan_half_fov;
@@ -497,7 +627,7 @@ int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_t
     if (out_on_device) {
         a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts; a.rgb = out->rgb;
         if (heat) SQ_TRY(heat_launch(c, b, a, p));
-        else SQ_TRY(trace_launch(c, b, a, p));
+        else SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
         return trace_finish(c, heat ? nullptr : p);
     }
     // Host outputs: stage through workspace buffers.
@@ -511,7 +641,7 @@ int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_t
     a.counts = out->counts ? (uint32_t*)(base + off_c) : nullptr;
     a.rgb = out->rgb ? (uint8_t*)(base + off_rgb) : nullptr;
     if (heat) SQ_TRY(heat_launch(c, b, a, p));
-    else SQ_TRY(trace_launch(c, b, a, p));
+    else SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
     if (out->hit_id) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_id, a.hit_id, nray * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->hit_t) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_t, a.hit_t, nray * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->counts) SQ_CHECK_HIP(hipMemcpyAsync(out->counts, a.counts, npix * 4, hipMemcpyDeviceToHost, c->stream));
@@ -519,26 +649,22 @@ int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_t
     return trace_finish(c, heat ? nullptr : p);
 }
 
-int sq_trace_rays(sq_ctx* c, const sq_bvh* b, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
-                  const sq_trace_out* out, int io_on_device) {
-    if (!c || !b || !rays || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_rays: NULL argument");
+static int rays_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray* rays, uint32_t n,
+                     const sq_trace_params* p, const sq_trace_out* out, int io_on_device) {
+    if (!c || !(b || t2) || !rays || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_rays: NULL argument");
     if (p && p->heatmap != SQ_HEATMAP_RAY) SQ_RET(SQ_ERR_UNSUPPORTED, "sq_trace_rays: heatmap mode %u", p->heatmap);
     if (n == 0) return SQ_OK;
     SQ_CHECK_HIP(hipSetDevice(c->device));
     phases_reset(c);
     TraceArgs a{};
-    a.nodes = b->d_nodes;
-    a.tris = b->scene->d_tris;
-    a.root = b->root;
-    a.node_count = b->capacity;
-    a.tri_count = b->scene->n;
+    SQ_TRY(target_args(b, t2, a));
     a.camera = 0;
     a.nrays = n;
     a.num_tiles = (n + kRayTile - 1) / kRayTile;
     if (io_on_device) {
         a.rays = rays;
         a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts;
-        SQ_TRY(trace_launch(c, b, a, p));
+        SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
         return trace_finish(c, p);
     }
     const size_t nr = n;
@@ -550,11 +676,35 @@ int sq_trace_rays(sq_ctx* c, const sq_bvh* b, const sq_ray* rays, uint32_t n, co
     a.hit_id = out->hit_id ? (uint32_t*)(base + off_id) : nullptr;
     a.hit_t = out->hit_t ? (float*)(base + off_t) : nullptr;
     a.counts = out->counts ? (uint32_t*)(base + off_c) : nullptr;
-    SQ_TRY(trace_launch(c, b, a, p));
+    SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
     if (out->hit_id) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_id, a.hit_id, nr * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->hit_t) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_t, a.hit_t, nr * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->counts) SQ_CHECK_HIP(hipMemcpyAsync(out->counts, a.counts, nr * 4, hipMemcpyDeviceToHost, c->stream));
     return trace_finish(c, p);
+}
+
+int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_trace_params* p,
+                    const sq_trace_out* out, int out_on_device) {
+    if (!b) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: NULL tree");
+    return camera_impl(c, b, nullptr, cam, p, out, out_on_device);
+}
+
+int sq_trace_rays(sq_ctx* c, const sq_bvh* b, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
+                  const sq_trace_out* out, int io_on_device) {
+    if (!b) SQ_RET(SQ_ERR_INVALID, "sq_trace_rays: NULL tree");
+    return rays_impl(c, b, nullptr, rays, n, p, out, io_on_device);
+}
+
+int sq_trace2_camera(sq_ctx* c, const sq_bvh2* t, const sq_camera* cam, const sq_trace_params* p,
+                     const sq_trace_out* out, int out_on_device) {
+    if (!t) SQ_RET(SQ_ERR_INVALID, "sq_trace2_camera: NULL tree");
+    return camera_impl(c, nullptr, t, cam, p, out, out_on_device);
+}
+
+int sq_trace2_rays(sq_ctx* c, const sq_bvh2* t, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
+                   const sq_trace_out* out, int io_on_device) {
+    if (!t) SQ_RET(SQ_ERR_INVALID, "sq_trace2_rays: NULL tree");
+    return rays_impl(c, nullptr, t, rays, n, p, out, io_on_device);
 }
 
 }  // extern "C"

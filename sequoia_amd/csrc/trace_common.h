@@ -34,6 +34,13 @@ struct TraceArgs {
     const uint32_t* job_tiles;
     uint32_t njobs;
     uint32_t* job_fail;   // optional: fault bits per job (njobs entries, zeroed by the caller)
+    // Node2 trees: nodes, original triangle ids of the reordered triangles,
+    // root box (tested as stored) and root kind (1: leaf group at `root`,
+    // root_count triangles; 2: Node2 `root`)
+    const sq_node2* nodes2;
+    const uint32_t* perm;
+    float rb[6];
+    uint32_t root_kind, root_count;
 };
 
 // Why a ray was abandoned (its results are then invalid).
@@ -210,6 +217,8 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
 int trace_fault(uint32_t bits, const char* what);
 // Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).
 int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi);
+// Same over a Node2 tree (a.nodes2, a.perm, root fields).
+int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a);
 
 // Wave-packet traversal of camera rays, exhaustive semantics (trace_packet.hip):
 // grid size for a launch, and the launch (zeroes a.work / a.stats first).
