@@ -215,8 +215,10 @@ typedef struct {
 } sq_node2_root;
 typedef struct sq_bvh2 sq_bvh2;
 /* Converts a device Node1 tree (needs leaves of at most 7 triangles, as every
- * builder here makes; breadth-first, deterministic). */
-int sq_bvh2_create(sq_ctx* ctx, const sq_bvh* bvh, sq_bvh2** out);
+ * builder here makes; breadth-first, deterministic). A list of several Node1
+ * leaves becomes one leaf group when it holds at most max_leaf triangles
+ * (1..7; 0 = 1, measured fastest), else a subtree of Node2s over them. */
+int sq_bvh2_create(sq_ctx* ctx, const sq_bvh* bvh, uint32_t max_leaf, sq_bvh2** out);
 int sq_bvh2_info(const sq_bvh2* t, uint32_t* nodes, uint32_t* tris, uint32_t* levels, sq_node2_root* root);
 /* Host copies of the nodes, the reordered triangles and their original indices (any may be NULL). */
 int sq_bvh2_download(const sq_bvh2* t, sq_node2* nodes, uint32_t node_cap, sq_triangle* tris, uint32_t* perm,

@@ -80,7 +80,7 @@ def oracle():
             "sqo_trace_packet": (None, [V, U, V, C.POINTER(Camera), U, V, V]),
             "sqo_trace_bundle": (None, [V, U, V, C.POINTER(Camera), U, V]),
             "sqo_heat_byte": (C.c_uint8, [U, U]),
-            "sqo_node2_convert": (C.c_int, [V, U, U, V, U, V, U, V, V, V, U, V, V]),
+            "sqo_node2_convert": (C.c_int, [V, U, U, V, U, U, V, U, V, V, V, U, V, V]),
             "sqo_node2_decode": (None, [V, V, V, V, V]),
             "sqo_node2_trace": (None, [V, V, V, V, V, U, U, V, V, V]),
         }
@@ -298,7 +298,7 @@ def ref_write_bmp(path, rgb):
     assert reference().ref_write_bmp(os.fsencode(path), w, h, _p(rgb)) == 1
 
 
-def node2_convert(nodes, root, tris):
+def node2_convert(nodes, root, tris, max_leaf=0):
     """Node1 tree -> (Node2 array, root record, reordered triangles, perm)."""
     nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
     tris = tris32(tris)
@@ -311,7 +311,8 @@ def node2_convert(nodes, root, tris):
         perm = np.zeros(tcap, np.uint32)
         nn, nt = C.c_uint32(), C.c_uint32()
         r = np.zeros(1, ROOT2_DTYPE)
-        rc = oracle().sqo_node2_convert(_p(nodes), len(nodes), root, _p(tris), len(tris), _p(out), cap, C.byref(nn),
+        rc = oracle().sqo_node2_convert(_p(nodes), len(nodes), root, _p(tris), len(tris), max_leaf, _p(out), cap,
+                                        C.byref(nn),
                                         _p(tris2), _p(perm), tcap, C.byref(nt), _p(r))
         if rc == -5:
             cap *= 2

@@ -21,8 +21,8 @@
 //     box always contains the child's Node1 box.
 // Conversion from a Node1 tree (deterministic breadth-first order): a list
 // of Node1 nodes with one interior node is replaced by that node's child list
-// (keeping the outer box); a list of leaves holding at most 7 triangles in
-// total becomes a leaf group; any other list becomes a Node2 whose halves
+// (keeping the outer box); a single leaf, or a list of leaves holding at most
+// max_leaf (1..7) triangles in total, becomes a leaf group; any other list becomes a Node2 whose halves
 // (first c/2 nodes, the rest) are its children. Node2 indices follow levels,
 // items in order, child a before b; leaf groups take triangle ranges in the
 // same order, the root's group first.
@@ -72,7 +72,8 @@ struct Child {
 };
 
 // A list of Node1 nodes as a child of a Node2 (or the root).
-int resolve(const sqo_node* nodes, uint32_t node_count, uint32_t f, uint32_t c, const B6& box, Child& out) {
+int resolve(const sqo_node* nodes, uint32_t node_count, uint32_t f, uint32_t c, const B6& box, uint32_t max_leaf,
+            Child& out) {
     for (uint32_t guard = 0; guard < 64 && c == 1; guard++) {
         if (f >= node_count) return -1;
         const sqo_node& x = nodes[f];
@@ -92,7 +93,7 @@ int resolve(const sqo_node* nodes, uint32_t node_count, uint32_t f, uint32_t c, 
     out.f = f;
     out.c = c;
     out.box = box;
-    if (all_leaves && tris <= 7) {
+    if (all_leaves && tris <= (c == 1 ? 7u : max_leaf)) {
         out.kind = KIND_LEAF;
         out.tris = tris;
         return 0;
@@ -152,9 +153,11 @@ void sqo_node2_decode(const sqo_node2* n, float a_min[3], float a_max[3], float 
 }
 
 int sqo_node2_convert(const sqo_node* nodes, uint32_t node_count, uint32_t root, const sqo_tri* tris,
-                      uint32_t ntris, sqo_node2* out, uint32_t node_cap, uint32_t* out_nodes, sqo_tri* tris2,
-                      uint32_t* perm, uint32_t tri_cap, uint32_t* out_tris, sqo_node2_root* root_out) {
+                      uint32_t ntris, uint32_t max_leaf, sqo_node2* out, uint32_t node_cap, uint32_t* out_nodes,
+                      sqo_tri* tris2, uint32_t* perm, uint32_t tri_cap, uint32_t* out_tris,
+                      sqo_node2_root* root_out) {
     if (root >= node_count) return -1;
+    max_leaf = max_leaf == 0 ? 1u : (max_leaf > 7 ? 7u : max_leaf);
     uint32_t tri_next = 0;
     auto emit_leaf = [&](const Child& ch, uint32_t* first) -> int {
         *first = tri_next;
@@ -173,7 +176,7 @@ int sqo_node2_convert(const sqo_node* nodes, uint32_t node_count, uint32_t root,
     };
     const B6 rbox = node_box(nodes[root]);
     Child rc;
-    int rc_ = resolve(nodes, node_count, root, 1, rbox, rc);
+    int rc_ = resolve(nodes, node_count, root, 1, rbox, max_leaf, rc);
     if (rc_) return rc_;
     std::memset(root_out, 0, sizeof *root_out);
     for (int k = 0; k < 3; k++) { root_out->min[k] = rbox.mn[k]; root_out->max[k] = rbox.mx[k]; }
@@ -199,7 +202,7 @@ int sqo_node2_convert(const sqo_node* nodes, uint32_t node_count, uint32_t root,
             const uint32_t hf[2] = {it.f, it.f + h}, hc[2] = {h, it.c - h};
             for (int s = 0; s < 2; s++) {
                 const B6 hb = list_box(nodes, hf[s], hc[s]);
-                if ((rc_ = resolve(nodes, node_count, hf[s], hc[s], hb, half[s]))) return rc_;
+                if ((rc_ = resolve(nodes, node_count, hf[s], hc[s], hb, max_leaf, half[s]))) return rc_;
             }
             sqo_node2& o = out[base + i];
             std::memset(&o, 0, sizeof o);

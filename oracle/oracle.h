@@ -108,8 +108,9 @@ typedef struct { float min[3], max[3]; uint32_t kind, child, count, pad; } sqo_n
  * Returns 0, or < 0 for an unsupported tree (leaf of > 7 triangles, index out
  * of range, non-finite box, capacity). */
 int sqo_node2_convert(const sqo_node* nodes, uint32_t node_count, uint32_t root, const sqo_tri* tris,
-                      uint32_t ntris, sqo_node2* out, uint32_t node_cap, uint32_t* out_nodes, sqo_tri* tris2,
-                      uint32_t* perm, uint32_t tri_cap, uint32_t* out_tris, sqo_node2_root* root_out);
+                      uint32_t ntris, uint32_t max_leaf, sqo_node2* out, uint32_t node_cap, uint32_t* out_nodes,
+                      sqo_tri* tris2, uint32_t* perm, uint32_t tri_cap, uint32_t* out_tris,
+                      sqo_node2_root* root_out);
 void sqo_node2_decode(const sqo_node2* n, float a_min[3], float a_max[3], float b_min[3], float b_max[3]);
 /* counts: box tests per ray (root + two per visited Node2); hits report original triangle ids. */
 void sqo_node2_trace(const sqo_node2* nodes, const sqo_node2_root* root, const sqo_tri* tris2, const uint32_t* perm,

@@ -176,7 +176,7 @@ _SIGS = {
     "sq_reference_camera": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Camera)]),
     "sq_pinhole_camera": (C.c_int, [_P, _U32, _U32, _U32, C.c_float, C.c_float, _U32, _U32, _U32,
                                     C.POINTER(Camera)]),
-    "sq_bvh2_create": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "sq_bvh2_create": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
     "sq_bvh2_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32), _P]),
     "sq_bvh2_download": (C.c_int, [_P, _P, _U32, _P, _P, _U32]),
     "sq_bvh2_free": (None, [_P]),
@@ -700,8 +700,8 @@ class Context:
         _check(f(self.h, bvh.h, _ptr(rays), n, C.byref(p), C.byref(out), 0))
         return {"hit_id": hid, "hit_t": ht, "counts": cnt}
 
-    def node2(self, bvh):
+    def node2(self, bvh, max_leaf=0):
         """Converts a Node1 tree into a Node2 tree on the device."""
         h = C.c_void_p()
-        _check(lib().sq_bvh2_create(self.h, bvh.h, C.byref(h)))
+        _check(lib().sq_bvh2_create(self.h, bvh.h, max_leaf, C.byref(h)))
         return Bvh2(self, h)

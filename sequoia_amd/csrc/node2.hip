@@ -19,7 +19,8 @@
 //     fetch instead of two.
 // Conversion (breadth-first, deterministic): a list holding one interior
 // Node1 node stands for that node's child list (keeping the outer box); a
-// list of leaves with at most 7 triangles becomes a leaf group; any other
+// single leaf, or a list of leaves with at most max_leaf (1..7) triangles in
+// all, becomes a leaf group; any other
 // list becomes a Node2 whose children are its first c/2 nodes and the rest.
 // Node2 indices follow levels, items in order, child a before b; leaf groups
 // take triangle ranges in the same order, the root's group first. One level
@@ -63,6 +64,7 @@ struct N2Args {
     uint32_t* perm;
     uint32_t* counters;  // [0] next items, [1] triangles of the level, [2] error code
     uint32_t node_cap, item_cap, tri_cap;
+    uint32_t max_leaf;   // largest leaf group made of several Node1 leaves (1..7)
     sq_node2_root* root;
 };
 
@@ -97,7 +99,7 @@ __device__ uint32_t n2_resolve(const N2Args& a, uint32_t f, uint32_t c, const fl
     out.f = f;
     out.c = c;
     for (int k = 0; k < 6; k++) out.box[k] = box[k];
-    if (all_leaves && tris <= 7) {
+    if (all_leaves && tris <= (c == 1 ? 7u : a.max_leaf)) {
         out.kind = 1;
         out.tris = tris;
         return 0;
@@ -334,7 +336,7 @@ static int n2_error(uint32_t code) {
 // Node1 tree -> Node2 tree on the device. A reference-layout tree can reach a
 // subtree through two paths (right-singleton lists); the conversion then
 // holds it twice, so the output arrays grow level by level as needed.
-int node2_convert(sq_ctx* c, const sq_bvh* b, sq_bvh2* out) {
+int node2_convert(sq_ctx* c, const sq_bvh* b, uint32_t max_leaf, sq_bvh2* out) {
     const uint32_t ntris = b->scene->n;
     size_t node_cap = b->capacity ? b->capacity : 1, tri_cap = ntris > 8 ? ntris : 8;
     SQ_CHECK_HIP(hipMalloc(&out->d_nodes, node_cap * sizeof(sq_node2)));
@@ -356,6 +358,9 @@ int node2_convert(sq_ctx* c, const sq_bvh* b, sq_bvh2* out) {
     a.perm = out->d_perm;
     a.node_cap = (uint32_t)node_cap;
     a.tri_cap = (uint32_t)tri_cap;
+    // default 1: larger groups trade box tests for triangle tests, which cost
+    // more (1M soup, closest hits: 2.11 ms at 1, 2.35 at 2, 3.33 at 7)
+    a.max_leaf = max_leaf == 0 ? 1u : (max_leaf > 7 ? 7u : max_leaf);
     sq::DevBuf* items[2] = {&c->aux[0], &c->aux[1]};
     uint32_t* h = (uint32_t*)c->pinned;
     hipLaunchKernelGGL(k_n2_root, dim3(1), dim3(64), 0, c->stream, a, b->root, items[0]->as<N2Item>());
@@ -433,14 +438,14 @@ using namespace sq;
 
 extern "C" {
 
-int sq_bvh2_create(sq_ctx* c, const sq_bvh* b, sq_bvh2** out) {
+int sq_bvh2_create(sq_ctx* c, const sq_bvh* b, uint32_t max_leaf, sq_bvh2** out) {
     if (!c || !b || !out) SQ_RET(SQ_ERR_INVALID, "sq_bvh2_create: NULL argument");
     if (!b->scene) SQ_RET(SQ_ERR_INVALID, "sq_bvh2_create: the tree has no scene");
     *out = nullptr;
     SQ_CHECK_HIP(hipSetDevice(c->device));
     sq_bvh2* t = new sq_bvh2();
     t->ctx = c;
-    int rc = node2_convert(c, b, t);
+    int rc = node2_convert(c, b, max_leaf, t);
     if (rc != SQ_OK) {
         sq_bvh2_free(t);
         return rc;

@@ -208,8 +208,70 @@ __device__ __forceinline__ void leaf_tris2(const TraceArgs& a, uint32_t first, u
     }
 }
 
+// Decoded child boxes of a Node2 (c + (q - 128) * 2^(size - 127)).
+struct Node2V {
+    NodeV a, b;
+    uint32_t ca, cb, cnta, cntb;
+    bool la, lb;
+};
+
+__device__ __forceinline__ Node2V load_node2(const sq_node2* nodes, uint32_t e) {
+    const float4* p = (const float4*)(nodes + e);
+    const float4 A = p[0], B = p[1];
+    const uint32_t w0 = __float_as_uint(A.w), w1 = __float_as_uint(B.x);
+    const uint32_t qw[3] = {__float_as_uint(B.y), __float_as_uint(B.z), __float_as_uint(B.w)};
+    const float c[3] = {A.x, A.y, A.z};
+    const float step = __uint_as_float((w0 >> 24) << 23);
+    Node2V n;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        n.a.mn[k] = c[k] + (float)((int)(qw[k] & 0xFFu) - 128) * step;
+        n.a.mx[k] = c[k] + (float)((int)((qw[k] >> 8) & 0xFFu) - 128) * step;
+        n.b.mn[k] = c[k] + (float)((int)((qw[k] >> 16) & 0xFFu) - 128) * step;
+        n.b.mx[k] = c[k] + (float)((int)(qw[k] >> 24) - 128) * step;
+    }
+    n.ca = w0 & 0xFFFFFFu;
+    n.cb = w1 & 0xFFFFFFu;
+    n.cnta = (w1 >> 24) & 7u;
+    n.cntb = (w1 >> 28) & 7u;
+    n.la = (w1 >> 27) & 1u;
+    n.lb = (w1 >> 31) & 1u;
+    return n;
+}
+
+// Exhaustive Node2 step: up to two Node2s (two independent 32-byte fetches)
+// per step, the current one and the next on the stack.
+__device__ __forceinline__ bool lane_step2_wide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+    float te = 0.f;
+    if (st.sp == 0) return false;
+    const uint32_t e0 = pop<false>(st, te);
+    const bool two = st.sp > 0;
+    const uint32_t e1 = two ? pop<false>(st, te) : 0u;
+    if (e0 >= a.node_count || (two && e1 >= a.node_count)) { st.fault |= kFaultNode; return false; }
+    const Node2V n0 = load_node2(a.nodes2, e0);
+    Node2V n1;
+    if (two) n1 = load_node2(a.nodes2, e1);
+    L.tests += two ? 4u : 2u;
+    float f;
+    const bool h0 = slab(n0.a, L.r, L.ix, L.iy, L.iz, L.fast, f);
+    const bool h1 = slab(n0.b, L.r, L.ix, L.iy, L.iz, L.fast, f);
+    const bool h2 = two && slab(n1.a, L.r, L.ix, L.iy, L.iz, L.fast, f);
+    const bool h3 = two && slab(n1.b, L.r, L.ix, L.iy, L.iz, L.fast, f);
+    bool bad = false;
+    if (h0 && n0.la) leaf_tris2(a, n0.ca, n0.cnta, L, tri_tests, bad);
+    if (h1 && n0.lb) leaf_tris2(a, n0.cb, n0.cntb, L, tri_tests, bad);
+    if (h2 && n1.la) leaf_tris2(a, n1.ca, n1.cnta, L, tri_tests, bad);
+    if (h3 && n1.lb) leaf_tris2(a, n1.cb, n1.cntb, L, tri_tests, bad);
+    if (bad) { st.fault |= kFaultTri; return false; }
+    if (h0 && !n0.la) push<false>(st, n0.ca, 0.f);
+    if (h1 && !n0.lb) push<false>(st, n0.cb, 0.f);
+    if (h2 && !n1.la) push<false>(st, n1.ca, 0.f);
+    if (h3 && !n1.lb) push<false>(st, n1.cb, 0.f);
+    return st.fault == 0;
+}
+
 // Node2 step: the root box on the first step, then one Node2 (both children
-// from one 32-byte fetch) per step.
+// from one 32-byte fetch) per step; exhaustive semantics take two.
 template <bool CLOSEST>
 __device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     if (L.cnt) {
@@ -229,6 +291,7 @@ __device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack
         push<CLOSEST>(st, a.root, f);
         return st.fault == 0;
     }
+    if (!CLOSEST) return lane_step2_wide(a, L, st, tri_tests);
     const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
     uint32_t e = 0;
     while (true) {
@@ -239,31 +302,17 @@ __device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack
         break;
     }
     if (e >= a.node_count) { st.fault |= kFaultNode; return false; }
-    const float4* p = (const float4*)(a.nodes2 + e);
-    const float4 A = p[0], B = p[1];
-    const uint32_t w0 = __float_as_uint(A.w), w1 = __float_as_uint(B.x);
-    const uint32_t qw[3] = {__float_as_uint(B.y), __float_as_uint(B.z), __float_as_uint(B.w)};
-    const float c[3] = {A.x, A.y, A.z};
-    const float step = __uint_as_float((w0 >> 24) << 23);
-    NodeV na, nb;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        na.mn[k] = c[k] + (float)((int)(qw[k] & 0xFFu) - 128) * step;
-        na.mx[k] = c[k] + (float)((int)((qw[k] >> 8) & 0xFFu) - 128) * step;
-        nb.mn[k] = c[k] + (float)((int)((qw[k] >> 16) & 0xFFu) - 128) * step;
-        nb.mx[k] = c[k] + (float)((int)(qw[k] >> 24) - 128) * step;
-    }
+    const Node2V n = load_node2(a.nodes2, e);
     L.tests += 2;
     float fa, fb;
-    bool ha = slab(na, L.r, L.ix, L.iy, L.iz, L.fast, fa);
-    bool hb = slab(nb, L.r, L.ix, L.iy, L.iz, L.fast, fb);
+    bool ha = slab(n.a, L.r, L.ix, L.iy, L.iz, L.fast, fa);
+    bool hb = slab(n.b, L.r, L.ix, L.iy, L.iz, L.fast, fb);
     if (CLOSEST) {
         ha = ha && !(fa > cull);
         hb = hb && !(fb > cull);
     }
-    const uint32_t ca = w0 & 0xFFFFFFu, cb = w1 & 0xFFFFFFu;
-    const uint32_t cnta = (w1 >> 24) & 7u, cntb = (w1 >> 28) & 7u;
-    const bool la = (w1 >> 27) & 1u, lb = (w1 >> 31) & 1u;
+    const uint32_t ca = n.ca, cb = n.cb, cnta = n.cnta, cntb = n.cntb;
+    const bool la = n.la, lb = n.lb;
     bool bad = false;
     if (ha && la) leaf_tris2(a, ca, cnta, L, tri_tests, bad);
     if (hb && lb) leaf_tris2(a, cb, cntb, L, tri_tests, bad);

@@ -28,13 +28,13 @@ CASES = [
 ]
 
 
-def convert_both(ctx, tris, kind, layout):
+def convert_both(ctx, tris, kind, layout, max_leaf=0):
     s = ctx.upload(tris)
     b = ctx.build(s, kind, layout=layout)
     nodes = b.download()
-    t2 = ctx.node2(b)
+    t2 = ctx.node2(b, max_leaf=max_leaf)
     got = t2.download()
-    want = ob.node2_convert(nodes, b.root, tris)
+    want = ob.node2_convert(nodes, b.root, tris, max_leaf=max_leaf)
     return s, b, t2, got, want
 
 
@@ -79,6 +79,21 @@ def test_node2_trace_matches_oracle(ctx, name, kind, layout):
     oc = ob.node2_trace(n2, r2, tris2, perm, ob.camera_rays(cam))
     assert np.array_equal(gc["counts"], oc["counts"].reshape(-1, 2).sum(1))
     check_hits(gc["hit_id"], gc["hit_t"], oc["hit_id"], oc["hit_t"])
+    t2.free(); b.free(); s.free()
+
+
+@pytest.mark.parametrize("max_leaf", [1, 2, 3])
+def test_node2_leaf_group_sizes(ctx, max_leaf):
+    say("node2 max_leaf", max_leaf)
+    tris = GOLDEN_SCENES["clus3000"]()
+    s, b, t2, got, want = convert_both(ctx, tris, "morton63", sq.LAYOUT_REFERENCE, max_leaf)
+    assert got[0].tobytes() == want[0].tobytes()
+    assert np.array_equal(got[3], want[3])
+    rays = sq.gen_secondary_rays(tris, 5000, seed=6, tmin=1e-4)
+    g = ctx.trace_rays(t2, rays)
+    o = ob.node2_trace(*want, rays)
+    assert np.array_equal(g["counts"], o["counts"])
+    check_hits(g["hit_id"], g["hit_t"], o["hit_id"], o["hit_t"])
     t2.free(); b.free(); s.free()
 
 
