@@ -11,7 +11,9 @@ resident in HBM. `value` is Mrays/s of the trace (HIP events on the library
 stream around the traversal kernel, summed over the timed steps).
 
 The same JSON line carries more sections: "sah32" (the same soup built with
-32-bin binned SAH and the same rays traced on that tree), "incoherent" (10M
+32-bin binned SAH and the same rays traced on that tree), "closest" (the
+headline frame's closest hits through the Morton tree and through its
+quantised two-child Node2 conversion), "incoherent" (10M
 clustered triangles built with the grid and binned-SAH builders, 8M
 cosine-weighted secondary rays traced to their closest hits), "batched" (256
 scenes built and traced in one captured graph against 256 sequential plain
@@ -108,6 +110,37 @@ def sah_section(sq, ctx, scene, cam, nrays, reps=3):
     bms, tm = float(np.median(ms)), float(np.median(tms))
     return {"build_ms": round(bms, 3), "build_mtri_s": round(N_TRI / (bms / 1e3) / 1e6, 1), "tree_depth": depth,
             "trace_ms": round(tm, 3), "mrays_s": round(nrays / (tm / 1e3) / 1e6, 1)}
+
+
+def closest_section(sq, ctx, scene, cam, nrays, reps=3):
+    """Closest hits of the headline frame through the Morton63 tree (Node1)
+    and through its quantised two-child conversion (Node2), device outputs;
+    the Node2 conversion time is reported with it."""
+    log("closest hits: Node1 and Node2")
+    b = ctx.build(scene, "morton63")
+    t0 = time.perf_counter()
+    t2 = ctx.node2(b)
+    conv_ms = (time.perf_counter() - t0) * 1e3
+    d_id, d_t = ctx.alloc(nrays * 4), ctx.alloc(nrays * 4)
+    out = {"workload": "headline frame, closest-hit semantics"}
+    try:
+        for tree, name in ((b, "node1"), (t2, "node2")):
+            ms = []
+            for r in range(reps + 1):
+                ctx.trace_camera_device(tree, cam, d_id, d_t, None, None, semantics=sq.TRACE_CLOSEST)
+                if r:
+                    ms.append(ctx.timings()["trace_ms"])
+            m = float(np.median(ms))
+            out[name + "_trace_ms"] = round(m, 3)
+            out[name + "_mrays_s"] = round(nrays / (m / 1e3) / 1e6, 1)
+        out["node2_convert_ms"] = round(conv_ms, 2)
+        out["node2_nodes"] = t2.info()["nodes"]
+    finally:
+        ctx.free(d_id)
+        ctx.free(d_t)
+        t2.free()
+        b.free()
+    return out
 
 
 INC_TRI, INC_RAYS, INC_SUB = 10_000_000, 8_000_000, 100_000
@@ -509,6 +542,11 @@ def main():
             line["sah32"] = sah_section(sq, ctx, scene, cam, nrays)
         except Exception as e:  # noqa: BLE001
             line["sah32"] = {"error": repr(e)}
+    if not a.no_sah:
+        try:
+            line["closest"] = closest_section(sq, ctx, scene, cam, nrays)
+        except Exception as e:  # noqa: BLE001
+            line["closest"] = {"error": repr(e)}
     if not a.no_incoherent:
         try:
             line["incoherent"] = incoherent_section(sq, ctx)

@@ -172,6 +172,8 @@ typedef struct {
     uint32_t kernel;      /* camera rays: SQ_KERNEL_* (packets: exhaustive semantics only) */
     uint32_t blocks_per_cu; /* per-ray kernel: 0 = default (8 waves per CU for one camera's rays, 16
                                otherwise, at most what fits); else 1..64 */
+    uint32_t tri_test;    /* 0: reference Moller-Trumbore; 1: precomputed triangles (sq_bvh_plucker) */
+    uint32_t reserved;
 } sq_trace_params;
 
 /* Outputs (any may be NULL). hit_id/hit_t: one per ray (SQ_MISS / +inf on
@@ -191,6 +193,27 @@ int sq_trace_camera(sq_ctx* ctx, const sq_bvh* bvh, const sq_camera* cam, const 
 /* rays: host (io_on_device = 0) or device (1) array of n rays. */
 int sq_trace_rays(sq_ctx* ctx, const sq_bvh* bvh, const sq_ray* rays, uint32_t n, const sq_trace_params* params,
                   const sq_trace_out* out, int io_on_device);
+
+/* ---- precomputed triangles (PluckerTriangle, aabb.h:182-241) ----
+ * The reference declares the 48-byte record and leaves its conversion
+ * commented out, with no intersection routine. Here it is the projection
+ * method of that layout (Wald 2004, 7.1): the plane as P[w] + nu P[u] +
+ * nv P[v] = np along the dominant normal axis w = ci, and the two edges
+ * scaled by the projected determinant (e0u, e0v, e1u, e1v). A hit needs
+ * beta, gamma >= 0, beta + gamma <= 1, t > 1e-9, tmin < t < tmax. It changes
+ * hit arithmetic: t differs from Moller-Trumbore in the last bits and hits
+ * within rounding of an edge may differ; box tests (counts) do not change. */
+typedef struct {
+    float nu, nv, np, pu, pv;
+    int32_t ci;               /* dominant normal axis; 3 = degenerate, never hit */
+    float e0u, e0v, e1u, e1v;
+    int32_t pad0, pad1;
+} sq_plucker;
+/* Precomputes the records of the tree's scene on the device (kept with the
+ * tree, as the reference's Hierarchy::plucker_tri_base; freed with it);
+ * *out receives their device address (may be NULL). Traces with
+ * sq_trace_params.tri_test = 1 then use them (per-ray kernel, Node1 trees). */
+int sq_bvh_plucker(sq_ctx* ctx, sq_bvh* bvh, const sq_plucker** out);
 
 /* ---- quantised two-child nodes (Node2, aabb.h:44-74) ----
  * The reference declares Node2 (a float centre, 24-bit child fields, an 8-bit

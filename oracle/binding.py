@@ -16,6 +16,9 @@ REF_SO = os.path.join(HERE, "_ref", "libseqref.so")
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 
+PLUCKER_DTYPE = np.dtype([("nu", "<f4"), ("nv", "<f4"), ("np", "<f4"), ("pu", "<f4"), ("pv", "<f4"), ("ci", "<i4"),
+                          ("e0u", "<f4"), ("e0v", "<f4"), ("e1u", "<f4"), ("e1v", "<f4"), ("pad0", "<i4"),
+                          ("pad1", "<i4")])
 NODE2_DTYPE = np.dtype([("c", "<f4", 3), ("w0", "<u4"), ("w1", "<u4"), ("q", "u1", 12)])
 ROOT2_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("kind", "<u4"), ("child", "<u4"), ("count", "<u4"),
                         ("pad", "<u4")])
@@ -83,6 +86,9 @@ def oracle():
             "sqo_node2_convert": (C.c_int, [V, U, U, V, U, U, V, U, V, V, V, U, V, V]),
             "sqo_node2_decode": (None, [V, V, V, V, V]),
             "sqo_node2_trace": (None, [V, V, V, V, V, U, U, V, V, V]),
+            "sqo_plucker_precompute": (None, [V, U, V]),
+            "sqo_plucker_test": (C.c_int, [V, V, V]),
+            "sqo_trace_rays_plucker": (None, [V, U, V, V, U, V, V, V]),
         }
         for k, (res, args) in sig.items():
             f = getattr(_o, k)
@@ -344,4 +350,30 @@ def node2_trace(n2, root, tris2, perm, rays, closest=False):
     counts, hid, ht = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.float32)
     oracle().sqo_node2_trace(_p(n2), _p(r), _p(tris2), _p(perm), _p(rays), n, 1 if closest else 0, _p(counts),
                              _p(hid), _p(ht))
+    return {"counts": counts, "hit_id": hid, "hit_t": ht}
+
+
+def plucker_precompute(tris):
+    tris = tris32(tris)
+    out = np.zeros(len(tris), PLUCKER_DTYPE)
+    oracle().sqo_plucker_precompute(_p(tris), len(tris), _p(out))
+    return out
+
+
+def plucker_test(rec, ray):
+    """(hit, t) of one precomputed triangle and one ray."""
+    rec = np.ascontiguousarray(rec, PLUCKER_DTYPE).reshape(1)
+    ray = np.ascontiguousarray(ray, RAY_DTYPE).reshape(1)
+    t = C.c_float()
+    h = oracle().sqo_plucker_test(_p(rec), _p(ray), C.byref(t))
+    return bool(h), t.value
+
+
+def trace_rays_plucker(nodes, root, pl, rays):
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    pl = np.ascontiguousarray(pl, PLUCKER_DTYPE)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = len(rays)
+    counts, hid, ht = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.float32)
+    oracle().sqo_trace_rays_plucker(_p(nodes), root, _p(pl), _p(rays), n, _p(counts), _p(hid), _p(ht))
     return {"counts": counts, "hit_id": hid, "hit_t": ht}

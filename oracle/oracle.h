@@ -117,6 +117,18 @@ void sqo_node2_trace(const sqo_node2* nodes, const sqo_node2_root* root, const s
                      const sqo_ray* rays, uint32_t nrays, uint32_t closest, uint32_t* counts, uint32_t* hit_id,
                      float* hit_t);
 
+/* ---- Precomputed-triangle (PluckerTriangle, aabb.h:182-195) test; plucker.cpp ---- */
+typedef struct {
+    float nu, nv, np, pu, pv;
+    int32_t ci;                 /* dominant normal axis, 3 = never hit */
+    float e0u, e0v, e1u, e1v;
+    int32_t pad0, pad1;
+} sqo_plucker;                  /* 48 B */
+void sqo_plucker_precompute(const sqo_tri* tris, uint32_t n, sqo_plucker* out);
+int sqo_plucker_test(const sqo_plucker* p, const sqo_ray* r, float* t_out);
+void sqo_trace_rays_plucker(const sqo_node* nodes, uint32_t root, const sqo_plucker* pl, const sqo_ray* rays,
+                            uint32_t nrays, uint32_t* counts, uint32_t* hit_id, float* hit_t);
+
 /* Sizing helpers. */
 uint32_t sqo_sah_node_capacity(uint32_t n);
 uint32_t sqo_grid_node_capacity(uint32_t n, uint32_t grid_dim);

@@ -30,6 +30,9 @@ MISS = 0xFFFFFFFF
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
 NODE2_DTYPE = np.dtype([("c", "<f4", 3), ("w0", "<u4"), ("w1", "<u4"), ("q", "u1", 12)])
+PLUCKER_DTYPE = np.dtype([("nu", "<f4"), ("nv", "<f4"), ("np", "<f4"), ("pu", "<f4"), ("pv", "<f4"), ("ci", "<i4"),
+                          ("e0u", "<f4"), ("e0v", "<f4"), ("e1u", "<f4"), ("e1v", "<f4"), ("pad0", "<i4"),
+                          ("pad1", "<i4")])
 assert NODE_DTYPE.itemsize == 32 and RAY_DTYPE.itemsize == 32
 
 
@@ -58,7 +61,8 @@ class BuildParams(C.Structure):
 class TraceParams(C.Structure):
     _fields_ = [("semantics", C.c_uint32), ("heatmap", C.c_uint32), ("width", C.c_uint32),
                 ("block_size", C.c_uint32), ("stack_slots", C.c_uint32), ("count_work", C.c_uint32),
-                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("tri_test", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class TraceOut(C.Structure):
@@ -69,7 +73,8 @@ class TuneResult(C.Structure):
     _fields_ = [("radix_digit_bits", C.c_uint32), ("subtree_threshold", C.c_uint32), ("stack_slots", C.c_uint32),
                 ("block_size", C.c_uint32), ("build_ms", C.c_float), ("trace_ms", C.c_float),
                 ("default_build_ms", C.c_float), ("default_trace_ms", C.c_float), ("candidates_tried", C.c_uint32),
-                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("tri_test", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -176,6 +181,7 @@ _SIGS = {
     "sq_reference_camera": (C.c_int, [_P, _U32, _U32, _U32, C.POINTER(Camera)]),
     "sq_pinhole_camera": (C.c_int, [_P, _U32, _U32, _U32, C.c_float, C.c_float, _U32, _U32, _U32,
                                     C.POINTER(Camera)]),
+    "sq_bvh_plucker": (C.c_int, [_P, _P, C.POINTER(_P)]),
     "sq_bvh2_create": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
     "sq_bvh2_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32), _P]),
     "sq_bvh2_download": (C.c_int, [_P, _P, _U32, _P, _P, _U32]),
@@ -631,7 +637,8 @@ class Context:
         """Trace with outputs already in device memory (no host copies)."""
         out = TraceOut(hit_id, hit_t, counts, rgb)
         p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work, kernel)
-        _check(lib().sq_trace_camera(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 1))
+        f = lib().sq_trace2_camera if isinstance(bvh, Bvh2) else lib().sq_trace_camera
+        _check(f(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 1))
 
     def upload(self, tris):
         tris = as_tris(tris)
@@ -668,11 +675,11 @@ class Context:
         _check(lib().sq_refit(self.h, bvh.h))
 
     @staticmethod
-    def _params(semantics, heatmap, width, stack_slots, count_work, kernel=0):
-        return TraceParams(semantics, heatmap, width, 0, stack_slots, 1 if count_work else 0, kernel, 0)
+    def _params(semantics, heatmap, width, stack_slots, count_work, kernel=0, tri_test=0):
+        return TraceParams(semantics, heatmap, width, 0, stack_slots, 1 if count_work else 0, kernel, 0, tri_test, 0)
 
     def trace_camera(self, bvh, cam, semantics=TRACE_EXHAUSTIVE, heatmap=HEATMAP_RAY, width=0, want_hits=True,
-                     want_counts=True, want_rgb=True, stack_slots=0, count_work=False, kernel=0):
+                     want_counts=True, want_rgb=True, stack_slots=0, count_work=False, kernel=0, tri_test=0):
         npix = cam.width * cam.height
         nray = npix * cam.spp
         res = {}
@@ -681,13 +688,14 @@ class Context:
         cnt = np.empty(npix, np.uint32) if want_counts else None
         rgb = np.empty((cam.height, cam.width, 3), np.uint8) if want_rgb else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), _ptr(rgb))
-        p = self._params(semantics, heatmap, width, stack_slots, count_work, kernel)
+        p = self._params(semantics, heatmap, width, stack_slots, count_work, kernel, tri_test)
         f = lib().sq_trace2_camera if isinstance(bvh, Bvh2) else lib().sq_trace_camera
         _check(f(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 0))
         res.update(hit_id=hid, hit_t=ht, counts=cnt, rgb=rgb)
         return res
 
-    def trace_rays(self, bvh, rays, semantics=TRACE_EXHAUSTIVE, want_counts=True, stack_slots=0, count_work=False):
+    def trace_rays(self, bvh, rays, semantics=TRACE_EXHAUSTIVE, want_counts=True, stack_slots=0, count_work=False,
+                   tri_test=0):
         """Ray list through a Node1 tree (Bvh) or a Node2 tree (Bvh2)."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         n = len(rays)
@@ -695,10 +703,17 @@ class Context:
         ht = np.empty(n, np.float32)
         cnt = np.empty(n, np.uint32) if want_counts else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), None)
-        p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work)
+        p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work, 0, tri_test)
         f = lib().sq_trace2_rays if isinstance(bvh, Bvh2) else lib().sq_trace_rays
         _check(f(self.h, bvh.h, _ptr(rays), n, C.byref(p), C.byref(out), 0))
         return {"hit_id": hid, "hit_t": ht, "counts": cnt}
+
+    def plucker(self, bvh):
+        """Precomputes the tree's triangles for tri_test=1; returns them (host copy)."""
+        p = C.c_void_p()
+        _check(lib().sq_bvh_plucker(self.h, bvh.h, C.byref(p)))
+        n = lib().sq_scene_count(bvh.scene.h)
+        return self.to_host(p.value, n * PLUCKER_DTYPE.itemsize, np.uint8).view(PLUCKER_DTYPE)
 
     def node2(self, bvh, max_leaf=0):
         """Converts a Node1 tree into a Node2 tree on the device."""

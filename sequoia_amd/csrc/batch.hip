@@ -522,6 +522,7 @@ int sq_batch_set_cameras(sq_ctx* c, sq_batch* b, const sq_camera* cams, const sq
     if (!c || !b || !cams || b->ctx != c) SQ_RET(SQ_ERR_INVALID, "sq_batch_set_cameras: bad argument");
     if (p && p->heatmap != SQ_HEATMAP_RAY) SQ_RET(SQ_ERR_UNSUPPORTED, "batches trace per-ray heatmaps only");
     if (p && p->kernel > 2) SQ_RET(SQ_ERR_INVALID, "sq_batch_set_cameras: kernel %u", p->kernel);
+    if (p && p->tri_test) SQ_RET(SQ_ERR_UNSUPPORTED, "batches use the reference triangle test");
     if (outputs & ~0xFu) SQ_RET(SQ_ERR_INVALID, "sq_batch_set_cameras: output mask %#x", outputs);
     SQ_CHECK_HIP(hipSetDevice(c->device));
     sq_trace_params tp{};

@@ -411,6 +411,7 @@ const sq_node* sq_bvh_device_nodes(const sq_bvh* b) { return b ? b->d_nodes : nu
 void sq_bvh_free(sq_bvh* b) {
     if (!b || b->view) return;
     if (b->d_nodes) (void)hipFree(b->d_nodes);
+    if (b->d_plucker) (void)hipFree(b->d_plucker);
     delete b;
 }
 

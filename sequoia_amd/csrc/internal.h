@@ -102,6 +102,7 @@ struct sq_bvh {
     uint32_t root = 0;
     uint32_t depth = 0;  // levels from the root to the deepest node (root = 1)
     bool view = false;   // part of a batch: storage owned by the batch
+    sq_plucker* d_plucker = nullptr;  // precomputed triangles (sq_bvh_plucker)
 };
 
 // A Node2 tree (node2.hip): nodes, triangles in leaf-group order with their

@@ -57,10 +57,15 @@ __device__ __forceinline__ void leaf_tris(const TraceArgs& a, const NodeV& n, La
         const uint32_t tid = n.child + j;
         if (tid >= a.tri_count) { bad = true; break; }
         float v[9];
-        sqd::load_tri(a.tris, tid, v);
         float t;
         tri_tests++;
-        bool m = tri_test(v, L.r, t);
+        bool m;
+        if (a.plk) {
+            m = plucker_test(a.plk, tid, L.r, t);
+        } else {
+            sqd::load_tri(a.tris, tid, v);
+            m = tri_test(v, L.r, t);
+        }
         if (m && t > L.r.tmin && t < L.r.tmax && (t < L.best_t || (t == L.best_t && tid < L.best_id))) {
             L.best_t = t;
             L.best_id = tid;
@@ -573,7 +578,8 @@ bool use_packets(const sq_trace_params* p) {
 }
 
 // Tree fields of the launch arguments (Node1 tree or Node2 tree).
-static int target_args(const sq_bvh* b, const sq_bvh2* t2, TraceArgs& a) {
+static int target_args(const sq_bvh* b, const sq_bvh2* t2, const sq_trace_params* p, TraceArgs& a) {
+    if (p && p->tri_test > 1) SQ_RET(SQ_ERR_INVALID, "trace: tri_test %u", p->tri_test);
     if (b) {
         if (!b->scene) SQ_RET(SQ_ERR_INVALID, "trace: the tree has no scene");
         a.nodes = b->d_nodes;
@@ -581,8 +587,13 @@ static int target_args(const sq_bvh* b, const sq_bvh2* t2, TraceArgs& a) {
         a.root = b->root;
         a.node_count = b->capacity;
         a.tri_count = b->scene->n;
+        if (p && p->tri_test == 1) {
+            if (!b->d_plucker) SQ_RET(SQ_ERR_INVALID, "trace: tri_test 1 needs sq_bvh_plucker first");
+            a.plk = b->d_plucker;
+        }
         return SQ_OK;
     }
+    if (p && p->tri_test) SQ_RET(SQ_ERR_UNSUPPORTED, "trace: Node2 trees use the reference triangle test");
     a.nodes2 = t2->d_nodes;
     a.perm = t2->d_perm;
     a.tris = t2->d_tris;
@@ -661,11 +672,13 @@ static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_c
         SQ_RET(SQ_ERR_UNSUPPORTED, "packet/bundle heatmaps take one sample per pixel and produce counts/rgb only");
     if (t2 && (heat || (p && p->kernel == SQ_KERNEL_PACKETS)))
         SQ_RET(SQ_ERR_UNSUPPORTED, "Node2 trees are traced by the per-ray kernel with ray heatmaps only");
+    if (p && p->tri_test && (heat || p->kernel == SQ_KERNEL_PACKETS))
+        SQ_RET(SQ_ERR_UNSUPPORTED, "precomputed triangles are tested by the per-ray kernel only");
     SQ_CHECK_HIP(hipSetDevice(c->device));
     phases_reset(c);
     const size_t npix = (size_t)cam->width * cam->height, nray = npix * cam->spp;
     TraceArgs a{};
-    SQ_TRY(target_args(b, t2, a));
+    SQ_TRY(target_args(b, t2, p, a));
     a.camera = 1;
     a.ox = cam->origin.x; a.oy = cam->origin.y; a.oz = cam->origin.z;
     a.tanh = cam->tan_half_fov;
@@ -706,7 +719,7 @@ static int rays_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray
     SQ_CHECK_HIP(hipSetDevice(c->device));
     phases_reset(c);
     TraceArgs a{};
-    SQ_TRY(target_args(b, t2, a));
+    SQ_TRY(target_args(b, t2, p, a));
     a.camera = 0;
     a.nrays = n;
     a.num_tiles = (n + kRayTile - 1) / kRayTile;

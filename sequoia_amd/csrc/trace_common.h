@@ -41,6 +41,7 @@ struct TraceArgs {
     const uint32_t* perm;
     float rb[6];
     uint32_t root_kind, root_count;
+    const sq_plucker* plk;  // non-null: precomputed triangle test
 };
 
 // Why a ray was abandoned (its results are then invalid).
@@ -103,6 +104,31 @@ __device__ __forceinline__ bool tri_test(const float v[9], const RayF& r, float&
     miss |= (vv < 0.0f) | ((u + vv) > 1.0f);
     t = f * __fmaf_rn(e2z, qz, __fmaf_rn(e2y, qy, e2x * qx));
     return (!miss) & (t > eps);
+}
+
+// Precomputed-triangle test (projection method; layout of the reference's
+// PluckerTriangle): plane along the dominant axis w, edges scaled by the
+// projected determinant. No fused multiply-adds: the CPU restatement and the
+// precompute use the same plain operations.
+__device__ __forceinline__ bool plucker_test(const sq_plucker* __restrict__ pl, uint32_t i, const RayF& r, float& t) {
+    const float4* p = (const float4*)(pl + i);
+    const float4 A = p[0], B = p[1], C = p[2];
+    const float nu = A.x, nv = A.y, np = A.z, pu = A.w, pv = B.x;
+    const int w = __float_as_int(B.y);
+    const float e0u = B.z, e0v = B.w, e1u = C.x, e1v = C.y;
+    t = __int_as_float(0x7f800000);
+    if (w > 2 || w < 0) return false;
+    const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+    const int u = w == 0 ? 1 : 0, v = w == 2 ? 1 : 2;
+    const float ow = w == 0 ? o[0] : (w == 1 ? o[1] : o[2]), dw = w == 0 ? d[0] : (w == 1 ? d[1] : d[2]);
+    const float ou = u == 0 ? o[0] : o[1], du = u == 0 ? d[0] : d[1];
+    const float ov = v == 1 ? o[1] : o[2], dv = v == 1 ? d[1] : d[2];
+    const float den = (dw + nu * du) + nv * dv;
+    if (den == 0.0f) return false;
+    t = (np - ((ow + nu * ou) + nv * ov)) / den;
+    const float hu = (ou + t * du) - pu, hv = (ov + t * dv) - pv;
+    const float beta = hu * e1v - hv * e1u, gamma = hv * e0u - hu * e0v;
+    return (beta >= 0.0f) & (gamma >= 0.0f) & ((beta + gamma) <= 1.0f) & (t > 0.000000001f);
 }
 
 struct NodeV {
