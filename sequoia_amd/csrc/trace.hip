@@ -96,9 +96,8 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
     const uint32_t i0 = L.first;
     const bool two = L.cnt >= 2;
     if (i0 >= a.node_count || (two && i0 + 1 >= a.node_count)) { st.fault |= kFaultNode; return false; }
-    NodeV n0 = load_node(a.nodes, i0);
-    NodeV n1;
-    if (two) n1 = load_node(a.nodes, i0 + 1);
+    const NodeV n0 = load_node(a.nodes, i0);
+    const NodeV n1 = load_node(a.nodes, i0 + (two ? 1u : 0u));  // unconditional: no branch between fetches
     L.first += two ? 2 : 1;
     L.cnt -= two ? 2 : 1;
     L.tests += two ? 2 : 1;
@@ -170,10 +169,13 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
         st.fault |= kFaultNode;
         return false;
     }
-    NodeV n0 = load_node(a.nodes, ia), n1, n2, n3;
-    if (na == 2) n1 = load_node(a.nodes, ia + 1);
-    if (nb >= 1) n2 = load_node(a.nodes, ib);
-    if (nb == 2) n3 = load_node(a.nodes, ib + 1);
+    // all four fetches unconditional (in-bounds indices; unused ones repeat a
+    // node) so that they issue back to back without branches between them
+    const uint32_t ib0 = nb ? ib : ia;
+    const NodeV n0 = load_node(a.nodes, ia);
+    const NodeV n1 = load_node(a.nodes, ia + (na == 2 ? 1u : 0u));
+    const NodeV n2 = load_node(a.nodes, ib0);
+    const NodeV n3 = load_node(a.nodes, ib0 + (nb == 2 ? 1u : 0u));
     L.tests += na + nb;
     float f;
     const bool h0 = slab(n0, L.r, L.ix, L.iy, L.iz, L.fast, f);
@@ -254,8 +256,7 @@ __device__ __forceinline__ bool lane_step2_wide(const TraceArgs& a, LaneRay& L, 
     const uint32_t e1 = two ? pop<false>(st, te) : 0u;
     if (e0 >= a.node_count || (two && e1 >= a.node_count)) { st.fault |= kFaultNode; return false; }
     const Node2V n0 = load_node2(a.nodes2, e0);
-    Node2V n1;
-    if (two) n1 = load_node2(a.nodes2, e1);
+    const Node2V n1 = load_node2(a.nodes2, two ? e1 : e0);  // unconditional: no branch between fetches
     L.tests += two ? 4u : 2u;
     float f;
     const bool h0 = slab(n0.a, L.r, L.ix, L.iy, L.iz, L.fast, f);
