@@ -130,11 +130,13 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
     return true;
 }
 
-// Exhaustive semantics: one step tests up to four nodes, the next two of the
-// current list and the next two after them (from the same list or, once it
-// is used up, from the list on top of the stack when that names its nodes
-// directly), so a lane has two independent node fetches in flight instead of
-// one. Counts and the closest hit do not depend on the visiting order.
+// Exhaustive semantics: one step tests up to kWideLists x 2 nodes, the next
+// two of the current list and the next two after them (from the same list or,
+// once it is used up, from the list on top of the stack when that names its
+// nodes directly), so a lane has several independent node fetches in flight
+// instead of one. Counts and the closest hit do not depend on the order.
+constexpr int kWideLists = 2;  // three lists measured slower with 16 stack slots (more spills)
+
 __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     while (L.cnt == 0) {
         if (st.sp == 0) return false;
@@ -150,48 +152,56 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
             L.cnt = p.count;
         }
     }
-    const uint32_t ia = L.first, na = L.cnt >= 2 ? 2u : 1u;
-    L.first += na;
-    L.cnt -= na;
-    if (L.cnt == 0 && st.sp > 0) {
-        const uint32_t top = st.lds[((st.sp - 1) & st.mask) * st.stride];
-        if ((top & 15u) != 15u) {
-            float te = 0.f;
-            (void)pop<false>(st, te);
-            L.first = top >> 4;
-            L.cnt = (top & 15u) + 1;
+    uint32_t idx[kWideLists], num[kWideLists];
+#pragma unroll
+    for (int g = 0; g < kWideLists; g++) {
+        if (g > 0 && L.cnt == 0 && st.sp > 0) {
+            const uint32_t top = st.lds[((st.sp - 1) & st.mask) * st.stride];
+            if ((top & 15u) != 15u) {
+                float te = 0.f;
+                (void)pop<false>(st, te);
+                L.first = top >> 4;
+                L.cnt = (top & 15u) + 1;
+            }
         }
+        idx[g] = L.first;
+        num[g] = L.cnt >= 2 ? 2u : L.cnt;
+        L.first += num[g];
+        L.cnt -= num[g];
+        if (num[g] && (idx[g] >= a.node_count || num[g] > a.node_count - idx[g])) {
+            st.fault |= kFaultNode;
+            return false;
+        }
+        if (num[g] == 0) idx[g] = idx[0];  // in-bounds repeat, never tested
     }
-    const uint32_t ib = L.first, nb = L.cnt >= 2 ? 2u : L.cnt;
-    L.first += nb;
-    L.cnt -= nb;
-    if (ia >= a.node_count || na > a.node_count - ia || (nb && (ib >= a.node_count || nb > a.node_count - ib))) {
-        st.fault |= kFaultNode;
-        return false;
+    // all fetches unconditional (in-bounds indices; unused ones repeat a node)
+    // so that they issue back to back without branches between them
+    NodeV n[kWideLists][2];
+#pragma unroll
+    for (int g = 0; g < kWideLists; g++) {
+        n[g][0] = load_node(a.nodes, idx[g]);
+        n[g][1] = load_node(a.nodes, idx[g] + (num[g] == 2 ? 1u : 0u));
     }
-    // all four fetches unconditional (in-bounds indices; unused ones repeat a
-    // node) so that they issue back to back without branches between them
-    const uint32_t ib0 = nb ? ib : ia;
-    const NodeV n0 = load_node(a.nodes, ia);
-    const NodeV n1 = load_node(a.nodes, ia + (na == 2 ? 1u : 0u));
-    const NodeV n2 = load_node(a.nodes, ib0);
-    const NodeV n3 = load_node(a.nodes, ib0 + (nb == 2 ? 1u : 0u));
-    L.tests += na + nb;
+    bool h[kWideLists][2];
     float f;
-    const bool h0 = slab(n0, L.r, L.ix, L.iy, L.iz, L.fast, f);
-    const bool h1 = na == 2 && slab(n1, L.r, L.ix, L.iy, L.iz, L.fast, f);
-    const bool h2 = nb >= 1 && slab(n2, L.r, L.ix, L.iy, L.iz, L.fast, f);
-    const bool h3 = nb == 2 && slab(n3, L.r, L.ix, L.iy, L.iz, L.fast, f);
+#pragma unroll
+    for (int g = 0; g < kWideLists; g++) {
+        L.tests += num[g];
+        h[g][0] = num[g] >= 1 && slab(n[g][0], L.r, L.ix, L.iy, L.iz, L.fast, f);
+        h[g][1] = num[g] == 2 && slab(n[g][1], L.r, L.ix, L.iy, L.iz, L.fast, f);
+    }
     bool bad = false;
-    if (h0 && n0.leaf) leaf_tris(a, n0, L, tri_tests, bad);
-    if (h1 && n1.leaf) leaf_tris(a, n1, L, tri_tests, bad);
-    if (h2 && n2.leaf) leaf_tris(a, n2, L, tri_tests, bad);
-    if (h3 && n3.leaf) leaf_tris(a, n3, L, tri_tests, bad);
+#pragma unroll
+    for (int g = 0; g < kWideLists; g++)
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            if (h[g][k] && n[g][k].leaf) leaf_tris(a, n[g][k], L, tri_tests, bad);
     if (bad) { st.fault |= kFaultTri; return false; }
-    if (h0 && !n0.leaf) push<false>(st, encode_list(ia, n0), 0.f);
-    if (h1 && !n1.leaf) push<false>(st, encode_list(ia + 1, n1), 0.f);
-    if (h2 && !n2.leaf) push<false>(st, encode_list(ib, n2), 0.f);
-    if (h3 && !n3.leaf) push<false>(st, encode_list(ib + 1, n3), 0.f);
+#pragma unroll
+    for (int g = 0; g < kWideLists; g++)
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            if (h[g][k] && !n[g][k].leaf) push<false>(st, encode_list(idx[g] + k, n[g][k]), 0.f);
     return st.fault == 0;
 }
 
