@@ -73,8 +73,7 @@ class TuneResult(C.Structure):
     _fields_ = [("radix_digit_bits", C.c_uint32), ("subtree_threshold", C.c_uint32), ("stack_slots", C.c_uint32),
                 ("block_size", C.c_uint32), ("build_ms", C.c_float), ("trace_ms", C.c_float),
                 ("default_build_ms", C.c_float), ("default_trace_ms", C.c_float), ("candidates_tried", C.c_uint32),
-                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("tri_test", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
