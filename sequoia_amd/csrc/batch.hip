@@ -459,7 +459,10 @@ int sq_batch_create(sq_ctx* c, const sq_triangle* tris, const uint32_t* counts, 
     if ((rc = balloc_t(b, nscenes, &b->d_desc)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, 6ull * nscenes, &b->d_ord)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, nscenes, &b->d_depth)) != SQ_OK) return fail(rc);
-    if ((rc = balloc_t(b, 2ull * total, &b->d_nodes)) != SQ_OK) return fail(rc);
+    // Morton trees: every scene's first child list is at its slot 1 and
+    // scenes start at even slots, so one slot of offset aligns the lists.
+    if ((rc = balloc_t(b, 2ull * total + 1, &b->d_nodes)) != SQ_OK) return fail(rc);
+    b->d_nodes += 1;
     if ((rc = balloc_t(b, 16, &b->d_work)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, 4, &b->d_stats)) != SQ_OK) return fail(rc);
     if ((rc = balloc_t(b, nscenes, &b->d_job_fail)) != SQ_OK) return fail(rc);
@@ -480,8 +483,8 @@ int sq_batch_create(sq_ctx* c, const sq_triangle* tris, const uint32_t* counts, 
         bv.root = 0;
         bv.view = true;
     }
-    (void)hipEventCreate(&b->ev[0]);
-    (void)hipEventCreate(&b->ev[1]);
+    (void)hipEventCreateWithFlags(&b->ev[0], hipEventReleaseToDevice);  // timing only
+    (void)hipEventCreateWithFlags(&b->ev[1], hipEventReleaseToDevice);
     *out = b;
     return SQ_OK;
 }

@@ -127,9 +127,12 @@ int sq_ctx_create(int device, sq_ctx** out) {
         delete c;
         SQ_RET(SQ_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
     }
+    // Timing events release to device scope only: a system-scope release
+    // writes the whole L2 back, which costs tens of microseconds and would be
+    // charged to the kernel before the event.
     for (int i = 0; i < PH_COUNT; i++) {
-        (void)hipEventCreate(&c->ev_begin[i]);
-        (void)hipEventCreate(&c->ev_end[i]);
+        (void)hipEventCreateWithFlags(&c->ev_begin[i], hipEventReleaseToDevice);
+        (void)hipEventCreateWithFlags(&c->ev_end[i], hipEventReleaseToDevice);
     }
     *out = c;
     return SQ_OK;
@@ -264,7 +267,7 @@ static int build_morton(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_
     SQ_TRY(c->keys[1].ensure(kb * n));
     SQ_TRY(c->vals[0].ensure(4ull * n));
     SQ_TRY(c->vals[1].ensure(4ull * n));
-    SQ_TRY(c->aux[0].ensure(64));
+    SQ_TRY(c->aux[0].ensure(scene_bounds_bytes(c, n)));
 
     phase_begin(c, PH_BOUNDS);
     SQ_TRY(scene_bounds(c, s->d_tris, n, c->aux[0].as<uint32_t>()));
@@ -290,6 +293,18 @@ static int build_morton(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_
     return SQ_OK;
 }
 
+// Allocates b's node array of `count` slots; odd_lists: child lists start at
+// odd slots (shift the array by one slot so they start 64-byte aligned).
+static hipError_t alloc_nodes(sq_bvh* b, size_t count, bool odd_lists) {
+    hipError_t e = hipMalloc(&b->d_alloc, sizeof(sq_node) * (count + 1));
+    if (e != hipSuccess) {
+        b->d_alloc = nullptr;
+        return e;
+    }
+    b->d_nodes = (sq_node*)b->d_alloc + (odd_lists ? 1 : 0);
+    return hipSuccess;
+}
+
 int sq_build(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_build_params* p, sq_bvh** out) {
     if (!c || !s || !out) SQ_RET(SQ_ERR_INVALID, "sq_build: NULL argument");
     *out = nullptr;
@@ -303,7 +318,8 @@ int sq_build(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_build_param
     b->ctx = c;
     b->scene = s;
     b->capacity = cap;
-    hipError_t e = hipMalloc((void**)&b->d_nodes, sizeof(sq_node) * (size_t)cap);
+    const bool morton = kind == SQ_BUILDER_MORTON || kind == SQ_BUILDER_MORTON2 || kind == SQ_BUILDER_MORTON63;
+    hipError_t e = alloc_nodes(b, cap, morton);
     if (e != hipSuccess) {
         delete b;
         SQ_RET(e == hipErrorOutOfMemory ? SQ_ERR_NOMEM : SQ_ERR_HIP, "node alloc: %s", hipGetErrorString(e));
@@ -311,8 +327,8 @@ int sq_build(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_build_param
     phases_reset(c);
     c->last = sq_timings{};
     hipEvent_t t0, t1;
-    (void)hipEventCreate(&t0);
-    (void)hipEventCreate(&t1);
+    (void)hipEventCreateWithFlags(&t0, hipEventReleaseToDevice);
+    (void)hipEventCreateWithFlags(&t1, hipEventReleaseToDevice);
     (void)hipEventRecord(t0, c->stream);
     int rc = launch_fill_u32(c, b->d_nodes, 0u, (size_t)cap * 8);
     if (rc == SQ_OK) {
@@ -340,7 +356,7 @@ int sq_build(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_build_param
     (void)hipEventDestroy(t1);
     if (rc != SQ_OK) {
         (void)hipStreamSynchronize(c->stream);
-        (void)hipFree(b->d_nodes);
+        (void)hipFree(b->d_alloc);
         delete b;
         return rc;
     }
@@ -375,12 +391,18 @@ int sq_bvh_from_host(sq_ctx* c, const sq_scene* s, const sq_node* nodes, uint32_
     b->capacity = count;
     b->root = root;
     b->depth = host_depth(nodes, count, root);
-    hipError_t e = hipMalloc((void**)&b->d_nodes, sizeof(sq_node) * (size_t)count);
+    size_t odd = 0, lists = 0;
+    for (uint32_t i = 0; i < count; i++)
+        if (!(nodes[i].flags & SQ_NODE_LEAF) && nodes[i].count > 0) {
+            lists++;
+            odd += nodes[i].child & 1u;
+        }
+    hipError_t e = alloc_nodes(b, count, 2 * odd > lists);
     if (e == hipSuccess)
         e = hipMemcpyAsync(b->d_nodes, nodes, sizeof(sq_node) * (size_t)count, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
-        if (b->d_nodes) (void)hipFree(b->d_nodes);
+        if (b->d_alloc) (void)hipFree(b->d_alloc);
         delete b;
         SQ_RET(SQ_ERR_HIP, "sq_bvh_from_host: %s", hipGetErrorString(e));
     }
@@ -410,7 +432,7 @@ const sq_node* sq_bvh_device_nodes(const sq_bvh* b) { return b ? b->d_nodes : nu
 
 void sq_bvh_free(sq_bvh* b) {
     if (!b || b->view) return;
-    if (b->d_nodes) (void)hipFree(b->d_nodes);
+    if (b->d_alloc) (void)hipFree(b->d_alloc);
     if (b->d_plucker) (void)hipFree(b->d_plucker);
     delete b;
 }
@@ -421,7 +443,7 @@ int sq_morton_codes(sq_ctx* c, const sq_scene* s, uint32_t bits, uint64_t* host_
     const uint32_t n = s->n;
     SQ_TRY(c->keys[0].ensure(8ull * n));
     SQ_TRY(c->vals[0].ensure(4ull * n));
-    SQ_TRY(c->aux[0].ensure(64));
+    SQ_TRY(c->aux[0].ensure(scene_bounds_bytes(c, n)));
     SQ_TRY(scene_bounds(c, s->d_tris, n, c->aux[0].as<uint32_t>()));
     SQ_TRY(morton_codes(c, s->d_tris, n, bits, c->aux[0].as<uint32_t>(), c->keys[0].p, c->vals[0].as<uint32_t>(), true));
     SQ_CHECK_HIP(hipMemcpyAsync(host_codes, c->keys[0].p, 8ull * n, hipMemcpyDeviceToHost, c->stream));

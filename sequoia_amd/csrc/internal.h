@@ -97,7 +97,12 @@ struct sq_scene {
 struct sq_bvh {
     sq_ctx* ctx = nullptr;
     const sq_scene* scene = nullptr;
+    // Node1 array. It starts one 32-byte slot into its allocation (d_alloc)
+    // when most child lists begin at odd slots (Morton trees: root 0, first
+    // list at 1), so that a sibling pair sits in one 64-byte segment instead
+    // of straddling two.
     sq_node* d_nodes = nullptr;
+    void* d_alloc = nullptr;
     uint32_t capacity = 0;
     uint32_t root = 0;
     uint32_t depth = 0;  // levels from the root to the deepest node (root = 1)
@@ -127,6 +132,9 @@ int pinned_ensure(sq_ctx* c, size_t bytes);
 
 // Kernel launchers implemented across the .hip files.
 int launch_fill_u32(sq_ctx* c, void* dst, uint32_t value, size_t words);
+// Scene box of n triangles into d_ord6[0..5] (ordered uints); d_ord6 must hold
+// scene_bounds_bytes(c, n) bytes (ticket and per-block partials follow).
+size_t scene_bounds_bytes(const sq_ctx* c, uint32_t n);
 int scene_bounds(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t* d_ord6);
 int morton_codes(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t bits, const uint32_t* d_ord6,
                  void* d_keys, uint32_t* d_vals, bool key64);

@@ -14,8 +14,11 @@ namespace sq {
 
 constexpr int kBoundsBlock = 256;
 
-// Block-reduces the six box extremes into ordered-uint atomics in ord6[0..5]
-// (min x,y,z then max x,y,z). ord6 must be pre-set to (UINT_MAX x3, 0 x3).
+// Block-reduces the six box extremes (ordered uints: min x,y,z then max
+// x,y,z). Each block writes its six values to ord6[16 + 6 * block + k]; the
+// block that takes the last ticket (ord6[8], zeroed by k_bounds_init) folds
+// all partials into ord6[0..5]. Atomics on the six result words from every
+// block serialise at L2 and cost about 50 us whatever the triangle count.
 __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle* __restrict__ tris, uint32_t n,
                                                                 uint32_t* __restrict__ ord6) {
     uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
@@ -31,43 +34,74 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle
             hi[k] = c > hi[k] ? c : hi[k];
         }
     }
-    // wave reduction
+    __shared__ uint32_t s[6][kBoundsBlock / 64];
+    __shared__ uint32_t s_last;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    auto block_reduce = [&]() {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                uint32_t a = __shfl_xor(lo[k], off, 64), c = __shfl_xor(hi[k], off, 64);
+                lo[k] = a < lo[k] ? a : lo[k];
+                hi[k] = c > hi[k] ? c : hi[k];
+            }
+        }
+        if (lane == 0) {
+            for (int k = 0; k < 3; k++) { s[k][wave] = lo[k]; s[3 + k][wave] = hi[k]; }
+        }
+        __syncthreads();
+    };
+    block_reduce();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t r = s[k][0];
+        for (int w = 1; w < kBoundsBlock / 64; w++) r = (k < 3) ? (s[k][w] < r ? s[k][w] : r) : (s[k][w] > r ? s[k][w] : r);
+        ord6[16 + 6 * blockIdx.x + k] = r;
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&ord6[8], 1u) == gridDim.x - 1 ? 1u : 0u;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    // last block: fold every block's partial (volatile loads read L2, not a
+    // stale L1 line)
+    const volatile uint32_t* part = ord6 + 16;
+    for (int k = 0; k < 3; k++) { lo[k] = 0xFFFFFFFFu; hi[k] = 0u; }
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            uint32_t a = __shfl_xor(lo[k], off, 64), c = __shfl_xor(hi[k], off, 64);
+            const uint32_t a = part[6 * b + k], c = part[6 * b + 3 + k];
             lo[k] = a < lo[k] ? a : lo[k];
             hi[k] = c > hi[k] ? c : hi[k];
         }
     }
-    __shared__ uint32_t s[6][kBoundsBlock / 64];
-    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) {
-        for (int k = 0; k < 3; k++) { s[k][wave] = lo[k]; s[3 + k][wave] = hi[k]; }
-    }
-    __syncthreads();
+    block_reduce();
     if (threadIdx.x < 6) {
-        int k = threadIdx.x;
+        const int k = threadIdx.x;
         uint32_t r = s[k][0];
         for (int w = 1; w < kBoundsBlock / 64; w++) r = (k < 3) ? (s[k][w] < r ? s[k][w] : r) : (s[k][w] > r ? s[k][w] : r);
-        if (k < 3) atomicMin(&ord6[k], r);
-        else atomicMax(&ord6[k], r);
+        ord6[k] = r;
     }
 }
 
 __global__ void k_bounds_init(uint32_t* ord6) {
-    int k = threadIdx.x;
-    if (k < 6) ord6[k] = k < 3 ? 0xFFFFFFFFu : 0u;
+    if (threadIdx.x == 0) ord6[8] = 0u;
 }
+
+static uint32_t bounds_blocks(const sq_ctx* c, uint32_t n) {
+    uint32_t blocks = (n + kBoundsBlock - 1) / kBoundsBlock;
+    const uint32_t cap = (uint32_t)c->num_cus * 8;
+    if (blocks > cap) blocks = cap;
+    return blocks ? blocks : 1;
+}
+
+size_t scene_bounds_bytes(const sq_ctx* c, uint32_t n) { return (16 + 6 * (size_t)bounds_blocks(c, n)) * 4; }
 
 int scene_bounds(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t* d_ord6) {
     hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, c->stream, d_ord6);
-    uint32_t blocks = (n + kBoundsBlock - 1) / kBoundsBlock;
-    uint32_t cap = (uint32_t)c->num_cus * 8;
-    if (blocks > cap) blocks = cap;
-    if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k_scene_bounds, dim3(blocks), dim3(kBoundsBlock), 0, c->stream, tris, n, d_ord6);
+    hipLaunchKernelGGL(k_scene_bounds, dim3(bounds_blocks(c, n)), dim3(kBoundsBlock), 0, c->stream, tris, n, d_ord6);
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
 }

@@ -52,8 +52,8 @@ __device__ __forceinline__ void lane_begin(LaneRay& L, const RayF& r, uint32_t r
 }
 
 __device__ __forceinline__ void leaf_tris(const TraceArgs& a, const NodeV& n, LaneRay& L, uint32_t& tri_tests,
-                                          bool& bad) {
-    for (uint32_t j = 0; j < n.count; j++) {
+                                          bool& bad, uint32_t j0 = 0) {
+    for (uint32_t j = j0; j < n.count; j++) {
         const uint32_t tid = n.child + j;
         if (tid >= a.tri_count) { bad = true; break; }
         float v[9];
@@ -71,6 +71,45 @@ __device__ __forceinline__ void leaf_tris(const TraceArgs& a, const NodeV& n, La
             L.best_id = tid;
         }
     }
+}
+
+// Leaf triangles of up to N hit leaves (nodes nd[j] with hit[j]):
+// the first triangle of every hit leaf is fetched before any is tested, so the
+// fetches are in flight together (the reference's leaves hold one triangle;
+// longer leaves finish in order after their first).
+template <int N>
+__device__ __forceinline__ void leaf_batch(const TraceArgs& a, const NodeV* nd, const bool* hit, LaneRay& L,
+                                           uint32_t& tri_tests, bool& bad) {
+    if (a.plk) {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (hit[j] && nd[j].leaf) leaf_tris(a, nd[j], L, tri_tests, bad);
+        return;
+    }
+    float tv[N][9];
+    bool lf[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        lf[j] = hit[j] && nd[j].leaf && nd[j].count > 0;
+        if (lf[j]) {
+            if (nd[j].child >= a.tri_count) bad = true;
+            else sqd::load_tri(a.tris, nd[j].child, tv[j]);
+        }
+    }
+    if (bad) return;
+#pragma unroll
+    for (int j = 0; j < N; j++)
+        if (lf[j]) {
+            const uint32_t tid = nd[j].child;
+            float t;
+            tri_tests++;
+            if (tri_test(tv[j], L.r, t) && t > L.r.tmin && t < L.r.tmax &&
+                (t < L.best_t || (t == L.best_t && tid < L.best_id))) {
+                L.best_t = t;
+                L.best_id = tid;
+            }
+            if (nd[j].count > 1) leaf_tris(a, nd[j], L, tri_tests, bad, 1);
+        }
 }
 
 // One step: pop a list if the current one is exhausted, then test up to two
@@ -109,8 +148,11 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
         h1 = h1 && !(f1 > cull);
     }
     bool bad = false;
-    if (h0 && n0.leaf) leaf_tris(a, n0, L, tri_tests, bad);
-    if (h1 && n1.leaf) leaf_tris(a, n1, L, tri_tests, bad);
+    {
+        const NodeV nd[2] = {n0, n1};
+        const bool hh[2] = {h0, h1};
+        leaf_batch<2>(a, nd, hh, L, tri_tests, bad);
+    }
     if (bad) { st.fault |= kFaultTri; return false; }
     const bool c0 = h0 && !n0.leaf, c1 = h1 && !n1.leaf;
     if (c0 || c1) {
@@ -191,11 +233,18 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
         h[g][1] = num[g] == 2 && slab(n[g][1], L.r, L.ix, L.iy, L.iz, L.fast, f);
     }
     bool bad = false;
+    {
+        NodeV nd[kWideLists * 2];
+        bool hh[kWideLists * 2];
 #pragma unroll
-    for (int g = 0; g < kWideLists; g++)
+        for (int g = 0; g < kWideLists; g++)
 #pragma unroll
-        for (int k = 0; k < 2; k++)
-            if (h[g][k] && n[g][k].leaf) leaf_tris(a, n[g][k], L, tri_tests, bad);
+            for (int k = 0; k < 2; k++) {
+                nd[2 * g + k] = n[g][k];
+                hh[2 * g + k] = h[g][k];
+            }
+        leaf_batch<kWideLists * 2>(a, nd, hh, L, tri_tests, bad);
+    }
     if (bad) { st.fault |= kFaultTri; return false; }
 #pragma unroll
     for (int g = 0; g < kWideLists; g++)
