@@ -172,13 +172,23 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
     return true;
 }
 
-// Exhaustive semantics: one step tests up to kWideLists x 2 nodes, the next
-// two of the current list and the next two after them (from the same list or,
-// once it is used up, from the list on top of the stack when that names its
-// nodes directly), so a lane has several independent node fetches in flight
-// instead of one. Counts and the closest hit do not depend on the order.
-constexpr int kWideLists = 2;  // three lists measured slower with 16 stack slots (more spills)
+// Exhaustive semantics: one step tests up to WL x 2 nodes, the next two of
+// the current list and the next two after them (from the same list or, once it
+// is used up, from the list on top of the stack when that names its nodes
+// directly), so a lane has several independent node fetches in flight instead
+// of one. Counts and the closest hit do not depend on the order.
+//
+// Stack bound: a step pops at most WL lists, all from the top (the deepest
+// level), and pushes at most 2 WL lists of the next level, so at most WL
+// lists per level stay behind: WL x depth + 2 WL entries.
+//
+// WL = 4 for one camera's rays (199 VGPRs, two waves per SIMD: the launch
+// shape those rays want anyway; 1080p x 4 on a 1M-triangle Morton tree:
+// 10.35 ms at 4 lists, 10.65 at 3, 11.28 at 2, with 32 stack slots), WL = 2
+// for ray lists and many-job launches, which want more waves per CU.
+constexpr int kCameraLists = 4, kListLists = 2;
 
+template <int WL>
 __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     while (L.cnt == 0) {
         if (st.sp == 0) return false;
@@ -194,9 +204,9 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
             L.cnt = p.count;
         }
     }
-    uint32_t idx[kWideLists], num[kWideLists];
+    uint32_t idx[WL], num[WL];
 #pragma unroll
-    for (int g = 0; g < kWideLists; g++) {
+    for (int g = 0; g < WL; g++) {
         if (g > 0 && L.cnt == 0 && st.sp > 0) {
             const uint32_t top = st.lds[((st.sp - 1) & st.mask) * st.stride];
             if ((top & 15u) != 15u) {
@@ -218,36 +228,36 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
     }
     // all fetches unconditional (in-bounds indices; unused ones repeat a node)
     // so that they issue back to back without branches between them
-    NodeV n[kWideLists][2];
+    NodeV n[WL][2];
 #pragma unroll
-    for (int g = 0; g < kWideLists; g++) {
+    for (int g = 0; g < WL; g++) {
         n[g][0] = load_node(a.nodes, idx[g]);
         n[g][1] = load_node(a.nodes, idx[g] + (num[g] == 2 ? 1u : 0u));
     }
-    bool h[kWideLists][2];
+    bool h[WL][2];
     float f;
 #pragma unroll
-    for (int g = 0; g < kWideLists; g++) {
+    for (int g = 0; g < WL; g++) {
         L.tests += num[g];
         h[g][0] = num[g] >= 1 && slab(n[g][0], L.r, L.ix, L.iy, L.iz, L.fast, f);
         h[g][1] = num[g] == 2 && slab(n[g][1], L.r, L.ix, L.iy, L.iz, L.fast, f);
     }
     bool bad = false;
     {
-        NodeV nd[kWideLists * 2];
-        bool hh[kWideLists * 2];
+        NodeV nd[WL * 2];
+        bool hh[WL * 2];
 #pragma unroll
-        for (int g = 0; g < kWideLists; g++)
+        for (int g = 0; g < WL; g++)
 #pragma unroll
             for (int k = 0; k < 2; k++) {
                 nd[2 * g + k] = n[g][k];
                 hh[2 * g + k] = h[g][k];
             }
-        leaf_batch<kWideLists * 2>(a, nd, hh, L, tri_tests, bad);
+        leaf_batch<WL * 2>(a, nd, hh, L, tri_tests, bad);
     }
     if (bad) { st.fault |= kFaultTri; return false; }
 #pragma unroll
-    for (int g = 0; g < kWideLists; g++)
+    for (int g = 0; g < WL; g++)
 #pragma unroll
         for (int k = 0; k < 2; k++)
             if (h[g][k] && !n[g][k].leaf) push<false>(st, encode_list(idx[g] + k, n[g][k]), 0.f);
@@ -399,7 +409,7 @@ constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode
 
 // One tile of job J: the wave's lanes take the tile's rays one after
 // another (per-lane refill) until all are traced.
-template <bool CLOSEST, int FMT>
+template <bool CLOSEST, int FMT, int WL>
 __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, uint32_t lane, Stack& st,
                                            uint32_t* s_cnt, uint32_t& tests_total, uint32_t& tris_total,
                                            uint32_t& faults, uint32_t* job_fail) {
@@ -454,7 +464,7 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
         if (active) {
             const bool more = FMT == 2 ? lane_step2<CLOSEST>(a, L, st, tris_total)
                                        : (CLOSEST ? lane_step<CLOSEST>(a, L, st, tris_total)
-                                                  : lane_step_wide(a, L, st, tris_total));
+                                                  : lane_step_wide<WL>(a, L, st, tris_total));
             if (!more) {
                 // ray finished
                 if (st.fault) {
@@ -498,7 +508,7 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
 // MULTI: the launch covers a.njobs jobs (tree + rays + outputs each); tile t
 // of the launch is tile t - job_tiles[j] of the job j with job_tiles[j] <= t
 // < job_tiles[j + 1]. Waves steal tiles from one counter across all jobs.
-template <bool CLOSEST, bool MULTI, int FMT = 1>
+template <bool CLOSEST, bool MULTI, int FMT = 1, int WL = kListLists>
 __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
     extern __shared__ uint32_t s_dyn[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -539,10 +549,10 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
                 else hi = mid;
             }
             const TraceArgs J = jobs[lo];
-            trace_tile<CLOSEST, FMT>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, faults,
+            trace_tile<CLOSEST, FMT, WL>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, faults,
                                      a.job_fail ? a.job_fail + lo : nullptr);
         } else {
-            trace_tile<CLOSEST, FMT>(a, tile, lane, st, s_cnt, tests_total, tris_total, faults, nullptr);
+            trace_tile<CLOSEST, FMT, WL>(a, tile, lane, st, s_cnt, tests_total, tris_total, faults, nullptr);
         }
     }
     if (faults) atomicOr(&a.work[1], faults);
@@ -575,7 +585,8 @@ constexpr uint32_t kCameraWavesPerCu = 8, kWavesPerCu = 16;
 int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out) {
     TracePlan L;
     L.closest = p && p->semantics == SQ_TRACE_CLOSEST;
-    uint32_t slots = p && p->stack_slots ? p->stack_slots : 16;
+    L.lists = !L.closest && coherent ? kCameraLists : kListLists;
+    uint32_t slots = p && p->stack_slots ? p->stack_slots : (L.lists == kCameraLists ? 32 : 16);
     uint32_t pow2 = 1;
     while (pow2 < slots) pow2 <<= 1;
     L.slots = pow2 > 64 ? 64 : pow2;
@@ -585,6 +596,9 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     int per_cu = 0;
     if (L.closest)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false>, L.block, L.lds_bytes);
+    else if (L.lists == kCameraLists)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 1, kCameraLists>, L.block,
+                                                           L.lds_bytes);
     else
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, L.block, L.lds_bytes);
     if (per_cu < 1) per_cu = 8;
@@ -603,7 +617,9 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     }
     L.grid = (uint32_t)c->num_cus * (uint32_t)per_cu;
     if (depth == 0) depth = 64;
-    uint32_t cap = 2 * depth + 16;
+    // stack entries a ray can need: WL x depth + 2 WL for the exhaustive step,
+    // depth + 2 for the closest step
+    uint32_t cap = (L.closest ? 2 : L.lists) * depth + 16;
     L.spill_cap = cap > L.slots ? cap - L.slots + 8 : 8;
     L.spill_words = (size_t)L.grid * L.block * L.spill_cap * (L.closest ? 2 : 1);
     *out = L;
@@ -617,6 +633,9 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
         else hipLaunchKernelGGL((k_trace<false, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     } else {
         if (L.closest) hipLaunchKernelGGL((k_trace<true, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        else if (L.lists == kCameraLists)
+            hipLaunchKernelGGL((k_trace<false, false, 1, kCameraLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
+                               stream, a);
         else hipLaunchKernelGGL((k_trace<false, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     }
     SQ_CHECK_HIP(hipGetLastError());

@@ -234,6 +234,7 @@ __device__ __forceinline__ uint8_t heat_byte(uint32_t count, uint32_t spp) {
 // Launch geometry and stack sizing of the traversal kernel.
 struct TracePlan {
     bool closest;
+    uint32_t lists;  // child lists per exhaustive step (k_trace's WL)
     uint32_t slots, block, grid, spill_cap;
     size_t lds_bytes, spill_words;
 };
