@@ -173,7 +173,8 @@ typedef struct {
     uint32_t blocks_per_cu; /* per-ray kernel: 0 = default (8 waves per CU for one camera's rays, 16
                                otherwise, at most what fits); else 1..64 */
     uint32_t tri_test;    /* 0: reference Moller-Trumbore; 1: precomputed triangles (sq_bvh_plucker) */
-    uint32_t reserved;
+    uint32_t lists_per_step; /* per-ray kernel, exhaustive semantics, Node1 trees: child lists tested per
+                                step, 0 = default (4 for one camera's rays, 2 otherwise), else 2 or 4 */
 } sq_trace_params;
 
 /* Outputs (any may be NULL). hit_id/hit_t: one per ray (SQ_MISS / +inf on
@@ -300,9 +301,9 @@ int sq_batch_times(const sq_batch* b, float* build_ms, float* trace_ms, float* r
 
 /* Autotuner: times candidate radix digit widths (4..8) and subtree
  * thresholds (512..2048) for sq_batch_build, and the traversal kernel
- * (per-lane rays with stack slots 8..32 and launch shapes of 8 to 24 waves
- * per CU in workgroups of 64 or 128, or wave packets for exhaustive
- * semantics) for sq_batch_trace, on this batch (which must have cameras
+ * (per-lane rays with stack slots 8..32, launch shapes of 8 to 24 waves
+ * per CU in workgroups of 64 or 128 and 2 or 4 child lists per step, or wave
+ * packets for exhaustive semantics) for sq_batch_trace, on this batch (which must have cameras
  * set), keeps the fastest in the batch and reports them. None of them
  * changes any result. */
 typedef struct {
@@ -312,6 +313,7 @@ typedef struct {
     uint32_t candidates_tried;
     uint32_t kernel;                    /* SQ_KERNEL_RAYS or SQ_KERNEL_PACKETS */
     uint32_t blocks_per_cu;             /* 0 = the default launch shape */
+    uint32_t lists_per_step;            /* per-ray kernel: 2 or 4 */
 } sq_tune_result;
 int sq_batch_autotune(sq_ctx* ctx, sq_batch* b, sq_builder kind, uint32_t repeats, sq_tune_result* out);
 

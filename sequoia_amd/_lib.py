@@ -62,7 +62,7 @@ class TraceParams(C.Structure):
     _fields_ = [("semantics", C.c_uint32), ("heatmap", C.c_uint32), ("width", C.c_uint32),
                 ("block_size", C.c_uint32), ("stack_slots", C.c_uint32), ("count_work", C.c_uint32),
                 ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("tri_test", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("lists_per_step", C.c_uint32)]
 
 
 class TraceOut(C.Structure):
@@ -73,7 +73,7 @@ class TuneResult(C.Structure):
     _fields_ = [("radix_digit_bits", C.c_uint32), ("subtree_threshold", C.c_uint32), ("stack_slots", C.c_uint32),
                 ("block_size", C.c_uint32), ("build_ms", C.c_float), ("trace_ms", C.c_float),
                 ("default_build_ms", C.c_float), ("default_trace_ms", C.c_float), ("candidates_tried", C.c_uint32),
-                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32)]
+                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lists_per_step", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -458,10 +458,10 @@ class Batch:
         return BatchBvh(self.ctx, h, None)
 
     def set_cameras(self, cams, semantics=TRACE_EXHAUSTIVE, outputs=OUT_ALL, stack_slots=0, block_size=0,
-                    count_work=False, kernel=0):
+                    count_work=False, kernel=0, lists_per_step=0):
         arr = (Camera * len(cams))(*cams)
         p = TraceParams(semantics, HEATMAP_RAY, 0, block_size, stack_slots, 1 if count_work else 0,
-                        kernel, 0)
+                        kernel, 0, 0, lists_per_step)
         _check(lib().sq_batch_set_cameras(self.ctx.h, self.h, arr, C.byref(p), outputs))
         self.cams = list(cams)
         self._cam_arr = arr
@@ -674,11 +674,13 @@ class Context:
         _check(lib().sq_refit(self.h, bvh.h))
 
     @staticmethod
-    def _params(semantics, heatmap, width, stack_slots, count_work, kernel=0, tri_test=0):
-        return TraceParams(semantics, heatmap, width, 0, stack_slots, 1 if count_work else 0, kernel, 0, tri_test, 0)
+    def _params(semantics, heatmap, width, stack_slots, count_work, kernel=0, tri_test=0, lists_per_step=0):
+        return TraceParams(semantics, heatmap, width, 0, stack_slots, 1 if count_work else 0, kernel, 0, tri_test,
+                           lists_per_step)
 
     def trace_camera(self, bvh, cam, semantics=TRACE_EXHAUSTIVE, heatmap=HEATMAP_RAY, width=0, want_hits=True,
-                     want_counts=True, want_rgb=True, stack_slots=0, count_work=False, kernel=0, tri_test=0):
+                     want_counts=True, want_rgb=True, stack_slots=0, count_work=False, kernel=0, tri_test=0,
+                     lists_per_step=0):
         npix = cam.width * cam.height
         nray = npix * cam.spp
         res = {}
@@ -687,14 +689,14 @@ class Context:
         cnt = np.empty(npix, np.uint32) if want_counts else None
         rgb = np.empty((cam.height, cam.width, 3), np.uint8) if want_rgb else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), _ptr(rgb))
-        p = self._params(semantics, heatmap, width, stack_slots, count_work, kernel, tri_test)
+        p = self._params(semantics, heatmap, width, stack_slots, count_work, kernel, tri_test, lists_per_step)
         f = lib().sq_trace2_camera if isinstance(bvh, Bvh2) else lib().sq_trace_camera
         _check(f(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 0))
         res.update(hit_id=hid, hit_t=ht, counts=cnt, rgb=rgb)
         return res
 
     def trace_rays(self, bvh, rays, semantics=TRACE_EXHAUSTIVE, want_counts=True, stack_slots=0, count_work=False,
-                   tri_test=0):
+                   tri_test=0, lists_per_step=0):
         """Ray list through a Node1 tree (Bvh) or a Node2 tree (Bvh2)."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         n = len(rays)
@@ -702,7 +704,7 @@ class Context:
         ht = np.empty(n, np.float32)
         cnt = np.empty(n, np.uint32) if want_counts else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), None)
-        p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work, 0, tri_test)
+        p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work, 0, tri_test, lists_per_step)
         f = lib().sq_trace2_rays if isinstance(bvh, Bvh2) else lib().sq_trace_rays
         _check(f(self.h, bvh.h, _ptr(rays), n, C.byref(p), C.byref(out), 0))
         return {"hit_id": hid, "hit_t": ht, "counts": cnt}

@@ -730,13 +730,15 @@ int sq_batch_autotune(sq_ctx* c, sq_batch* b, sq_builder kind, uint32_t repeats,
     // (workgroup size, workgroups per CU; 0 = the default, 16 waves per CU)
     const uint32_t shapes[][2] = {{64, 0}, {64, 8}, {64, 12}, {64, 24}, {128, 0}};
     float best_t = 1e30f;
-    uint32_t bs = 16, bb = 64, bw = 0, bk = SQ_KERNEL_RAYS;
-    auto time_trace = [&](uint32_t kernel, uint32_t sl, uint32_t bl, uint32_t bpc, float* best) -> int {
+    uint32_t bs = 16, bb = 64, bw = 0, bk = SQ_KERNEL_RAYS, bl4 = 0;
+    auto time_trace = [&](uint32_t kernel, uint32_t sl, uint32_t bl, uint32_t bpc, uint32_t lists,
+                          float* best) -> int {
         sq_trace_params tp = base;
         tp.kernel = kernel;
         tp.stack_slots = sl;
         tp.block_size = bl;
         tp.blocks_per_cu = bpc;
+        tp.lists_per_step = lists;
         SQ_TRY(sq_batch_set_cameras(c, b, cams.data(), &tp, outputs));
         float m = 1e30f;
         for (uint32_t r = 0; r < repeats + 1; r++) {
@@ -750,22 +752,31 @@ int sq_batch_autotune(sq_ctx* c, sq_batch* b, sq_builder kind, uint32_t repeats,
     for (uint32_t sl : slots)
         for (const auto& sh : shapes) {
             float m = 0;
-            SQ_TRY(time_trace(SQ_KERNEL_RAYS, sl, sh[0], sh[1], &m));
+            SQ_TRY(time_trace(SQ_KERNEL_RAYS, sl, sh[0], sh[1], 2, &m));
             if (sl == 16 && sh[0] == 64 && sh[1] == 0) out->default_trace_ms = m;
-            if (m < best_t) { best_t = m; bs = sl; bb = sh[0]; bw = sh[1]; bk = SQ_KERNEL_RAYS; }
+            if (m < best_t) { best_t = m; bs = sl; bb = sh[0]; bw = sh[1]; bk = SQ_KERNEL_RAYS; bl4 = 2; }
         }
-    if (base.semantics == SQ_TRACE_EXHAUSTIVE) {  // wave packets: exhaustive semantics only
+    if (base.semantics == SQ_TRACE_EXHAUSTIVE) {
+        // four lists per step (two waves per SIMD at most)
+        for (uint32_t sl : {32u, 16u}) {
+            float m = 0;
+            SQ_TRY(time_trace(SQ_KERNEL_RAYS, sl, 64, 8, 4, &m));
+            if (m < best_t) { best_t = m; bs = sl; bb = 64; bw = 8; bk = SQ_KERNEL_RAYS; bl4 = 4; }
+        }
+        // wave packets: exhaustive semantics only
         float m = 0;
-        SQ_TRY(time_trace(SQ_KERNEL_PACKETS, 0, 0, 0, &m));
-        if (m < best_t) { best_t = m; bk = SQ_KERNEL_PACKETS; }
+        SQ_TRY(time_trace(SQ_KERNEL_PACKETS, 0, 0, 0, 0, &m));
+        if (m < best_t) { best_t = m; bk = SQ_KERNEL_PACKETS; bl4 = 0; }
     }
     sq_trace_params tp = base;
     tp.kernel = bk;
     tp.stack_slots = bs;
     tp.block_size = bb;
     tp.blocks_per_cu = bw;
+    tp.lists_per_step = bl4;
     SQ_TRY(sq_batch_set_cameras(c, b, cams.data(), &tp, outputs));
     out->kernel = bk;
+    out->lists_per_step = bl4;
     out->radix_digit_bits = bd;
     out->subtree_threshold = bt;
     out->stack_slots = bs;

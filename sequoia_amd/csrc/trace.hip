@@ -586,6 +586,11 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     TracePlan L;
     L.closest = p && p->semantics == SQ_TRACE_CLOSEST;
     L.lists = !L.closest && coherent ? kCameraLists : kListLists;
+    if (p && p->lists_per_step) {
+        if (p->lists_per_step != kListLists && p->lists_per_step != kCameraLists)
+            SQ_RET(SQ_ERR_INVALID, "trace lists_per_step %u (2 or 4)", p->lists_per_step);
+        if (!L.closest) L.lists = p->lists_per_step;
+    }
     uint32_t slots = p && p->stack_slots ? p->stack_slots : (L.lists == kCameraLists ? 32 : 16);
     uint32_t pow2 = 1;
     while (pow2 < slots) pow2 <<= 1;
@@ -630,6 +635,9 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     if (multi) {
         if (L.closest) hipLaunchKernelGGL((k_trace<true, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        else if (L.lists == kCameraLists)
+            hipLaunchKernelGGL((k_trace<false, true, 1, kCameraLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
+                               stream, a);
         else hipLaunchKernelGGL((k_trace<false, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     } else {
         if (L.closest) hipLaunchKernelGGL((k_trace<true, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);

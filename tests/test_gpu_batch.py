@@ -110,15 +110,16 @@ def assert_same(got, want, tag):
         assert np.array_equal(np.ascontiguousarray(g).view(np.uint8), np.ascontiguousarray(w).view(np.uint8)), (tag, k)
 
 
-@pytest.mark.parametrize("semantics,kernel", [(sq.TRACE_EXHAUSTIVE, sq.KERNEL_RAYS),
-                                              (sq.TRACE_EXHAUSTIVE, sq.KERNEL_PACKETS),
-                                              (sq.TRACE_CLOSEST, sq.KERNEL_AUTO)])
-def test_batch_trace_equals_plain(ctx, mixed, semantics, kernel):
+@pytest.mark.parametrize("semantics,kernel,lists", [(sq.TRACE_EXHAUSTIVE, sq.KERNEL_RAYS, 0),
+                                                    (sq.TRACE_EXHAUSTIVE, sq.KERNEL_RAYS, 4),
+                                                    (sq.TRACE_EXHAUSTIVE, sq.KERNEL_PACKETS, 0),
+                                                    (sq.TRACE_CLOSEST, sq.KERNEL_AUTO, 0)])
+def test_batch_trace_equals_plain(ctx, mixed, semantics, kernel, lists):
     scenes, b = mixed
     cams = cameras_for(scenes)
-    say("batch trace", semantics, kernel)
+    say("batch trace", semantics, kernel, lists)
     b.build("morton63")
-    b.set_cameras(cams, semantics=semantics, kernel=kernel)
+    b.set_cameras(cams, semantics=semantics, kernel=kernel, lists_per_step=lists)
     b.trace()
     for i, t in enumerate(scenes):
         assert_same(b.download(i), plain_trace(ctx, t, cams[i], "morton63", semantics), (semantics, i))
@@ -165,9 +166,10 @@ def test_batch_autotune_keeps_results(ctx, mixed):
     trees = [b.bvh(i).download() for i in range(len(scenes))]
     r = b.autotune("morton63", repeats=1)
     say("autotune", r)
-    assert r["candidates_tried"] == 31 and r["kernel"] in (sq.KERNEL_RAYS, sq.KERNEL_PACKETS)
+    assert r["candidates_tried"] == 33 and r["kernel"] in (sq.KERNEL_RAYS, sq.KERNEL_PACKETS)
     assert 4 <= r["radix_digit_bits"] <= 8 and r["stack_slots"] in (8, 16, 32) and r["block_size"] in (64, 128)
     assert r["blocks_per_cu"] in (0, 8, 12, 24)
+    assert r["lists_per_step"] in ((2, 4) if r["kernel"] == sq.KERNEL_RAYS else (0,))
     b.run("morton63")
     for i in range(len(scenes)):
         assert len(nodes_equal(b.bvh(i).download(), trees[i])) == 0

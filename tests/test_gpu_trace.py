@@ -87,10 +87,11 @@ def test_ray_list_secondary_rays(ctx):
     scene = ctx.upload(tris)
     bvh = ctx.build(scene, "morton63")
     nodes = bvh.download()
-    got = ctx.trace_rays(bvh, rays)
     want = ob.trace_rays(nodes, bvh.root, tris, rays)
-    assert np.array_equal(got["counts"], want["counts"])
-    check_hits(got["hit_id"], got["hit_t"], want["hit_id"], want["hit_t"])
+    for lists in (0, 4):  # two child lists per step by default for ray lists; four on request
+        got = ctx.trace_rays(bvh, rays, lists_per_step=lists)
+        assert np.array_equal(got["counts"], want["counts"]), lists
+        check_hits(got["hit_id"], got["hit_t"], want["hit_id"], want["hit_t"])
     cl = ctx.trace_rays(bvh, rays, semantics=sq.TRACE_CLOSEST)
     check_hits(cl["hit_id"], cl["hit_t"], want["hit_id"], want["hit_t"])
 
@@ -114,9 +115,19 @@ def test_small_stack_spills_correctly(ctx):
     scene = ctx.upload(tris)
     bvh = ctx.build(scene, "morton63")
     cam = sq.pinhole_camera(tris, 128, 96, 60.0, 3.0)
-    a = ctx.trace_camera(bvh, cam, stack_slots=2)
     b = ctx.trace_camera(bvh, cam, stack_slots=32)
-    assert np.array_equal(a["counts"], b["counts"]) and np.array_equal(a["hit_id"], b["hit_id"])
+    for lists in (2, 4):  # spills under both step widths
+        a = ctx.trace_camera(bvh, cam, stack_slots=2, lists_per_step=lists)
+        assert np.array_equal(a["counts"], b["counts"]) and np.array_equal(a["hit_id"], b["hit_id"]), lists
+
+
+def test_lists_per_step_rejects_other_values(ctx):
+    tris = GOLDEN_SCENES["ico1280"]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    cam = sq.pinhole_camera(tris, 16, 16, 60.0, 3.0)
+    with pytest.raises(sq.SqError):
+        ctx.trace_camera(bvh, cam, lists_per_step=3)
 
 
 @pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000", "ico69632"])
