@@ -73,10 +73,13 @@ __device__ __forceinline__ void leaf_tris(const TraceArgs& a, const NodeV& n, La
     }
 }
 
-// Leaf triangles of up to N hit leaves (nodes nd[j] with hit[j]):
-// the first triangle of every hit leaf is fetched before any is tested, so the
-// fetches are in flight together (the reference's leaves hold one triangle;
-// longer leaves finish in order after their first).
+// Leaf triangles of up to N hit leaves (nodes nd[j] with hit[j]). The
+// lane's hit leaves are first compacted in slot order, so that the k-th test
+// block runs only when some lane has more than k hit leaves in this step
+// (rather than whenever some lane hit a leaf in slot k); the first triangle of
+// every hit leaf is fetched before any is tested, so the fetches are in flight
+// together. The reference's leaves hold one triangle; the rest of a longer
+// leaf is tested after the others.
 template <int N>
 __device__ __forceinline__ void leaf_batch(const TraceArgs& a, const NodeV* nd, const bool* hit, LaneRay& L,
                                            uint32_t& tri_tests, bool& bad) {
@@ -86,30 +89,47 @@ __device__ __forceinline__ void leaf_batch(const TraceArgs& a, const NodeV* nd, 
             if (hit[j] && nd[j].leaf) leaf_tris(a, nd[j], L, tri_tests, bad);
         return;
     }
-    float tv[N][9];
     bool lf[N];
+    uint32_t pos[N], c = 0;
+    bool longer = false;
 #pragma unroll
     for (int j = 0; j < N; j++) {
         lf[j] = hit[j] && nd[j].leaf && nd[j].count > 0;
-        if (lf[j]) {
-            if (nd[j].child >= a.tri_count) bad = true;
-            else sqd::load_tri(a.tris, nd[j].child, tv[j]);
-        }
+        if (lf[j] && nd[j].child >= a.tri_count) bad = true;
+        pos[j] = c;
+        c += lf[j] ? 1u : 0u;
+        longer |= lf[j] && nd[j].count > 1;
     }
     if (bad) return;
+    uint32_t ids[N];
 #pragma unroll
-    for (int j = 0; j < N; j++)
-        if (lf[j]) {
-            const uint32_t tid = nd[j].child;
+    for (int k = 0; k < N; k++) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = k; j < N; j++) v = (lf[j] && pos[j] == (uint32_t)k) ? nd[j].child : v;
+        ids[k] = v;
+    }
+    float tv[N][9];
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if ((uint32_t)k < c) sqd::load_tri(a.tris, ids[k], tv[k]);
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if ((uint32_t)k < c) {
+            const uint32_t tid = ids[k];
             float t;
             tri_tests++;
-            if (tri_test(tv[j], L.r, t) && t > L.r.tmin && t < L.r.tmax &&
+            if (tri_test(tv[k], L.r, t) && t > L.r.tmin && t < L.r.tmax &&
                 (t < L.best_t || (t == L.best_t && tid < L.best_id))) {
                 L.best_t = t;
                 L.best_id = tid;
             }
-            if (nd[j].count > 1) leaf_tris(a, nd[j], L, tri_tests, bad, 1);
         }
+    if (longer) {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (lf[j] && nd[j].count > 1) leaf_tris(a, nd[j], L, tri_tests, bad, 1);
+    }
 }
 
 // One step: pop a list if the current one is exhausted, then test up to two
