@@ -45,10 +45,30 @@ void phase_begin(sq_ctx* c, Phase p) {
 void phase_end(sq_ctx* c, Phase p) { (void)hipEventRecord(c->ev_end[p], c->stream); }
 void phases_reset(sq_ctx* c) {
     for (int i = 0; i < PH_COUNT; i++) c->phase_used[i] = false;
+    // calls on a ctx end with a sync, so no kernel of an earlier call can
+    // still write the sort error word
+    c->sort_err_pending = false;
+    if (c->h_sort_err) *c->h_sort_err = 0;
+}
+
+int sort_error_word(sq_ctx* c, uint32_t** word) {
+    if (!c->h_sort_err) {
+        void* p = nullptr;
+        SQ_CHECK_HIP(hipHostMalloc(&p, 64, hipHostMallocDefault));
+        c->h_sort_err = (uint32_t*)p;
+        *c->h_sort_err = 0;  // cleared again by phases_reset at the start of each call
+    }
+    c->sort_err_pending = true;
+    *word = c->h_sort_err;
+    return SQ_OK;
 }
 
 int phases_collect(sq_ctx* c) {
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (c->sort_err_pending) {
+        c->sort_err_pending = false;
+        if (*c->h_sort_err) SQ_RET(SQ_ERR_HIP, "radix sort: a look-back wait exceeded its spin limit");
+    }
     float ms[PH_COUNT] = {};
     for (int i = 0; i < PH_COUNT; i++) {
         if (!c->phase_used[i]) continue;
@@ -144,8 +164,10 @@ void sq_ctx_destroy(sq_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto* b : {&c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1], &c->hist, &c->scratch, &c->tasks[0],
                     &c->tasks[1], &c->tasks[2], &c->counters, &c->spill, &c->stats, &c->aux[0], &c->aux[1],
-                    &c->aux[2], &c->aux[3], &c->sah[0], &c->sah[1], &c->sah[2], &c->sah[3], &c->sah[4], &c->sah[5]})
+                    &c->aux[2], &c->aux[3], &c->sah[0], &c->sah[1], &c->sah[2], &c->sah[3], &c->sah[4], &c->sah[5],
+                    &c->sort_os})
         b->release();
+    if (c->h_sort_err) (void)hipHostFree(c->h_sort_err);
     for (int i = 0; i < PH_COUNT; i++) {
         (void)hipEventDestroy(c->ev_begin[i]);
         (void)hipEventDestroy(c->ev_end[i]);

@@ -78,6 +78,7 @@ struct sq_ctx {
     int num_cus = 256;
     // Workspace reused across calls (grow-only).
     sq::DevBuf keys[2], vals[2], hist, scratch, tasks[3], counters, spill, stats, aux[4];
+    sq::DevBuf sort_os;  // single-pass radix sort area (sort_onesweep_bytes)
     sq::DevBuf sah[6];  // binned-SAH large-task queues, plans, partial bins, counts, bin ids
     hipEvent_t ev_begin[sq::PH_COUNT] = {};
     hipEvent_t ev_end[sq::PH_COUNT] = {};
@@ -85,6 +86,8 @@ struct sq_ctx {
     sq_timings last = {};
     void* pinned = nullptr;  // small pinned host staging area
     size_t pinned_bytes = 0;
+    uint32_t* h_sort_err = nullptr;  // pinned: the last single-pass sort's error word
+    bool sort_err_pending = false;
 };
 
 struct sq_scene {
@@ -144,7 +147,16 @@ struct SortBufs {
     void* keys[2];
     uint32_t* vals[2];
     uint32_t* hist;  // sort_hist_bytes(n, digit_bits)
+    // non-null: key passes run as single-pass look-back kernels in this area
+    // (sort_onesweep_bytes); null: three kernels per pass
+    uint32_t* os = nullptr;
+    uint32_t* os_err = nullptr;  // pinned host word the look-back sets on a failed wait
+    int num_cus = 256;
 };
+size_t sort_onesweep_bytes(uint32_t n, uint32_t key_bits, uint32_t digit_bits);
+// Clears and returns the ctx's pinned sort error word; the next
+// phases_collect (after its sync) reports a set word as a failed call.
+int sort_error_word(sq_ctx* c, uint32_t** word);
 size_t sort_hist_bytes(uint32_t n, uint32_t digit_bits);
 uint32_t sort_segment_passes(uint32_t nseg, uint32_t digit_bits);
 // Key passes, then (d_seg_off != NULL, nseg > 1) segment passes that group

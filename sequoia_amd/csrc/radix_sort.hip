@@ -4,7 +4,10 @@
 // every pass is a stable counting sort on one digit, so the output is the
 // input ordered by (key, original position).
 //
-// Per pass (reduce-then-scan, three kernels):
+// Plain builds and sq_radix_sort_pairs run each key pass as one kernel with
+// decoupled look-back (k_rs_onesweep, below), after one read of the keys
+// that counts the digits of every pass. Batched and served builds, and the
+// segment passes, use reduce-then-scan, three kernels per pass:
 //   upsweep   each 4096-key tile builds its digit histogram; each wave counts
 //             its own 1024 keys with ballot-matched peer groups into a
 //             wave-private LDS row (no atomics),
@@ -248,6 +251,190 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const K* __restrict
     }
 }
 
+// ---- single-pass key passes (plain builds) ----
+// Digit totals of every pass come from one read of the keys up front
+// (k_rs_ghist). Each pass is then one kernel: a workgroup takes the next
+// tile by ticket, ranks its keys as the downsweep does, publishes its digit
+// counts, and finds the keys of earlier tiles with each digit by looking back
+// over the earlier tiles' published words (decoupled look-back): a word
+// holds either that tile's own count (flag 1) or the count of all tiles up to
+// and including it (flag 2); the walk stops at the first flag-2 word. Tiles
+// are taken in ticket order, and a tile publishes its own counts before it
+// looks back, so every word it waits for belongs to a workgroup that is
+// already running and never waits for it: the wait always ends. A wait that
+// nevertheless spins kSpinLimit times sets the error word (pinned host
+// memory) and the pass continues with a zero count (the host then reports
+// the sort as failed).
+constexpr uint32_t kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kValueMask = (1u << 30) - 1u;
+constexpr uint32_t kSpinLimit = 1u << 20;  // about 30 ms of s_sleep(1) per word
+constexpr uint32_t kMaxKeyPasses = 64;
+
+// Layout of the single-pass area (words): digit totals [kMaxKeyPasses][256],
+// tickets [kMaxKeyPasses], a spare word, then per pass num_tiles x radix
+// status words.
+__host__ __device__ constexpr size_t os_ghist() { return 0; }
+__host__ __device__ constexpr size_t os_tickets() { return (size_t)kMaxKeyPasses * kRadixMax; }
+__host__ __device__ constexpr size_t os_spare() { return os_tickets() + kMaxKeyPasses; }
+__host__ __device__ constexpr size_t os_status() { return os_spare() + 64; }
+
+template <typename K>
+__global__ __launch_bounds__(kRsThreads) void k_rs_ghist(const K* __restrict__ keys, uint32_t n, uint32_t bits,
+                                                         uint32_t kpasses, uint32_t* __restrict__ ghist) {
+    __shared__ uint32_t s_h[kMaxKeyPasses * 2 > 8 * kRadixMax ? kMaxKeyPasses * 2 : 8 * kRadixMax];
+    const uint32_t radix = 1u << bits, mask = radix - 1, cells = kpasses * radix;
+    for (uint32_t i = threadIdx.x; i < cells; i += kRsThreads) s_h[i] = 0;
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * kRsThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kRsThreads) {
+        const K k = keys[i];
+        for (uint32_t p = 0; p < kpasses; p++) atomicAdd(&s_h[p * radix + digit_of(k, p * bits, mask)], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < cells; i += kRsThreads) {
+        const uint32_t v = s_h[i];
+        if (v) atomicAdd(&ghist[(i / radix) * kRadixMax + (i % radix)], v);
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(const K* __restrict__ keys_in,
+                                                            const uint32_t* __restrict__ vals_in,
+                                                            K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                                                            uint32_t n, uint32_t shift, uint32_t bits,
+                                                            const uint32_t* __restrict__ ghist_pass,
+                                                            uint32_t* __restrict__ ticket,
+                                                            uint32_t* __restrict__ status, uint32_t* __restrict__ err) {
+    __shared__ uint32_t s_cnt[kRsWaves][kRadixMax];
+    __shared__ uint32_t s_start[kRadixMax];
+    __shared__ uint32_t s_gbase[kRadixMax];
+    __shared__ uint32_t s_hist[kRadixMax];
+    __shared__ uint32_t s_dbase[kRadixMax];
+    __shared__ K s_keys[kRsTile];
+    __shared__ uint32_t s_vals[kRsTile];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t radix = 1u << bits, mask = radix - 1;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    for (uint32_t i = threadIdx.x; i < kRsWaves * kRadixMax; i += kRsThreads) (&s_cnt[0][0])[i] = 0;
+    // digit bases: exclusive scan of this pass's digit totals (one wave)
+    if (wave == 0) {
+        uint32_t v[kRadixMax / 64], run = 0;
+#pragma unroll
+        for (int j = 0; j < kRadixMax / 64; j++) {
+            const uint32_t d = (uint32_t)lane * (kRadixMax / 64) + j;
+            v[j] = d < radix ? ghist_pass[d] : 0u;
+            run += v[j];
+        }
+        const uint32_t incl = wave_incl_scan(run, lane);
+        uint32_t ex = incl - run;
+#pragma unroll
+        for (int j = 0; j < kRadixMax / 64; j++) {
+            const uint32_t d = (uint32_t)lane * (kRadixMax / 64) + j;
+            s_dbase[d] = ex;
+            ex += v[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t tile_base = (size_t)tile * kRsTile;
+    const size_t base = tile_base + (size_t)wave * kRsWaveKeys;
+    K key[kRsRounds];
+    uint32_t val[kRsRounds], rank[kRsRounds], dig[kRsRounds];
+#pragma unroll
+    for (int r = 0; r < kRsRounds; r++) {
+        size_t idx = base + (size_t)r * 64 + lane;
+        bool valid = idx < n;
+        key[r] = valid ? keys_in[idx] : (K)0;
+        val[r] = valid ? vals_in[idx] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < kRsRounds; r++) {
+        size_t idx = base + (size_t)r * 64 + lane;
+        bool valid = idx < n;
+        uint64_t active = __ballot(valid);
+        uint32_t d = digit_of(key[r], shift, mask);
+        dig[r] = d;
+        rank[r] = 0;
+        if (active == 0) continue;
+        uint64_t peers = match_digit(d, bits, active);
+        uint64_t lower = peers & ((1ull << lane) - 1ull);
+        uint32_t prior = valid ? s_cnt[wave][d] : 0u;
+        rank[r] = prior + (uint32_t)__popcll(lower);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && lower == 0) s_cnt[wave][d] = prior + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < radix; d += kRsThreads) {
+        uint32_t run = 0;
+        for (int w = 0; w < kRsWaves; w++) {
+            uint32_t c = s_cnt[w][d];
+            s_cnt[w][d] = run;
+            run += c;
+        }
+        s_hist[d] = run;
+        // publish this tile's own count (tile 0: its count is the prefix)
+        __hip_atomic_store(&status[(size_t)tile * radix + d], (tile == 0 ? kFlagPrefix : kFlagAgg) | run,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // look back: keys with digit d in earlier tiles (a window of 16 words
+    // read at once measured no faster)
+    for (uint32_t d = threadIdx.x; d < radix; d += kRsThreads) {
+        uint32_t excl = 0;
+        for (uint32_t t = tile; t-- > 0;) {
+            uint32_t w, spins = 0;
+            while (true) {
+                w = __hip_atomic_load(&status[(size_t)t * radix + d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (w & (kFlagAgg | kFlagPrefix)) break;
+                if (++spins > kSpinLimit) {
+                    *err = 1u;  // host-visible word, read after the call's final sync
+                    w = kFlagPrefix;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            excl += w & kValueMask;
+            if (w & kFlagPrefix) break;
+        }
+        if (tile > 0)
+            __hip_atomic_store(&status[(size_t)tile * radix + d], kFlagPrefix | ((excl + s_hist[d]) & kValueMask),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_gbase[d] = s_dbase[d] + excl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (uint32_t d = 0; d < radix; d++) {
+            s_start[d] = run;
+            run += s_hist[d];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRsRounds; r++) {
+        size_t idx = base + (size_t)r * 64 + lane;
+        if (idx < n) {
+            uint32_t d = dig[r];
+            uint32_t pos = s_start[d] + s_cnt[wave][d] + rank[r];
+            if (pos < (uint32_t)kRsTile) {
+                s_keys[pos] = key[r];
+                s_vals[pos] = val[r];
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t tile_count = (uint32_t)((n - tile_base) < (size_t)kRsTile ? (n - tile_base) : (size_t)kRsTile);
+    for (uint32_t i = threadIdx.x; i < tile_count; i += kRsThreads) {
+        K k = s_keys[i];
+        uint32_t d = digit_of(k, shift, mask);
+        size_t out = (size_t)s_gbase[d] + (i - s_start[d]);
+        if (out < n) {
+            keys_out[out] = k;
+            vals_out[out] = s_vals[i];
+        }
+    }
+}
+
 template <typename K>
 static int sort_passes(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_t bits, uint32_t seg_passes,
                        SegTable seg, int in, int* out_slot, uint32_t* passes_out) {
@@ -257,8 +444,28 @@ static int sort_passes(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_
     uint32_t* totals = hist + (size_t)radix * num_tiles;
     const uint32_t kpasses = (key_bits + bits - 1) / bits;
     int cur = in;
+    if (b.os) {
+        // single-pass key passes: zero the area, count every pass's digits once
+        const size_t words = os_status() + (size_t)kpasses * num_tiles * radix;
+        SQ_TRY(launch_fill_u32_on(b.stream, b.num_cus, b.os, 0u, words));
+        uint32_t gb = (n + kRsThreads * 16 - 1) / (kRsThreads * 16);
+        if (gb > (uint32_t)b.num_cus * 4) gb = (uint32_t)b.num_cus * 4;
+        hipLaunchKernelGGL((k_rs_ghist<K>), dim3(gb), dim3(kRsThreads), 0, b.stream, (const K*)b.keys[cur], n, bits,
+                           kpasses, b.os + os_ghist());
+        SQ_CHECK_HIP(hipGetLastError());
+    }
     for (uint32_t p = 0; p < kpasses + seg_passes; p++) {
         const bool segp = p >= kpasses;
+        if (b.os && !segp) {
+            const K* kin = (const K*)b.keys[cur];
+            hipLaunchKernelGGL((k_rs_onesweep<K>), dim3(num_tiles), dim3(kRsThreads), 0, b.stream, kin, b.vals[cur],
+                               (K*)b.keys[cur ^ 1], b.vals[cur ^ 1], n, p * bits, bits,
+                               b.os + os_ghist() + (size_t)p * kRadixMax, b.os + os_tickets() + p,
+                               b.os + os_status() + (size_t)p * num_tiles * radix, b.os_err);
+            SQ_CHECK_HIP(hipGetLastError());
+            cur ^= 1;
+            continue;
+        }
         const uint32_t shift = (segp ? p - kpasses : p) * bits;
         const K* kin = (const K*)b.keys[cur];
         const uint32_t* vin = b.vals[cur];
@@ -291,6 +498,12 @@ size_t sort_hist_bytes(uint32_t n, uint32_t digit_bits) {
     return ((size_t)1 << digit_bits) * num_tiles * 4 + kRadixMax * 4;
 }
 
+size_t sort_onesweep_bytes(uint32_t n, uint32_t key_bits, uint32_t digit_bits) {
+    const size_t num_tiles = (n + kRsTile - 1) / kRsTile;
+    const size_t kpasses = (key_bits + digit_bits - 1) / digit_bits;
+    return (os_status() + kpasses * num_tiles * ((size_t)1 << digit_bits)) * 4;
+}
+
 uint32_t sort_segment_passes(uint32_t nseg, uint32_t digit_bits) {
     uint32_t segbits = 0;
     while ((1u << segbits) < nseg) segbits++;
@@ -310,9 +523,17 @@ int radix_sort_bufs(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_t d
 
 int radix_sort(sq_ctx* c, uint32_t n, uint32_t key_bits, uint32_t digit_bits, bool key64, int in, int* out_slot) {
     if (n == 0) { *out_slot = in; return SQ_OK; }
+    if (key_bits > kMaxKeyPasses) SQ_RET(SQ_ERR_INVALID, "radix sort: %u key bits", key_bits);
     SQ_TRY(c->hist.ensure(sort_hist_bytes(n, digit_bits)));
     SortBufs b{c->stream, {c->keys[0].p, c->keys[1].p}, {c->vals[0].as<uint32_t>(), c->vals[1].as<uint32_t>()},
-               c->hist.as<uint32_t>()};
+               c->hist.as<uint32_t>(), nullptr, nullptr, c->num_cus};
+    // single-pass key passes while prefix counts fit the 30-bit status words
+    const bool os = n < (1u << 30);
+    if (os) {
+        SQ_TRY(c->sort_os.ensure(sort_onesweep_bytes(n, key_bits, digit_bits)));
+        b.os = c->sort_os.as<uint32_t>();
+        SQ_TRY(sort_error_word(c, &b.os_err));  // checked after the call's final sync
+    }
     uint32_t passes = 0;
     SQ_TRY(radix_sort_bufs(b, n, key_bits, digit_bits, key64, nullptr, 0, in, out_slot, &passes));
     c->last.sort_passes = passes;
