@@ -188,6 +188,8 @@ struct LbvhArgs {
     uint32_t* depth_out;       // per scene: tree depth (root = 1), may be null
     uint32_t* counters;        // [0] small-task count, [1] max small-range levels, [2] error flag
     uint32_t small_cap;        // entries of `small`
+    uint32_t* topm;            // one-scene top phase over many workgroups: per-level range
+                               // counts [kMaxLevels + 2], then per-level barrier words
 };
 
 __device__ __forceinline__ EmitArgs scene_emit(const LbvhArgs& a, const SceneDesc& d) {
@@ -273,6 +275,108 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
     if (threadIdx.x == 0) {
         if (level < kMaxLevels) level_off[level] = s_ntop;
         a.scene_levels[scene] = level;
+    }
+}
+
+// One scene's top phase spread over gridDim.x workgroups (all resident: the
+// grid is small): the ranges of a level are dealt to every wave of the grid,
+// and the workgroups meet at a barrier (an arrival counter per level) before
+// the next level. The ranges, node slots and writes are those of
+// k_lbvh_top; only the order in which ranges of a level are split differs
+// (node slots do not depend on it). A barrier wait is bounded: past
+// kBarrierSpins it raises the error flag (counters[2] bit 4) and goes on, so
+// the launch always ends and the host reports the build as failed.
+constexpr uint32_t kBarrierSpins = 1u << 20;
+
+__device__ __forceinline__ void grid_barrier(uint32_t* word, uint32_t* err) {
+    // every wave waits for its own stores to be acknowledged (L1 writes
+    // through to this XCD's L2) before the workgroup meets
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // One agent-scope fence per workgroup: it writes this XCD's L2 back
+        // (and invalidates it) for the whole workgroup; a fence per thread
+        // would repeat that a thousand times.
+        __threadfence();
+        atomicAdd(word, 1u);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x) {
+            if (++spins > kBarrierSpins) {
+                atomicOr(err, 4u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __threadfence();  // acquire: invalidates this CU's L1 and this XCD's L2
+    }
+    __syncthreads();
+}
+
+template <typename K>
+__global__ __launch_bounds__(kTopThreads) void k_lbvh_top_grid(LbvhArgs a) {
+    const SceneDesc d = a.scenes[0];
+    const EmitArgs e = scene_emit(a, d);
+    const K* codes = (const K*)a.codes;
+    Task* q[2] = {a.qa + d.q_base, a.qb + d.q_base};
+    const uint32_t qcap = d.n / a.threshold + 4;  // ranges > T in one level are disjoint
+    uint32_t* top_list = a.top_list + d.top_base;
+    uint32_t* level_off = a.level_off + d.lvl_base;
+    uint32_t* cnt = a.topm;
+    uint32_t* arrive = a.topm + kMaxLevels + 2;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = kTopThreads / 64;
+    const uint32_t gw = blockIdx.x * nwaves + wave, total_waves = gridDim.x * nwaves;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (d.n < 8) {
+            sqd::Box3 u, lb;
+            for (uint32_t i = 0; i < d.n; i++) {
+                write_leaf(e, 1 + i, a.idx[d.seg_begin + i], &lb);
+                if (i == 0) u = lb; else sqd::box_grow(u, lb);
+            }
+            if (d.n > 0) {
+                if (a.refit) sqd::store_node(e.nodes, 0, u, 1, d.n, 0);
+                else sqd::store_node_links(e.nodes, 0, 1, d.n, 0);
+            }
+        } else {
+            sqd::store_node_links(e.nodes, 0, 1, 2, 0);
+            Task t{d.seg_begin, d.seg_begin + d.n, 1, 0, 0};
+            if (d.n > a.threshold) { q[0][0] = t; cnt[0] = 1; }
+            else push_small(a, t);
+        }
+    }
+    grid_barrier(&arrive[0], &a.counters[2]);
+    uint32_t level = 0, off = 0;
+    while (level < (uint32_t)kMaxLevels) {
+        const uint32_t ncur = __hip_atomic_load(&cnt[level], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ncur == 0) break;
+        if (blockIdx.x == 0 && threadIdx.x == 0) level_off[level] = off;
+        const Task* cur = q[level & 1];
+        Task* nxt = q[(level + 1) & 1];
+        for (uint32_t t = gw; t < ncur; t += total_waves) {
+            Task tk = cur[t];
+            uint32_t split = wave_find_split<K>(codes, tk.begin, tk.end, lane);
+            if (lane == 0) {
+                emit_children(e, tk.begin, tk.end, tk.free, split,
+                              [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
+                                  Task c{b, en, f, self, 0};
+                                  if (en - b > a.threshold) {
+                                      const uint32_t k = atomicAdd(&cnt[level + 1], 1u);
+                                      if (k < qcap) nxt[k] = c;
+                                      else atomicOr(&a.counters[2], 1u);
+                                  } else {
+                                      push_small(a, c);
+                                  }
+                              });
+                if (off + t < d.top_cap) top_list[off + t] = tk.self;
+                else atomicOr(&a.counters[2], 1u);
+            }
+        }
+        off += ncur;
+        grid_barrier(&arrive[level + 1], &a.counters[2]);
+        level++;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (level < kMaxLevels) level_off[level] = off;
+        a.scene_levels[0] = level;
     }
 }
 
@@ -420,7 +524,7 @@ void lbvh_fill_descs(SceneDesc* d, const uint32_t* counts, uint32_t nscenes, uin
 }
 
 struct WsLayout {
-    size_t qa, qb, small, top, lvl, counters, scene_levels, scene_small, total;
+    size_t qa, qb, small, top, lvl, counters, scene_levels, scene_small, topm, total;
     size_t small_cap;
 };
 static WsLayout ws_layout(uint32_t total, uint32_t nscenes, uint32_t T) {
@@ -436,13 +540,15 @@ static WsLayout ws_layout(uint32_t total, uint32_t nscenes, uint32_t T) {
     w.small_cap = sm;
     w.top = take(tp * 4);
     w.lvl = take((size_t)nscenes * (kMaxLevels + 2) * 4);
-    // counters, scene_levels and scene_small are contiguous (zeroed together)
+    // counters, scene_levels, scene_small and topm are contiguous (zeroed together)
     w.counters = off;
     off += 64;
     w.scene_levels = off;
     off += (size_t)nscenes * 4;
     w.scene_small = off;
     off += (size_t)nscenes * 4;
+    w.topm = off;
+    off += 2 * ((size_t)kMaxLevels + 2) * 4;
     off = (off + 255) & ~(size_t)255;
     w.total = off;
     return w;
@@ -454,7 +560,8 @@ size_t lbvh_batch_workspace(uint32_t total, uint32_t nscenes, uint32_t threshold
 
 static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total, const void* d_codes,
                        const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
-                       bool refit, uint32_t T, sq_node* d_nodes, void* ws, uint32_t* d_depth, bool split_refit_phase) {
+                       bool refit, uint32_t T, sq_node* d_nodes, void* ws, uint32_t* d_depth, bool split_refit_phase,
+                       bool top_grid) {
     WsLayout w = ws_layout(total, nscenes, T);
     char* base = (char*)ws;
     LbvhArgs a;
@@ -476,12 +583,24 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     a.depth_out = d_depth;
     a.counters = (uint32_t*)(base + w.counters);
     a.small_cap = (uint32_t)w.small_cap;
-    const uint32_t zero_words = 16 + 2 * nscenes;
+    a.topm = (uint32_t*)(base + w.topm);
+    const uint32_t zero_words = (uint32_t)((w.topm - w.counters) / 4) + 2 * (kMaxLevels + 2);
     uint32_t zb = (zero_words + 255) / 256;
     zb = zb > 64 ? 64 : zb;
     hipLaunchKernelGGL(k_zero_counters, dim3(zb), dim3(256), 0, stream, a.counters, zero_words);
-    if (key64) hipLaunchKernelGGL(k_lbvh_top<uint64_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
-    else hipLaunchKernelGGL(k_lbvh_top<uint32_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+    if (nscenes == 1 && top_grid) {
+        // one scene: its top levels over a few workgroups (enough waves for a
+        // level of n / T ranges, at most one workgroup per 8 CUs)
+        uint32_t wgs = (uint32_t)(total / T / (kTopThreads / 64)) + 1;
+        const uint32_t lim = (uint32_t)c->num_cus / 8 > 1 ? (uint32_t)c->num_cus / 8 : 1;
+        if (wgs > lim) wgs = lim;
+        if (key64) hipLaunchKernelGGL(k_lbvh_top_grid<uint64_t>, dim3(wgs), dim3(kTopThreads), 0, stream, a);
+        else hipLaunchKernelGGL(k_lbvh_top_grid<uint32_t>, dim3(wgs), dim3(kTopThreads), 0, stream, a);
+    } else if (key64) {
+        hipLaunchKernelGGL(k_lbvh_top<uint64_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+    } else {
+        hipLaunchKernelGGL(k_lbvh_top<uint32_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+    }
     uint32_t max_tasks = (uint32_t)(total / 8 + nscenes);
     uint32_t grid = (uint32_t)c->num_cus * 8;
     if (grid > max_tasks) grid = max_tasks;
@@ -498,8 +617,10 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
 int lbvh_emit_batch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total, const void* d_codes,
                     const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
                     uint32_t threshold, sq_node* d_nodes, void* ws, uint32_t* d_depth_out) {
+    // batched and served builds may share the device with other streams'
+    // work, so they keep the one-workgroup-per-scene top phase (no barrier)
     return lbvh_launch(c, stream, tris, total, d_codes, d_idx, key64, d_scenes, nscenes, layout, true,
-                       clamp_threshold(threshold), d_nodes, ws, d_depth_out, false);
+                       clamp_threshold(threshold), d_nodes, ws, d_depth_out, false, false);
 }
 
 int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_codes, const uint32_t* d_idx, bool key64,
@@ -515,13 +636,14 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
     char* base = c->aux[1].as<char>();
     uint32_t* d_depth = (uint32_t*)(base + w.total);
     SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit, T,
-                       d_nodes, c->aux[1].p, d_depth, true));
+                       d_nodes, c->aux[1].p, d_depth, true, true));
     uint32_t* h = (uint32_t*)c->pinned;
     SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 12, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipMemcpyAsync(h + 4, base + w.scene_levels, 4, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipMemcpyAsync(h + 5, d_depth, 4, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
     const uint32_t nsmall = h[0], top_levels = h[4];
+    if (h[2] & 4u) SQ_RET(SQ_ERR_HIP, "lbvh: a top-phase barrier wait exceeded its spin limit");
     if (h[2]) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: emission workspace overflow");
     if (top_levels >= (uint32_t)kMaxLevels) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: more than %d top levels", kMaxLevels);
     c->last.top_levels = top_levels;
