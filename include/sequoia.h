@@ -65,8 +65,11 @@ int sq_ctx_sync(sq_ctx* ctx);
 /* Opaque hipStream_t of the context (work of this library is ordered on it). */
 void* sq_ctx_stream(sq_ctx* ctx);
 
-/* Per-phase device times (ms) of the last build/trace on this context,
- * measured with HIP events recorded on the context stream. */
+/* Device times (ms) of the last build/trace on this context, measured with
+ * HIP events recorded on the context stream. total_build_ms and trace_ms are
+ * always measured; the build's phase times (bounds_ms .. grid_ms) only after
+ * sq_ctx_set_phase_timing(ctx, 1), since every phase boundary's event leaves
+ * the device idle for about 10 us. */
 typedef struct {
     float bounds_ms, morton_ms, sort_ms, emit_ms, refit_ms, sah_ms, grid_ms, total_build_ms;
     float trace_ms;
@@ -74,6 +77,8 @@ typedef struct {
     uint32_t sort_passes, tree_depth, top_levels, small_tasks;
 } sq_timings;
 int sq_get_timings(sq_ctx* ctx, sq_timings* out);
+/* 1: time each build phase separately from now on; 0 (the default): not. */
+int sq_ctx_set_phase_timing(sq_ctx* ctx, int enable);
 
 /* ---- device memory for callers without their own allocator ---- */
 int sq_device_alloc(sq_ctx* ctx, size_t bytes, void** out);

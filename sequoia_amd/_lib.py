@@ -131,6 +131,7 @@ _SIGS = {
     "sq_ctx_sync": (C.c_int, [_P]),
     "sq_ctx_stream": (_P, [_P]),
     "sq_get_timings": (C.c_int, [_P, C.POINTER(Timings)]),
+    "sq_ctx_set_phase_timing": (C.c_int, [_P, C.c_int]),
     "sq_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "sq_device_free": (None, [_P, _P]),
     "sq_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
@@ -614,6 +615,10 @@ class Context:
         t = Timings()
         _check(lib().sq_get_timings(self.h, C.byref(t)))
         return t.as_dict()
+
+    def set_phase_timing(self, enable=True):
+        """Time each build phase (bounds_ms .. grid_ms) from now on; off by default."""
+        _check(lib().sq_ctx_set_phase_timing(self.h, 1 if enable else 0))
 
     def alloc(self, nbytes):
         p = C.c_void_p()

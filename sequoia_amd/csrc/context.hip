@@ -38,11 +38,17 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+// Build phases are timed only on request (each event leaves the device idle
+// for ~10 us between kernels); traces are always timed.
 void phase_begin(sq_ctx* c, Phase p) {
+    if (p != PH_TRACE && !c->phase_timing) return;
     (void)hipEventRecord(c->ev_begin[p], c->stream);
     c->phase_used[p] = true;
 }
-void phase_end(sq_ctx* c, Phase p) { (void)hipEventRecord(c->ev_end[p], c->stream); }
+void phase_end(sq_ctx* c, Phase p) {
+    if (p != PH_TRACE && !c->phase_timing) return;
+    (void)hipEventRecord(c->ev_end[p], c->stream);
+}
 void phases_reset(sq_ctx* c) {
     for (int i = 0; i < PH_COUNT; i++) c->phase_used[i] = false;
     // calls on a ctx end with a sync, so no kernel of an earlier call can
@@ -184,6 +190,12 @@ int sq_ctx_sync(sq_ctx* c) {
 }
 
 void* sq_ctx_stream(sq_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int sq_ctx_set_phase_timing(sq_ctx* c, int enable) {
+    if (!c) SQ_RET(SQ_ERR_INVALID, "sq_ctx_set_phase_timing: NULL context");
+    c->phase_timing = enable != 0;
+    return SQ_OK;
+}
 
 int sq_get_timings(sq_ctx* c, sq_timings* out) {
     if (!c || !out) SQ_RET(SQ_ERR_INVALID, "sq_get_timings: NULL argument");

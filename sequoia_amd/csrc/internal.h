@@ -86,6 +86,7 @@ struct sq_ctx {
     sq_timings last = {};
     void* pinned = nullptr;  // small pinned host staging area
     size_t pinned_bytes = 0;
+    bool phase_timing = false;       // build phases timed separately (sq_ctx_set_phase_timing)
     uint32_t* h_sort_err = nullptr;  // pinned: the last single-pass sort's error word
     bool sort_err_pending = false;
 };

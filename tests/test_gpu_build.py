@@ -132,3 +132,21 @@ def test_large_icosphere_many_small_tasks(ctx, kind, bits):
         got = bvh.download()
         assert len(nodes_equal(got, want)) == 0, (kind, thr, nodes_equal(got, want)[:5])
         bvh.free()
+
+
+def test_phase_timing_opt_in(ctx):
+    """Build phases are timed only on request; the total is always timed."""
+    tris = sq.gen_soup(50_000, 7, 0.01)
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    t = ctx.timings()
+    assert t["total_build_ms"] > 0 and t["sort_ms"] == 0
+    bvh.free()
+    ctx.set_phase_timing(True)
+    try:
+        bvh = ctx.build(scene, "morton63")
+        t = ctx.timings()
+        assert t["bounds_ms"] > 0 and t["sort_ms"] > 0 and t["emit_ms"] > 0
+        bvh.free()
+    finally:
+        ctx.set_phase_timing(False)
