@@ -64,6 +64,33 @@ __device__ __forceinline__ void write_leaf(const EmitArgs& a, uint32_t slot, uin
     }
 }
 
+// A fan of m < 8 single-triangle leaves at slots f.. for sorted positions
+// b..b+m-1, and the union of their boxes. All index and triangle fetches are
+// issued before any result is used (one thread writes the whole fan, so
+// fetching leaf by leaf would wait for each in turn).
+__device__ __forceinline__ void write_fan(const EmitArgs& a, uint32_t f, uint32_t b, uint32_t m, sqd::Box3* u) {
+    uint32_t gt[7];
+#pragma unroll
+    for (uint32_t i = 0; i < 7; i++) gt[i] = i < m ? a.idx[b + i] : 0u;
+    if (!a.refit) {
+#pragma unroll
+        for (uint32_t i = 0; i < 7; i++)
+            if (i < m) write_leaf(a, f + i, gt[i], nullptr);
+        return;
+    }
+    float v[7][9];
+#pragma unroll
+    for (uint32_t i = 0; i < 7; i++)
+        if (i < m) sqd::load_tri(a.tris, gt[i], v[i]);
+#pragma unroll
+    for (uint32_t i = 0; i < 7; i++) {
+        if (i >= m) break;
+        const sqd::Box3 lb = sqd::tri_box(v[i]);
+        if (f + i < a.cap) sqd::store_node(a.nodes, f + i, lb, gt[i] - a.tri_base, 1, 1);
+        if (i == 0) *u = lb; else sqd::box_grow(*u, lb);
+    }
+}
+
 // Writes the two children of range [b, e) split after `split`; hands the
 // child ranges that still need splitting (count >= 8) to the callback.
 template <typename Push>
@@ -83,11 +110,8 @@ __device__ void emit_children(const EmitArgs& a, uint32_t b, uint32_t e, uint32_
     if (L == 1) {
         write_leaf(a, lfree, a.idx[b], nullptr);
     } else if (L < 8) {
-        sqd::Box3 u, lb;
-        for (uint32_t i = 0; i < L; i++) {
-            write_leaf(a, lfree + i, a.idx[b + i], &lb);
-            if (i == 0) u = lb; else sqd::box_grow(u, lb);
-        }
+        sqd::Box3 u;
+        write_fan(a, lfree, b, L, &u);
         if (lslot < a.cap) {
             if (a.refit) sqd::store_node(a.nodes, lslot, u, lfree, L, 0);
             else sqd::store_node_links(a.nodes, lslot, lfree, L, 0);
@@ -100,11 +124,8 @@ __device__ void emit_children(const EmitArgs& a, uint32_t b, uint32_t e, uint32_
     if (R == 1) {
         write_leaf(a, rfree, a.idx[b + L], nullptr);
     } else if (R < 8) {
-        sqd::Box3 u, lb;
-        for (uint32_t i = 0; i < R; i++) {
-            write_leaf(a, rfree + i, a.idx[b + L + i], &lb);
-            if (i == 0) u = lb; else sqd::box_grow(u, lb);
-        }
+        sqd::Box3 u;
+        write_fan(a, rfree, b + L, R, &u);
         if (rslot < a.cap) {
             if (a.refit) sqd::store_node(a.nodes, rslot, u, rfree, R, 0);
             else sqd::store_node_links(a.nodes, rslot, rfree, R, 0);
@@ -609,7 +630,7 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     else hipLaunchKernelGGL(k_lbvh_small<uint32_t>, dim3(grid), dim3(kSmallThreads), 0, stream, a);
     if (refit && split_refit_phase) { phase_end(c, PH_EMIT); phase_begin(c, PH_REFIT); }
     hipLaunchKernelGGL(k_lbvh_finish, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
-    if (refit && split_refit_phase) { phase_end(c, PH_REFIT); phase_begin(c, PH_EMIT); }
+    if (refit && split_refit_phase) phase_end(c, PH_REFIT);  // the emit phase ended before the refit
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
 }
