@@ -335,39 +335,110 @@ __device__ __forceinline__ Node2V load_node2(const sq_node2* nodes, uint32_t e) 
     return n;
 }
 
-// Exhaustive Node2 step: up to two Node2s (two independent 32-byte fetches)
-// per step, the current one and the next on the stack.
+// Leaf groups of a Node2 step (first triangle first[j], cnt[j] triangles,
+// hit[j]): the lane's hit groups are compacted, the first triangle of each is
+// fetched before any is tested, and the rest of longer groups follow.
+template <int N>
+__device__ __forceinline__ void leaf_batch2(const TraceArgs& a, const uint32_t* first, const uint32_t* cnt,
+                                            const bool* hit, LaneRay& L, uint32_t& tri_tests, bool& bad) {
+    bool lf[N];
+    uint32_t pos[N], c = 0;
+    bool longer = false;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        lf[j] = hit[j] && cnt[j] > 0;
+        if (lf[j] && first[j] >= a.tri_count) bad = true;
+        pos[j] = c;
+        c += lf[j] ? 1u : 0u;
+        longer |= lf[j] && cnt[j] > 1;
+    }
+    if (bad) return;
+    uint32_t ids[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int j = k; j < N; j++) v = (lf[j] && pos[j] == (uint32_t)k) ? first[j] : v;
+        ids[k] = v;
+    }
+    float tv[N][9];
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if ((uint32_t)k < c) sqd::load_tri(a.tris, ids[k], tv[k]);
+#pragma unroll
+    for (int k = 0; k < N; k++)
+        if ((uint32_t)k < c) {
+            float t;
+            tri_tests++;
+            if (tri_test(tv[k], L.r, t) && t > L.r.tmin && t < L.r.tmax && t <= L.best_t) {
+                const uint32_t id = a.perm[ids[k]];
+                if (t < L.best_t || id < L.best_id) {
+                    L.best_t = t;
+                    L.best_id = id;
+                }
+            }
+        }
+    if (longer) {
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (lf[j] && cnt[j] > 1) leaf_tris2(a, first[j] + 1, cnt[j] - 1, L, tri_tests, bad);
+    }
+}
+
+// Exhaustive Node2 step: up to WL Node2s (independent 32-byte fetches) per
+// step, the current one and the next ones on the stack; a step pops at most
+// WL entries and pushes at most 2 WL, so at most WL per level stay behind.
+template <int WL>
 __device__ __forceinline__ bool lane_step2_wide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     float te = 0.f;
     if (st.sp == 0) return false;
-    const uint32_t e0 = pop<false>(st, te);
-    const bool two = st.sp > 0;
-    const uint32_t e1 = two ? pop<false>(st, te) : 0u;
-    if (e0 >= a.node_count || (two && e1 >= a.node_count)) { st.fault |= kFaultNode; return false; }
-    const Node2V n0 = load_node2(a.nodes2, e0);
-    const Node2V n1 = load_node2(a.nodes2, two ? e1 : e0);  // unconditional: no branch between fetches
-    L.tests += two ? 4u : 2u;
+    uint32_t e[WL];
+    bool use[WL];
+#pragma unroll
+    for (int g = 0; g < WL; g++) {
+        use[g] = g == 0 || st.sp > 0;
+        e[g] = use[g] ? pop<false>(st, te) : 0u;
+        if (use[g] && e[g] >= a.node_count) { st.fault |= kFaultNode; return false; }
+        if (!use[g]) e[g] = e[0];  // in-bounds repeat, never tested
+    }
+    Node2V n[WL];
+#pragma unroll
+    for (int g = 0; g < WL; g++) n[g] = load_node2(a.nodes2, e[g]);  // unconditional: no branch between fetches
+    bool h[2 * WL];
+    uint32_t first[2 * WL], cnt[2 * WL];
     float f;
-    const bool h0 = slab(n0.a, L.r, L.ix, L.iy, L.iz, L.fast, f);
-    const bool h1 = slab(n0.b, L.r, L.ix, L.iy, L.iz, L.fast, f);
-    const bool h2 = two && slab(n1.a, L.r, L.ix, L.iy, L.iz, L.fast, f);
-    const bool h3 = two && slab(n1.b, L.r, L.ix, L.iy, L.iz, L.fast, f);
+#pragma unroll
+    for (int g = 0; g < WL; g++) {
+        L.tests += use[g] ? 2u : 0u;
+        h[2 * g] = use[g] && slab(n[g].a, L.r, L.ix, L.iy, L.iz, L.fast, f);
+        h[2 * g + 1] = use[g] && slab(n[g].b, L.r, L.ix, L.iy, L.iz, L.fast, f);
+        first[2 * g] = n[g].ca;
+        first[2 * g + 1] = n[g].cb;
+        cnt[2 * g] = n[g].la ? n[g].cnta : 0u;
+        cnt[2 * g + 1] = n[g].lb ? n[g].cntb : 0u;
+    }
     bool bad = false;
-    if (h0 && n0.la) leaf_tris2(a, n0.ca, n0.cnta, L, tri_tests, bad);
-    if (h1 && n0.lb) leaf_tris2(a, n0.cb, n0.cntb, L, tri_tests, bad);
-    if (h2 && n1.la) leaf_tris2(a, n1.ca, n1.cnta, L, tri_tests, bad);
-    if (h3 && n1.lb) leaf_tris2(a, n1.cb, n1.cntb, L, tri_tests, bad);
+    {
+        bool hl[2 * WL];
+#pragma unroll
+        for (int g = 0; g < WL; g++) {
+            hl[2 * g] = h[2 * g] && n[g].la;
+            hl[2 * g + 1] = h[2 * g + 1] && n[g].lb;
+        }
+        leaf_batch2<2 * WL>(a, first, cnt, hl, L, tri_tests, bad);
+    }
     if (bad) { st.fault |= kFaultTri; return false; }
-    if (h0 && !n0.la) push<false>(st, n0.ca, 0.f);
-    if (h1 && !n0.lb) push<false>(st, n0.cb, 0.f);
-    if (h2 && !n1.la) push<false>(st, n1.ca, 0.f);
-    if (h3 && !n1.lb) push<false>(st, n1.cb, 0.f);
+#pragma unroll
+    for (int g = 0; g < WL; g++) {
+        if (h[2 * g] && !n[g].la) push<false>(st, n[g].ca, 0.f);
+        if (h[2 * g + 1] && !n[g].lb) push<false>(st, n[g].cb, 0.f);
+    }
     return st.fault == 0;
 }
 
 // Node2 step: the root box on the first step, then one Node2 (both children
 // from one 32-byte fetch) per step; exhaustive semantics take two.
-template <bool CLOSEST>
+template <bool CLOSEST, int WL>
 __device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     if (L.cnt) {
         L.cnt = 0;
@@ -386,7 +457,7 @@ __device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack
         push<CLOSEST>(st, a.root, f);
         return st.fault == 0;
     }
-    if (!CLOSEST) return lane_step2_wide(a, L, st, tri_tests);
+    if (!CLOSEST) return lane_step2_wide<WL>(a, L, st, tri_tests);
     const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
     uint32_t e = 0;
     while (true) {
@@ -409,8 +480,11 @@ __device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack
     const uint32_t ca = n.ca, cb = n.cb, cnta = n.cnta, cntb = n.cntb;
     const bool la = n.la, lb = n.lb;
     bool bad = false;
-    if (ha && la) leaf_tris2(a, ca, cnta, L, tri_tests, bad);
-    if (hb && lb) leaf_tris2(a, cb, cntb, L, tri_tests, bad);
+    {
+        const uint32_t fs[2] = {ca, cb}, cs[2] = {la ? cnta : 0u, lb ? cntb : 0u};
+        const bool hl[2] = {ha && la, hb && lb};
+        leaf_batch2<2>(a, fs, cs, hl, L, tri_tests, bad);
+    }
     if (bad) { st.fault |= kFaultTri; return false; }
     const bool pa = ha && !la, pb = hb && !lb;
     if (pa && pb) {
@@ -482,7 +556,7 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
             continue;
         }
         if (active) {
-            const bool more = FMT == 2 ? lane_step2<CLOSEST>(a, L, st, tris_total)
+            const bool more = FMT == 2 ? lane_step2<CLOSEST, WL>(a, L, st, tris_total)
                                        : (CLOSEST ? lane_step<CLOSEST>(a, L, st, tris_total)
                                                   : lane_step_wide<WL>(a, L, st, tris_total));
             if (!more) {
@@ -673,6 +747,9 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
 int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     if (L.closest) hipLaunchKernelGGL((k_trace<true, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+    else if (L.lists == kCameraLists)
+        hipLaunchKernelGGL((k_trace<false, false, 2, kCameraLists>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream,
+                           a);
     else hipLaunchKernelGGL((k_trace<false, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
