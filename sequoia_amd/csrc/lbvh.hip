@@ -609,18 +609,24 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     uint32_t zb = (zero_words + 255) / 256;
     zb = zb > 64 ? 64 : zb;
     hipLaunchKernelGGL(k_zero_counters, dim3(zb), dim3(256), 0, stream, a.counters, zero_words);
+    bool grid_done = false;
     if (nscenes == 1 && top_grid) {
-        // one scene: its top levels over a few workgroups (enough waves for a
-        // level of n / T ranges, at most one workgroup per 8 CUs)
+        // One scene: its top levels over a few workgroups (enough waves for a
+        // level of n / T ranges, at most one workgroup per 8 CUs). The
+        // cooperative launch makes the runtime start all of them together,
+        // which the level barriers rely on; where it is refused, the
+        // one-workgroup kernel below runs instead.
         uint32_t wgs = (uint32_t)(total / T / (kTopThreads / 64)) + 1;
         const uint32_t lim = (uint32_t)c->num_cus / 8 > 1 ? (uint32_t)c->num_cus / 8 : 1;
         if (wgs > lim) wgs = lim;
-        if (key64) hipLaunchKernelGGL(k_lbvh_top_grid<uint64_t>, dim3(wgs), dim3(kTopThreads), 0, stream, a);
-        else hipLaunchKernelGGL(k_lbvh_top_grid<uint32_t>, dim3(wgs), dim3(kTopThreads), 0, stream, a);
-    } else if (key64) {
-        hipLaunchKernelGGL(k_lbvh_top<uint64_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
-    } else {
-        hipLaunchKernelGGL(k_lbvh_top<uint32_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+        void* args[] = {&a};
+        const void* fn = key64 ? (const void*)k_lbvh_top_grid<uint64_t> : (const void*)k_lbvh_top_grid<uint32_t>;
+        grid_done = hipLaunchCooperativeKernel(fn, dim3(wgs), dim3(kTopThreads), args, 0, stream) == hipSuccess;
+        if (!grid_done) (void)hipGetLastError();  // clear the refusal
+    }
+    if (!grid_done) {
+        if (key64) hipLaunchKernelGGL(k_lbvh_top<uint64_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+        else hipLaunchKernelGGL(k_lbvh_top<uint32_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
     }
     uint32_t max_tasks = (uint32_t)(total / 8 + nscenes);
     uint32_t grid = (uint32_t)c->num_cus * 8;
