@@ -305,9 +305,10 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
 // the next level. The ranges, node slots and writes are those of
 // k_lbvh_top; only the order in which ranges of a level are split differs
 // (node slots do not depend on it). A barrier wait is bounded: past
-// kBarrierSpins it raises the error flag (counters[2] bit 4) and goes on, so
-// the launch always ends and the host reports the build as failed.
-constexpr uint32_t kBarrierSpins = 1u << 20;
+// kBarrierSpins (seconds: long enough to wait out other streams' kernels
+// that hold the CUs) it raises the error flag (counters[2] bit 4) and goes
+// on, so the launch always ends and the host reports the build as failed.
+constexpr uint32_t kBarrierSpins = 1u << 26;
 
 __device__ __forceinline__ void grid_barrier(uint32_t* word, uint32_t* err) {
     // every wave waits for its own stores to be acknowledged (L1 writes
@@ -612,17 +613,15 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     bool grid_done = false;
     if (nscenes == 1 && top_grid) {
         // One scene: its top levels over a few workgroups (enough waves for a
-        // level of n / T ranges, at most one workgroup per 8 CUs). The
-        // cooperative launch makes the runtime start all of them together,
-        // which the level barriers rely on; where it is refused, the
-        // one-workgroup kernel below runs instead.
+        // level of n / T ranges, at most one workgroup per 8 CUs, so the grid
+        // always fits the device at once; work of other streams can delay a
+        // workgroup's start, which the bounded barrier waits out).
         uint32_t wgs = (uint32_t)(total / T / (kTopThreads / 64)) + 1;
         const uint32_t lim = (uint32_t)c->num_cus / 8 > 1 ? (uint32_t)c->num_cus / 8 : 1;
         if (wgs > lim) wgs = lim;
-        void* args[] = {&a};
-        const void* fn = key64 ? (const void*)k_lbvh_top_grid<uint64_t> : (const void*)k_lbvh_top_grid<uint32_t>;
-        grid_done = hipLaunchCooperativeKernel(fn, dim3(wgs), dim3(kTopThreads), args, 0, stream) == hipSuccess;
-        if (!grid_done) (void)hipGetLastError();  // clear the refusal
+        if (key64) hipLaunchKernelGGL(k_lbvh_top_grid<uint64_t>, dim3(wgs), dim3(kTopThreads), 0, stream, a);
+        else hipLaunchKernelGGL(k_lbvh_top_grid<uint32_t>, dim3(wgs), dim3(kTopThreads), 0, stream, a);
+        grid_done = true;
     }
     if (!grid_done) {
         if (key64) hipLaunchKernelGGL(k_lbvh_top<uint64_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
