@@ -702,11 +702,12 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, L.block, L.lds_bytes);
     if (per_cu < 1) per_cu = 8;
     // Default waves per CU, measured: one camera's primary rays run best at 8
-    // (1080p x 4 jittered over 1M-triangle Morton and SAH trees: fewer rays in
-    // flight keep more of their nodes in L1/L2, 16.1 ms against 19.0 ms at
-    // full occupancy and 27 ms at 4); ray lists (incoherent secondary rays on
-    // 10M clustered triangles: 0.78 s at 16 against 1.17 s at 8) and
-    // many-scene launches (256 scenes: 30 ms at 16 against 49 ms at 8) want 16.
+    // (1080p x 4 jittered over 1M-triangle Morton and SAH trees, four lists
+    // per step: 10.35 ms at 8, 10.7 at 7; with two lists per step 16.1 ms at 8
+    // against 19.0 ms at full occupancy and 27 ms at 4: more rays in flight
+    // cost more than they hide); ray lists (incoherent secondary rays on 10M
+    // clustered triangles: 0.78 s at 16 against 1.17 s at 8) and many-scene
+    // launches (256 scenes: 30 ms at 16 against 49 ms at 8) want 16.
     const uint32_t waves = coherent ? kCameraWavesPerCu : kWavesPerCu;
     const int target = (int)(waves * 64 / L.block);
     if (per_cu > target) per_cu = target;
