@@ -682,22 +682,34 @@ __global__ __launch_bounds__(kSahThreads) void k_big_bin(SahArgs a) {
         a.part_bins[(size_t)chunk * words + w] = s_bins[0][w / kBinWords][w % kBinWords];
 }
 
-// Folds a task's chunk bins (all threads over (chunk, word) pairs, LDS
-// atomics on the ordered-integer images) and chooses its plane.
+// Folds a task's chunk bins and chooses its plane. Each thread owns bin words
+// and folds them over all chunks in registers (a chunk's words are adjacent,
+// so the threads' loads of one chunk are coalesced); LDS atomics from every
+// (chunk, word) pair serialised on the task's few hundred words (33 us for
+// the 488 chunks of a 1M-primitive root at 32 bins, against 5 us this way).
 __global__ __launch_bounds__(kSahThreads) void k_big_plane(SahArgs a) {
     __shared__ uint32_t s_red[kMaxBins][kBinWords];
     const SahTask t = a.big[blockIdx.x];
     const uint32_t words = a.nb * kBinWords;
-    for (uint32_t w = threadIdx.x; w < words; w += kSahThreads) s_red[w / kBinWords][w % kBinWords] = bin_identity(w % kBinWords);
-    __syncthreads();
     const uint32_t* src = a.part_bins + (size_t)t.chunk0 * words;
-    const uint32_t total = t.nchunks * words;
-    for (uint32_t i = threadIdx.x; i < total; i += kSahThreads) {
-        const uint32_t w = i % words, f = w % kBinWords, v = src[i];
-        uint32_t* d = &s_red[w / kBinWords][f];
-        if (f == 0) atomicAdd(d, v);
-        else if ((f - 1) % 6 < 3) atomicMin(d, v);
-        else atomicMax(d, v);
+    for (uint32_t w = threadIdx.x; w < words; w += kSahThreads) {
+        const uint32_t f = w % kBinWords;
+        const uint32_t op = f == 0 ? 0u : ((f - 1) % 6 < 3 ? 1u : 2u);  // sum, min, max
+        uint32_t acc = bin_identity(f);
+        // eight chunks' words fetched before any is folded
+        uint32_t c = 0;
+        for (; c + 8 <= t.nchunks; c += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = src[(size_t)(c + k) * words + w];
+#pragma unroll
+            for (int k = 0; k < 8; k++) acc = op == 0 ? acc + v[k] : (op == 1 ? min(acc, v[k]) : max(acc, v[k]));
+        }
+        for (; c < t.nchunks; c++) {
+            const uint32_t v = src[(size_t)c * words + w];
+            acc = op == 0 ? acc + v : (op == 1 ? min(acc, v) : max(acc, v));
+        }
+        s_red[w / kBinWords][f] = acc;
     }
     __syncthreads();
     if (threadIdx.x < 64) {
