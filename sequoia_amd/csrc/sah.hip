@@ -106,9 +106,13 @@ __device__ __forceinline__ int choose_axis(const float* cb) {
     return r;
 }
 
+// v[k] for k in 0..2 by selects: an index known only at run time into a
+// small array would put the array in scratch memory.
+__device__ __forceinline__ float sel3(const float* v, int k) { return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]); }
+
 __device__ __forceinline__ float bin_k1(uint32_t nb, const float* cb, int axis) {
     const float epsilon = 1.1920929e-7f;
-    return (float)nb * (1 - epsilon) / (cb[3 + axis] - cb[axis]);
+    return (float)nb * (1 - epsilon) / (sel3(cb + 3, axis) - sel3(cb, axis));
 }
 
 __device__ __forceinline__ void prim_box(const float4* boxes, uint32_t id, float mn[3], float mx[3]) {
@@ -197,7 +201,7 @@ __device__ __forceinline__ void bin_range(const SahArgs& a, uint32_t lo, uint32_
         prim_box(a.boxes, id, mn, mx);
 #pragma unroll
         for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
-        int bi = (int)(k1 * (c[axis] - cmin));
+        int bi = (int)(k1 * (sel3(c, axis) - cmin));
         bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
         if (keep) a.bin_of[i] = (uint8_t)bi;
         uint32_t* B = s[wave][bi];
@@ -377,7 +381,7 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
     const uint32_t b = t.b, e = t.e, nb = a.nb;
     const int axis = choose_axis(t.cb);
     const float k1 = bin_k1(nb, t.cb, axis);
-    const float cmin = t.cb[axis];
+    const float cmin = sel3(t.cb, axis);
     uint32_t(*B)[kBinWords] = s_bins[wave];
 
     for (uint32_t i = lane; i < nb * kBinWords; i += 64) B[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
@@ -388,7 +392,7 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
         prim_box(a.boxes, id, mn, mx);
 #pragma unroll
         for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
-        int bi = (int)(k1 * (c[axis] - cmin));
+        int bi = (int)(k1 * (sel3(c, axis) - cmin));
         bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
         a.bin_of[i] = (uint8_t)bi;
         uint32_t* W = B[bi];
@@ -513,7 +517,7 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
         const uint32_t lo = E.lo, hi = E.hi, cnt = hi - lo;
         const int axis = choose_axis(E.cb);
         const float k1 = bin_k1(nb, E.cb, axis);
-        const float cmin = E.cb[axis];
+        const float cmin = sel3(E.cb, axis);
         for (uint32_t i = lane; i < nb * kBinWords; i += 64) s_bins[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
         wave_sync();
         for (uint32_t j = lo + lane; j < hi; j += 64) {
@@ -525,7 +529,7 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
                 mx[k] = s_box[sl][3 + k];
                 c[k] = (mn[k] + mx[k]) * 0.5f;
             }
-            int bi = (int)(k1 * (c[axis] - cmin));
+            int bi = (int)(k1 * (sel3(c, axis) - cmin));
             bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
             s_bin[j] = (uint8_t)bi;
             uint32_t* W = s_bins[bi];
@@ -615,27 +619,29 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
             if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2);
             if (R != 1) write_inner_box(a, right_index, P.rp, P.rp + 3, right_free, right_leaf ? R : 2);
         }
-        SubEntry C[2];
-        uint32_t nc = 0;
-        // larger child first: the smaller one is popped next
+        // larger child first: the smaller one is popped next (entries built
+        // in registers and written to fixed positions: an array indexed by a
+        // running count would live in scratch memory)
         const bool left_first = L >= R;
-        for (int side = 0; side < 2; side++) {
-            const bool left = (side == 0) == left_first;
-            if (left ? left_leaf : right_leaf) continue;
-            SubEntry& c = C[nc++];
-            c.lo = left ? lo : lo + L;
-            c.hi = left ? lo + L : hi;
-            c.free = left ? left_free : right_free;
-            c.self = left ? left_index : right_index;
-            c.level = E.level + 1;
-            for (int k = 0; k < 6; k++) c.cb[k] = left ? P.lc[k] : P.rc[k];
-        }
+        const bool push_l = !left_leaf, push_r = !right_leaf;
+        const uint32_t nc = (push_l ? 1u : 0u) + (push_r ? 1u : 0u);
         if (sp + nc > (uint32_t)kSubStack) {
             if (lane == 0) atomicOr(&a.counters[4], 2u);
             break;
         }
-        if (lane == 0)
-            for (uint32_t i = 0; i < nc; i++) s_stack[sp + i] = C[i];
+        if (lane == 0) {
+            SubEntry cl, cr;
+            cl.lo = lo; cl.hi = lo + L; cl.free = left_free; cl.self = left_index; cl.level = E.level + 1;
+            cr.lo = lo + L; cr.hi = hi; cr.free = right_free; cr.self = right_index; cr.level = E.level + 1;
+            for (int k = 0; k < 6; k++) {
+                cl.cb[k] = P.lc[k];
+                cr.cb[k] = P.rc[k];
+            }
+            const uint32_t pos_l = sp + (left_first ? 0u : (push_r ? 1u : 0u));
+            const uint32_t pos_r = sp + (left_first ? (push_l ? 1u : 0u) : 0u);
+            if (push_l) s_stack[pos_l] = cl;
+            if (push_r) s_stack[pos_r] = cr;
+        }
         sp += nc;
         wave_sync();
     }
@@ -673,7 +679,7 @@ __global__ __launch_bounds__(kSahThreads) void k_big_bin(SahArgs a) {
     const float k1 = bin_k1(a.nb, t.cb, axis);
     bins_clear(s_bins, a.nb);
     __syncthreads();
-    bin_range(a, lo, hi, axis, k1, t.cb[axis], s_bins, true);
+    bin_range(a, lo, hi, axis, k1, sel3(t.cb, axis), s_bins, true);
     __syncthreads();
     bins_merge_waves(s_bins, a.nb);
     __syncthreads();
