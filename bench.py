@@ -534,7 +534,7 @@ def main():
         "heatmap_mean_tests_per_pixel": round(float(heat.mean()) / SPP, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_trace<false, false, 1, 4>", "algorithmic_bytes_per_launch": alg_bytes},
+                     "kernel": "k_trace<false, false, 1, 2>", "algorithmic_bytes_per_launch": alg_bytes},
     }
     # the secondary sections never take the headline line down with them
     if not a.no_sah:

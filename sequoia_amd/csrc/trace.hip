@@ -202,11 +202,13 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
 // level), and pushes at most 2 WL lists of the next level, so at most WL
 // lists per level stay behind: WL x depth + 2 WL entries.
 //
-// WL = 4 for one camera's rays (199 VGPRs, two waves per SIMD: the launch
-// shape those rays want anyway; 1080p x 4 on a 1M-triangle Morton tree:
-// 10.35 ms at 4 lists, 10.65 at 3, 11.28 at 2, with 32 stack slots), WL = 2
-// for ray lists and many-job launches, which want more waves per CU.
-constexpr int kCameraLists = 4, kListLists = 2;
+// WL = 2 by default (three or four waves per SIMD); WL = 4 (199 VGPRs, two
+// waves per SIMD) on request. With 8x8-pixel camera tiles, whose last costly
+// tiles left half the waves idle, four lists at 8 waves per CU ran best
+// (1080p x 4 on a 1M-triangle Morton tree: 10.35 ms at 4 lists, 11.28 at 2);
+// with 4x4 tiles two lists at 12 waves per CU are faster (6.35 against
+// 7.07 ms).
+constexpr int kWideLists = 4, kListLists = 2;
 
 template <int WL>
 __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
@@ -684,18 +686,18 @@ int trace_fault(uint32_t bits, const char* what) {
            (bits & kFaultTri) ? " triangle index outside the scene" : "");
 }
 
-constexpr uint32_t kCameraWavesPerCu = 8, kWavesPerCu = 16;
+constexpr uint32_t kCameraWavesPerCu = 12, kCameraClosestWavesPerCu = 8, kWavesPerCu = 16;
 
-int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out) {
+int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out, bool node2) {
     TracePlan L;
     L.closest = p && p->semantics == SQ_TRACE_CLOSEST;
-    L.lists = !L.closest && coherent ? kCameraLists : kListLists;
+    L.lists = kListLists;
     if (p && p->lists_per_step) {
-        if (p->lists_per_step != kListLists && p->lists_per_step != kCameraLists)
+        if (p->lists_per_step != kListLists && p->lists_per_step != kWideLists)
             SQ_RET(SQ_ERR_INVALID, "trace lists_per_step %u (2 or 4)", p->lists_per_step);
         if (!L.closest) L.lists = p->lists_per_step;
     }
-    uint32_t slots = p && p->stack_slots ? p->stack_slots : (L.lists == kCameraLists ? 32 : 16);
+    uint32_t slots = p && p->stack_slots ? p->stack_slots : (L.lists == kWideLists ? 32 : 16);
     uint32_t pow2 = 1;
     while (pow2 < slots) pow2 <<= 1;
     L.slots = pow2 > 64 ? 64 : pow2;
@@ -703,22 +705,34 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     if (L.block != 64 && L.block != 128) SQ_RET(SQ_ERR_INVALID, "trace block size %u (64 or 128)", L.block);
     L.lds_bytes = ((size_t)L.slots * (L.closest ? 2 : 1) + 1) * L.block * 4;
     int per_cu = 0;
-    if (L.closest)
+    if (node2 && L.closest)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 2>, L.block, L.lds_bytes);
+    else if (node2 && L.lists == kWideLists)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 2, kWideLists>, L.block,
+                                                           L.lds_bytes);
+    else if (node2)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 2>, L.block, L.lds_bytes);
+    else if (L.closest)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false>, L.block, L.lds_bytes);
-    else if (L.lists == kCameraLists)
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 1, kCameraLists>, L.block,
+    else if (L.lists == kWideLists)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 1, kWideLists>, L.block,
                                                            L.lds_bytes);
     else
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, L.block, L.lds_bytes);
     if (per_cu < 1) per_cu = 8;
-    // Default waves per CU, measured: one camera's primary rays run best at 8
-    // (1080p x 4 jittered over 1M-triangle Morton and SAH trees, four lists
-    // per step: 10.35 ms at 8, 10.7 at 7; with two lists per step 16.1 ms at 8
-    // against 19.0 ms at full occupancy and 27 ms at 4: more rays in flight
-    // cost more than they hide); ray lists (incoherent secondary rays on 10M
-    // clustered triangles: 0.78 s at 16 against 1.17 s at 8) and many-scene
-    // launches (256 scenes: 30 ms at 16 against 49 ms at 8) want 16.
-    const uint32_t waves = coherent ? kCameraWavesPerCu : kWavesPerCu;
+    // Default waves per CU, measured on 1080p x 4 jittered over the 1M soup
+    // (Morton63 tree; 4x4-pixel tiles): exhaustive Node1 steps run best at 12
+    // (two lists per step: 6.35 ms at 12, 6.59 at 16, 7.25 at 10, 7.77 at 8;
+    // four lists per step reach only 8 waves, 7.07 ms), Node2 steps at 16
+    // (5.78 ms against 5.91 at 12); closest hits on 8x8 tiles at 8 (Morton
+    // 2.36 ms, SAH 1.96 ms; more waves than that help the one and hurt the
+    // other); ray lists (incoherent secondary rays on 10M clustered
+    // triangles: 0.78 s at 16 against 1.17 s at 8) and many-scene launches
+    // (256 scenes: 30 ms at 16 against 49 ms at 8) want 16.
+    const uint32_t waves = !coherent ? kWavesPerCu
+                           : L.closest ? kCameraClosestWavesPerCu
+                           : node2     ? kWavesPerCu
+                                       : kCameraWavesPerCu;
     const int target = (int)(waves * 64 / L.block);
     if (per_cu > target) per_cu = target;
     if (p && p->blocks_per_cu) {
@@ -740,14 +754,14 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     if (multi) {
         if (L.closest) hipLaunchKernelGGL((k_trace<true, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-        else if (L.lists == kCameraLists)
-            hipLaunchKernelGGL((k_trace<false, true, 1, kCameraLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
+        else if (L.lists == kWideLists)
+            hipLaunchKernelGGL((k_trace<false, true, 1, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
                                stream, a);
         else hipLaunchKernelGGL((k_trace<false, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     } else {
         if (L.closest) hipLaunchKernelGGL((k_trace<true, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-        else if (L.lists == kCameraLists)
-            hipLaunchKernelGGL((k_trace<false, false, 1, kCameraLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
+        else if (L.lists == kWideLists)
+            hipLaunchKernelGGL((k_trace<false, false, 1, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
                                stream, a);
         else hipLaunchKernelGGL((k_trace<false, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     }
@@ -758,8 +772,8 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
 int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     if (L.closest) hipLaunchKernelGGL((k_trace<true, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-    else if (L.lists == kCameraLists)
-        hipLaunchKernelGGL((k_trace<false, false, 2, kCameraLists>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream,
+    else if (L.lists == kWideLists)
+        hipLaunchKernelGGL((k_trace<false, false, 2, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream,
                            a);
     else hipLaunchKernelGGL((k_trace<false, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     SQ_CHECK_HIP(hipGetLastError());
@@ -819,7 +833,7 @@ static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, con
         return rc;
     }
     TracePlan L;
-    SQ_TRY(trace_plan(c, p, depth, a.camera != 0u, &L));
+    SQ_TRY(trace_plan(c, p, depth, a.camera != 0u, &L, node2));
     SQ_TRY(c->spill.ensure(L.spill_words * 4));
     SQ_TRY(c->counters.ensure(64));
     SQ_TRY(c->stats.ensure(64));
@@ -884,7 +898,8 @@ static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_c
     // frame are short. 1080p x 4 over the 1M soup: 7.08 ms, against 9.55 ms
     // with 8x8 tiles, whose ~2,500 costly tiles fell on 2,048 waves (half the
     // waves idle for the last fifth of the launch). Packets keep 8x8.
-    a.tile_shift = (cam->spp >= 4 && (t2 || !use_packets(p))) ? 2u : 3u;
+    const bool exhaustive = !p || p->semantics == SQ_TRACE_EXHAUSTIVE;
+    a.tile_shift = (exhaustive && cam->spp >= 4 && (t2 || !use_packets(p))) ? 2u : 3u;
     const uint64_t tile_edge = 1ull << a.tile_shift;
     const uint64_t tx = (cam->width + tile_edge - 1) >> a.tile_shift, ty = (cam->height + tile_edge - 1) >> a.tile_shift;
     if (tx * ty >= (1ull << 31)) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: %ux%u image has too many tiles", cam->width,

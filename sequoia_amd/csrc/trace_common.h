@@ -240,7 +240,8 @@ struct TracePlan {
     size_t lds_bytes, spill_words;
 };
 // coherent: one camera's primary rays (a smaller default launch).
-int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out);
+int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out,
+               bool node2 = false);
 // SQ_ERR_OVERFLOW with a message naming the fault bits.
 int trace_fault(uint32_t bits, const char* what);
 // Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).

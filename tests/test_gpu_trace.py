@@ -67,6 +67,27 @@ def test_jittered_pinhole_on_gpu_tree(ctx, name, kind, kernel):
     check_hits(got["hit_id"], got["hit_t"], want["hit_id"], want["hit_t"])
 
 
+@pytest.mark.parametrize("lists", [0, 4])
+@pytest.mark.parametrize("spp", [1, 4, 5])
+def test_ragged_image_tiles(ctx, spp, lists):
+    """Image edges that cut camera tiles (203 x 117: 4x4 tiles for 4+
+    samples, 8x8 below, odd and even tile-row counts for the rows dealt from
+    the middle outwards) against the oracle, both step widths."""
+    say("ragged image", spp, lists)
+    tris = GOLDEN_SCENES["clus3000"]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    nodes = bvh.download()
+    cam = sq.pinhole_camera(tris, 203, 117, 60.0, 3.0, spp=spp, jitter=1, seed=11)
+    got = ctx.trace_camera(bvh, cam, lists_per_step=lists)
+    want = ob.trace_camera(nodes, bvh.root, tris, cam)
+    assert np.array_equal(got["counts"], want["counts"])
+    assert np.array_equal(got["rgb"], want["rgb"])
+    check_hits(got["hit_id"], got["hit_t"], want["hit_id"], want["hit_t"])
+    cl = ctx.trace_camera(bvh, cam, semantics=sq.TRACE_CLOSEST)
+    check_hits(cl["hit_id"], cl["hit_t"], want["hit_id"], want["hit_t"])
+
+
 @pytest.mark.parametrize("name", ["ico69632", "clus3000"])
 def test_closest_mode_matches_exhaustive_hits(ctx, name):
     say("closest", name)
@@ -165,7 +186,8 @@ def test_packet_and_bundle_on_gpu_morton_tree(ctx):
 def test_packets_equal_rays_at_full_size(ctx, kind):
     """The bench frame (1M soup, 1920x1080x4 jittered): the wave-packet
     kernel's per-pixel counts, heatmap bytes and per-ray hits equal the
-    per-lane kernel's bit for bit, on a Morton and on a SAH tree."""
+    per-lane kernel's bit for bit (two and four lists per step), on a Morton
+    and on a SAH tree."""
     say("packets vs rays", kind)
     tris = sq.gen_soup(1_000_000, 2, 0.01)
     scene = ctx.upload(tris)
@@ -173,5 +195,7 @@ def test_packets_equal_rays_at_full_size(ctx, kind):
     cam = sq.pinhole_camera(tris, 1920, 1080, 60.0, 3.0, spp=4, jitter=1, seed=2)
     a = ctx.trace_camera(bvh, cam, kernel=sq.KERNEL_RAYS)
     b = ctx.trace_camera(bvh, cam, kernel=sq.KERNEL_PACKETS)
+    c = ctx.trace_camera(bvh, cam, kernel=sq.KERNEL_RAYS, lists_per_step=4)
     for k in ("hit_id", "hit_t", "counts", "rgb"):
         assert np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8), np.ascontiguousarray(b[k]).view(np.uint8)), k
+        assert np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8), np.ascontiguousarray(c[k]).view(np.uint8)), k
