@@ -536,6 +536,10 @@ int sq_batch_set_cameras(sq_ctx* c, sq_batch* b, const sq_camera* cams, const sq
     std::vector<size_t> ray_off(b->nscenes + 1), pix_off(b->nscenes + 1);
     std::vector<uint32_t> job_tiles(b->nscenes + 1);
     uint64_t tiles = 0;
+    // 8x8-pixel tiles: with hundreds of jobs in one launch the tiles are
+    // plentiful, and 4x4 ones only add per-tile work (256 scenes: 29.9 ms
+    // trace with 4x4 tiles against 23.4 ms).
+    const bool quad = false;
     for (uint32_t s = 0; s < b->nscenes; s++) {
         const sq_camera& k = cams[s];
         if (k.width == 0 || k.height == 0 || k.spp == 0) SQ_RET(SQ_ERR_INVALID, "camera %u: empty image", s);
@@ -544,7 +548,9 @@ int sq_batch_set_cameras(sq_ctx* c, sq_batch* b, const sq_camera* cams, const sq
         job_tiles[s] = (uint32_t)tiles;
         pix += (size_t)k.width * k.height;
         rays += (size_t)k.width * k.height * k.spp;
-        tiles += (uint64_t)((k.width + 7) / 8) * ((k.height + 7) / 8);
+        TraceArgs g{};
+        tiles += camera_tiling(k.width, k.height, k.spp, quad, g);
+        if (tiles >= (1ull << 32)) break;
     }
     ray_off[b->nscenes] = rays;
     pix_off[b->nscenes] = pix;
@@ -576,8 +582,7 @@ int sq_batch_set_cameras(sq_ctx* c, sq_batch* b, const sq_camera* cams, const sq
         a.tanh = k.tan_half_fov;
         a.w = k.width; a.h = k.height; a.spp = k.spp; a.jitter = k.jitter; a.seed = k.seed;
         a.tmin = k.tmin; a.tmax = k.tmax;
-        a.tiles_x = (k.width + 7) / 8;
-        a.num_tiles = a.tiles_x * ((k.height + 7) / 8);
+        camera_tiling(k.width, k.height, k.spp, quad, a);
         a.hit_id = b->d_hit_id ? b->d_hit_id + ray_off[s] : nullptr;
         a.hit_t = b->d_hit_t ? b->d_hit_t + ray_off[s] : nullptr;
         a.counts = b->d_counts ? b->d_counts + pix_off[s] : nullptr;

@@ -337,8 +337,7 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
             a.tanh = c.tan_half_fov;
             a.w = c.width; a.h = c.height; a.spp = c.spp; a.jitter = c.jitter; a.seed = c.seed;
             a.tmin = c.tmin; a.tmax = c.tmax;
-            a.tiles_x = (c.width + 7) / 8;
-            a.num_tiles = a.tiles_x * ((c.height + 7) / 8);
+            camera_tiling(c.width, c.height, c.spp, false, a);  // 8x8: many jobs per launch, as in batches
         }
         a.hit_id = q->o_hit_id != SIZE_MAX ? (uint32_t*)(outd + q->o_hit_id) : nullptr;
         a.hit_t = q->o_hit_t != SIZE_MAX ? (float*)(outd + q->o_hit_t) : nullptr;
@@ -655,7 +654,7 @@ int sq_server_submit(sq_server* s, const sq_request* r, uint64_t* id) {
         if (c.width == 0 || c.height == 0 || c.spp == 0) SQ_RET(SQ_ERR_INVALID, "sq_server_submit: empty image");
         pix = (uint64_t)c.width * c.height;
         rays = pix * c.spp;
-        if (((uint64_t)(c.width + 7) / 8) * ((c.height + 7) / 8) >= (1ull << 31))
+        if (((uint64_t)(c.width + 3) / 4) * ((c.height + 3) / 4) >= (1ull << 31))
             SQ_RET(SQ_ERR_INVALID, "sq_server_submit: image too large");
     }
     if (rays >= (1ull << 32)) SQ_RET(SQ_ERR_INVALID, "sq_server_submit: too many rays");

@@ -893,19 +893,14 @@ static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_c
     a.tanh = cam->tan_half_fov;
     a.w = cam->width; a.h = cam->height; a.spp = cam->spp; a.jitter = cam->jitter; a.seed = cam->seed;
     a.tmin = cam->tmin; a.tmax = cam->tmax;
-    // The per-ray kernel takes 4x4-pixel tiles once they hold 64 rays or
-    // more: four times as many tiles as 8x8 ones, so the last tiles of a
-    // frame are short. 1080p x 4 over the 1M soup: 7.08 ms, against 9.55 ms
-    // with 8x8 tiles, whose ~2,500 costly tiles fell on 2,048 waves (half the
-    // waves idle for the last fifth of the launch). Packets keep 8x8.
+    // 4x4-pixel tiles for the per-ray kernel's exhaustive traversal: four
+    // times as many tiles as 8x8 ones, so the last tiles of a frame are
+    // short. 1080p x 4 over the 1M soup: 7.08 ms, against 9.55 ms with 8x8
+    // tiles, whose ~2,500 costly tiles fell on 2,048 waves (half the waves
+    // idle for the last fifth of the launch).
     const bool exhaustive = !p || p->semantics == SQ_TRACE_EXHAUSTIVE;
-    a.tile_shift = (exhaustive && cam->spp >= 4 && (t2 || !use_packets(p))) ? 2u : 3u;
-    const uint64_t tile_edge = 1ull << a.tile_shift;
-    const uint64_t tx = (cam->width + tile_edge - 1) >> a.tile_shift, ty = (cam->height + tile_edge - 1) >> a.tile_shift;
-    if (tx * ty >= (1ull << 31)) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: %ux%u image has too many tiles", cam->width,
-                                        cam->height);
-    a.tiles_x = (uint32_t)tx;
-    a.num_tiles = (uint32_t)(tx * ty);
+    if (camera_tiling(cam->width, cam->height, cam->spp, exhaustive && (t2 || !use_packets(p)), a) >= (1ull << 31))
+        SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: %ux%u image has too many tiles", cam->width, cam->height);
     if (out_on_device) {
         a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts; a.rgb = out->rgb;
         if (heat) SQ_TRY(heat_launch(c, b, a, p));

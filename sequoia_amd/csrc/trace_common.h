@@ -45,6 +45,20 @@ struct TraceArgs {
     const sq_plucker* plk;  // non-null: precomputed triangle test
 };
 
+// Camera tile geometry of a job. `quad`: one camera traced by the per-ray
+// kernel with exhaustive semantics, which takes 4x4-pixel tiles once a tile
+// holds 64 rays or more (more, shorter tiles even out the end of a launch);
+// closest hits, wave packets and many-job launches take 8x8. Returns the
+// tile count; callers reject 2^31 or more.
+inline uint64_t camera_tiling(uint32_t w, uint32_t h, uint32_t spp, bool quad, TraceArgs& a) {
+    a.tile_shift = (quad && spp >= 4) ? 2u : 3u;
+    const uint64_t e = 1ull << a.tile_shift;
+    const uint64_t tx = (w + e - 1) >> a.tile_shift, ty = (h + e - 1) >> a.tile_shift;
+    a.tiles_x = (uint32_t)tx;
+    a.num_tiles = (uint32_t)(tx * ty);
+    return tx * ty;
+}
+
 // Why a ray was abandoned (its results are then invalid).
 constexpr uint32_t kFaultStack = 1;  // deeper than LDS slots + spill capacity
 constexpr uint32_t kFaultNode = 2;   // child index outside the node array
