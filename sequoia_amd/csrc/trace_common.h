@@ -30,6 +30,7 @@ struct TraceArgs {
     uint32_t num_tiles;
     uint32_t tiles_x;
     uint32_t tile_shift;  // camera tiles of (1 << tile_shift)^2 pixels, all samples; 0 = 8x8
+    uint32_t ray_tile;    // rays per ray-list tile; 0 = 256
     // multi-job launches: job table and its tile prefix (njobs + 1 entries)
     const TraceArgs* jobs;
     const uint32_t* job_tiles;

@@ -117,6 +117,22 @@ def test_ray_list_secondary_rays(ctx):
     check_hits(cl["hit_id"], cl["hit_t"], want["hit_id"], want["hit_t"])
 
 
+def test_ray_list_tile_sizes(ctx):
+    """Ray lists of 1 to 1.1M rays: tiles of 64, 128, 192 and 256 rays (one
+    tile per wave for short lists), ragged last tiles, against the oracle."""
+    say("ray list tile sizes")
+    tris = GOLDEN_SCENES["clus3000"]()
+    rays = sq.gen_secondary_rays(tris, 1_100_000, seed=5, tmin=1e-4)
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, "morton63")
+    nodes = bvh.download()
+    want = ob.trace_rays(nodes, bvh.root, tris, rays)
+    for n in (1, 65, 300_000, 700_000, 1_100_000):
+        got = ctx.trace_rays(bvh, rays[:n])
+        assert np.array_equal(got["counts"], want["counts"][:n]), n
+        check_hits(got["hit_id"], got["hit_t"], want["hit_id"][:n], want["hit_t"][:n])
+
+
 def test_camera_rays_equal_uploaded_oracle_rays(ctx):
     """GPU-generated camera rays produce the same results as the oracle's ray bits."""
     tris = GOLDEN_SCENES["ico1280"]()
