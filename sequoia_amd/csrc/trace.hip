@@ -286,6 +286,111 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
     return st.fault == 0;
 }
 
+// Closest semantics, two lists per step: the current list and, once it is
+// used up, the list on top of the stack when that names its nodes directly;
+// lists whose entry distance lies beyond the best hit (2^-20 relative) are
+// dropped when popped, as in lane_step. The four node fetches issue back to
+// back; the hit leaves are tested first, then the surviving child lists are
+// pushed farthest first (the nearest is popped next), culled against the
+// best hit those leaves left. Same stack bound as the two-list exhaustive
+// step: 2 x depth + 4 entries.
+__device__ __forceinline__ void cswap(float& fa, uint32_t& ea, float& fb, uint32_t& eb) {
+    const bool s = fb > fa;  // larger distance first
+    const float tf = s ? fb : fa;
+    const uint32_t te = s ? eb : ea;
+    fb = s ? fa : fb;
+    eb = s ? ea : eb;
+    fa = tf;
+    ea = te;
+}
+
+__device__ __forceinline__ bool lane_step_cwide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+    const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
+    while (L.cnt == 0) {
+        if (st.sp == 0) return false;
+        float te = 0.f;
+        const uint32_t e = pop<true>(st, te);
+        if (te > cull) continue;
+        if ((e & 15u) != 15u) {
+            L.first = e >> 4;
+            L.cnt = (e & 15u) + 1;
+        } else {
+            if ((e >> 4) >= a.node_count) { st.fault |= kFaultNode; return false; }
+            const NodeV p = load_node(a.nodes, e >> 4);
+            L.first = p.child;
+            L.cnt = p.count;
+        }
+    }
+    uint32_t idx[2], num[2];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+        if (g > 0 && L.cnt == 0 && st.sp > 0) {
+            const uint32_t top = st.lds[((st.sp - 1) & st.mask) * st.stride];
+            if ((top & 15u) != 15u) {
+                float te = 0.f;
+                (void)pop<true>(st, te);
+                if (!(te > cull)) {
+                    L.first = top >> 4;
+                    L.cnt = (top & 15u) + 1;
+                }
+            }
+        }
+        idx[g] = L.first;
+        num[g] = L.cnt >= 2 ? 2u : L.cnt;
+        L.first += num[g];
+        L.cnt -= num[g];
+        if (num[g] && (idx[g] >= a.node_count || num[g] > a.node_count - idx[g])) {
+            st.fault |= kFaultNode;
+            return false;
+        }
+        if (num[g] == 0) idx[g] = idx[0];  // in-bounds repeat, never tested
+    }
+    NodeV n[2][2];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+        n[g][0] = load_node(a.nodes, idx[g]);
+        n[g][1] = load_node(a.nodes, idx[g] + (num[g] == 2 ? 1u : 0u));
+    }
+    bool h[2][2];
+    float f[2][2];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+        L.tests += num[g];
+        f[g][0] = f[g][1] = 0.f;
+        h[g][0] = num[g] >= 1 && slab(n[g][0], L.r, L.ix, L.iy, L.iz, L.fast, f[g][0]) && !(f[g][0] > cull);
+        h[g][1] = num[g] == 2 && slab(n[g][1], L.r, L.ix, L.iy, L.iz, L.fast, f[g][1]) && !(f[g][1] > cull);
+    }
+    bool bad = false;
+    {
+        const NodeV nd[4] = {n[0][0], n[0][1], n[1][0], n[1][1]};
+        const bool hh[4] = {h[0][0], h[0][1], h[1][0], h[1][1]};
+        leaf_batch<4>(a, nd, hh, L, tri_tests, bad);
+    }
+    if (bad) { st.fault |= kFaultTri; return false; }
+    const float cull2 = L.best_t + L.best_t * (1.0f / 1048576.0f);
+    const float inf = __int_as_float(0x7f800000);
+    float fk[4];
+    uint32_t ek[4];
+#pragma unroll
+    for (int g = 0; g < 2; g++)
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const bool c = h[g][k] && !n[g][k].leaf && !(f[g][k] > cull2);
+            fk[2 * g + k] = c ? f[g][k] : -inf;  // absent entries sort last and are not pushed
+            ek[2 * g + k] = c ? encode_list(idx[g] + k, n[g][k]) : 0u;
+        }
+    // sorting network, largest distance first
+    cswap(fk[0], ek[0], fk[1], ek[1]);
+    cswap(fk[2], ek[2], fk[3], ek[3]);
+    cswap(fk[0], ek[0], fk[2], ek[2]);
+    cswap(fk[1], ek[1], fk[3], ek[3]);
+    cswap(fk[1], ek[1], fk[2], ek[2]);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (fk[k] != -inf) push<true>(st, ek[k], fk[k]);
+    return st.fault == 0;
+}
+
 // Node2 trees: triangles of a leaf group; hit ids are original indices.
 __device__ __forceinline__ void leaf_tris2(const TraceArgs& a, uint32_t first, uint32_t count, LaneRay& L,
                                            uint32_t& tri_tests, bool& bad) {
@@ -570,7 +675,8 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
         }
         if (active) {
             const bool more = FMT == 2 ? lane_step2<CLOSEST, WL>(a, L, st, tris_total)
-                                       : (CLOSEST ? lane_step<CLOSEST>(a, L, st, tris_total)
+                                       : (CLOSEST ? (WL >= 2 && a.camera ? lane_step_cwide(a, L, st, tris_total)
+                                                                         : lane_step<CLOSEST>(a, L, st, tris_total))
                                                   : lane_step_wide<WL>(a, L, st, tris_total));
             if (!more) {
                 // ray finished
@@ -687,12 +793,20 @@ int trace_fault(uint32_t bits, const char* what) {
            (bits & kFaultTri) ? " triangle index outside the scene" : "");
 }
 
-constexpr uint32_t kCameraWavesPerCu = 12, kCameraClosestWavesPerCu = 8, kWavesPerCu = 16;
+constexpr uint32_t kCameraWavesPerCu = 12, kCameraClosestWavesPerCu = 12, kCameraClosest2WavesPerCu = 8,
+                   kWavesPerCu = 16;
 
-int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out, bool node2) {
+int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out, bool node2,
+               bool ray_list) {
     TracePlan L;
     L.closest = p && p->semantics == SQ_TRACE_CLOSEST;
-    L.lists = kListLists;
+    // closest hits: camera rays take two lists per step (134 VGPRs, three
+    // waves per SIMD), ray lists one. A launch of one ray list uses the
+    // one-list kernel, which fits 16 waves per CU (1M incoherent closest rays:
+    // 772 ms at 16 waves, 1,100 ms at 12 with one or two lists per step);
+    // many-job launches use the two-list kernel, which steps each job's rays
+    // the way a launch of that job alone would.
+    L.lists = L.closest && ray_list ? 1 : kListLists;
     if (p && p->lists_per_step) {
         if (p->lists_per_step != kListLists && p->lists_per_step != kWideLists)
             SQ_RET(SQ_ERR_INVALID, "trace lists_per_step %u (2 or 4)", p->lists_per_step);
@@ -708,6 +822,8 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     int per_cu = 0;
     if (node2 && L.closest)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 2>, L.block, L.lds_bytes);
+    else if (L.closest && L.lists == 1)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 1, 1>, L.block, L.lds_bytes);
     else if (node2 && L.lists == kWideLists)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 2, kWideLists>, L.block,
                                                            L.lds_bytes);
@@ -725,13 +841,15 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     // (Morton63 tree; 4x4-pixel tiles): exhaustive Node1 steps run best at 12
     // (two lists per step: 6.35 ms at 12, 6.59 at 16, 7.25 at 10, 7.77 at 8;
     // four lists per step reach only 8 waves, 7.07 ms), Node2 steps at 16
-    // (5.78 ms against 5.91 at 12); closest hits on 8x8 tiles at 8 (Morton
-    // 2.36 ms, SAH 1.96 ms; more waves than that help the one and hurt the
-    // other); ray lists (incoherent secondary rays on 10M clustered
+    // (5.78 ms against 5.91 at 12); closest hits on 8x8 tiles, two lists
+    // per step, at 12, all that fits (Morton 1.56 ms against 1.68 at 8, SAH
+    // 1.55 / 1.54; one list per step: 2.34 / 1.97 ms at 8), Node2 closest
+    // hits at 8 (1.84 ms against 1.92 at 12); ray lists
+    // (incoherent secondary rays on 10M clustered
     // triangles: 0.78 s at 16 against 1.17 s at 8) and many-scene launches
     // (256 scenes: 30 ms at 16 against 49 ms at 8) want 16.
     const uint32_t waves = !coherent ? kWavesPerCu
-                           : L.closest ? kCameraClosestWavesPerCu
+                           : L.closest ? (node2 ? kCameraClosest2WavesPerCu : kCameraClosestWavesPerCu)
                            : node2     ? kWavesPerCu
                                        : kCameraWavesPerCu;
     const int target = (int)(waves * 64 / L.block);
@@ -754,13 +872,17 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
 int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     if (multi) {
-        if (L.closest) hipLaunchKernelGGL((k_trace<true, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        if (L.closest && L.lists == 1)
+            hipLaunchKernelGGL((k_trace<true, true, 1, 1>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        else if (L.closest) hipLaunchKernelGGL((k_trace<true, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
         else if (L.lists == kWideLists)
             hipLaunchKernelGGL((k_trace<false, true, 1, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
                                stream, a);
         else hipLaunchKernelGGL((k_trace<false, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     } else {
-        if (L.closest) hipLaunchKernelGGL((k_trace<true, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        if (L.closest && L.lists == 1)
+            hipLaunchKernelGGL((k_trace<true, false, 1, 1>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        else if (L.closest) hipLaunchKernelGGL((k_trace<true, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
         else if (L.lists == kWideLists)
             hipLaunchKernelGGL((k_trace<false, false, 1, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
                                stream, a);
@@ -834,7 +956,7 @@ static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, con
         return rc;
     }
     TracePlan L;
-    SQ_TRY(trace_plan(c, p, depth, a.camera != 0u, &L, node2));
+    SQ_TRY(trace_plan(c, p, depth, a.camera != 0u, &L, node2, a.camera == 0u));
     if (!a.camera) {
         // Tiles of 64 k rays, k = min(4, ceil(rays / (64 x waves))): a list
         // too short to give every wave a 256-ray tile is cut into at most one

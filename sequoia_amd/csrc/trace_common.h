@@ -256,7 +256,7 @@ struct TracePlan {
 };
 // coherent: one camera's primary rays (a smaller default launch).
 int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out,
-               bool node2 = false);
+               bool node2 = false, bool ray_list = false);
 // SQ_ERR_OVERFLOW with a message naming the fault bits.
 int trace_fault(uint32_t bits, const char* what);
 // Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).
