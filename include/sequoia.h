@@ -175,8 +175,8 @@ typedef struct {
     uint32_t stack_slots; /* 0 = default; LDS stack entries per lane */
     uint32_t count_work;  /* 1 = accumulate total node/triangle tests into sq_timings */
     uint32_t kernel;      /* camera rays: SQ_KERNEL_* (packets: exhaustive semantics only) */
-    uint32_t blocks_per_cu; /* per-ray kernel: 0 = default (one camera's rays: 12 waves per CU, 8 for
-                               Node2 closest hits; 16 otherwise; at most what fits); else 1..64 */
+    uint32_t blocks_per_cu; /* per-ray kernel: 0 = default (one camera's rays: 12 waves per CU, 16 for
+                               exhaustive Node2 steps; 16 otherwise; at most what fits); else 1..64 */
     uint32_t tri_test;    /* 0: reference Moller-Trumbore; 1: precomputed triangles (sq_bvh_plucker) */
     uint32_t lists_per_step; /* per-ray kernel, exhaustive semantics: child lists (Node1) or Node2s
                                 tested per step, 0 = default (2), else 2 or 4 */

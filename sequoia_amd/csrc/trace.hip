@@ -545,6 +545,63 @@ __device__ __forceinline__ bool lane_step2_wide(const TraceArgs& a, LaneRay& L, 
 
 // Node2 step: the root box on the first step, then one Node2 (both children
 // from one 32-byte fetch) per step; exhaustive semantics take two.
+// Node2 trees, closest semantics, camera rays: two Node2s per step (four
+// child boxes, both fetches issued together), leaf groups tested first, then
+// the surviving children pushed farthest first, as in lane_step_cwide.
+__device__ __forceinline__ bool lane_step2_cwide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+    const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
+    uint32_t e[2];
+    uint32_t got = 0;
+    while (got < 2 && st.sp > 0) {
+        float te = 0.f;
+        const uint32_t x = pop<true>(st, te);
+        if (te > cull) continue;
+        if (x >= a.node_count) { st.fault |= kFaultNode; return false; }
+        e[got++] = x;
+        if (got == 1 && st.sp == 0) break;
+    }
+    if (got == 0) return false;
+    if (got == 1) e[1] = e[0];  // in-bounds repeat, never tested
+    const Node2V n0 = load_node2(a.nodes2, e[0]);
+    const Node2V n1 = load_node2(a.nodes2, e[1]);
+    const bool two = got == 2;
+    L.tests += two ? 4 : 2;
+    float f[4];
+    bool h[4];
+    h[0] = slab(n0.a, L.r, L.ix, L.iy, L.iz, L.fast, f[0]) && !(f[0] > cull);
+    h[1] = slab(n0.b, L.r, L.ix, L.iy, L.iz, L.fast, f[1]) && !(f[1] > cull);
+    h[2] = two && slab(n1.a, L.r, L.ix, L.iy, L.iz, L.fast, f[2]) && !(f[2] > cull);
+    h[3] = two && slab(n1.b, L.r, L.ix, L.iy, L.iz, L.fast, f[3]) && !(f[3] > cull);
+    const uint32_t cc[4] = {n0.ca, n0.cb, n1.ca, n1.cb}, cn[4] = {n0.cnta, n0.cntb, n1.cnta, n1.cntb};
+    const bool lf[4] = {n0.la, n0.lb, n1.la, n1.lb};
+    bool bad = false;
+    {
+        const uint32_t cs[4] = {lf[0] ? cn[0] : 0u, lf[1] ? cn[1] : 0u, lf[2] ? cn[2] : 0u, lf[3] ? cn[3] : 0u};
+        const bool hl[4] = {h[0] && lf[0], h[1] && lf[1], h[2] && lf[2], h[3] && lf[3]};
+        leaf_batch2<4>(a, cc, cs, hl, L, tri_tests, bad);
+    }
+    if (bad) { st.fault |= kFaultTri; return false; }
+    const float cull2 = L.best_t + L.best_t * (1.0f / 1048576.0f);
+    const float inf = __int_as_float(0x7f800000);
+    float fk[4];
+    uint32_t ek[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const bool c = h[k] && !lf[k] && !(f[k] > cull2);
+        fk[k] = c ? f[k] : -inf;  // absent entries sort last and are not pushed
+        ek[k] = c ? cc[k] : 0u;
+    }
+    cswap(fk[0], ek[0], fk[1], ek[1]);
+    cswap(fk[2], ek[2], fk[3], ek[3]);
+    cswap(fk[0], ek[0], fk[2], ek[2]);
+    cswap(fk[1], ek[1], fk[3], ek[3]);
+    cswap(fk[1], ek[1], fk[2], ek[2]);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (fk[k] != -inf) push<true>(st, ek[k], fk[k]);
+    return st.fault == 0;
+}
+
 template <bool CLOSEST, int WL>
 __device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     if (L.cnt) {
@@ -565,6 +622,7 @@ __device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack
         return st.fault == 0;
     }
     if (!CLOSEST) return lane_step2_wide<WL>(a, L, st, tri_tests);
+    if (WL >= 2 && a.camera) return lane_step2_cwide(a, L, st, tri_tests);
     const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
     uint32_t e = 0;
     while (true) {
@@ -793,8 +851,7 @@ int trace_fault(uint32_t bits, const char* what) {
            (bits & kFaultTri) ? " triangle index outside the scene" : "");
 }
 
-constexpr uint32_t kCameraWavesPerCu = 12, kCameraClosestWavesPerCu = 12, kCameraClosest2WavesPerCu = 8,
-                   kWavesPerCu = 16;
+constexpr uint32_t kCameraWavesPerCu = 12, kCameraClosestWavesPerCu = 12, kWavesPerCu = 16;
 
 int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out, bool node2,
                bool ray_list) {
@@ -820,7 +877,9 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     if (L.block != 64 && L.block != 128) SQ_RET(SQ_ERR_INVALID, "trace block size %u (64 or 128)", L.block);
     L.lds_bytes = ((size_t)L.slots * (L.closest ? 2 : 1) + 1) * L.block * 4;
     int per_cu = 0;
-    if (node2 && L.closest)
+    if (node2 && L.closest && L.lists == 1)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 2, 1>, L.block, L.lds_bytes);
+    else if (node2 && L.closest)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 2>, L.block, L.lds_bytes);
     else if (L.closest && L.lists == 1)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 1, 1>, L.block, L.lds_bytes);
@@ -843,13 +902,13 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     // four lists per step reach only 8 waves, 7.07 ms), Node2 steps at 16
     // (5.78 ms against 5.91 at 12); closest hits on 8x8 tiles, two lists
     // per step, at 12, all that fits (Morton 1.56 ms against 1.68 at 8, SAH
-    // 1.55 / 1.54; one list per step: 2.34 / 1.97 ms at 8), Node2 closest
-    // hits at 8 (1.84 ms against 1.92 at 12); ray lists
+    // 1.55 / 1.54; one list per step: 2.34 / 1.97 ms at 8; Node2 trees, two
+    // Node2s per step: 1.67 ms at 12, 1.77 at 8, 2.01 at 16); ray lists
     // (incoherent secondary rays on 10M clustered
     // triangles: 0.78 s at 16 against 1.17 s at 8) and many-scene launches
     // (256 scenes: 30 ms at 16 against 49 ms at 8) want 16.
     const uint32_t waves = !coherent ? kWavesPerCu
-                           : L.closest ? (node2 ? kCameraClosest2WavesPerCu : kCameraClosestWavesPerCu)
+                           : L.closest ? kCameraClosestWavesPerCu
                            : node2     ? kWavesPerCu
                                        : kCameraWavesPerCu;
     const int target = (int)(waves * 64 / L.block);
@@ -894,7 +953,9 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
 
 int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
-    if (L.closest) hipLaunchKernelGGL((k_trace<true, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+    if (L.closest && L.lists == 1)
+        hipLaunchKernelGGL((k_trace<true, false, 2, 1>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+    else if (L.closest) hipLaunchKernelGGL((k_trace<true, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
     else if (L.lists == kWideLists)
         hipLaunchKernelGGL((k_trace<false, false, 2, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream,
                            a);

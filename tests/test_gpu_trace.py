@@ -156,6 +156,18 @@ def test_small_stack_spills_correctly(ctx):
     for lists in (2, 4):  # spills under both step widths
         a = ctx.trace_camera(bvh, cam, stack_slots=2, lists_per_step=lists)
         assert np.array_equal(a["counts"], b["counts"]) and np.array_equal(a["hit_id"], b["hit_id"]), lists
+    # closest hits: camera rays (two lists per step, up to four pushes) and a
+    # ray list (one list per step) through a two-entry LDS stack
+    cb = ctx.trace_camera(bvh, cam, semantics=sq.TRACE_CLOSEST, stack_slots=32)
+    ca = ctx.trace_camera(bvh, cam, semantics=sq.TRACE_CLOSEST, stack_slots=2)
+    for k in ("counts", "hit_id", "hit_t"):
+        assert np.array_equal(ca[k], cb[k]), k
+    check_hits(ca["hit_id"], ca["hit_t"], b["hit_id"], b["hit_t"])
+    rays = sq.gen_secondary_rays(tris, 20000, seed=7, tmin=1e-4)
+    rb = ctx.trace_rays(bvh, rays, semantics=sq.TRACE_CLOSEST, stack_slots=32)
+    ra = ctx.trace_rays(bvh, rays, semantics=sq.TRACE_CLOSEST, stack_slots=2)
+    for k in ("counts", "hit_id", "hit_t"):
+        assert np.array_equal(ra[k], rb[k]), k
 
 
 def test_lists_per_step_rejects_other_values(ctx):
