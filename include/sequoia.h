@@ -184,8 +184,11 @@ typedef struct {
 
 /* Outputs (any may be NULL). hit_id/hit_t: one per ray (SQ_MISS / +inf on
  * miss). counts: one per pixel (camera) or per ray (ray list), node tests
- * summed over samples. rgb: width*height*3 heatmap bytes
- * uint8(min(1, tests/200)*255) (tracer.cpp:222-225). */
+ * summed over samples; with exhaustive semantics they are the reference's
+ * counts, with closest semantics the box tests this traversal made (they
+ * depend on the step: camera rays test two child lists per step, ray lists
+ * one). rgb: width*height*3 heatmap bytes uint8(min(1, tests/200)*255)
+ * (tracer.cpp:222-225). */
 typedef struct {
     uint32_t* hit_id;
     float* hit_t;
