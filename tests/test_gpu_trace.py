@@ -88,6 +88,31 @@ def test_ragged_image_tiles(ctx, spp, lists):
     check_hits(cl["hit_id"], cl["hit_t"], want["hit_id"], want["hit_t"])
 
 
+@pytest.mark.parametrize("kind", ["morton63", "recursive"])
+def test_closest_steps_with_finite_tmax(ctx, kind):
+    """Camera rays whose tmax cuts through the scene: the two-list closest
+    step (Node1) and the two-Node2 closest step equal the exhaustive hits and
+    the oracle's, with culling and tmax acting together."""
+    say("closest tmax", kind)
+    tris = GOLDEN_SCENES["clus3000"]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, kind)
+    nodes = bvh.download()
+    cam = sq.pinhole_camera(tris, 160, 96, 60.0, 3.0, spp=4, jitter=1, seed=21)
+    cam.tmax = 3.0  # about the distance to the scene-box centre
+    want = ob.trace_camera(nodes, bvh.root, tris, cam)
+    ex = ctx.trace_camera(bvh, cam)
+    assert np.array_equal(ex["counts"], want["counts"])
+    check_hits(ex["hit_id"], ex["hit_t"], want["hit_id"], want["hit_t"])
+    assert (want["hit_id"] != sq.MISS).any() and (want["hit_id"] == sq.MISS).any()
+    cl = ctx.trace_camera(bvh, cam, semantics=sq.TRACE_CLOSEST)
+    check_hits(cl["hit_id"], cl["hit_t"], want["hit_id"], want["hit_t"])
+    t2 = ctx.node2(bvh)
+    c2 = ctx.trace_camera(t2, cam, semantics=sq.TRACE_CLOSEST)
+    check_hits(c2["hit_id"], c2["hit_t"], want["hit_id"], want["hit_t"])
+    t2.free()
+
+
 @pytest.mark.parametrize("name", ["ico69632", "clus3000"])
 def test_closest_mode_matches_exhaustive_hits(ctx, name):
     say("closest", name)
