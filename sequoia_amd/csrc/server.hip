@@ -337,7 +337,9 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
             a.tanh = c.tan_half_fov;
             a.w = c.width; a.h = c.height; a.spp = c.spp; a.jitter = c.jitter; a.seed = c.seed;
             a.tmin = c.tmin; a.tmax = c.tmax;
-            camera_tiling(c.width, c.height, c.spp, false, a);  // 8x8: many jobs per launch, as in batches
+            // a wave of one camera request tiles it as a launch of that camera
+            // alone would; waves of several jobs take 8x8 tiles, as batches do
+            camera_tiling(c.width, c.height, c.spp, rq.size() == 1 && !s->plan.closest, a);
         }
         a.hit_id = q->o_hit_id != SIZE_MAX ? (uint32_t*)(outd + q->o_hit_id) : nullptr;
         a.hit_t = q->o_hit_t != SIZE_MAX ? (float*)(outd + q->o_hit_t) : nullptr;
