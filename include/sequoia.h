@@ -88,6 +88,8 @@ void sq_device_free(sq_ctx* ctx, void* ptr);
 int sq_device_copy(sq_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Copies device -> host on the context stream and waits. */
 int sq_copy_to_host(sq_ctx* ctx, void* host_dst, const void* dev_src, size_t bytes);
+/* Copies host -> device on the context stream and waits. */
+int sq_copy_to_device(sq_ctx* ctx, void* dev_dst, const void* host_src, size_t bytes);
 
 /* ---- scenes ---- */
 int sq_scene_upload(sq_ctx* ctx, const sq_triangle* host_tris, uint32_t n, sq_scene** out);

@@ -135,6 +135,7 @@ _SIGS = {
     "sq_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "sq_device_free": (None, [_P, _P]),
     "sq_copy_to_host": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "sq_copy_to_device": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sq_device_copy": (C.c_int, [_P, _P, _P, C.c_size_t]),
     "sq_scene_upload": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
     "sq_scene_wrap_device": (C.c_int, [_P, _P, _U32, C.POINTER(_P)]),
@@ -635,6 +636,11 @@ class Context:
         out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
         _check(lib().sq_copy_to_host(self.h, _ptr(out), dev, nbytes))
         return out
+
+    def to_device(self, dev, host):
+        """Copies a contiguous host array into device memory at `dev`."""
+        host = np.ascontiguousarray(host)
+        _check(lib().sq_copy_to_device(self.h, dev, _ptr(host), host.nbytes))
 
     def trace_camera_device(self, bvh, cam, hit_id=None, hit_t=None, counts=None, rgb=None,
                             semantics=TRACE_EXHAUSTIVE, stack_slots=0, count_work=False, kernel=0):

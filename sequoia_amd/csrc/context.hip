@@ -160,6 +160,13 @@ int sq_ctx_create(int device, sq_ctx** out) {
         (void)hipEventCreateWithFlags(&c->ev_begin[i], hipEventReleaseToDevice);
         (void)hipEventCreateWithFlags(&c->ev_end[i], hipEventReleaseToDevice);
     }
+    // Pinned status words (trace fault bits and work totals are read back
+    // through them), set up here so that later calls need not allocate them.
+    int rc = pinned_ensure(c, 4096);
+    if (rc != SQ_OK) {
+        sq_ctx_destroy(c);
+        return rc;
+    }
     *out = c;
     return SQ_OK;
 }
@@ -231,6 +238,13 @@ int sq_device_copy(sq_ctx* c, void* dst, const void* src, size_t bytes) {
 int sq_copy_to_host(sq_ctx* c, void* dst, const void* src, size_t bytes) {
     if (!c || !dst || !src) SQ_RET(SQ_ERR_INVALID, "sq_copy_to_host: NULL argument");
     SQ_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return SQ_OK;
+}
+
+int sq_copy_to_device(sq_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src) SQ_RET(SQ_ERR_INVALID, "sq_copy_to_device: NULL argument");
+    SQ_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
     return SQ_OK;
 }

@@ -255,6 +255,7 @@ int heat_launch(sq_ctx* c, const sq_bvh* b, TraceArgs& t, const sq_trace_params*
     h.t.spill = c->spill.as<uint32_t>();
     h.t.spill_cap = cap;
     h.t.work = c->counters.as<uint32_t>();
+    t.work = h.t.work;  // the caller reads the fault bits back from here
     h.t.stats = nullptr;
     h.t.hit_id = nullptr;
     h.t.hit_t = nullptr;

@@ -975,6 +975,8 @@ static int target_args(const sq_bvh* b, const sq_bvh2* t2, const sq_trace_params
     if (p && p->tri_test > 1) SQ_RET(SQ_ERR_INVALID, "trace: tri_test %u", p->tri_test);
     if (b) {
         if (!b->scene) SQ_RET(SQ_ERR_INVALID, "trace: the tree has no scene");
+        if (b->capacity > kMaxTraceNodes)
+            SQ_RET(SQ_ERR_UNSUPPORTED, "trace: %u nodes; stack entries hold node indices below 2^28", b->capacity);
         a.nodes = b->d_nodes;
         a.tris = b->scene->d_tris;
         a.root = b->root;
@@ -1004,8 +1006,10 @@ static int target_args(const sq_bvh* b, const sq_bvh2* t2, const sq_trace_params
 
 static uint32_t target_depth(const sq_bvh* b, const sq_bvh2* t2) { return b ? b->depth : t2->levels + 2; }
 
-static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, const sq_trace_params* p) {
+static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, const sq_trace_params* p,
+                        const TraceScratch* ext = nullptr) {
     if (!node2 && a.camera && use_packets(p)) {
+        if (ext) SQ_RET(SQ_ERR_UNSUPPORTED, "trace: wave packets need the context workspace");
         SQ_TRY(c->counters.ensure(64));
         SQ_TRY(c->stats.ensure(64));
         a.work = c->counters.as<uint32_t>();
@@ -1018,6 +1022,22 @@ static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, con
     }
     TracePlan L;
     SQ_TRY(trace_plan(c, p, depth, a.camera != 0u, &L, node2, a.camera == 0u));
+    if (ext) {
+        // Caller scratch: counters first, then per-lane spill areas. A
+        // persistent launch stays correct with fewer workgroups (tiles are
+        // dealt from one counter), so the grid shrinks to what the spill
+        // area holds, down to one workgroup per CU.
+        const size_t per_block = (size_t)L.block * L.spill_cap * (L.closest ? 2 : 1);
+        const size_t avail = ext->words > kScratchSpillWord ? ext->words - kScratchSpillWord : 0;
+        const size_t fit = avail / per_block;
+        if (fit < (size_t)c->num_cus)
+            SQ_RET(SQ_ERR_BUDGET,
+                   "trace: a tree of depth %u needs %zu bytes of stack spill per workgroup; the %zu-byte scratch "
+                   "holds %zu workgroups, fewer than one per CU (%d)",
+                   depth, per_block * 4, ext->words * 4, fit, c->num_cus);
+        if (L.grid > fit) L.grid = (uint32_t)fit;
+        L.spill_words = (size_t)L.grid * per_block;
+    }
     if (!a.camera) {
         // Tiles of 64 k rays, k = min(4, ceil(rays / (64 x waves))): a list
         // too short to give every wave a 256-ray tile is cut into at most one
@@ -1030,34 +1050,56 @@ static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, con
         a.ray_tile = 64u * (uint32_t)(k < 1 ? 1 : (k > kRayTile / 64 ? kRayTile / 64 : k));
         a.num_tiles = (uint32_t)(((uint64_t)a.nrays + a.ray_tile - 1) / a.ray_tile);
     }
-    SQ_TRY(c->spill.ensure(L.spill_words * 4));
-    SQ_TRY(c->counters.ensure(64));
-    SQ_TRY(c->stats.ensure(64));
+    if (ext) {
+        a.work = ext->base;
+        a.stats = (p && p->count_work) ? (unsigned long long*)(ext->base + kScratchStatsWord) : nullptr;
+        a.spill = ext->base + kScratchSpillWord;
+    } else {
+        SQ_TRY(c->spill.ensure(L.spill_words * 4));
+        SQ_TRY(c->counters.ensure(64));
+        SQ_TRY(c->stats.ensure(64));
+        a.work = c->counters.as<uint32_t>();
+        a.stats = (p && p->count_work) ? c->stats.as<unsigned long long>() : nullptr;
+        a.spill = c->spill.as<uint32_t>();
+    }
     a.slots = L.slots;
-    a.spill = c->spill.as<uint32_t>();
     a.spill_cap = L.spill_cap;
-    a.work = c->counters.as<uint32_t>();
-    a.stats = (p && p->count_work) ? c->stats.as<unsigned long long>() : nullptr;
     phase_begin(c, PH_TRACE);
     int rc = node2 ? trace_launch_plan2(c->stream, L, a) : trace_launch_plan(c->stream, L, a, false);
     phase_end(c, PH_TRACE);
     return rc;
 }
 
-static int trace_finish(sq_ctx* c, const sq_trace_params* p) {
-    SQ_TRY(pinned_ensure(c, 64));
+// Waits for the trace and reads its fault bits (and work totals) back from
+// a.work / a.stats through the context's pinned words.
+static int trace_finish(sq_ctx* c, const sq_trace_params* p, const TraceArgs& a) {
+    if (!c->pinned || c->pinned_bytes < 64) SQ_RET(SQ_ERR_HIP, "trace: the context has no pinned status words");
     uint32_t* h = (uint32_t*)c->pinned;
-    SQ_CHECK_HIP(hipMemcpyAsync(h, c->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
-    if (p && p->count_work)
-        SQ_CHECK_HIP(hipMemcpyAsync(h + 8, c->stats.p, 16, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipMemcpyAsync(h, a.work, 16, hipMemcpyDeviceToHost, c->stream));
+    if (p && p->count_work && a.stats)
+        SQ_CHECK_HIP(hipMemcpyAsync(h + 8, a.stats, 16, hipMemcpyDeviceToHost, c->stream));
     SQ_TRY(phases_collect(c));
-    if (p && p->count_work) {
+    if (p && p->count_work && a.stats) {
         unsigned long long* s = (unsigned long long*)(h + 8);
         c->last.node_tests = s[0];
         c->last.tri_tests = s[1];
     }
     if (h[1]) return trace_fault(h[1], "trace");
     return SQ_OK;
+}
+
+int trace_rays_device(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray* rays, uint32_t n,
+                      const sq_trace_params* p, const sq_trace_out* out, const TraceScratch* ext) {
+    SQ_CHECK_HIP(hipSetDevice(c->device));
+    phases_reset(c);
+    TraceArgs a{};
+    SQ_TRY(target_args(b, t2, p, a));
+    a.camera = 0;
+    a.nrays = n;
+    a.rays = rays;
+    a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts;
+    SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p, ext));
+    return trace_finish(c, p, a);
 }
 
 }  // namespace sq
@@ -1101,7 +1143,7 @@ static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_c
         a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts; a.rgb = out->rgb;
         if (heat) SQ_TRY(heat_launch(c, b, a, p));
         else SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
-        return trace_finish(c, heat ? nullptr : p);
+        return trace_finish(c, heat ? nullptr : p, a);
     }
     // Host outputs: stage through workspace buffers.
     size_t off_id = 0, off_t = off_id + (out->hit_id ? nray * 4 : 0);
@@ -1119,7 +1161,7 @@ static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_c
     if (out->hit_t) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_t, a.hit_t, nray * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->counts) SQ_CHECK_HIP(hipMemcpyAsync(out->counts, a.counts, npix * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->rgb) SQ_CHECK_HIP(hipMemcpyAsync(out->rgb, a.rgb, npix * 3, hipMemcpyDeviceToHost, c->stream));
-    return trace_finish(c, heat ? nullptr : p);
+    return trace_finish(c, heat ? nullptr : p, a);
 }
 
 static int rays_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray* rays, uint32_t n,
@@ -1127,19 +1169,13 @@ static int rays_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray
     if (!c || !(b || t2) || !rays || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_rays: NULL argument");
     if (p && p->heatmap != SQ_HEATMAP_RAY) SQ_RET(SQ_ERR_UNSUPPORTED, "sq_trace_rays: heatmap mode %u", p->heatmap);
     if (n == 0) return SQ_OK;
+    if (io_on_device) return trace_rays_device(c, b, t2, rays, n, p, out, nullptr);
     SQ_CHECK_HIP(hipSetDevice(c->device));
     phases_reset(c);
     TraceArgs a{};
     SQ_TRY(target_args(b, t2, p, a));
     a.camera = 0;
     a.nrays = n;
-    a.num_tiles = (n + kRayTile - 1) / kRayTile;
-    if (io_on_device) {
-        a.rays = rays;
-        a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts;
-        SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
-        return trace_finish(c, p);
-    }
     const size_t nr = n;
     size_t off_r = 0, off_id = nr * sizeof(sq_ray), off_t = off_id + nr * 4, off_c = off_t + nr * 4;
     SQ_TRY(c->scratch.ensure(off_c + nr * 4 + 256));
@@ -1153,7 +1189,7 @@ static int rays_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray
     if (out->hit_id) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_id, a.hit_id, nr * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->hit_t) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_t, a.hit_t, nr * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->counts) SQ_CHECK_HIP(hipMemcpyAsync(out->counts, a.counts, nr * 4, hipMemcpyDeviceToHost, c->stream));
-    return trace_finish(c, p);
+    return trace_finish(c, p, a);
 }
 
 int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_trace_params* p,

@@ -233,6 +233,11 @@ __device__ __forceinline__ uint32_t pop(Stack& st, float& t) {
     return e;
 }
 
+// A stack entry keeps a node index in its upper 28 bits, so Node1 trees
+// are traced only when every node index is below 2^28 (node capacity
+// checked at every launch: single trees, batches and served scenes).
+constexpr uint32_t kMaxTraceNodes = 1u << 28;
+
 __device__ __forceinline__ uint32_t encode_list(uint32_t parent, const NodeV& n) {
     if (n.count >= 1 && n.count <= 15 && n.child < (1u << 28)) return (n.child << 4) | (n.count - 1);
     return (parent << 4) | 15u;
@@ -246,6 +251,19 @@ __device__ __forceinline__ uint8_t heat_byte(uint32_t count, uint32_t spp) {
     return (uint8_t)(v * 255.0f);
 }
 
+
+// Caller-provided trace scratch (instead of the context's workspace): the
+// tile counter and fault bits at word 0, work totals (two u64) at word
+// kScratchStatsWord, per-lane stack spill areas from word kScratchSpillWord.
+struct TraceScratch {
+    uint32_t* base;
+    size_t words;
+};
+constexpr size_t kScratchStatsWord = 16, kScratchSpillWord = 64;
+// Traces n device rays into device outputs (io on the device); ext == NULL
+// uses the context's grow-only workspace, otherwise only ext's memory.
+int trace_rays_device(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray* rays, uint32_t n,
+                      const sq_trace_params* p, const sq_trace_out* out, const TraceScratch* ext);
 
 // Launch geometry and stack sizing of the traversal kernel.
 struct TracePlan {
