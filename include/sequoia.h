@@ -205,6 +205,24 @@ int sq_trace_camera(sq_ctx* ctx, const sq_bvh* bvh, const sq_camera* cam, const 
 int sq_trace_rays(sq_ctx* ctx, const sq_bvh* bvh, const sq_ray* rays, uint32_t n, const sq_trace_params* params,
                   const sq_trace_out* out, int io_on_device);
 
+/* ---- allocation-free tracing ----
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sq_trace_rays with device rays and device outputs
+ * (io_on_device = 1) whose scratch (tile counter, fault bits, work totals,
+ * per-lane stack spill areas) is one fixed array of SQ_NOALLOC_SCRATCH_BYTES
+ * per device, set up by sq_ctx_create, instead of the context's workspace;
+ * it calls no allocating routine. Node1 trees, ray heatmaps. The launch
+ * takes as many workgroups as the spill area holds for the tree's depth (at
+ * most sq_trace_rays' launch); a tree so deep that fewer than one workgroup
+ * per CU fits fails with SQ_ERR_BUDGET. Every call on a device shares that
+ * device's array, so calls that overlap on one device (from other contexts
+ * or threads) race: the caller serialises them. Outputs equal
+ * sq_trace_rays(..., io_on_device = 1). */
+#define SQ_NOALLOC_SCRATCH_BYTES (64u << 20)
+int sq_trace_rays_noalloc(sq_ctx* ctx, const sq_bvh* bvh, const sq_ray* dev_rays, uint32_t n,
+                          const sq_trace_params* params, const sq_trace_out* dev_out);
+
 /* ---- precomputed triangles (PluckerTriangle, aabb.h:182-241) ----
  * The reference declares the 48-byte record and leaves its conversion
  * commented out, with no intersection routine. Here it is the projection

@@ -153,6 +153,7 @@ _SIGS = {
     "sq_refit": (C.c_int, [_P, _P]),
     "sq_trace_camera": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
     "sq_trace_rays": (C.c_int, [_P, _P, _P, _U32, C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
+    "sq_trace_rays_noalloc": (C.c_int, [_P, _P, _P, _U32, C.POINTER(TraceParams), C.POINTER(TraceOut)]),
     "sq_batch_create": (C.c_int, [_P, _P, _P, _U32, C.c_int, C.POINTER(_P)]),
     "sq_batch_free": (None, [_P]),
     "sq_batch_count": (_U32, [_P]),
@@ -649,6 +650,17 @@ class Context:
         p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work, kernel)
         f = lib().sq_trace2_camera if isinstance(bvh, Bvh2) else lib().sq_trace_camera
         _check(f(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 1))
+
+    def trace_rays_device(self, bvh, rays, n, hit_id=None, hit_t=None, counts=None, semantics=TRACE_EXHAUSTIVE,
+                          count_work=False, lists_per_step=0, noalloc=False):
+        """Traces n rays already in device memory into device outputs; noalloc
+        selects the allocation-free entry point (fixed per-device scratch)."""
+        out = TraceOut(hit_id, hit_t, counts, None)
+        p = self._params(semantics, HEATMAP_RAY, 0, 0, count_work, 0, 0, lists_per_step)
+        if noalloc:
+            _check(lib().sq_trace_rays_noalloc(self.h, bvh.h, rays, n, C.byref(p), C.byref(out)))
+        else:
+            _check(lib().sq_trace_rays(self.h, bvh.h, rays, n, C.byref(p), C.byref(out), 1))
 
     def upload(self, tris):
         tris = as_tris(tris)

@@ -161,8 +161,10 @@ int sq_ctx_create(int device, sq_ctx** out) {
         (void)hipEventCreateWithFlags(&c->ev_end[i], hipEventReleaseToDevice);
     }
     // Pinned status words (trace fault bits and work totals are read back
-    // through them), set up here so that later calls need not allocate them.
+    // through them) and the allocation-free entry points' scratch address
+    // on this device: both set up here, so later calls need not allocate.
     int rc = pinned_ensure(c, 4096);
+    if (rc == SQ_OK) rc = noalloc_setup(device);
     if (rc != SQ_OK) {
         sq_ctx_destroy(c);
         return rc;

@@ -133,6 +133,9 @@ void phase_end(sq_ctx* c, Phase p);
 void phases_reset(sq_ctx* c);
 int phases_collect(sq_ctx* c);
 int pinned_ensure(sq_ctx* c, size_t bytes);
+// Looks up, once per device (with that device current), the device address
+// of the allocation-free entry points' scratch array (noalloc.hip).
+int noalloc_setup(int device);
 
 // Kernel launchers implemented across the .hip files.
 int launch_fill_u32(sq_ctx* c, void* dst, uint32_t value, size_t words);
