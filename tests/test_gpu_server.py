@@ -7,6 +7,8 @@ import time
 import numpy as np
 import pytest
 
+from oracle import binding as ob
+
 import sequoia_amd as sq
 
 pytestmark = pytest.mark.gpu
@@ -22,7 +24,7 @@ def same(a, b):
     return np.array_equal(np.ascontiguousarray(a).view(np.uint8), np.ascontiguousarray(b).view(np.uint8))
 
 
-def plain(ctx, tris, camera=None, rays=None, semantics=sq.TRACE_EXHAUSTIVE):
+def plain(ctx, tris, camera=None, rays=None, semantics=sq.TRACE_EXHAUSTIVE, with_nodes=False):
     s = ctx.upload(tris)
     b = ctx.build(s, "morton63")
     if rays is not None:
@@ -30,9 +32,27 @@ def plain(ctx, tris, camera=None, rays=None, semantics=sq.TRACE_EXHAUSTIVE):
         r["rgb"] = None
     else:
         r = ctx.trace_camera(b, camera, semantics=semantics)
+    if with_nodes:
+        r["nodes"], r["root"] = b.download(), b.root
     b.free()
     s.free()
     return r
+
+
+def oracle_check(out, tris, nodes, root, camera=None, rays=None, semantics=sq.TRACE_EXHAUSTIVE):
+    """Served outputs against the oracle's trace of the same tree; closest
+    semantics are checked on hits only (their counts are the traversal's own
+    box tests)."""
+    if rays is not None:
+        ref = ob.trace_rays(nodes, root, tris, rays)
+    else:
+        ref = ob.trace_camera(nodes, root, tris, camera)
+        if semantics == sq.TRACE_EXHAUSTIVE:
+            assert np.array_equal(out["rgb"], ref["rgb"])
+    if semantics == sq.TRACE_EXHAUSTIVE:
+        assert np.array_equal(out["counts"], ref["counts"])
+    assert np.array_equal(out["hit_id"], ref["hit_id"])
+    assert np.array_equal(out["hit_t"].view(np.uint32), ref["hit_t"].view(np.uint32))
 
 
 def mix_requests():
@@ -75,11 +95,13 @@ def test_mix_equals_alone(ctx, semantics):
         st = srv.stats()
     say("stats", st)
     assert st["completed"] == len(reqs) and st["failed"] == 0
-    for (k, t, c, r), (status, out) in zip(reqs, got):
+    for j, ((k, t, c, r), (status, out)) in enumerate(zip(reqs, got)):
         assert status["state"] == sq.REQ_DONE and status["status"] == 0
-        want = plain(ctx, scenes[k], camera=c, rays=r, semantics=semantics)
+        want = plain(ctx, scenes[k], camera=c, rays=r, semantics=semantics, with_nodes=True)
         for f in ("hit_id", "hit_t", "counts", "rgb"):
             assert same(out[f], want[f]), (k, f)
+        if j % 3 == 0:
+            oracle_check(out, scenes[k], want["nodes"], want["root"], camera=c, rays=r, semantics=semantics)
     # the same requests one at a time on a fresh server
     with sq.Server(0, semantics=semantics) as srv:
         for (k, t, c, r), (_, mixed) in zip(reqs[:6], got[:6]):
@@ -87,6 +109,25 @@ def test_mix_equals_alone(ctx, semantics):
             _, alone = srv.wait(rid)
             for f in ("hit_id", "hit_t", "counts", "rgb"):
                 assert same(alone[f], mixed[f]), (k, f)
+
+
+@pytest.mark.parametrize("spp", [4, 5])
+def test_single_camera_wave_equals_plain_and_oracle(ctx, spp):
+    """A wave holding one exhaustive camera request of 4+ samples per pixel
+    tiles it 4x4 as a plain call does; ragged image sides (not multiples of
+    the tile) and a trace-only repeat in a wave of its own."""
+    tris = sq.gen_icosphere(30000, 77, 0.1)
+    cam = sq.pinhole_camera(tris, 203, 117, 60.0, 3.0, spp=spp, jitter=1, seed=spp)
+    cam2 = sq.pinhole_camera(tris, 61, 35, 45.0, 2.5, spp=spp, jitter=1, seed=10 + spp)
+    with sq.Server(0, inflight_waves=1, max_wave_requests=1) as srv:
+        s1, out1 = srv.wait(srv.submit(1, tris=tris, camera=cam))
+        s2, out2 = srv.wait(srv.submit(1, camera=cam2))
+    assert s1["state"] == sq.REQ_DONE and s2["state"] == sq.REQ_DONE and s1["wave"] != s2["wave"]
+    for c, out in ((cam, out1), (cam2, out2)):
+        want = plain(ctx, tris, camera=c, with_nodes=True)
+        for f in ("hit_id", "hit_t", "counts", "rgb"):
+            assert same(out[f], want[f]), (spp, f)
+        oracle_check(out, tris, want["nodes"], want["root"], camera=c)
 
 
 def test_batched_waves_overlap():
