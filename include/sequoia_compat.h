@@ -3,7 +3,9 @@
  * library. A caller written against the reference's headers (aabb.h,
  * builder.h, tracer.h) includes this header instead and relinks against
  * libsequoia_amd.so; the types keep the reference's layout and the functions
- * its signatures:
+ * its signatures and mangled names, so objects compiled against the
+ * reference's own headers (its main.cpp, for one) link against the library
+ * unchanged:
  *
  *   Triangle, Node1, Hierarchy ............. aabb.h:17-24, 34-42, 197-205
  *   Builder::build_hierarchy{,_morton,...} . builder.h:81-104
@@ -27,12 +29,29 @@
 #include <cstdint>
 #include <vector>
 
+/* The reference's vec3f1 lives in namespace vec (vec3.h:10,130) and has a
+ * user-provided copy constructor (vec3.h:143), which makes it non-trivial for
+ * the purpose of calls: the Itanium C++ ABI passes such an argument as a
+ * pointer to a caller-made copy, and mangles it as vec::vec3f1. This type
+ * keeps both properties, so trace(..., vec3f1) and its templates link and
+ * pass the origin exactly as in objects compiled against the reference's
+ * tracer.h:129-135. */
+namespace vec {
 struct vec3f1 {
     float x, y, z;
     vec3f1() {}
     vec3f1(float a) : x(a), y(a), z(a) {}
     vec3f1(float x_, float y_, float z_) : x(x_), y(y_), z(z_) {}
+    vec3f1(const vec3f1& a) : x(a.x), y(a.y), z(a.z) {}
+    vec3f1& operator=(const vec3f1& o) {
+        x = o.x;
+        y = o.y;
+        z = o.z;
+        return *this;
+    }
 };
+}  // namespace vec
+using vec::vec3f1;
 
 struct Triangle {
     vec3f1 v0, v1, v2;
@@ -62,6 +81,7 @@ struct Hierarchy {
     unsigned root_index;
 };
 
+static_assert(sizeof(vec3f1) == 12, "vec3f1 layout");
 static_assert(sizeof(Triangle) == 36, "Triangle layout");
 static_assert(sizeof(Node1) == 32, "Node1 layout");
 

@@ -12,6 +12,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libseqref.so")
+# the reference compiled with NUM_BINS (builder.cpp:11) set to 16 or 32
+REF_SO_BINS = {8: REF_SO, 16: os.path.join(HERE, "_ref", "bins16", "libseqref.so"),
+               32: os.path.join(HERE, "_ref", "bins32", "libseqref.so")}
 
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
@@ -50,7 +53,7 @@ def _p(a):
 
 
 _o = None
-_r = None
+_r = {}
 
 
 def oracle():
@@ -96,16 +99,17 @@ def oracle():
     return _o
 
 
-def reference_available():
-    return os.path.exists(REF_SO)
+def reference_available(bins=8):
+    return os.path.exists(REF_SO_BINS[bins])
 
 
-def reference():
-    global _r
-    if _r is None:
-        if not reference_available():
+def reference(bins=8):
+    """The compiled reference (its SAH builders with `bins` bins)."""
+    r = _r.get(bins)
+    if r is None:
+        if not reference_available(bins):
             raise RuntimeError("reference library not built (make -C oracle ref)")
-        _r = C.CDLL(REF_SO)
+        r = _r[bins] = C.CDLL(REF_SO_BINS[bins])
         V, U = C.c_void_p, C.c_uint32
         sig = {
             "ref_sizeof_node": (U, []), "ref_sizeof_triangle": (U, []),
@@ -122,9 +126,9 @@ def reference():
             "ref_cli_main": (C.c_int, [C.c_int, C.POINTER(C.c_char_p)]),
         }
         for k, (res, args) in sig.items():
-            f = getattr(_r, k)
+            f = getattr(r, k)
             f.restype, f.argtypes = res, args
-    return _r
+    return r
 
 
 # ---------------- oracle helpers ----------------
@@ -240,13 +244,13 @@ def trace_bundle(nodes, root, tris, cam, K):
 REF_KINDS = {"recursive": 0, "horizontal": 1, "grid": 2, "grid_sse": 3, "grid_avx": 4, "morton": 5, "morton2": 6}
 
 
-def ref_build(tris, kind):
+def ref_build(tris, kind, bins=8):
     tris = tris32(tris)
     n = len(tris)
     cap = 2 * n + (128 if kind.startswith("grid") else 0)
     nodes = np.zeros(cap, NODE_DTYPE)
     root = C.c_uint32()
-    rc = reference().ref_build(REF_KINDS[kind], _p(tris), n, _p(nodes), C.byref(root))
+    rc = reference(bins).ref_build(REF_KINDS[kind], _p(tris), n, _p(nodes), C.byref(root))
     assert rc == 0
     return nodes, root.value
 

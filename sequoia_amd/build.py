@@ -88,6 +88,8 @@ def build_oracle(with_reference=True):
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
     if with_reference and os.path.isdir("/root/reference/src"):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+        if os.path.exists(LIB):
+            subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "relink"], check=True)
 
 
 def main(argv=None):

@@ -4,7 +4,12 @@ Runs only where the reference sources exist (they are compiled by
 oracle/Makefile into oracle/_ref/libseqref.so); the fixtures it writes are
 plain data (inputs and the reference's outputs) and travel with the repo.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py             # everything
+    python tests/golden/make_golden.py --bins      # only the 16/32-bin SAH entries
+
+The reference fixes its SAH bin count at compile time (builder.cpp:11); the
+16- and 32-bin entries ("<scene>/<builder>@<bins>") come from the reference
+compiled with that constant changed (oracle/Makefile, bins16/bins32).
 
 Each reference build runs in a fresh subprocess: the reference's grid_sse /
 grid_avx partition stores four lanes past the end of its scratch vectors
@@ -31,6 +36,8 @@ SCENES = {
 }
 SMALL = ["ico1280", "soup2000", "clus3000"]
 KINDS = ["morton", "morton2", "recursive", "horizontal", "grid", "grid_sse", "grid_avx"]
+SAH_KINDS = ["recursive", "horizontal", "grid", "grid_sse", "grid_avx"]
+BIN_COUNTS = [16, 32]
 TRACE_W, TRACE_H = 160, 96
 
 
@@ -44,22 +51,39 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
-def child_build(name, kind, path):
+def child_build(name, kind, path, bins=8):
     from oracle import binding as ob
     tris = scene(name)
-    nodes, root = ob.ref_build(tris, kind)
+    nodes, root = ob.ref_build(tris, kind, bins)
     np.save(path, nodes)
     print(root)
 
 
-def build_in_subprocess(name, kind):
-    path = os.path.join("/tmp", "golden_%s_%s.npy" % (name, kind))
-    r = subprocess.run([sys.executable, __file__, "--child", name, kind, path], capture_output=True, text=True)
+def build_in_subprocess(name, kind, bins=8):
+    path = os.path.join("/tmp", "golden_%s_%s_%d.npy" % (name, kind, bins))
+    r = subprocess.run([sys.executable, __file__, "--child", name, kind, path, str(bins)], capture_output=True,
+                       text=True)
     if r.returncode != 0:
         raise RuntimeError("reference build %s/%s failed: %s" % (name, kind, r.stderr[-400:]))
     nodes = np.load(path)
     os.remove(path)
     return nodes, int(r.stdout.strip().splitlines()[-1])
+
+
+def bin_entries(manifest):
+    """Whole-tree SHA-256s of the reference's SAH and grid builders at 16 and
+    32 bins on every golden scene."""
+    for name in SCENES:
+        for bins in BIN_COUNTS:
+            for kind in SAH_KINDS:
+                key = "%s/%s@%d" % (name, kind, bins)
+                try:
+                    nodes, root = build_in_subprocess(name, kind, bins)
+                except RuntimeError as e:
+                    manifest["builds"][key] = {"reference_error": str(e)[-160:]}
+                    continue
+                manifest["builds"][key] = {"root": root, "nodes_sha256": sha(nodes), "capacity": int(len(nodes))}
+        print(name, "bins done", flush=True)
 
 
 def main():
@@ -135,12 +159,24 @@ def main():
         kat64.append([x, y, z, int(ob.reference().ref_interleave64(x, y, z))])
     manifest["interleave32"] = kat
     manifest["interleave64"] = kat64
+    bin_entries(manifest)
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
+        json.dump(manifest, f, indent=1, sort_keys=True)
+
+
+def main_bins():
+    path = os.path.join(OUT, "manifest.json")
+    with open(path) as f:
+        manifest = json.load(f)
+    bin_entries(manifest)
+    with open(path, "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "--child":
-        child_build(sys.argv[2], sys.argv[3], sys.argv[4])
+        child_build(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]))
+    elif len(sys.argv) > 1 and sys.argv[1] == "--bins":
+        main_bins()
     else:
         main()

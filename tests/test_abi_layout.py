@@ -43,3 +43,33 @@ def test_struct_layouts_match_header(tmp_path):
             assert int(out["%s.%s" % (cname, f)]) == getattr(cls, f).offset, (cname, f)
     for cname, dt in ARRAYS.items():
         assert int(out[cname]) == np.dtype(dt).itemsize, cname
+
+
+REF_MAIN_OBJ = os.path.join(ROOT, "oracle", "_ref", "main_cli.o")
+LIBSQ = os.path.join(ROOT, "sequoia_amd", "libsequoia_amd.so")
+
+
+def _syms(args):
+    r = subprocess.run(["nm", "-C"] + args, capture_output=True, text=True, check=True)
+    return r.stdout.splitlines()
+
+
+@pytest.mark.skipif(not os.path.exists(REF_MAIN_OBJ) or not os.path.exists(LIBSQ),
+                    reason="needs the reference's main.cpp compiled by oracle/Makefile (relink) and the library")
+def test_reference_main_links_against_library(tmp_path):
+    """The reference's main.cpp, compiled against its own headers, links
+    against libsequoia_amd.so with no builder.o, tracer.o or algorithms.o:
+    every Builder::* and trace* symbol it leaves undefined is defined by the
+    library under the same mangled name (vec::vec3f1 passed as the reference
+    passes it)."""
+    need = [l.split(None, 1)[1] for l in _syms(["-u", REF_MAIN_OBJ])
+            if "Builder::" in l or "trace" in l]
+    assert len(need) == 13, need  # six builders, trace, trace_packet<1,2,4,8>, trace_bundle<16,64>
+    assert all("vec::vec3f1" in s for s in need if "trace" in s)
+    have = {l.split(None, 2)[2] for l in _syms(["-D", "--defined-only", LIBSQ]) if len(l.split(None, 2)) == 3}
+    missing = [s for s in need if s not in have]
+    assert not missing, missing
+    exe = tmp_path / "relinked"
+    r = subprocess.run(["g++", "-fopenmp", REF_MAIN_OBJ, "-o", str(exe), "-L" + os.path.dirname(LIBSQ),
+                        "-lsequoia_amd"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]

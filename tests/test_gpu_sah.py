@@ -38,6 +38,26 @@ def test_sah_reference_bins_bit_exact(ctx, name, kind):
 
 
 @pytest.mark.parametrize("nb", [16, 32])
+@pytest.mark.parametrize("kind", list(ORACLE))
+@pytest.mark.parametrize("name", ["ico1280", "soup2000", "clus3000", "ico69632"])
+def test_sah_16_32_bins_match_reference(ctx, manifest, name, kind, nb):
+    """GPU trees at 16 and 32 bins against the reference compiled with that
+    bin count (whole Node1[] SHA-256)."""
+    import hashlib
+    m = manifest["builds"]["%s/%s@%d" % (name, kind, nb)]
+    if "reference_error" in m:
+        pytest.skip("the reference build aborts on this scene: " + m["reference_error"][:60])
+    tris = GOLDEN_SCENES[name]()
+    scene = ctx.upload(tris)
+    bvh = ctx.build(scene, kind, num_bins=nb)
+    got = bvh.download()
+    assert bvh.root == m["root"] and len(got) == m["capacity"]
+    assert hashlib.sha256(np.ascontiguousarray(got).tobytes()).hexdigest() == m["nodes_sha256"]
+    bvh.free()
+    scene.free()
+
+
+@pytest.mark.parametrize("nb", [16, 32])
 @pytest.mark.parametrize("layout", [sq.LAYOUT_REFERENCE, sq.LAYOUT_FIXED])
 def test_sah_bins_and_layouts(ctx, nb, layout):
     say("sah bins", nb, layout)

@@ -94,6 +94,42 @@ def test_builder_matches_reference(name, kind, manifest):
                 assert len(nodes_equal(nodes, g[key])) == 0
 
 
+SAH_ARGS = {
+    "recursive": lambda t, nb: ob.build_sah(t, nb, ob.PART_HOARE, ob.SA_SCALAR),
+    "horizontal": lambda t, nb: ob.build_sah(t, nb, ob.PART_HOARE, ob.SA_SCALAR),
+    "grid": lambda t, nb: ob.build_grid(t, 4, nb, ob.PART_HOARE, ob.SA_SCALAR, ob.CELLBOX_SCENE),
+    "grid_sse": lambda t, nb: ob.build_grid(t, 4, nb, ob.PART_STABLE, ob.SA_SSE, ob.CELLBOX_TIGHT),
+    "grid_avx": lambda t, nb: ob.build_grid(t, 4, nb, ob.PART_STABLE, ob.SA_M128, ob.CELLBOX_TIGHT),
+}
+
+
+@pytest.mark.parametrize("bins", [16, 32])
+@pytest.mark.parametrize("kind", list(SAH_ARGS))
+@pytest.mark.parametrize("name", ALL)
+def test_sah_bins_match_reference(name, kind, bins, manifest):
+    """16 and 32 bins: whole Node1[] against the reference compiled with
+    NUM_BINS = 16 / 32 (builder.cpp:11)."""
+    m = manifest["builds"]["%s/%s@%d" % (name, kind, bins)]
+    if "reference_error" in m:
+        pytest.skip("the reference build aborts on this scene: " + m["reference_error"][:60])
+    tris = GOLDEN_SCENES[name]()
+    nodes, root = SAH_ARGS[kind](tris, bins)
+    assert root == m["root"]
+    assert len(nodes) == m["capacity"]
+    assert sha(nodes) == m["nodes_sha256"]
+
+
+def test_bin_count_changes_the_trees(manifest):
+    """The 16/32-bin fixtures are not the 8-bin trees again: the bin count
+    reached the reference build."""
+    b = manifest["builds"]
+    for name in ALL:
+        s8 = b["%s/recursive" % name]["nodes_sha256"]
+        s16 = b["%s/recursive@16" % name]["nodes_sha256"]
+        s32 = b["%s/recursive@32" % name]["nodes_sha256"]
+        assert len({s8, s16, s32}) == 3, name
+
+
 @pytest.mark.parametrize("name", ALL)
 def test_traces_match_reference(name, manifest):
     import sequoia_amd as sq
