@@ -19,11 +19,14 @@ cosine-weighted secondary rays traced to their closest hits), "batched" (256
 scenes built and traced in one captured graph against 256 sequential plain
 calls), "served" (a Poisson request stream through the serving loop against
 one plain call per request, throughput and p50/p99 latency at three rates),
-and "cpu_baseline" (the reference's own CPU build + traverse on a bounded
-sample).
+"build_roofline" (per-phase times of the Morton63 build and the 32-bin SAH
+build against their algorithmic HBM bytes), "config1" (the 69,632-triangle
+displaced icosphere: reference CPU builds and trace beside the GPU's), and
+"cpu_baseline" (the reference's own CPU build + traverse on a bounded sample,
+with the GPU trace of the same tree beside it).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-sah] [--no-incoherent]
-                    [--no-batched] [--no-served]
+                    [--no-batched] [--no-served] [--no-build-roofline] [--no-config1]
 """
 import argparse
 import json
@@ -40,16 +43,43 @@ N_TRI = 1_000_000
 W, H, SPP = 1920, 1080, 4
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "trace_traffic_soup1m.json")
+# the sources whose code the committed traffic counters measured
+TRACE_SOURCES = ("sequoia_amd/csrc/trace.hip", "sequoia_amd/csrc/trace_common.h")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(tris, cam, gpu_rays_per_s, budget_s):
-    """Times the reference build + traverse on a bounded sample (host cores)."""
+def sources_sha256(paths=TRACE_SOURCES):
+    import hashlib
+    h = hashlib.sha256()
+    for p in paths:
+        with open(os.path.join(ROOT, p), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+
+
+def cpu_baseline(sq, ctx, scene, tris, cam, gpu_rays_per_s, budget_s):
+    """Times the reference build + traverse on a bounded sample (host cores),
+    and the GPU trace of the full frame through that same tree."""
     from oracle import binding as ob
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    threads = cpu_threads()
     kind = "reference" if ob.reference_available() else "port"
     t0 = time.perf_counter()
     if kind == "reference":
@@ -77,14 +107,184 @@ def cpu_baseline(tris, cam, gpu_rays_per_s, budget_s):
         ob.trace_rays(nodes, root, tris, sample)
     trace_s = time.perf_counter() - t0
     mrays = len(sample) / trace_s / 1e6
+    # the GPU on the very same tree (uploaded), full frame, device outputs
+    same = ctx.bvh_from_host(scene, nodes, root)
+    nr = cam.width * cam.height * cam.spp
+    d_id, d_t = ctx.alloc(nr * 4), ctx.alloc(nr * 4)
+    try:
+        ms = []
+        for r in range(4):
+            ctx.trace_camera_device(same, cam, d_id, d_t, None, None)
+            if r:
+                ms.append(ctx.timings()["trace_ms"])
+    finally:
+        ctx.free(d_id)
+        ctx.free(d_t)
+        same.free()
+    gpu_same = nr / (float(np.median(ms)) / 1e3) / 1e6
     return {
         "value": round(mrays, 3), "unit": "Mrays/s", "cores": threads, "kind": kind,
+        "cpu_model": cpu_model(), "host_threads_available": os.cpu_count(),
         "sample": "%d of %d config rays (every %d-th) traced exhaustively on the reference's 8-bin recursive SAH "
                   "tree of the same 1M soup, %.1f s; that build took %.2f s (%.2f Mtri/s)"
                   % (len(sample), len(rays), stride, trace_s, build_s, len(tris) / build_s / 1e6),
         "build_mtri_s": round(len(tris) / build_s / 1e6, 3),
         "gpu_over_cpu": round(gpu_rays_per_s / 1e6 / mrays, 1) if mrays > 0 else None,
+        "gpu_same_tree_mrays_s": round(gpu_same, 1),
+        "gpu_over_cpu_same_tree": round(gpu_same / mrays, 1) if mrays > 0 else None,
     }
+
+
+def prim_levels(nodes, root):
+    """Sum over leaves of (triangles in the leaf) x (depth of the leaf), the
+    root at depth 0: the number of binning/partition levels every primitive
+    passes through in a top-down SAH build."""
+    frontier, depth, total = np.array([root], np.int64), 0, 0
+    while len(frontier):
+        leaf = (nodes["flags"][frontier] & 1) != 0
+        total += depth * int(nodes["count"][frontier[leaf]].astype(np.int64).sum())
+        inner = frontier[~leaf]
+        cnt = nodes["count"][inner].astype(np.int64)
+        first = nodes["child"][inner].astype(np.int64)
+        frontier = np.repeat(first, cnt) + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt))
+        depth += 1
+    return total
+
+
+def _rl(nbytes, ms):
+    gbs = nbytes / (ms / 1e3) / 1e9
+    return {"ms": round(ms, 4), "bytes": int(nbytes), "gb_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def build_roofline_section(sq, ctx, scene, reps=5):
+    """Each Morton63 build phase (HIP events per phase, median of `reps`
+    builds after one warm-up, outside the timed steps) against its
+    algorithmic HBM bytes, and the 32-bin SAH build against 52 B per
+    primitive per level over the levels its tree actually has."""
+    log("build roofline: Morton63 phases")
+    n = N_TRI
+    keys = ("bounds_ms", "morton_ms", "sort_ms", "emit_ms", "refit_ms", "total_build_ms")
+    ph = {k: [] for k in keys}
+    ctx.set_phase_timing(True)
+    try:
+        for r in range(reps + 1):
+            b = ctx.build(scene, "morton63")
+            t = ctx.timings()
+            b.free()
+            if r:
+                for k in keys:
+                    ph[k].append(t[k])
+        passes = int(t["sort_passes"])
+    finally:
+        ctx.set_phase_timing(False)
+    med = {k: float(np.median(v)) for k, v in ph.items()}
+    m = 2 * n - 1  # nodes written
+    alg = {"bounds": 36 * n, "morton": (36 + 12) * n, "sort": (8 + 24 * passes) * n,
+           "emit": 12 * n + 32 * m, "refit": 36 * n + 36 * m}
+    out = {"workload": "1M-triangle soup (seed 2), morton63", "sort_passes": passes,
+           "formulas": {"bounds": "36 B/tri triangle read", "morton": "36 B/tri read + 12 B/tri key+value write",
+                        "sort": "8 B/key histogram read + %d passes x (12 B read + 12 B write)" % passes,
+                        "emit": "12 B/tri sorted key+value read + 32 B per node written (2N-1 nodes)",
+                        "refit": "36 B/tri read + 36 B per node (32 B box write + 4 B link read)"}}
+    tot_b, tot_ms = 0, 0.0
+    for k in ("bounds", "morton", "sort", "emit", "refit"):
+        out[k] = _rl(alg[k], med[k + "_ms"])
+        tot_b += alg[k]
+        tot_ms += med[k + "_ms"]
+    out["phases_sum"] = _rl(tot_b, tot_ms)
+    out["build_ms_phase_timed"] = round(med["total_build_ms"], 4)
+    log("build roofline: 32-bin SAH")
+    ms = []
+    for r in range(4):
+        b = ctx.build(scene, "recursive", num_bins=32)
+        if r:
+            ms.append(ctx.timings()["total_build_ms"])
+        if r < 3:
+            b.free()
+    nodes, root = b.download(), b.root
+    b.free()
+    pl = prim_levels(nodes, root)
+    out["sah32"] = dict(_rl(52 * pl, float(np.median(ms))), prim_levels=int(pl), mean_levels=round(pl / n, 2),
+                        formula="52 B per primitive per level (12 centroid + 24 box + 4 id binned; 4 id + 4 key + "
+                                "4 id write partitioned) x levels summed over primitives")
+    return out
+
+
+C1_TRI = 69632
+
+
+def config1_section(sq, ctx, reps=5):
+    """The 69,632-triangle displaced icosphere (seed 1, amplitude 0.1): the
+    reference's Morton (30-bit codes, 8-bit digits) and 16-bin SAH builds and
+    the 1920x1080 60-degree primary-ray trace on host cores, beside the
+    library's same builds and trace on the GPU (median of `reps` after one
+    warm-up each)."""
+    from oracle import binding as ob
+    log("config 1: %d-triangle icosphere" % C1_TRI)
+    tris = sq.gen_icosphere(C1_TRI, 1, 0.1)
+    n = len(tris)
+    cam = sq.pinhole_camera(tris, W, H, 60.0, 3.0)
+    nr = W * H
+    sc = ctx.upload(tris)
+    d_id, d_t, d_cnt, d_rgb = ctx.alloc(nr * 4), ctx.alloc(nr * 4), ctx.alloc(nr * 4), ctx.alloc(nr * 3)
+    gpu = {}
+    try:
+        for name, kind, kw in (("morton30", "morton", {}), ("sah16", "recursive", {"num_bins": 16})):
+            ms = []
+            for r in range(reps + 1):
+                b = ctx.build(sc, kind, **kw)
+                if r:
+                    ms.append(ctx.timings()["total_build_ms"])
+                if r < reps:
+                    b.free()
+            bm = float(np.median(ms))
+            tms = []
+            for r in range(reps + 1):
+                ctx.trace_camera_device(b, cam, d_id, d_t, d_cnt, d_rgb)
+                if r:
+                    tms.append(ctx.timings()["trace_ms"])
+            b.free()
+            tm = float(np.median(tms))
+            gpu[name] = {"build_ms": round(bm, 4), "build_mtri_s": round(n / (bm / 1e3) / 1e6, 1),
+                         "trace_ms": round(tm, 4), "trace_mrays_s": round(nr / (tm / 1e3) / 1e6, 1)}
+    finally:
+        for p in (d_id, d_t, d_cnt, d_rgb):
+            ctx.free(p)
+        sc.free()
+    out = {"workload": "69,632-triangle icosphere (3-octave value noise, amplitude 0.1, seed 1); 1920x1080 "
+                       "primary rays, 60-degree pinhole at distance 3, exhaustive with heatmap",
+           "triangles": n, "rays": nr, "gpu": gpu}
+    if not ob.reference_available():
+        out["cpu"] = None
+        return out
+    cpu = {"kind": "reference", "cores": cpu_threads(), "cpu_model": cpu_model()}
+    mt = ob.ref_time_build(tris, "morton", reps + 1)[1:]
+    cpu["morton30_build_ms"] = round(float(np.median(mt)), 3)
+    cpu["morton30_build_mtri_s"] = round(n / (np.median(mt) / 1e3) / 1e6, 2)
+    bins = 16 if ob.reference_available(16) else 8
+    st = ob.ref_time_build(tris, "recursive", reps + 1, bins)[1:]
+    cpu["sah_bins"] = bins
+    cpu["sah_build_ms"] = round(float(np.median(st)), 3)
+    cpu["sah_build_mtri_s"] = round(n / (np.median(st) / 1e3) / 1e6, 2)
+    nodes, root = ob.ref_build(tris, "recursive", bins)
+    rays = ob.camera_rays(cam)
+    ob.ref_traverse_rays(nodes, root, tris, rays)
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ob.ref_traverse_rays(nodes, root, tris, rays)
+        ts.append(time.perf_counter() - t0)
+    tm = float(np.median(ts))
+    cpu["trace_ms"] = round(tm * 1e3, 2)
+    cpu["trace_mrays_s"] = round(nr / tm / 1e6, 2)
+    cpu["trace_note"] = "the reference's traverse() per ray over the config camera's rays, OpenMP over rays, " \
+                        "on its %d-bin SAH tree" % bins
+    out["cpu"] = cpu
+    g = gpu["sah16"]
+    out["gpu_over_cpu"] = {"morton_build": round(cpu["morton30_build_ms"] / gpu["morton30"]["build_ms"], 1),
+                           "sah_build": round(cpu["sah_build_ms"] / g["build_ms"], 1),
+                           "trace_sah_tree": round(cpu["trace_ms"] / g["trace_ms"], 1)}
+    return out
 
 
 BATCH_SCENES = 256
@@ -174,6 +374,11 @@ def incoherent_section(sq, ctx, reps=3):
         out["closest_trace_ms"] = round(t["trace_ms"], 1)
         out["closest_mrays_s"] = round(INC_RAYS / (t["trace_ms"] / 1e3) / 1e6, 3)
         out["closest_node_tests_per_ray"] = round(t["node_tests"] / INC_RAYS, 1)
+        out["closest_tri_tests_per_ray"] = round(t["tri_tests"] / INC_RAYS, 1)
+        alg = 32.0 * t["node_tests"] + 36.0 * t["tri_tests"]
+        out["closest_roofline"] = dict(_rl(alg, t["trace_ms"]), kernel="k_trace<true, false, 1, 1>",
+                                       formula="32 B per node test + 36 B per triangle test (counted on the device "
+                                               "in the same launch)")
         out["hit_fraction"] = round(float((r["hit_id"] != sq.MISS).mean()), 4)
         sub = rays[:INC_SUB]
         ctx.trace_rays(b, sub, count_work=True)
@@ -231,6 +436,15 @@ def batched_section(sq, ctx, steps, warmup):
     batch.build("morton63")
     batch.trace()
     eager = batch.times()
+    # node and triangle tests of the batched trace (a separate counting pass)
+    tuned = dict(outputs=out_mask, block_size=tune["block_size"], stack_slots=tune["stack_slots"],
+                 kernel=tune["kernel"], lists_per_step=tune["lists_per_step"], blocks_per_cu=tune["blocks_per_cu"])
+    batch.set_cameras(cams, count_work=True, **tuned)
+    batch.trace()
+    work = ctx.timings()
+    batch.set_cameras(cams, **tuned)
+    batch.trace()  # outputs again (set_cameras allocates fresh output buffers)
+    batch_alg = 32.0 * work["node_tests"] + 36.0 * work["tri_tests"]
 
     # plain path: one build + one trace call per scene
     log("batched: plain per-scene calls")
@@ -248,7 +462,7 @@ def batched_section(sq, ctx, steps, warmup):
                 check[i] = ctx.to_host(d_cnt, npix * 4, np.uint32)
             bvh.free()
 
-    check = {i: None for i in range(0, BATCH_SCENES, 37)}
+    check = {i: None for i in range(BATCH_SCENES)}
     plain_all(check)
     parity = all(np.array_equal(check[i], batch.download(i, hits=False, rgb=False)["counts"]) for i in check)
     ctx.sync()
@@ -273,6 +487,10 @@ def batched_section(sq, ctx, steps, warmup):
         "plain_wall_ms": round(plain_wall * 1e3, 2),
         "plain_mrays_s": round(nrays / plain_wall / 1e6, 1),
         "speedup_vs_plain": round(plain_wall / batch_wall, 2),
+        "trace_roofline": dict(_rl(batch_alg, eager["trace_ms"]), kernel="k_trace<false, true, 1, 2>",
+                               node_tests=int(work["node_tests"]), tri_tests=int(work["tri_tests"]),
+                               formula="32 B per node test + 36 B per triangle test over all 256 scenes, "
+                                       "against the eager batched trace time"),
         "counts_equal_plain": bool(parity), "checked_scenes": len(check),
         "autotune": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()},
     }
@@ -446,6 +664,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-scene batched-vs-plain comparison")
     ap.add_argument("--batched-steps", type=int, default=3)
     ap.add_argument("--no-served", action="store_true", help="skip the Poisson-stream served-vs-plain comparison")
+    ap.add_argument("--no-build-roofline", action="store_true", help="skip the per-phase build roofline")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 69,632-triangle CPU-vs-GPU comparison")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (byte-counter calibration)")
     a = ap.parse_args()
@@ -503,10 +723,17 @@ def main():
     achieved = alg_bytes / (trace_ms / 1e3) / 1e9
 
     heat = ctx.to_host(d_cnt, W * H * 4, np.uint32)
-    traffic = None
+    # traffic below L2 comes from the committed counter passes; it is used only
+    # when they measured the trace sources of this tree
+    traffic, traffic_note = None, "no counter profile"
     if a.semantics == "exhaustive" and os.path.exists(TRAFFIC_PROFILE):
         with open(TRAFFIC_PROFILE) as f:
-            traffic = json.load(f).get("traffic_bytes_per_launch")
+            prof = json.load(f)
+        if prof.get("sources_sha256") == sources_sha256():
+            traffic = prof.get("traffic_bytes_per_launch")
+            traffic_note = "FETCH_SIZE + WRITE_SIZE passes on these trace sources (%s)" % TRAFFIC_PROFILE[len(ROOT) + 1:]
+        else:
+            traffic_note = "stale: the counter profile measured other trace sources"
     line = {
         "metric": "Mrays/s & build Mtri/s on 1 GPU; served p50/p99 latency, batched vs plain",
         "value": round(rays_per_s / 1e6, 2),
@@ -533,10 +760,20 @@ def main():
         "tri_tests_per_ray": round(tri_tests / nrays, 2),
         "heatmap_mean_tests_per_pixel": round(float(heat.mean()) / SPP, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": "k_trace<false, false, 1, 2>", "algorithmic_bytes_per_launch": alg_bytes},
     }
     # the secondary sections never take the headline line down with them
+    if not a.no_build_roofline:
+        try:
+            line["build_roofline"] = build_roofline_section(sq, ctx, scene)
+        except Exception as e:  # noqa: BLE001
+            line["build_roofline"] = {"error": repr(e)}
+    if not a.no_config1:
+        try:
+            line["config1"] = config1_section(sq, ctx)
+        except Exception as e:  # noqa: BLE001
+            line["config1"] = {"error": repr(e)}
     if not a.no_sah:
         try:
             line["sah32"] = sah_section(sq, ctx, scene, cam, nrays)
@@ -564,7 +801,7 @@ def main():
             line["served"] = {"error": repr(e)}
     if not a.no_cpu:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(tris, cam, rays_per_s, a.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(sq, ctx, scene, tris, cam, rays_per_s, a.cpu_budget)
     for p in (d_id, d_t, d_cnt, d_rgb):
         ctx.free(p)
     scene.free()

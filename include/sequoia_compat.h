@@ -10,6 +10,7 @@
  *   Triangle, Node1, Hierarchy ............. aabb.h:17-24, 34-42, 197-205
  *   Builder::build_hierarchy{,_morton,...} . builder.h:81-104
  *   trace / trace_packet<K> / trace_bundle<K>  tracer.h:129-135
+ *   radix_sort<Key, T>, bit_interleave_32/64 .. algorithms.h:89-116
  *
  * Differences a caller can observe:
  *   - Hierarchy::node_count and tri_count are filled in (the reference leaves
@@ -102,11 +103,47 @@ uint8_t* trace_packet(Hierarchy& hierarchy, uint32_t w, uint32_t h, vec3f1 origi
 template <size_t K>
 uint8_t* trace_bundle(Hierarchy& hierarchy, uint32_t w, uint32_t h, vec3f1 origin);
 
+/* algorithms.cpp:8-22: bit i of x, y, z placed at bits 3i, 3i+1, 3i+2 (bits
+ * beyond the word dropped), as the reference's pdep masks place them. Host
+ * functions. */
+uint32_t bit_interleave_32(const uint32_t& x, const uint32_t& y, const uint32_t& z);
+uint64_t bit_interleave_64(const uint64_t& x, const uint64_t& y, const uint64_t& z);
+
 namespace sequoia_compat {
 /* Device used by the shim (default 0); call before the first build. */
 void set_device(int device);
 /* Frees node_base and the device-resident copy of the hierarchy. */
 void release(Hierarchy& h);
+/* Stable LSD sort of (key, value) pairs in host memory on the GPU, by the
+ * low key_bits bits in passes of digit_bits (1..8) bits. */
+void sort_pairs(uint64_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, uint32_t digit_bits);
 }  // namespace sequoia_compat
+
+/* algorithms.h:89-113: stable LSD radix sort of [begin, end) by (Key)(*it),
+ * four passes of b bits for a 32-bit Key and eight for a 64-bit one (so the
+ * low 4b / 8b bits of the key order the elements), result in [begin, end).
+ * Here the passes run on the GPU; the scratch range is not used and its
+ * contents are left unchanged (the reference leaves intermediate passes
+ * there). b must be 1..8 (the reference's counts[1<<b] table). */
+template <typename Key, typename T>
+static void radix_sort(typename std::vector<T>::iterator begin, typename std::vector<T>::iterator end,
+                       typename std::vector<T>::iterator scratch, typename std::vector<T>::iterator scratch_end,
+                       unsigned b) {
+    (void)scratch;
+    (void)scratch_end;
+    static_assert(sizeof(Key) == 4 || sizeof(Key) == 8, "radix_sort: 32- or 64-bit keys");
+    const size_t n = (size_t)(end - begin);
+    if (n == 0) return;
+    std::vector<uint64_t> keys(n);
+    std::vector<uint32_t> order(n);
+    std::vector<T> copy(begin, end);
+    for (size_t i = 0; i < n; i++) {
+        keys[i] = (uint64_t)(Key)copy[i];
+        order[i] = (uint32_t)i;
+    }
+    const unsigned passes = sizeof(Key) == 4 ? 4u : 8u;
+    sequoia_compat::sort_pairs(keys.data(), order.data(), (uint32_t)n, passes * b, b);
+    for (size_t i = 0; i < n; i++) begin[i] = copy[order[i]];
+}
 
 #endif

@@ -116,6 +116,7 @@ def reference(bins=8):
             "ref_interleave32": (U, [U, U, U]),
             "ref_interleave64": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_uint64]),
             "ref_build": (C.c_int, [C.c_int, V, U, V, C.POINTER(U)]),
+            "ref_time_build": (C.c_int, [C.c_int, V, U, U, V]),
             "ref_morton32": (None, [V, U, V, V, V]),
             "ref_morton64": (None, [V, U, V, V, V]),
             "ref_radix_sort32": (None, [V, U, V, V]),
@@ -253,6 +254,16 @@ def ref_build(tris, kind, bins=8):
     rc = reference(bins).ref_build(REF_KINDS[kind], _p(tris), n, _p(nodes), C.byref(root))
     assert rc == 0
     return nodes, root.value
+
+
+def ref_time_build(tris, kind, reps, bins=8):
+    """Milliseconds of each of `reps` reference builds (steady_clock around
+    the builder call only)."""
+    tris = tris32(tris)
+    out = np.zeros(reps, np.float64)
+    rc = reference(bins).ref_time_build(REF_KINDS[kind], _p(tris), len(tris), reps, _p(out))
+    assert rc == 0
+    return out
 
 
 def ref_morton32(tris):

@@ -12,6 +12,8 @@
 #include "ray.h"
 #include "tracer.h"
 
+#include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -86,6 +88,32 @@ int ref_build(int kind, const float* tris, uint32_t n, void* nodes_out, uint32_t
     std::memcpy(nodes_out, h.node_base, cap * sizeof(Node1));
     *root = h.root_index;
     delete[] h.node_base;
+    return 0;
+}
+
+// Wall time (steady_clock, ms) of `reps` builds of one kind, each on a fresh
+// copy of the triangles made outside the timed region; times[r] per build.
+// The CPU baseline's build figures, measured the way main.cpp:376-392 brackets
+// the builder call (with steady_clock in place of clock()).
+int ref_time_build(int kind, const float* tris, uint32_t n, uint32_t reps, double* times) {
+    for (uint32_t r = 0; r < reps; r++) {
+        std::vector<Triangle> prims = to_tris(tris, n);
+        Hierarchy h;
+        auto t0 = std::chrono::steady_clock::now();
+        switch (kind) {
+            case 0: h = Builder::build_hierarchy(prims); break;
+            case 1: h = Builder::build_hierarchy_horizontal(prims); break;
+            case 2: h = Builder::build_hierarchy_grid(prims); break;
+            case 3: h = Builder::build_hierarchy_grid_sse(prims); break;
+            case 4: h = Builder::build_hierarchy_grid_m128(prims); break;
+            case 5: h = Builder::build_hierarchy_morton(prims); break;
+            case 6: h = Builder::build_hierarchy_morton2(prims); break;
+            default: return -1;
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        times[r] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        delete[] h.node_base;
+    }
     return 0;
 }
 

@@ -461,10 +461,10 @@ class Batch:
         return BatchBvh(self.ctx, h, None)
 
     def set_cameras(self, cams, semantics=TRACE_EXHAUSTIVE, outputs=OUT_ALL, stack_slots=0, block_size=0,
-                    count_work=False, kernel=0, lists_per_step=0):
+                    count_work=False, kernel=0, lists_per_step=0, blocks_per_cu=0):
         arr = (Camera * len(cams))(*cams)
         p = TraceParams(semantics, HEATMAP_RAY, 0, block_size, stack_slots, 1 if count_work else 0,
-                        kernel, 0, 0, lists_per_step)
+                        kernel, blocks_per_cu, 0, lists_per_step)
         _check(lib().sq_batch_set_cameras(self.ctx.h, self.h, arr, C.byref(p), outputs))
         self.cams = list(cams)
         self._cam_arr = arr

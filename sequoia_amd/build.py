@@ -75,6 +75,11 @@ def build_library(jobs=8, verbose=False):
           os.path.join(ROOT, "tests", "compat", "compat_demo.cpp"), "-o", demo + ".tmp",
           "-L" + HERE, "-lsequoia_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(demo + ".tmp", demo)
+    alg = os.path.join(HERE, "compat_algorithms")
+    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "compat", "compat_algorithms.cpp"), "-o", alg + ".tmp",
+          "-L" + HERE, "-lsequoia_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(alg + ".tmp", alg)
     # the reference's command-line harness (sequoia <scene.obj> [-builder ...] ...)
     cli = os.path.join(HERE, "sequoia")
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),

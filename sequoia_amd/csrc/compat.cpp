@@ -165,7 +165,30 @@ template uint8_t* trace_packet<8>(Hierarchy&, uint32_t, uint32_t, vec3f1);
 template uint8_t* trace_bundle<16>(Hierarchy&, uint32_t, uint32_t, vec3f1);
 template uint8_t* trace_bundle<64>(Hierarchy&, uint32_t, uint32_t, vec3f1);
 
+// Bit i of x lands at bit 3i, of y at 3i+1, of z at 3i+2 (the reference's
+// pdep masks 0x49249249 / 0x92492492 / 0x24924924 and their 64-bit widening);
+// bits that fall beyond the word are dropped, as pdep drops them.
+template <typename U>
+static U interleave3(U x, U y, U z) {
+    const unsigned W = sizeof(U) * 8;
+    U r = 0;
+    for (unsigned i = 0; 3 * i < W; i++) {
+        r |= ((x >> i) & U(1)) << (3 * i);
+        if (3 * i + 1 < W) r |= ((y >> i) & U(1)) << (3 * i + 1);
+        if (3 * i + 2 < W) r |= ((z >> i) & U(1)) << (3 * i + 2);
+    }
+    return r;
+}
+
+uint32_t bit_interleave_32(const uint32_t& x, const uint32_t& y, const uint32_t& z) { return interleave3(x, y, z); }
+uint64_t bit_interleave_64(const uint64_t& x, const uint64_t& y, const uint64_t& z) { return interleave3(x, y, z); }
+
 namespace sequoia_compat {
+
+void sort_pairs(uint64_t* keys, uint32_t* vals, uint32_t n, uint32_t key_bits, uint32_t digit_bits) {
+    std::lock_guard<std::mutex> lock(g_mu);
+    check(sq_radix_sort_pairs(ctx(), keys, vals, n, key_bits, digit_bits), "radix_sort");
+}
 
 void set_device(int device) {
     std::lock_guard<std::mutex> lock(g_mu);

@@ -73,3 +73,28 @@ def test_reference_main_links_against_library(tmp_path):
     r = subprocess.run(["g++", "-fopenmp", REF_MAIN_OBJ, "-o", str(exe), "-L" + os.path.dirname(LIBSQ),
                         "-lsequoia_amd"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+ALG = os.path.join(ROOT, "sequoia_amd", "compat_algorithms")
+
+
+@pytest.mark.skipif(not os.path.exists(ALG), reason="compat_algorithms is built by sequoia_amd/build.py")
+def test_compat_interleave_known_answers(manifest):
+    """bit_interleave_32/64 through the C++ shim against the reference's
+    outputs and against the bit-placement rule on random inputs."""
+    def run(x, y, z):
+        out = subprocess.run([ALG, "interleave", str(x), str(y), str(z)], capture_output=True, text=True,
+                             check=True).stdout.split()
+        return int(out[0]), int(out[1])
+    for (x, y, z, w32), (x6, y6, z6, w64) in zip(manifest["interleave32"], manifest["interleave64"]):
+        assert run(x, y, z)[0] == w32
+        assert run(x6, y6, z6)[1] == w64
+    rng = np.random.default_rng(9)
+    for _ in range(8):
+        x, y, z = (int(v) for v in rng.integers(0, 1 << 21, 3))
+        want64 = sum((((x >> i) & 1) << (3 * i)) | (((y >> i) & 1) << (3 * i + 1)) | (((z >> i) & 1) << (3 * i + 2))
+                     for i in range(21))
+        assert run(x, y, z)[1] == want64
+        xs, ys, zs = x & 1023, y & 1023, z & 1023
+        assert run(xs, ys, zs)[0] == sum((((xs >> i) & 1) << (3 * i)) | (((ys >> i) & 1) << (3 * i + 1)) |
+                                         (((zs >> i) & 1) << (3 * i + 2)) for i in range(10))
