@@ -29,7 +29,7 @@
 namespace sq {
 
 constexpr int kTopThreads = 1024;
-constexpr int kSmallThreads = 256;
+constexpr int kSmallThreads = 512;
 constexpr uint32_t kDefaultThreshold = 2048;
 constexpr uint32_t kMaxThreshold = 2048;
 constexpr int kMaxLevels = 160;
@@ -410,8 +410,23 @@ struct SmallTask {
     uint32_t self;
 };
 
-// Persistent: one workgroup finishes one queued range at a time (codes in
-// LDS, the reference's binary search per thread), then refits it.
+// What each node slot of a small range holds, relative to the range's first
+// child slot (a range of c primitives places all its nodes and leaves in the
+// 2(c-1) slots from its root's `free`): a leaf (sorted position), a fan node
+// (first child slot, count), or a split node (written from the task list).
+constexpr uint32_t kSlotLeaf = 1u << 31, kSlotFan = 1u << 30;
+constexpr int kSmallSlots = 2 * kMaxThreshold;
+
+// Persistent: one workgroup finishes one queued range at a time, entirely in
+// LDS until its nodes are written:
+//   1. codes in LDS; level by level each task is split with the reference's
+//      binary search and its children recorded (slot tags, new tasks);
+//   2. every leaf's triangle is fetched at once and its node written; the
+//      leaf boxes go to an LDS box per slot (in the space the codes held);
+//   3. fan nodes, then the split nodes deepest level first, take the union of
+//      their child slots' LDS boxes in slot order, and are written.
+// Only phase 2 reads global memory after the codes, and every node is
+// written once, whole.
 //
 // Capacity: a range of c <= T primitives has at most c - 6 split tasks
 // (tasks are ranges of >= 8; leaves of the task tree are disjoint, and every
@@ -421,10 +436,15 @@ struct SmallTask {
 // launch's error flag instead of writing out of bounds.
 template <typename K>
 __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
-    __shared__ K s_codes[kMaxThreshold];
+    // the codes (emission) and the slot boxes (phases 2-3) share one area
+    constexpr size_t kCodeBytes = sizeof(K) * kMaxThreshold, kBoxBytes = sizeof(sqd::Box3) * kSmallSlots;
+    __shared__ __align__(16) unsigned char s_area[kCodeBytes > kBoxBytes ? kCodeBytes : kBoxBytes];
+    __shared__ uint32_t s_slot[kSmallSlots];
     __shared__ SmallTask s_tasks[kMaxThreshold];
     __shared__ uint32_t s_lvl[kMaxLevels + 1];
     __shared__ uint32_t s_total, s_err;
+    K* s_codes = (K*)s_area;
+    sqd::Box3* s_box = (sqd::Box3*)s_area;
     const K* codes = (const K*)a.codes;
     const uint32_t ntasks = a.counters[0];
     uint32_t max_lev = 0;
@@ -437,13 +457,16 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
             if (threadIdx.x == 0) atomicOr(&a.counters[2], 1u);
             continue;
         }
+        const uint32_t sbase = root.free, nslots = 2 * (count - 1);
         for (uint32_t i = threadIdx.x; i < count; i += kSmallThreads) s_codes[i] = codes[base + i];
+        for (uint32_t i = threadIdx.x; i < nslots; i += kSmallThreads) s_slot[i] = 0;
         if (threadIdx.x == 0) {
             s_tasks[0] = SmallTask{count << 16, root.free, root.self};
             s_total = 1;
             s_err = 0;
         }
         __syncthreads();
+        // 1. emission: slot tags and split tasks, level by level
         uint32_t lb = 0, le = 1, nlev = 0;
         while (lb < le) {
             if (threadIdx.x == 0) {
@@ -452,14 +475,31 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
             }
             for (uint32_t i = lb + threadIdx.x; i < le; i += kSmallThreads) {
                 const SmallTask tk = s_tasks[i];
-                const uint32_t tb = tk.be & 0xFFFFu, te = tk.be >> 16;
-                uint32_t split = find_split_ref<K>(s_codes + tb, te - tb);
-                emit_children(e, base + tb, base + te, tk.free, split,
-                              [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
-                                  const uint32_t k = atomicAdd(&s_total, 1u);
-                                  if (k < kMaxThreshold) s_tasks[k] = SmallTask{(b - base) | ((en - base) << 16), f, self};
-                                  else s_err = 1;
-                              });
+                const uint32_t tb = tk.be & 0xFFFFu, te = tk.be >> 16, f = tk.free;
+                const uint32_t cnt = te - tb, L = find_split_ref<K>(s_codes + tb, cnt) + 1, R = cnt - L;
+                const uint32_t nn = (L != 1) + (R != 1);
+                uint32_t lfree = f + nn, rfree = f + nn + (L - 1) * 2 + (L == 1);
+                const uint32_t lslot = f, rslot = f + (L != 1);
+                if (e.layout == SQ_LAYOUT_FIXED && R == 1 && L != 1) {
+                    lfree = f + 2;
+                    rfree = f + 1;
+                }
+                const uint32_t side_b[2] = {tb, tb + L}, side_n[2] = {L, R};
+                const uint32_t side_free[2] = {lfree, rfree}, side_slot[2] = {lslot, rslot};
+                for (int sd = 0; sd < 2; sd++) {
+                    const uint32_t cb = side_b[sd], cn = side_n[sd], cf = side_free[sd] - sbase;
+                    const uint32_t cs = side_slot[sd] - sbase;
+                    if (cn < 8) {
+                        // a leaf (cn == 1) or a fan of cn leaves at cf..
+                        if (cf + cn > nslots || (cn > 1 && cs >= nslots)) { s_err = 1; continue; }
+                        for (uint32_t j = 0; j < cn; j++) s_slot[cf + j] = kSlotLeaf | (cb + j);
+                        if (cn > 1) s_slot[cs] = kSlotFan | (cn << 16) | cf;
+                    } else {
+                        const uint32_t k = atomicAdd(&s_total, 1u);
+                        if (k < kMaxThreshold) s_tasks[k] = SmallTask{cb | ((cb + cn) << 16), side_free[sd], side_slot[sd]};
+                        else s_err = 1;
+                    }
+                }
             }
             __syncthreads();
             lb = le;
@@ -477,15 +517,55 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
         if (threadIdx.x == 0) s_lvl[nlev] = le;
         max_lev = nlev > max_lev ? nlev : max_lev;
         if (threadIdx.x == 0) atomicMax(&a.scene_small[root.scene], nlev);
-        __syncthreads();
-        if (a.refit) {
-            for (int lv = (int)nlev - 1; lv >= 0; lv--) {
-                for (uint32_t i = s_lvl[lv] + threadIdx.x; i < s_lvl[lv + 1]; i += kSmallThreads)
-                    refit_node(e.nodes, s_tasks[i].self, e.cap);
-                __syncthreads();
+        // 2. leaves: every triangle fetched at once (the codes are no longer needed)
+        for (uint32_t sl = threadIdx.x; sl < nslots; sl += kSmallThreads) {
+            const uint32_t tag = s_slot[sl];
+            if (!(tag & kSlotLeaf)) continue;
+            const uint32_t gtri = a.idx[base + (tag & 0xFFFFu)];
+            const uint32_t slot = sbase + sl;
+            if (a.refit) {
+                float v[9];
+                sqd::load_tri(a.tris, gtri, v);
+                const sqd::Box3 b = sqd::tri_box(v);
+                s_box[sl] = b;
+                if (slot < e.cap) sqd::store_node(e.nodes, slot, b, gtri - e.tri_base, 1, 1);
+            } else if (slot < e.cap) {
+                sqd::store_node_links(e.nodes, slot, gtri - e.tri_base, 1, 1);
             }
         }
         __syncthreads();
+        // 3. fans, then split nodes deepest first: unions of child slots in slot order
+        for (uint32_t sl = threadIdx.x; sl < nslots; sl += kSmallThreads) {
+            const uint32_t tag = s_slot[sl];
+            if (!(tag & kSlotFan)) continue;
+            const uint32_t c0 = tag & 0xFFFFu, cn = (tag >> 16) & 0x3FFFu, slot = sbase + sl;
+            if (c0 + cn > nslots) continue;  // never tagged so (checked at emission)
+            if (a.refit) {
+                sqd::Box3 u = s_box[c0];
+                for (uint32_t j = 1; j < cn; j++) sqd::box_grow(u, s_box[c0 + j]);
+                s_box[sl] = u;
+                if (slot < e.cap) sqd::store_node(e.nodes, slot, u, sbase + c0, cn, 0);
+            } else if (slot < e.cap) {
+                sqd::store_node_links(e.nodes, slot, sbase + c0, cn, 0);
+            }
+        }
+        __syncthreads();
+        for (int lv = (int)nlev - 1; lv >= 0; lv--) {
+            for (uint32_t i = s_lvl[lv] + threadIdx.x; i < s_lvl[lv + 1]; i += kSmallThreads) {
+                const SmallTask tk = s_tasks[i];
+                const uint32_t c0 = tk.free - sbase;
+                if (c0 + 1 >= nslots || (i > 0 && tk.self - sbase >= nslots)) continue;  // never queued so
+                if (a.refit) {
+                    sqd::Box3 u = s_box[c0];
+                    sqd::box_grow(u, s_box[c0 + 1]);
+                    if (i > 0) s_box[tk.self - sbase] = u;  // the range's root lies outside its slots
+                    if (tk.self < e.cap) sqd::store_node(e.nodes, tk.self, u, tk.free, 2, 0);
+                } else if (tk.self < e.cap) {
+                    sqd::store_node_links(e.nodes, tk.self, tk.free, 2, 0);
+                }
+            }
+            __syncthreads();
+        }
     }
     if (threadIdx.x == 0 && max_lev) atomicMax(&a.counters[1], max_lev);
 }

@@ -279,21 +279,22 @@ __host__ __device__ constexpr size_t os_tickets() { return (size_t)kMaxKeyPasses
 __host__ __device__ constexpr size_t os_spare() { return os_tickets() + kMaxKeyPasses; }
 __host__ __device__ constexpr size_t os_status() { return os_spare() + 64; }
 
+// Digit totals of passes p0 .. kpasses-1 (ghist row p = pass p).
 template <typename K>
 __global__ __launch_bounds__(kRsThreads) void k_rs_ghist(const K* __restrict__ keys, uint32_t n, uint32_t bits,
-                                                         uint32_t kpasses, uint32_t* __restrict__ ghist) {
+                                                         uint32_t p0, uint32_t kpasses, uint32_t* __restrict__ ghist) {
     __shared__ uint32_t s_h[kMaxKeyPasses * 2 > 8 * kRadixMax ? kMaxKeyPasses * 2 : 8 * kRadixMax];
-    const uint32_t radix = 1u << bits, mask = radix - 1, cells = kpasses * radix;
+    const uint32_t radix = 1u << bits, mask = radix - 1, cells = (kpasses - p0) * radix;
     for (uint32_t i = threadIdx.x; i < cells; i += kRsThreads) s_h[i] = 0;
     __syncthreads();
     for (size_t i = (size_t)blockIdx.x * kRsThreads + threadIdx.x; i < n; i += (size_t)gridDim.x * kRsThreads) {
         const K k = keys[i];
-        for (uint32_t p = 0; p < kpasses; p++) atomicAdd(&s_h[p * radix + digit_of(k, p * bits, mask)], 1u);
+        for (uint32_t p = p0; p < kpasses; p++) atomicAdd(&s_h[(p - p0) * radix + digit_of(k, p * bits, mask)], 1u);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < cells; i += kRsThreads) {
         const uint32_t v = s_h[i];
-        if (v) atomicAdd(&ghist[(i / radix) * kRadixMax + (i % radix)], v);
+        if (v) atomicAdd(&ghist[(p0 + i / radix) * kRadixMax + (i % radix)], v);
     }
 }
 
@@ -437,6 +438,273 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_onesweep(const K* __restrict_
     }
 }
 
+// ---- local finish (plain builds with 8-bit digits and three or more passes) ----
+// The two most significant digits run as global single-pass passes, which
+// leaves the pairs ordered by their top 16 key bits (stably). Each workgroup
+// then takes a span of about kLocSpan positions whose ends are moved back to
+// the start of their top-16-bit run, so spans hold whole runs and together
+// cover the array once, and sorts it by every digit in which its keys differ
+// (LSD, stable, in LDS). Every pass is a stable counting sort on one digit,
+// as in the global passes, so the result is the same (key, original position)
+// order. A span longer than kLocCap (one run of equal top bits longer than
+// the LDS holds) is sorted by the same workgroup in chunks through global
+// memory: per digit a count over the span, then the chunks in order, each
+// ranked stably and scattered behind the earlier chunks' keys.
+constexpr int kLocCap = 8192;                        // keys sorted in LDS by one workgroup
+constexpr int kLocRounds = kLocCap / kRsThreads;     // 8 keys per thread
+constexpr int kLocWaveKeys = kLocCap / kRsWaves;     // 512 keys per wave
+constexpr uint32_t kLocSpan = kLocCap / 2;           // nominal span per workgroup
+
+// Smallest q <= p whose key has the same prefix (key >> pshift) as position p
+// (keys ordered by prefix): one wave, 64-ary search, first over the kLocSpan
+// positions before p, then over all of [0, p] when the run reaches further.
+template <typename K>
+__device__ uint32_t run_start(const K* __restrict__ keys, uint32_t p, uint32_t pshift, int lane) {
+    if (p == 0) return 0;
+    const K target = keys[p] >> pshift;
+    uint32_t lo = p > kLocSpan ? p - kLocSpan : 0, hi = p;
+    bool widened = lo == 0;
+    while (lo < hi) {  // the answer lies in [lo, hi]; keys[hi] has the prefix
+        const uint32_t span = hi - lo;
+        const uint32_t pos = lo + (uint32_t)(((uint64_t)span * (uint32_t)lane) >> 6);
+        const uint64_t eq = __ballot((keys[pos] >> pshift) == target);
+        if (eq & 1ull) {  // lane 0 probes lo itself
+            if (widened) return lo;
+            hi = lo;
+            lo = 0;
+            widened = true;
+            continue;
+        }
+        const int k = eq ? __ffsll((long long)eq) - 1 : 64;
+        const uint32_t below = __shfl(pos, k - 1, 64);
+        const uint32_t at = __shfl(pos, k < 64 ? k : 63, 64);
+        lo = below + 1;
+        if (k < 64) hi = at;
+    }
+    return lo;
+}
+
+// Exclusive scan of s_hist[0..256) into s_start by wave 0 (4 digits per lane).
+__device__ __forceinline__ void digit_starts(const uint32_t* s_hist, uint32_t* s_start, int wave, int lane) {
+    if (wave != 0) return;
+    uint32_t v[kRadixMax / 64], run = 0;
+#pragma unroll
+    for (int j = 0; j < kRadixMax / 64; j++) {
+        v[j] = s_hist[lane * (kRadixMax / 64) + j];
+        run += v[j];
+    }
+    uint32_t ex = wave_incl_scan(run, lane) - run;
+#pragma unroll
+    for (int j = 0; j < kRadixMax / 64; j++) {
+        s_start[lane * (kRadixMax / 64) + j] = ex;
+        ex += v[j];
+    }
+}
+
+// Stable ranks of one workgroup's kLocCap keys (position w*512 + r*64 + lane,
+// the first m valid) on digit `shift`: dig[r], rank[r] within the wave, then
+// s_cnt[w][d] = keys of digit d in earlier waves, s_hist[d] = total of d.
+template <typename K>
+__device__ __forceinline__ void local_rank(const K (&key)[kLocRounds], uint32_t m, uint32_t shift,
+                                           uint32_t (&dig)[kLocRounds], uint32_t (&rank)[kLocRounds],
+                                           uint32_t (*s_cnt)[kRadixMax], uint32_t* s_hist, int wave, int lane) {
+    for (uint32_t i = threadIdx.x; i < kRsWaves * kRadixMax; i += kRsThreads) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kLocRounds; r++) {
+        const uint32_t i = (uint32_t)(wave * kLocWaveKeys + r * 64 + lane);
+        const bool valid = i < m;
+        const uint64_t active = __ballot(valid);
+        const uint32_t d = digit_of(key[r], shift, 0xFFu);
+        dig[r] = d;
+        rank[r] = 0;
+        if (active == 0) continue;
+        const uint64_t peers = match_digit(d, 8, active);
+        const uint64_t lower = peers & ((1ull << lane) - 1ull);
+        const uint32_t prior = valid ? s_cnt[wave][d] : 0u;
+        rank[r] = prior + (uint32_t)__popcll(lower);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && lower == 0) s_cnt[wave][d] = prior + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < kRadixMax; d += kRsThreads) {
+        uint32_t run = 0;
+        for (int w = 0; w < kRsWaves; w++) {
+            const uint32_t c = s_cnt[w][d];
+            s_cnt[w][d] = run;
+            run += c;
+        }
+        s_hist[d] = run;
+    }
+    __syncthreads();
+}
+
+// spans[j] = start of the top-bit run holding position j * kLocSpan (n past
+// the end), j = 0 .. nspans: one wave per entry. Computed before any span is
+// sorted, since a span longer than the LDS is sorted in place of its input.
+template <typename K>
+__global__ __launch_bounds__(256) void k_rs_spans(const K* __restrict__ keys, uint32_t n, uint32_t pshift,
+                                                  uint32_t nspans, uint32_t* __restrict__ spans) {
+    const uint32_t j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j > nspans) return;
+    const uint64_t nom = (uint64_t)j * kLocSpan;
+    const uint32_t v = nom >= n ? n : (nom == 0 ? 0u : run_start<K>(keys, (uint32_t)nom, pshift, lane));
+    if (lane == 0) spans[j] = v;
+}
+
+// Sorts span [spans[b], spans[b+1]) of workgroup b (all of [0, n) when spans
+// is null) by every digit in which its keys differ.
+template <typename K>
+__global__ __launch_bounds__(kRsThreads) void k_rs_local(const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+                                                         K* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+                                                         uint32_t n, uint32_t kpasses, const uint32_t* __restrict__ spans) {
+    __shared__ K s_k[kLocCap];
+    __shared__ uint32_t s_v[kLocCap];
+    __shared__ uint32_t s_cnt[kRsWaves][kRadixMax];
+    __shared__ uint32_t s_hist[kRadixMax];
+    __shared__ uint32_t s_start[kRadixMax];
+    __shared__ uint32_t s_span[2];
+    __shared__ uint32_t s_bits[4];  // OR low/high, AND low/high of the span's keys
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (threadIdx.x < 2) s_span[threadIdx.x] = spans ? spans[blockIdx.x + threadIdx.x] : (threadIdx.x ? n : 0u);
+    if (threadIdx.x >= 64 && threadIdx.x < 68) s_bits[threadIdx.x - 64] = threadIdx.x < 66 ? 0u : 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t s = s_span[0], m = s_span[1] > s ? s_span[1] - s : 0u;
+    if (m == 0) return;
+    const K* kin = keys_in + s;
+    const uint32_t* vin = vals_in + s;
+    K* kout = keys_out + s;
+    uint32_t* vout = vals_out + s;
+
+    // digits in which the span's keys differ
+    uint64_t any = 0, all = ~0ull;
+    for (uint32_t i = threadIdx.x; i < m && m > (uint32_t)kLocCap; i += kRsThreads) {
+        const uint64_t k = (uint64_t)kin[i];
+        any |= k;
+        all &= k;
+    }
+    K key[kLocRounds];
+    uint32_t val[kLocRounds], dig[kLocRounds], rank[kLocRounds];
+    if (m <= (uint32_t)kLocCap) {
+#pragma unroll
+        for (int r = 0; r < kLocRounds; r++) {
+            const uint32_t i = (uint32_t)(wave * kLocWaveKeys + r * 64 + lane);
+            key[r] = i < m ? kin[i] : (K)0;
+            val[r] = i < m ? vin[i] : 0u;
+            if (i < m) {
+                any |= (uint64_t)key[r];
+                all &= (uint64_t)key[r];
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        any |= __shfl_xor(any, off, 64);
+        all &= __shfl_xor(all, off, 64);
+    }
+    if (lane == 0) {
+        atomicOr(&s_bits[0], (uint32_t)any);
+        atomicOr(&s_bits[1], (uint32_t)(any >> 32));
+        atomicAnd(&s_bits[2], (uint32_t)all);
+        atomicAnd(&s_bits[3], (uint32_t)(all >> 32));
+    }
+    __syncthreads();
+    const uint64_t differ = (((uint64_t)s_bits[1] << 32) | s_bits[0]) ^ (((uint64_t)s_bits[3] << 32) | s_bits[2]);
+
+    if (m <= (uint32_t)kLocCap) {
+        for (uint32_t p = 0; p < kpasses; p++) {
+            const uint32_t shift = p * 8;
+            if (((differ >> shift) & 0xFFu) == 0) continue;  // one digit value: the pass keeps the order
+            local_rank<K>(key, m, shift, dig, rank, s_cnt, s_hist, wave, lane);
+            digit_starts(s_hist, s_start, wave, lane);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kLocRounds; r++) {
+                const uint32_t i = (uint32_t)(wave * kLocWaveKeys + r * 64 + lane);
+                if (i < m) {
+                    const uint32_t pos = s_start[dig[r]] + s_cnt[wave][dig[r]] + rank[r];
+                    if (pos < (uint32_t)kLocCap) {
+                        s_k[pos] = key[r];
+                        s_v[pos] = val[r];
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kLocRounds; r++) {
+                const uint32_t i = (uint32_t)(wave * kLocWaveKeys + r * 64 + lane);
+                if (i < m) {
+                    key[r] = s_k[i];
+                    val[r] = s_v[i];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kLocRounds; r++) {
+            const uint32_t i = (uint32_t)(wave * kLocWaveKeys + r * 64 + lane);
+            if (i < m) {
+                kout[i] = key[r];
+                vout[i] = val[r];
+            }
+        }
+        return;
+    }
+
+    // A span longer than the LDS: chunked passes through global memory,
+    // ping-ponging between the span's input and output ranges.
+    K* ka[2] = {const_cast<K*>(kin), kout};
+    uint32_t* va[2] = {const_cast<uint32_t*>(vin), vout};
+    int src = 0;
+    for (uint32_t p = 0; p < kpasses; p++) {
+        const uint32_t shift = p * 8;
+        if (((differ >> shift) & 0xFFu) == 0) continue;
+        const K* ks = ka[src];
+        const uint32_t* vs = va[src];
+        K* kd = ka[src ^ 1];
+        uint32_t* vd = va[src ^ 1];
+        for (uint32_t d = threadIdx.x; d < kRadixMax; d += kRsThreads) s_hist[d] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < m; i += kRsThreads) atomicAdd(&s_hist[digit_of(ks[i], shift, 0xFFu)], 1u);
+        __syncthreads();
+        digit_starts(s_hist, s_start, wave, lane);
+        __syncthreads();
+        // s_start[d]: where the next chunk's keys of digit d go (span-relative)
+        for (uint32_t c0 = 0; c0 < m; c0 += kLocCap) {
+            const uint32_t cm = m - c0 < (uint32_t)kLocCap ? m - c0 : (uint32_t)kLocCap;
+#pragma unroll
+            for (int r = 0; r < kLocRounds; r++) {
+                const uint32_t i = (uint32_t)(wave * kLocWaveKeys + r * 64 + lane);
+                key[r] = i < cm ? ks[c0 + i] : (K)0;
+                val[r] = i < cm ? vs[c0 + i] : 0u;
+            }
+            local_rank<K>(key, cm, shift, dig, rank, s_cnt, s_hist, wave, lane);
+#pragma unroll
+            for (int r = 0; r < kLocRounds; r++) {
+                const uint32_t i = (uint32_t)(wave * kLocWaveKeys + r * 64 + lane);
+                if (i < cm) {
+                    const uint32_t pos = s_start[dig[r]] + s_cnt[wave][dig[r]] + rank[r];
+                    if (pos < m) {
+                        kd[pos] = key[r];
+                        vd[pos] = val[r];
+                    }
+                }
+            }
+            __syncthreads();
+            for (uint32_t d = threadIdx.x; d < kRadixMax; d += kRsThreads) s_start[d] += s_hist[d];
+            __syncthreads();
+        }
+        src ^= 1;
+    }
+    if (src == 0) {  // an even number of passes ran: the sorted span is in the input range
+        for (uint32_t i = threadIdx.x; i < m; i += kRsThreads) {
+            kout[i] = kin[i];
+            vout[i] = vin[i];
+        }
+    }
+}
+
 template <typename K>
 static int sort_passes(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_t bits, uint32_t seg_passes,
                        SegTable seg, int in, int* out_slot, uint32_t* passes_out) {
@@ -446,17 +714,28 @@ static int sort_passes(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_
     uint32_t* totals = hist + (size_t)radix * num_tiles;
     const uint32_t kpasses = (key_bits + bits - 1) / bits;
     int cur = in;
+    // the two top digits as global passes, then spans of equal top bits sorted locally
+    const bool local = b.os && seg_passes == 0 && bits == 8 && kpasses >= 3;
+    if (local && n <= (uint32_t)kLocCap) {
+        hipLaunchKernelGGL((k_rs_local<K>), dim3(1), dim3(kRsThreads), 0, b.stream, (const K*)b.keys[cur],
+                           b.vals[cur], (K*)b.keys[cur ^ 1], b.vals[cur ^ 1], n, kpasses, (const uint32_t*)nullptr);
+        SQ_CHECK_HIP(hipGetLastError());
+        if (passes_out) *passes_out = kpasses;
+        *out_slot = cur ^ 1;
+        return SQ_OK;
+    }
+    const uint32_t p_first = local ? kpasses - 2 : 0;
     if (b.os) {
-        // single-pass key passes: zero the area, count every pass's digits once
+        // single-pass key passes: zero the area, count the passes' digits once
         const size_t words = os_status() + (size_t)kpasses * num_tiles * radix;
         SQ_TRY(launch_fill_u32_on(b.stream, b.num_cus, b.os, 0u, words));
         uint32_t gb = (n + kRsThreads * 16 - 1) / (kRsThreads * 16);
         if (gb > (uint32_t)b.num_cus * 4) gb = (uint32_t)b.num_cus * 4;
         hipLaunchKernelGGL((k_rs_ghist<K>), dim3(gb), dim3(kRsThreads), 0, b.stream, (const K*)b.keys[cur], n, bits,
-                           kpasses, b.os + os_ghist());
+                           p_first, kpasses, b.os + os_ghist());
         SQ_CHECK_HIP(hipGetLastError());
     }
-    for (uint32_t p = 0; p < kpasses + seg_passes; p++) {
+    for (uint32_t p = p_first; p < kpasses + seg_passes; p++) {
         const bool segp = p >= kpasses;
         if (b.os && !segp) {
             const K* kin = (const K*)b.keys[cur];
@@ -487,6 +766,16 @@ static int sort_passes(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_
         else
             hipLaunchKernelGGL((k_rs_downsweep<K, false>), dim3(num_tiles), dim3(kRsThreads), 0, b.stream, kin, vin,
                                kout, vout, n, shift, bits, hist, totals, num_tiles, seg);
+        SQ_CHECK_HIP(hipGetLastError());
+        cur ^= 1;
+    }
+    if (local) {
+        // span table in the (otherwise unused) three-kernel histogram area
+        const uint32_t grid = (uint32_t)(((uint64_t)n + kLocSpan - 1) / kLocSpan);
+        hipLaunchKernelGGL((k_rs_spans<K>), dim3(grid / 4 + 1), dim3(256), 0, b.stream, (const K*)b.keys[cur], n,
+                           p_first * bits, grid, hist);
+        hipLaunchKernelGGL((k_rs_local<K>), dim3(grid), dim3(kRsThreads), 0, b.stream, (const K*)b.keys[cur],
+                           b.vals[cur], (K*)b.keys[cur ^ 1], b.vals[cur ^ 1], n, kpasses, (const uint32_t*)hist);
         SQ_CHECK_HIP(hipGetLastError());
         cur ^= 1;
     }

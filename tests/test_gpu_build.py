@@ -40,6 +40,55 @@ def test_radix_sort_is_stable(ctx, key_bits, n):
     assert np.array_equal(v, vals[order])
 
 
+def _sort_case(name, n, key_bits, rng):
+    full = (1 << key_bits) - 1
+    if name == "uniform":
+        keys = rng.integers(0, 1 << 62, n, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, n, dtype=np.uint64)
+    elif name == "long_runs":
+        # runs of equal top 16 bits far longer than one workgroup's LDS, low bits random
+        top = rng.integers(0, 5, n, dtype=np.uint64) << np.uint64(key_bits - 16)
+        keys = top | rng.integers(0, 1 << (key_bits - 16), n, dtype=np.uint64)
+    elif name == "equal":
+        keys = np.full(n, 0x123456789 & full, np.uint64)
+    elif name == "low_bit":
+        keys = rng.integers(0, 2, n, dtype=np.uint64)
+    elif name == "two_digits":
+        # only digits 0 and 5 vary (every other pass keeps the order)
+        keys = (rng.integers(0, 256, n, dtype=np.uint64) | (rng.integers(0, 256, n, dtype=np.uint64) << np.uint64(40)))
+    return keys & np.uint64(full)
+
+
+@pytest.mark.parametrize("key_bits", [30, 63])
+@pytest.mark.parametrize("name,n", [("uniform", 1_000_000), ("uniform", 8192), ("uniform", 5000),
+                                    ("long_runs", 300_000), ("long_runs", 20_000), ("equal", 50_000),
+                                    ("low_bit", 100_000), ("two_digits", 200_000)])
+def test_radix_sort_spans(ctx, key_bits, name, n):
+    """Sorts whose two top digits run globally and whose spans of equal top
+    bits are finished per workgroup (in LDS, or in chunks through global
+    memory for runs longer than the LDS), against a stable argsort."""
+    say("radix sort spans", name, n, key_bits)
+    rng = np.random.default_rng(n + key_bits)
+    keys = _sort_case(name, n, key_bits, rng)
+    vals = rng.permutation(n).astype(np.uint32)
+    k, v = ctx.radix_sort(keys, vals, key_bits)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k, keys[order])
+    assert np.array_equal(v, vals[order])
+
+
+def test_radix_sort_morton_codes_of_surfaces(ctx):
+    """63-bit Morton codes of a 1.84M-triangle icosphere (runs of equal top
+    bits of tens of thousands of keys) sort as a stable argsort does."""
+    tris = sq.gen_icosphere(1837637, 6023, 0.1)
+    scene = ctx.upload(tris)
+    codes = ctx.morton_codes(scene, 21)
+    k, v = ctx.radix_sort(codes, np.arange(len(codes), dtype=np.uint32), 63)
+    order = np.argsort(codes, kind="stable")
+    assert np.array_equal(v, order.astype(np.uint32))
+    assert np.array_equal(k, codes[order])
+    scene.free()
+
+
 def test_radix_sort_matches_reference_fixture(ctx):
     g = golden("radix_sort32")
     k, v = ctx.radix_sort(g["keys"].astype(np.uint64), np.arange(len(g["keys"]), dtype=np.uint32), 32)
