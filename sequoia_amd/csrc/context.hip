@@ -335,7 +335,7 @@ static int build_morton(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_
     phase_begin(c, PH_EMIT);
     uint32_t depth = 0;
     uint32_t thr = p && p->subtree_threshold ? p->subtree_threshold : 0;
-    SQ_TRY(lbvh_emit(c, s->d_tris, n, c->keys[slot].p, c->vals[slot].as<uint32_t>(), key64, layout, refit, thr,
+    SQ_TRY(lbvh_emit(c, s->d_tris, n, c->keys[slot].p, c->vals[slot].as<uint32_t>(), key64, 3 * bits, layout, refit, thr,
                      b->d_nodes, &depth));
     if (!refit) phase_end(c, PH_EMIT);  // with refit, the emission ends the phase before the refit kernel
     b->root = 0;

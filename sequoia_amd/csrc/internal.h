@@ -168,8 +168,10 @@ uint32_t sort_segment_passes(uint32_t nseg, uint32_t digit_bits);
 int radix_sort_bufs(const SortBufs& b, uint32_t n, uint32_t key_bits, uint32_t digit_bits, bool key64,
                     const uint32_t* d_seg_off, uint32_t nseg, int in, int* out_slot, uint32_t* passes);
 int radix_sort(sq_ctx* c, uint32_t n, uint32_t key_bits, uint32_t digit_bits, bool key64, int in, int* out_slot);
+// Single-scene emission from sorted codes of code_bits bits.
 int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_codes, const uint32_t* d_idx,
-              bool key64, uint32_t layout, bool refit, uint32_t threshold, sq_node* d_nodes, uint32_t* depth);
+              bool key64, uint32_t code_bits, uint32_t layout, bool refit, uint32_t threshold, sq_node* d_nodes,
+              uint32_t* depth);
 // Batched emission: scenes[0..nscenes) on device; no host synchronisation.
 // `ws` must hold lbvh_batch_workspace(...) bytes.
 size_t lbvh_batch_workspace(uint32_t total_tris, uint32_t nscenes, uint32_t threshold);
@@ -234,6 +236,15 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
 struct Box3 {
     float mn[3], mx[3];
 };
+
+// A workgroup barrier that orders LDS accesses only: unlike __syncthreads it
+// does not wait for the thread's global loads and stores to complete, so
+// loads issued before it stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 __device__ __forceinline__ void load_tri(const sq_triangle* __restrict__ tris, uint32_t i, float v[9]) {
     const float* p = (const float*)(tris + i);

@@ -147,14 +147,69 @@ __device__ __forceinline__ void refit_node(sq_node* nodes, uint32_t self, uint32
     sqd::store_node_box(nodes, self, u);
 }
 
+// Top bits of a sorted code array: table[p] = the first position whose code
+// has top bits (code >> pshift) >= p, for p < 2^(key bits - pshift). Each
+// position that starts a run of equal top bits fills the entries from just
+// past the previous run's top bits up to its own (the last position also
+// fills the entries past it with n), so every entry is written once.
+struct PrefixTable {
+    const uint32_t* table;  // null: no table
+    uint32_t pshift;
+    uint32_t entries;
+};
+
+// A wave fills its lanes' entry ranges together, 64 entries per store, so a
+// long range of empty top bits (the inside of a closed surface) costs no one
+// lane thousands of stores.
+template <typename K>
+__global__ void k_prefix_table(const K* __restrict__ codes, uint32_t n, uint32_t pshift, uint32_t entries,
+                               uint32_t* __restrict__ table) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u), wstride = gridDim.x * blockDim.x;
+    for (uint32_t wb = wave0; wb <= n; wb += wstride) {  // wave-uniform loop over positions 0..n
+        const uint32_t i = wb + lane;
+        uint64_t lo = 1, hi = 0;  // empty
+        if (i <= n) {
+            const uint64_t cur = i < n ? (uint64_t)(codes[i] >> pshift) : (uint64_t)entries;
+            const uint64_t prev = i > 0 ? (uint64_t)(codes[i - 1] >> pshift) : ~0ull;
+            if (cur != prev) {
+                lo = prev + 1;  // ~0 + 1 == 0 for the first position
+                hi = cur < entries ? cur : (uint64_t)entries - 1;
+            }
+        }
+        uint64_t pending = __ballot(lo <= hi);
+        while (pending) {
+            const int l = __ffsll((long long)pending) - 1;
+            pending &= pending - 1;
+            const uint64_t plo = __shfl(lo, l, 64), phi = __shfl(hi, l, 64);
+            const uint32_t v = __shfl(i, l, 64);
+            for (uint64_t p = plo + lane; p <= phi; p += 64) table[p] = v;
+        }
+    }
+}
+
 // find_split_morton as a 65-ary search by one wave: the largest s < count-1
 // whose code shares more leading bits with the first code than the last does.
+// When the highest bit in which the range's first and last codes differ lies
+// within the table's top bits, the split is read from the table instead: it
+// is one before the first position whose code has that bit set above the
+// common prefix.
 template <typename K>
-__device__ uint32_t wave_find_split(const K* __restrict__ codes, uint32_t b, uint32_t e, int lane) {
+__device__ uint32_t wave_find_split(const K* __restrict__ codes, uint32_t b, uint32_t e, int lane,
+                                    const PrefixTable& pt) {
     const uint32_t count = e - b;
     const K first = codes[b], last = codes[e - 1];
     if (first == last) return count >> 1;
     const int cp = clz_k<K>(first ^ last);
+    const int h = (int)(8 * sizeof(K)) - 1 - cp;  // highest differing bit
+    if (pt.table && h >= (int)pt.pshift) {
+        const K x = ((first >> h) | (K)1) << h;
+        const uint64_t p = (uint64_t)(x >> pt.pshift);
+        if (p < pt.entries) {
+            const uint32_t lb = pt.table[p];
+            if (lb > b && lb < e) return lb - 1 - b;
+        }
+    }
     uint32_t lo = 0, hi = count - 1;
     while (hi - lo > 1) {
         uint32_t span = hi - lo;
@@ -211,6 +266,7 @@ struct LbvhArgs {
     uint32_t small_cap;        // entries of `small`
     uint32_t* topm;            // one-scene top phase over many workgroups: per-level range
                                // counts [kMaxLevels + 2], then per-level barrier words
+    PrefixTable pt;            // one-scene builds: split positions of the top bits
 };
 
 __device__ __forceinline__ EmitArgs scene_emit(const LbvhArgs& a, const SceneDesc& d) {
@@ -266,7 +322,7 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
         __syncthreads();
         for (uint32_t t = wave; t < ncur; t += nwaves) {
             Task tk = cur[t];
-            uint32_t split = wave_find_split<K>(codes, tk.begin, tk.end, lane);
+            uint32_t split = wave_find_split<K>(codes, tk.begin, tk.end, lane, a.pt);
             if (lane == 0) {
                 emit_children(e, tk.begin, tk.end, tk.free, split,
                               [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
@@ -375,7 +431,7 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top_grid(LbvhArgs a) {
         Task* nxt = q[(level + 1) & 1];
         for (uint32_t t = gw; t < ncur; t += total_waves) {
             Task tk = cur[t];
-            uint32_t split = wave_find_split<K>(codes, tk.begin, tk.end, lane);
+            uint32_t split = wave_find_split<K>(codes, tk.begin, tk.end, lane, a.pt);
             if (lane == 0) {
                 emit_children(e, tk.begin, tk.end, tk.free, split,
                               [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
@@ -440,6 +496,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
     constexpr size_t kCodeBytes = sizeof(K) * kMaxThreshold, kBoxBytes = sizeof(sqd::Box3) * kSmallSlots;
     __shared__ __align__(16) unsigned char s_area[kCodeBytes > kBoxBytes ? kCodeBytes : kBoxBytes];
     __shared__ uint32_t s_slot[kSmallSlots];
+    __shared__ uint16_t s_leafslot[kMaxThreshold];  // sorted position -> its leaf's slot (range-relative)
     __shared__ SmallTask s_tasks[kMaxThreshold];
     __shared__ uint32_t s_lvl[kMaxLevels + 1];
     __shared__ uint32_t s_total, s_err;
@@ -458,14 +515,28 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
             continue;
         }
         const uint32_t sbase = root.free, nslots = 2 * (count - 1);
-        for (uint32_t i = threadIdx.x; i < count; i += kSmallThreads) s_codes[i] = codes[base + i];
+        // this thread's leaves are positions threadIdx.x + j * kSmallThreads: their
+        // codes and triangle ids are loaded together (branch-free: positions past
+        // the range load the range's last entry)
+        constexpr int kLeafPer = (kMaxThreshold + kSmallThreads - 1) / kSmallThreads;
+        uint32_t lt[kLeafPer];
+        K ck[kLeafPer];
+#pragma unroll
+        for (int j = 0; j < kLeafPer; j++) {
+            const uint32_t i = threadIdx.x + j * kSmallThreads, ic = i < count ? i : count - 1;
+            ck[j] = codes[base + ic];
+            lt[j] = a.idx[base + ic];
+        }
+#pragma unroll
+        for (int j = 0; j < kLeafPer; j++)
+            if (threadIdx.x + j * kSmallThreads < count) s_codes[threadIdx.x + j * kSmallThreads] = ck[j];
         for (uint32_t i = threadIdx.x; i < nslots; i += kSmallThreads) s_slot[i] = 0;
         if (threadIdx.x == 0) {
             s_tasks[0] = SmallTask{count << 16, root.free, root.self};
             s_total = 1;
             s_err = 0;
         }
-        __syncthreads();
+        sqd::lds_barrier();
         // 1. emission: slot tags and split tasks, level by level
         uint32_t lb = 0, le = 1, nlev = 0;
         while (lb < le) {
@@ -492,7 +563,10 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
                     if (cn < 8) {
                         // a leaf (cn == 1) or a fan of cn leaves at cf..
                         if (cf + cn > nslots || (cn > 1 && cs >= nslots)) { s_err = 1; continue; }
-                        for (uint32_t j = 0; j < cn; j++) s_slot[cf + j] = kSlotLeaf | (cb + j);
+                        for (uint32_t j = 0; j < cn; j++) {
+                            s_slot[cf + j] = kSlotLeaf | (cb + j);
+                            s_leafslot[cb + j] = (uint16_t)(cf + j);
+                        }
                         if (cn > 1) s_slot[cs] = kSlotFan | (cn << 16) | cf;
                     } else {
                         const uint32_t k = atomicAdd(&s_total, 1u);
@@ -501,39 +575,43 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
                     }
                 }
             }
-            __syncthreads();
+            sqd::lds_barrier();
             lb = le;
             le = s_total < kMaxThreshold ? s_total : kMaxThreshold;
             nlev++;
             const bool stop = s_err != 0;
-            __syncthreads();
+            sqd::lds_barrier();
             if (stop) break;
         }
         if (s_err) {
             if (threadIdx.x == 0) atomicOr(&a.counters[2], 1u);
-            __syncthreads();
+            sqd::lds_barrier();
             continue;
         }
         if (threadIdx.x == 0) s_lvl[nlev] = le;
         max_lev = nlev > max_lev ? nlev : max_lev;
         if (threadIdx.x == 0) atomicMax(&a.scene_small[root.scene], nlev);
-        // 2. leaves: every triangle fetched at once (the codes are no longer needed)
-        for (uint32_t sl = threadIdx.x; sl < nslots; sl += kSmallThreads) {
-            const uint32_t tag = s_slot[sl];
-            if (!(tag & kSlotLeaf)) continue;
-            const uint32_t gtri = a.idx[base + (tag & 0xFFFFu)];
-            const uint32_t slot = sbase + sl;
+        // 2. leaves: every triangle of the range fetched at once (the codes are no
+        // longer needed)
+        float lv[kLeafPer][9];
+        if (a.refit) {
+#pragma unroll
+            for (int j = 0; j < kLeafPer; j++) sqd::load_tri(a.tris, lt[j], lv[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kLeafPer; j++) {
+            const uint32_t i = threadIdx.x + j * kSmallThreads;
+            if (i >= count) break;
+            const uint32_t sl = s_leafslot[i], slot = sbase + sl;
             if (a.refit) {
-                float v[9];
-                sqd::load_tri(a.tris, gtri, v);
-                const sqd::Box3 b = sqd::tri_box(v);
+                const sqd::Box3 b = sqd::tri_box(lv[j]);
                 s_box[sl] = b;
-                if (slot < e.cap) sqd::store_node(e.nodes, slot, b, gtri - e.tri_base, 1, 1);
+                if (slot < e.cap) sqd::store_node(e.nodes, slot, b, lt[j] - e.tri_base, 1, 1);
             } else if (slot < e.cap) {
-                sqd::store_node_links(e.nodes, slot, gtri - e.tri_base, 1, 1);
+                sqd::store_node_links(e.nodes, slot, lt[j] - e.tri_base, 1, 1);
             }
         }
-        __syncthreads();
+        sqd::lds_barrier();
         // 3. fans, then split nodes deepest first: unions of child slots in slot order
         for (uint32_t sl = threadIdx.x; sl < nslots; sl += kSmallThreads) {
             const uint32_t tag = s_slot[sl];
@@ -549,7 +627,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
                 sqd::store_node_links(e.nodes, slot, sbase + c0, cn, 0);
             }
         }
-        __syncthreads();
+        sqd::lds_barrier();
         for (int lv = (int)nlev - 1; lv >= 0; lv--) {
             for (uint32_t i = s_lvl[lv] + threadIdx.x; i < s_lvl[lv + 1]; i += kSmallThreads) {
                 const SmallTask tk = s_tasks[i];
@@ -564,7 +642,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
                     sqd::store_node_links(e.nodes, tk.self, tk.free, 2, 0);
                 }
             }
-            __syncthreads();
+            sqd::lds_barrier();
         }
     }
     if (threadIdx.x == 0 && max_lev) atomicMax(&a.counters[1], max_lev);
@@ -591,8 +669,11 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_finish(LbvhArgs a) {
     }
 }
 
-__global__ void k_zero_counters(uint32_t* c, uint32_t n) {
+// Zeroes the emission counters; `desc` non-null: also writes the one scene
+// descriptor of a plain build (no host-to-device copy for it).
+__global__ void k_zero_counters(uint32_t* c, uint32_t n, SceneDesc* desc, SceneDesc value) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c[i] = 0;
+    if (desc && blockIdx.x == 0 && threadIdx.x == 0) *desc = value;
 }
 
 static uint32_t clamp_threshold(uint32_t threshold) {
@@ -663,7 +744,7 @@ size_t lbvh_batch_workspace(uint32_t total, uint32_t nscenes, uint32_t threshold
 static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total, const void* d_codes,
                        const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
                        bool refit, uint32_t T, sq_node* d_nodes, void* ws, uint32_t* d_depth, bool split_refit_phase,
-                       bool top_grid) {
+                       bool top_grid, const SceneDesc* write_desc = nullptr, PrefixTable pt = PrefixTable{}) {
     WsLayout w = ws_layout(total, nscenes, T);
     char* base = (char*)ws;
     LbvhArgs a;
@@ -686,10 +767,12 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     a.counters = (uint32_t*)(base + w.counters);
     a.small_cap = (uint32_t)w.small_cap;
     a.topm = (uint32_t*)(base + w.topm);
+    a.pt = pt;
     const uint32_t zero_words = (uint32_t)((w.topm - w.counters) / 4) + 2 * (kMaxLevels + 2);
     uint32_t zb = (zero_words + 255) / 256;
     zb = zb > 64 ? 64 : zb;
-    hipLaunchKernelGGL(k_zero_counters, dim3(zb), dim3(256), 0, stream, a.counters, zero_words);
+    hipLaunchKernelGGL(k_zero_counters, dim3(zb), dim3(256), 0, stream, a.counters, zero_words,
+                       write_desc ? const_cast<SceneDesc*>(d_scenes) : nullptr, write_desc ? *write_desc : SceneDesc{});
     bool grid_done = false;
     if (nscenes == 1 && top_grid) {
         // One scene: its top levels over a few workgroups (enough waves for a
@@ -730,32 +813,49 @@ int lbvh_emit_batch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint
 }
 
 int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_codes, const uint32_t* d_idx, bool key64,
-              uint32_t layout, bool refit, uint32_t threshold, sq_node* d_nodes, uint32_t* depth) {
+              uint32_t code_bits, uint32_t layout, bool refit, uint32_t threshold, sq_node* d_nodes, uint32_t* depth) {
     const uint32_t T = clamp_threshold(threshold);
     WsLayout w = ws_layout(n, 1, T);
-    SQ_TRY(c->aux[1].ensure(w.total + 256));
+    // top-bit split table: 15 top bits of the code (all of them for short codes)
+    const uint32_t key_bits = code_bits;
+    PrefixTable pt{};
+    pt.pshift = key_bits > 15 ? key_bits - 15 : 0;
+    pt.entries = 1u << (key_bits - pt.pshift);
+    SQ_TRY(c->aux[1].ensure(w.total + 256 + (size_t)pt.entries * 4));
     SQ_TRY(c->aux[2].ensure(sizeof(SceneDesc)));
     SQ_TRY(pinned_ensure(c, 256));
-    SceneDesc* hd = (SceneDesc*)((char*)c->pinned + 128);
-    lbvh_fill_descs(hd, &n, 1, T);
-    SQ_CHECK_HIP(hipMemcpyAsync(c->aux[2].p, hd, sizeof(SceneDesc), hipMemcpyHostToDevice, c->stream));
+    SceneDesc hd;
+    lbvh_fill_descs(&hd, &n, 1, T);
     char* base = c->aux[1].as<char>();
-    uint32_t* d_depth = (uint32_t*)(base + w.total);
+    uint32_t* d_table = (uint32_t*)(base + w.total + 256);
+    pt.table = d_table;
+    {
+        uint32_t tb = (n + 1 + 255) / 256;
+        const uint32_t cap = (uint32_t)c->num_cus * 8;
+        if (tb > cap) tb = cap;
+        if (key64)
+            hipLaunchKernelGGL(k_prefix_table<uint64_t>, dim3(tb), dim3(256), 0, c->stream, (const uint64_t*)d_codes,
+                               n, pt.pshift, pt.entries, d_table);
+        else
+            hipLaunchKernelGGL(k_prefix_table<uint32_t>, dim3(tb), dim3(256), 0, c->stream, (const uint32_t*)d_codes,
+                               n, pt.pshift, pt.entries, d_table);
+    }
+    // the depth goes to a spare counter word, so that one copy brings back the
+    // counters (words 0-2), the depth (word 8) and the scene's top levels (word 16)
+    uint32_t* d_depth = (uint32_t*)(base + w.counters) + 8;
     SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit, T,
-                       d_nodes, c->aux[1].p, d_depth, true, true));
+                       d_nodes, c->aux[1].p, d_depth, true, true, &hd, pt));
     uint32_t* h = (uint32_t*)c->pinned;
-    SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 12, hipMemcpyDeviceToHost, c->stream));
-    SQ_CHECK_HIP(hipMemcpyAsync(h + 4, base + w.scene_levels, 4, hipMemcpyDeviceToHost, c->stream));
-    SQ_CHECK_HIP(hipMemcpyAsync(h + 5, d_depth, 4, hipMemcpyDeviceToHost, c->stream));
+    SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 68, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
-    const uint32_t nsmall = h[0], top_levels = h[4];
+    const uint32_t nsmall = h[0], top_levels = h[16];
     if (h[2] & 4u) SQ_RET(SQ_ERR_HIP, "lbvh: a top-phase barrier wait exceeded its spin limit");
     if (h[2]) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: emission workspace overflow");
     if (top_levels >= (uint32_t)kMaxLevels) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: more than %d top levels", kMaxLevels);
     c->last.top_levels = top_levels;
     c->last.small_tasks = nsmall;
     // levels of split ranges + fan/leaf level + root
-    *depth = h[5];
+    *depth = h[8];
     return SQ_OK;
 }
 

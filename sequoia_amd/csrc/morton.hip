@@ -19,20 +19,38 @@ constexpr int kBoundsBlock = 256;
 // block that takes the last ticket (ord6[8], zeroed by k_bounds_init) folds
 // all partials into ord6[0..5]. Atomics on the six result words from every
 // block serialise at L2 and cost about 50 us whatever the triangle count.
+// VEC: the triangle array is 16-byte aligned and read as four triangles per
+// thread and step (nine 16-byte loads); otherwise one triangle at a time.
+template <bool VEC>
 __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle* __restrict__ tris, uint32_t n,
                                                                 uint32_t* __restrict__ ord6) {
     uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
     uint32_t hi[3] = {0u, 0u, 0u};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float v[9];
-        sqd::load_tri(tris, i, v);
-        sqd::Box3 b = sqd::tri_box(v);
+    auto take = [&](const float* v) {
+        const sqd::Box3 b = sqd::tri_box(v);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            uint32_t a = sqd::f2ord(b.mn[k]), c = sqd::f2ord(b.mx[k]);
+            const uint32_t a = sqd::f2ord(b.mn[k]), c = sqd::f2ord(b.mx[k]);
             lo[k] = a < lo[k] ? a : lo[k];
             hi[k] = c > hi[k] ? c : hi[k];
         }
+    };
+    const uint32_t quads = VEC ? n / 4 : 0;
+    const float4* q4 = (const float4*)tris;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
+        float v[36];
+#pragma unroll
+        for (int j = 0; j < 9; j++) {
+            const float4 f = q4[(size_t)q * 9 + j];
+            v[4 * j] = f.x; v[4 * j + 1] = f.y; v[4 * j + 2] = f.z; v[4 * j + 3] = f.w;
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) take(v + 9 * t);
+    }
+    for (uint32_t i = 4 * quads + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float v[9];
+        sqd::load_tri(tris, i, v);
+        take(v);
     }
     __shared__ uint32_t s[6][kBoundsBlock / 64];
     __shared__ uint32_t s_last;
@@ -93,10 +111,9 @@ __global__ void k_bounds_init(uint32_t* ord6) {
 }
 
 // Two workgroups per CU keep enough loads in flight to stream the triangles
-// (a cold 1M-triangle pass takes the same ~19 us at 512 or 2048 blocks) and
-// leave the last block few partials to fold.
+// and leave the last block few partials to fold.
 static uint32_t bounds_blocks(const sq_ctx* c, uint32_t n) {
-    uint32_t blocks = (n + kBoundsBlock - 1) / kBoundsBlock;
+    uint32_t blocks = (n / 4 + 1 + kBoundsBlock - 1) / kBoundsBlock;
     const uint32_t cap = (uint32_t)c->num_cus * 2;
     if (blocks > cap) blocks = cap;
     return blocks ? blocks : 1;
@@ -106,7 +123,12 @@ size_t scene_bounds_bytes(const sq_ctx* c, uint32_t n) { return (16 + 6 * (size_
 
 int scene_bounds(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t* d_ord6) {
     hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, c->stream, d_ord6);
-    hipLaunchKernelGGL(k_scene_bounds, dim3(bounds_blocks(c, n)), dim3(kBoundsBlock), 0, c->stream, tris, n, d_ord6);
+    if (((uintptr_t)tris & 15u) == 0)
+        hipLaunchKernelGGL(k_scene_bounds<true>, dim3(bounds_blocks(c, n)), dim3(kBoundsBlock), 0, c->stream, tris, n,
+                           d_ord6);
+    else
+        hipLaunchKernelGGL(k_scene_bounds<false>, dim3(bounds_blocks(c, n)), dim3(kBoundsBlock), 0, c->stream, tris,
+                           n, d_ord6);
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
 }
