@@ -113,7 +113,7 @@ class Timings(C.Structure):
                 ("refit_ms", C.c_float), ("sah_ms", C.c_float), ("grid_ms", C.c_float),
                 ("total_build_ms", C.c_float), ("trace_ms", C.c_float), ("node_tests", C.c_uint64),
                 ("tri_tests", C.c_uint64), ("sort_passes", C.c_uint32), ("tree_depth", C.c_uint32),
-                ("top_levels", C.c_uint32), ("small_tasks", C.c_uint32)]
+                ("top_levels", C.c_uint32), ("small_tasks", C.c_uint32), ("top_retries", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -267,6 +267,7 @@ struct LbvhArgs {
     uint32_t* topm;            // one-scene top phase over many workgroups: per-level range
                                // counts [kMaxLevels + 2], then per-level barrier words
     PrefixTable pt;            // one-scene builds: split positions of the top bits
+    uint32_t spin_limit;       // k_lbvh_top_grid: spins before a barrier wait gives up
 };
 
 __device__ __forceinline__ EmitArgs scene_emit(const LbvhArgs& a, const SceneDesc& d) {
@@ -361,12 +362,16 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
 // the next level. The ranges, node slots and writes are those of
 // k_lbvh_top; only the order in which ranges of a level are split differs
 // (node slots do not depend on it). A barrier wait is bounded: past
-// kBarrierSpins (seconds: long enough to wait out other streams' kernels
-// that hold the CUs) it raises the error flag (counters[2] bit 4) and goes
-// on, so the launch always ends and the host reports the build as failed.
+// LbvhArgs::spin_limit spins (by default seconds: long enough to wait out
+// other streams' kernels that hold the CUs) the workgroup raises the error
+// flag (counters[2] bit 4) and stops, so it never reads a level whose tasks
+// may not all be written, and the launch always ends; the host then runs the
+// top phase again with k_lbvh_top, which needs no barrier.
 constexpr uint32_t kBarrierSpins = 1u << 26;
 
-__device__ __forceinline__ void grid_barrier(uint32_t* word, uint32_t* err) {
+// false: the wait gave up (the same answer for every thread of the workgroup)
+__device__ __forceinline__ bool grid_barrier(uint32_t* word, uint32_t* err, uint32_t spin_limit) {
+    __shared__ uint32_t s_ok;
     // every wave waits for its own stores to be acknowledged (L1 writes
     // through to this XCD's L2) before the workgroup meets
     __builtin_amdgcn_s_waitcnt(0);
@@ -377,17 +382,20 @@ __device__ __forceinline__ void grid_barrier(uint32_t* word, uint32_t* err) {
         // would repeat that a thousand times.
         __threadfence();
         atomicAdd(word, 1u);
-        uint32_t spins = 0;
+        uint32_t spins = 0, ok = 1;
         while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x) {
-            if (++spins > kBarrierSpins) {
+            if (++spins > spin_limit) {
                 atomicOr(err, 4u);
+                ok = 0;
                 break;
             }
             __builtin_amdgcn_s_sleep(2);
         }
         __threadfence();  // acquire: invalidates this CU's L1 and this XCD's L2
+        s_ok = ok;
     }
     __syncthreads();
+    return s_ok != 0;
 }
 
 template <typename K>
@@ -421,7 +429,7 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top_grid(LbvhArgs a) {
             else push_small(a, t);
         }
     }
-    grid_barrier(&arrive[0], &a.counters[2]);
+    if (!grid_barrier(&arrive[0], &a.counters[2], a.spin_limit)) return;
     uint32_t level = 0, off = 0;
     while (level < (uint32_t)kMaxLevels) {
         const uint32_t ncur = __hip_atomic_load(&cnt[level], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -449,7 +457,7 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top_grid(LbvhArgs a) {
             }
         }
         off += ncur;
-        grid_barrier(&arrive[level + 1], &a.counters[2]);
+        if (!grid_barrier(&arrive[level + 1], &a.counters[2], a.spin_limit)) return;
         level++;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -768,6 +776,7 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     a.small_cap = (uint32_t)w.small_cap;
     a.topm = (uint32_t*)(base + w.topm);
     a.pt = pt;
+    a.spin_limit = c->barrier_spins ? c->barrier_spins : kBarrierSpins;
     const uint32_t zero_words = (uint32_t)((w.topm - w.counters) / 4) + 2 * (kMaxLevels + 2);
     uint32_t zb = (zero_words + 255) / 256;
     zb = zb > 64 ? 64 : zb;
@@ -848,6 +857,18 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
     uint32_t* h = (uint32_t*)c->pinned;
     SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 68, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+    c->last.top_retries = 0;
+    if (h[2] == 4u) {
+        // a barrier wait of the many-workgroup top phase gave up (other work
+        // held CUs the grid needed): clear the nodes and run the emission again
+        // with the one-workgroup top phase, which waits for nothing
+        SQ_TRY(launch_fill_u32(c, d_nodes, 0u, (size_t)2 * n * 8));
+        SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit,
+                           T, d_nodes, c->aux[1].p, d_depth, false, false, &hd, pt));
+        SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 68, hipMemcpyDeviceToHost, c->stream));
+        SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+        c->last.top_retries = 1;
+    }
     const uint32_t nsmall = h[0], top_levels = h[16];
     if (h[2] & 4u) SQ_RET(SQ_ERR_HIP, "lbvh: a top-phase barrier wait exceeded its spin limit");
     if (h[2]) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: emission workspace overflow");

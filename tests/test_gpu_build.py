@@ -199,3 +199,34 @@ def test_phase_timing_opt_in(ctx):
         bvh.free()
     finally:
         ctx.set_phase_timing(False)
+
+
+def test_top_phase_falls_back_when_a_barrier_gives_up():
+    """With the grid barrier's spin limit cut to one spin (the test hook read
+    at context creation) the many-workgroup top phase gives up waiting; the
+    build runs the emission again with the one-workgroup top phase and the
+    tree still equals the oracle's."""
+    import os
+    tris = sq.gen_soup(1_000_000, 2, 0.01)
+    os.environ["SEQUOIA_TEST_BARRIER_SPINS"] = "1"
+    try:
+        c = sq.Context(0)
+    finally:
+        del os.environ["SEQUOIA_TEST_BARRIER_SPINS"]
+    try:
+        s = c.upload(tris)
+        b = c.build(s, "morton63")
+        t = c.timings()
+        say("top retries", t["top_retries"], "depth", t["tree_depth"])
+        assert t["top_retries"] == 1
+        got = b.download()
+        want, root, _, _ = ob.build_morton(tris, 21)
+        assert b.root == root and len(nodes_equal(got, want)) == 0
+        b.free()
+        b = c.build(s, "morton")  # 30-bit codes take the same path
+        want, _, _, _ = ob.build_morton(tris, 10)
+        assert len(nodes_equal(b.download(), want)) == 0
+        b.free()
+        s.free()
+    finally:
+        c.close()
