@@ -368,7 +368,9 @@ typedef struct {
     uint32_t max_wave_tris;     /* 0 = 8 Mi */
     uint32_t semantics;         /* sq_semantics of every trace */
     uint32_t inflight_waves;    /* wave slots on the device at once, 1 or 2; 0 = 2 */
-    uint32_t reserved[3];
+    uint32_t reserved[3];       /* [0]: MiB of pinned input and output staging per slot, allocated at
+                                   creation (0 = 256); larger waves grow it. [1]: GiB of device pool
+                                   reserved at creation (0 = 16, at most the budget); it grows as needed */
 } sq_server_config;
 
 /* A request. Host buffers (tris, rays, out) must stay valid until it completes. */
@@ -379,10 +381,15 @@ typedef struct {
     uint32_t nrays;            /* > 0: trace the ray list `rays`; 0: trace `camera` */
     const sq_ray* rays;
     sq_camera camera;
-    sq_trace_out out;          /* host destinations, any may be NULL (counts: per pixel / per listed ray) */
+    sq_trace_out out;          /* destinations, any may be NULL (counts: per pixel / per listed ray) */
     float timeout_ms;          /* > 0: fail with SQ_ERR_TIMEOUT if not admitted within this time */
-    uint32_t reserved;
+    uint32_t flags;            /* SQ_REQ_DEVICE_OUT: `out` holds device pointers */
 } sq_request;
+
+/* The request's outputs are device memory of the server's device: the trace
+ * writes them in place (no staging, no device-to-host copy); they hold the
+ * results once the request is DONE. Requests in flight must not share them. */
+#define SQ_REQ_DEVICE_OUT 1u
 
 #define SQ_REQ_QUEUED 0
 #define SQ_REQ_RUNNING 1

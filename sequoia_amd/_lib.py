@@ -88,7 +88,7 @@ class ServerConfig(C.Structure):
 class Request(C.Structure):
     _fields_ = [("key", C.c_uint64), ("tris", C.c_void_p), ("ntris", C.c_uint32), ("nrays", C.c_uint32),
                 ("rays", C.c_void_p), ("camera", Camera), ("out", TraceOut), ("timeout_ms", C.c_float),
-                ("reserved", C.c_uint32)]
+                ("flags", C.c_uint32)]
 
 
 class RequestStatus(C.Structure):
@@ -410,6 +410,7 @@ class Bvh2:
 
 
 OUT_HIT_ID, OUT_HIT_T, OUT_COUNTS, OUT_RGB = 1, 2, 4, 8
+SQ_REQ_DEVICE_OUT = 1
 OUT_ALL = 15
 
 
@@ -535,9 +536,13 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, key, tris=None, camera=None, rays=None, hits=True, counts=True, rgb=True, timeout_ms=0.0):
+    def submit(self, key, tris=None, camera=None, rays=None, hits=True, counts=True, rgb=True, timeout_ms=0.0,
+               device_out=None):
         """Build `tris` under `key` (or trace the resident scene `key` when
-        tris is None) and trace `camera` or the ray list `rays`."""
+        tris is None) and trace `camera` or the ray list `rays`. device_out:
+        a dict of device pointers (hit_id, hit_t, counts, rgb; missing or None
+        = not written) that the trace writes in place; wait() then returns no
+        host arrays."""
         keep = []
         r = Request()
         r.key = key
@@ -554,11 +559,19 @@ class Server:
             r.camera = camera
             npix = camera.width * camera.height
             nray = npix * camera.spp
-        out = {"hit_id": np.empty(nray, np.uint32) if hits else None,
-               "hit_t": np.empty(nray, np.float32) if hits else None,
-               "counts": np.empty(nray if rays is not None else npix, np.uint32) if counts else None,
-               "rgb": np.empty((camera.height, camera.width, 3), np.uint8) if (rgb and rays is None) else None}
-        r.out = TraceOut(_ptr(out["hit_id"]), _ptr(out["hit_t"]), _ptr(out["counts"]), _ptr(out["rgb"]))
+        if device_out is not None:
+            def dp(k):
+                v = device_out.get(k)
+                return v.value if isinstance(v, C.c_void_p) else v
+            out = {"hit_id": None, "hit_t": None, "counts": None, "rgb": None}
+            r.out = TraceOut(dp("hit_id"), dp("hit_t"), dp("counts"), dp("rgb") if rays is None else None)
+            r.flags = SQ_REQ_DEVICE_OUT
+        else:
+            out = {"hit_id": np.empty(nray, np.uint32) if hits else None,
+                   "hit_t": np.empty(nray, np.float32) if hits else None,
+                   "counts": np.empty(nray if rays is not None else npix, np.uint32) if counts else None,
+                   "rgb": np.empty((camera.height, camera.width, 3), np.uint8) if (rgb and rays is None) else None}
+            r.out = TraceOut(_ptr(out["hit_id"]), _ptr(out["hit_t"]), _ptr(out["counts"]), _ptr(out["rgb"]))
         r.timeout_ms = timeout_ms
         rid = C.c_uint64()
         _check(lib().sq_server_submit(self.h, C.byref(r), C.byref(rid)))

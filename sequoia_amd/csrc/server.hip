@@ -75,7 +75,8 @@ uint64_t work_bytes(const sq_request& r, uint64_t rays, uint64_t pix) {
     uint64_t b = 8192;
     if (r.tris) b += 168ull * r.ntris + 65536;   // tris, keys, vals, temporary nodes, sort and emit workspace
     if (r.nrays) b += 32ull * r.nrays;           // ray list
-    b += 8ull * rays + 4ull * (r.nrays ? rays : pix) + (r.nrays ? 0 : 3ull * pix) + 1024;
+    if (!(r.flags & SQ_REQ_DEVICE_OUT))          // outputs staged in the wave's block
+        b += 8ull * rays + 4ull * (r.nrays ? rays : pix) + (r.nrays ? 0 : 3ull * pix) + 1024;
     return b;
 }
 
@@ -176,15 +177,53 @@ int status_of(Verdict v) {
     }
 }
 
-int pinned_grow(char** p, size_t* have, size_t need) {
+constexpr size_t kDefaultStage = (size_t)256 << 20;  // pinned input and output staging per slot
+
+// exact: allocate `need` bytes (set-up); otherwise leave slack for later waves
+int pinned_grow(char** p, size_t* have, size_t need, bool exact = false) {
     if (*have >= need) return SQ_OK;
     if (*p) (void)hipHostFree(*p);
     *p = nullptr;
     *have = 0;
     // generous slack: a pinned reallocation stalls the loop
-    size_t sz = need * 2 > ((size_t)64 << 20) ? need * 2 : ((size_t)64 << 20);
+    size_t sz = exact ? need : (need * 2 > ((size_t)64 << 20) ? need * 2 : ((size_t)64 << 20));
     SQ_CHECK_HIP(hipHostMalloc((void**)p, sz, hipHostMallocDefault));
     *have = sz;
+    return SQ_OK;
+}
+
+void complete_wave(sq_server* s, Slot& sl, std::unique_lock<std::mutex>& lk);
+
+// Called between pieces of a long input copy (lock released): retires the
+// other slots' finished waves, so that a large upload does not hold back the
+// completion of the wave already on the device.
+void retire_others(sq_server* s, const Slot& self, std::unique_lock<std::mutex>& lk) {
+    lk.lock();
+    bool any = false;
+    for (Slot& o : s->slots) {
+        if (&o == &self || !o.busy) continue;
+        if (hipEventQuery(o.ev_done) == hipSuccess) {
+            complete_wave(s, o, lk);
+            any = true;
+        }
+    }
+    lk.unlock();
+    if (any) s->cv_done.notify_all();
+}
+
+// Stages an input in pieces of kStagePiece bytes: each piece is copied into
+// pinned memory and its host-to-device copy enqueued at once, so the DMA of
+// one piece overlaps the host copy of the next; finished waves are retired
+// between pieces.
+constexpr size_t kStagePiece = 16u << 20;
+int stage_copy(sq_server* s, const Slot& self, std::unique_lock<std::mutex>& lk, char* pinned, const void* src,
+               size_t bytes, char* dev) {
+    for (size_t off = 0; off < bytes; off += kStagePiece) {
+        const size_t n = bytes - off < kStagePiece ? bytes - off : kStagePiece;
+        par_memcpy(pinned + off, (const char*)src + off, n);
+        SQ_CHECK_HIP(hipMemcpyAsync(dev + off, pinned + off, n, hipMemcpyHostToDevice, self.stream));
+        if (off + n < bytes) retire_others(s, self, lk);
+    }
     return SQ_OK;
 }
 
@@ -241,11 +280,11 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     const size_t o_fail = take_out(4 * rq.size());
     const size_t o_dep = take_out(4ull * (S ? S : 1));  // per new scene: tree depth, 0 = build failed
     for (Req* q : rq) {
-        const bool cam = q->r.nrays == 0;
-        q->o_hit_id = q->r.out.hit_id ? take_out(4 * q->rays) : SIZE_MAX;
-        q->o_hit_t = q->r.out.hit_t ? take_out(4 * q->rays) : SIZE_MAX;
-        q->o_counts = q->r.out.counts ? take_out(4 * (cam ? q->pix : q->rays)) : SIZE_MAX;
-        q->o_rgb = (cam && q->r.out.rgb) ? take_out(3 * q->pix) : SIZE_MAX;
+        const bool cam = q->r.nrays == 0, dev_out = (q->r.flags & SQ_REQ_DEVICE_OUT) != 0;
+        q->o_hit_id = (q->r.out.hit_id && !dev_out) ? take_out(4 * q->rays) : SIZE_MAX;
+        q->o_hit_t = (q->r.out.hit_t && !dev_out) ? take_out(4 * q->rays) : SIZE_MAX;
+        q->o_counts = (q->r.out.counts && !dev_out) ? take_out(4 * (cam ? q->pix : q->rays)) : SIZE_MAX;
+        q->o_rgb = (cam && q->r.out.rgb && !dev_out) ? take_out(3 * q->pix) : SIZE_MAX;
     }
     const size_t out_bytes = oo;
     const size_t total = out_off + out_bytes;
@@ -293,13 +332,24 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     // lock is released (only the scheduler touches admitted requests).
     const double t_in = now_ms(s->t0);
     lk.unlock();
-    for (size_t k = 0; k < rq.size(); k++) {
+    double retire_ms = 0;  // time spent retiring other waves between pieces
+    int stage_rc = SQ_OK;
+    for (size_t k = 0; k < rq.size() && stage_rc == SQ_OK; k++) {
         const Req* q = rq[k];
-        if (q->r.tris) par_memcpy(h + o_tris + 36ull * offs[q->scene_idx], q->r.tris, 36ull * q->r.ntris);
-        if (q->r.nrays) par_memcpy(h + o_rays + 32ull * list_off[k], q->r.rays, 32ull * q->r.nrays);
+        const double r0 = s->stage_out_ms;
+        if (q->r.tris) {
+            const size_t o = o_tris + 36ull * offs[q->scene_idx];
+            stage_rc = stage_copy(s, sl, lk, h + o, q->r.tris, 36ull * q->r.ntris, dev + o);
+        }
+        if (q->r.nrays && stage_rc == SQ_OK) {
+            const size_t o = o_rays + 32ull * list_off[k];
+            stage_rc = stage_copy(s, sl, lk, h + o, q->r.rays, 32ull * q->r.nrays, dev + o);
+        }
+        retire_ms += s->stage_out_ms - r0;
     }
     lk.lock();
-    s->stage_in_ms += now_ms(s->t0) - t_in;
+    SQ_TRY(stage_rc);
+    s->stage_in_ms += now_ms(s->t0) - t_in - retire_ms;
     std::memcpy(h + o_off, offs.data(), 4ull * (S + 1));
     std::vector<SceneDesc> desc(S);
     if (S) lbvh_fill_descs(desc.data(), counts.data(), S, 0);
@@ -315,6 +365,14 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     std::vector<uint32_t> jt(rq.size() + 1, 0);
     uint64_t tiles = 0;
     char* outd = dev + out_off;
+    // ray lists take tiles of 64 k rays, k = min(4, ceil(wave rays / lanes)),
+    // as a single list's launch does: a wave too short to give every wave of
+    // the launch a 256-ray tile cuts its lists finer
+    uint64_t wave_rays = 0;
+    for (Req* q : rq) wave_rays += q->rays;
+    const uint64_t lanes = (uint64_t)s->plan.grid * s->plan.block;
+    const uint64_t kt = lanes ? (wave_rays + lanes - 1) / lanes : 4;
+    const uint32_t list_tile = 64u * (uint32_t)(kt < 1 ? 1 : (kt > 4 ? 4 : kt));
     for (size_t k = 0; k < rq.size(); k++) {
         Req* q = rq[k];
         const Resident& R = s->resident[q->r.key];
@@ -329,7 +387,8 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
             a.camera = 0;
             a.rays = (const sq_ray*)(dev + o_rays + 32ull * list_off[k]);
             a.nrays = q->r.nrays;
-            a.num_tiles = (q->r.nrays + 255) / 256;
+            a.ray_tile = list_tile;
+            a.num_tiles = (q->r.nrays + list_tile - 1) / list_tile;
         } else {
             const sq_camera& c = q->r.camera;
             a.camera = 1;
@@ -341,10 +400,17 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
             // alone would; waves of several jobs take 8x8 tiles, as batches do
             camera_tiling(c.width, c.height, c.spp, rq.size() == 1 && !s->plan.closest, a);
         }
-        a.hit_id = q->o_hit_id != SIZE_MAX ? (uint32_t*)(outd + q->o_hit_id) : nullptr;
-        a.hit_t = q->o_hit_t != SIZE_MAX ? (float*)(outd + q->o_hit_t) : nullptr;
-        a.counts = q->o_counts != SIZE_MAX ? (uint32_t*)(outd + q->o_counts) : nullptr;
-        a.rgb = q->o_rgb != SIZE_MAX ? (uint8_t*)(outd + q->o_rgb) : nullptr;
+        if (q->r.flags & SQ_REQ_DEVICE_OUT) {
+            a.hit_id = q->r.out.hit_id;
+            a.hit_t = q->r.out.hit_t;
+            a.counts = q->r.out.counts;
+            a.rgb = q->r.nrays ? nullptr : q->r.out.rgb;
+        } else {
+            a.hit_id = q->o_hit_id != SIZE_MAX ? (uint32_t*)(outd + q->o_hit_id) : nullptr;
+            a.hit_t = q->o_hit_t != SIZE_MAX ? (float*)(outd + q->o_hit_t) : nullptr;
+            a.counts = q->o_counts != SIZE_MAX ? (uint32_t*)(outd + q->o_counts) : nullptr;
+            a.rgb = q->o_rgb != SIZE_MAX ? (uint8_t*)(outd + q->o_rgb) : nullptr;
+        }
         jt[k] = (uint32_t)tiles;
         tiles += a.num_tiles;
     }
@@ -357,7 +423,8 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     // by this wave (the policy defers the rest), so the wave depends only on
     // earlier work of its own stream.
     hipStream_t st = sl.stream;
-    SQ_CHECK_HIP(hipMemcpyAsync(dev, h, in_bytes, hipMemcpyHostToDevice, st));
+    // the triangles and ray lists went up piece by piece; the tables now
+    SQ_CHECK_HIP(hipMemcpyAsync(dev + o_off, h + o_off, in_bytes - o_off, hipMemcpyHostToDevice, st));
     if (S) {
         BatchBuild j;
         j.tris = (const sq_triangle*)(dev + o_tris);
@@ -454,7 +521,7 @@ void complete_wave(sq_server* s, Slot& sl, std::unique_lock<std::mutex>& lk) {
     for (size_t k = 0; k < rq.size(); k++) {
         const Req& q = *rq[k];
         const bool cam = q.r.nrays == 0;
-        if (status[k] != SQ_OK) continue;
+        if (status[k] != SQ_OK || (q.r.flags & SQ_REQ_DEVICE_OUT)) continue;
         if (q.r.out.hit_id) par_memcpy(q.r.out.hit_id, sl.pin_out + q.o_hit_id, 4 * q.rays);
         if (q.r.out.hit_t) par_memcpy(q.r.out.hit_t, sl.pin_out + q.o_hit_t, 4 * q.rays);
         if (q.r.out.counts) par_memcpy(q.r.out.counts, sl.pin_out + q.o_counts, 4 * (cam ? q.pix : q.rays));
@@ -580,10 +647,19 @@ int sq_server_create(int device, const sq_server_config* cfg, sq_server** out) {
         *base = (uint64_t)p;
         return true;
     });
+    // device pool reserved up front (growing it while serving stalls the
+    // wave that needs the room)
+    const uint64_t pool0 = (c.reserved[1] ? (uint64_t)c.reserved[1] : 16ull) << 30;
+    if (!s->arena->reserve(pool0 < c.budget_bytes ? pool0 : c.budget_bytes)) {
+        sq_server_destroy(s.release());
+        SQ_RET(SQ_ERR_NOMEM, "server: could not reserve the initial device pool");
+    }
     WaveLimits lim;
     lim.max_requests = c.max_wave_requests;
     lim.max_rays = c.max_wave_rays;
     lim.max_tris = c.max_wave_tris;
+    lim.cost_ratio = 2;
+    lim.min_wave_cost = 2u << 20;
     s->adm = std::make_unique<Admission>(c.budget_bytes, lim);
     sq_trace_params tp{};
     tp.semantics = c.semantics;
@@ -606,6 +682,15 @@ int sq_server_create(int device, const sq_server_config* cfg, sq_server** out) {
             hipMalloc((void**)&sl.spill, s->plan.spill_words * 4) != hipSuccess) {
             set_error("server: event/spill setup failed");
             rc = SQ_ERR_HIP;
+            break;
+        }
+        // pinned staging sized up front: growing it while serving pins pages
+        // with the scheduler's lock held and stalls every request
+        const size_t stage = c.reserved[0] ? (size_t)c.reserved[0] << 20 : kDefaultStage;
+        if (pinned_grow(&sl.pin_in, &sl.pin_in_bytes, stage, true) != SQ_OK ||
+            pinned_grow(&sl.pin_out, &sl.pin_out_bytes, stage, true) != SQ_OK) {
+            rc = SQ_ERR_NOMEM;
+            break;
         }
     }
     if (rc != SQ_OK) {

@@ -52,6 +52,19 @@ class Arena {
         return true;
     }
 
+    // Obtains empty chunks until `bytes` are reserved (set-up: later waves
+    // then find room without growing). False if a chunk cannot be made.
+    bool reserve(uint64_t bytes) {
+        while (reserved_ < bytes) {
+            uint64_t base = 0;
+            if (!grow_(chunk_, &base)) return false;
+            chunks_.push_back(Chunk{base, chunk_, {}, {}});
+            chunks_.back().free[0] = chunk_;
+            reserved_ += chunk_;
+        }
+        return true;
+    }
+
     bool release(uint64_t addr) {
         for (auto& c : chunks_) {
             if (addr < c.base || addr >= c.base + c.size) continue;
@@ -125,7 +138,17 @@ struct WaveLimits {
     uint32_t max_requests = 256;
     uint64_t max_rays = 64ull << 20;
     uint64_t max_tris = 32ull << 20;
+    // Size grouping (cost_ratio 0: off). A request's cost is rays + 2 x
+    // triangles built; once a wave holds a request, later ones join only while
+    // the wave's cost stays within max(cost_ratio x the first one's cost,
+    // min_wave_cost). A request that does not fit stays queued for a later
+    // wave, so small requests are not held back behind a large one in its
+    // wave and a large one runs in a wave of its own.
+    uint32_t cost_ratio = 0;
+    uint64_t min_wave_cost = 0;
 };
+
+inline uint64_t request_cost(const PolicyRequest& r) { return r.rays + (r.build ? 2 * r.tris : 0); }
 
 // Scene residency with LRU eviction of unreferenced scenes, and wave
 // admission under a byte budget (resident scenes + in-flight waves).
@@ -166,7 +189,7 @@ class Admission {
                                     std::vector<uint64_t>* evicted) {
         std::vector<Decision> out;
         uint32_t nreq = 0;
-        uint64_t rays = 0, tris = 0, bytes = 0;
+        uint64_t rays = 0, tris = 0, bytes = 0, cost = 0, cap = 0;
         for (auto it = pending.begin(); it != pending.end();) {
             PolicyRequest& r = *it;
             if (r.cancelled) { out.push_back({r.id, Verdict::Cancelled}); it = pending.erase(it); continue; }
@@ -193,6 +216,10 @@ class Admission {
             }
             if (nreq >= lim_.max_requests || (nreq && (rays + r.rays > lim_.max_rays || tris + r.tris > lim_.max_tris)))
                 break;
+            if (lim_.cost_ratio && nreq && cost + request_cost(r) > cap) {
+                ++it;  // another size class: a later wave
+                continue;
+            }
             if (resident_ + inflight_ + bytes + need > budget_) {
                 make_room(resident_ + inflight_ + bytes + need - budget_, r.build ? r.key : ~0ull, evicted);
                 if (resident_ + inflight_ + bytes + need > budget_) break;  // wait for memory
@@ -224,6 +251,11 @@ class Admission {
             bytes += r.work_bytes;
             rays += r.rays;
             tris += r.build ? r.tris : 0;
+            if (nreq == 0) {
+                const uint64_t c = request_cost(r) * lim_.cost_ratio;
+                cap = c > lim_.min_wave_cost ? c : lim_.min_wave_cost;
+            }
+            cost += request_cost(r);
             nreq++;
             out.push_back({r.id, Verdict::Admit});
             it = pending.erase(it);

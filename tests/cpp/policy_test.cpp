@@ -180,8 +180,35 @@ static void test_trace_waits_for_build_wave() {
     CHECK(d.size() == 1 && d[0].id == 3 && d[0].v == Verdict::Admit);
 }
 
+static void test_size_grouping() {
+    WaveLimits lim;
+    lim.cost_ratio = 2;
+    lim.min_wave_cost = 10000;
+    Admission adm(1 << 30, lim);
+    std::deque<PolicyRequest> q;
+    q.push_back(build(1, 1, 100, 100, 1000, 2000));      // cost 4000: cap max(8000, 10000)
+    q.push_back(build(2, 2, 100, 100, 1000000, 500000));  // cost 2.5M: another size class
+    q.push_back(build(3, 3, 100, 100, 1000, 3000));       // joins: 9000
+    q.push_back(build(4, 4, 100, 100, 1000, 3000));       // would make 14000: waits
+    std::vector<uint64_t> ev;
+    auto d = adm.form_wave(q, 0, 1, &ev);
+    CHECK(d.size() == 2 && d[0].id == 1 && d[1].id == 3);
+    CHECK(q.size() == 2 && q[0].id == 2 && q[1].id == 4);
+    // the large request heads the queue: it forms a wave with what fits 2x its cost
+    d = adm.form_wave(q, 0, 2, &ev);
+    CHECK(d.size() == 2 && d[0].id == 2 && d[1].id == 4);
+    // grouping off: one wave takes everything that fits the other limits
+    Admission all(1 << 30);
+    std::deque<PolicyRequest> q2;
+    q2.push_back(build(1, 1, 100, 100, 1000, 2000));
+    q2.push_back(build(2, 2, 100, 100, 1000000, 500000));
+    d = all.form_wave(q2, 0, 1, &ev);
+    CHECK(d.size() == 2 && q2.empty());
+}
+
 int main() {
     test_arena();
+    test_size_grouping();
     test_trace_waits_for_build_wave();
     test_budget_and_queue();
     test_trace_only_and_lru();

@@ -208,3 +208,35 @@ def test_timeout_and_cancel():
         assert sd["state"] == sq.REQ_DONE
         with pytest.raises(sq.SqError):
             srv.cancel(d)
+
+
+@pytest.mark.parametrize("semantics", [sq.TRACE_EXHAUSTIVE, sq.TRACE_CLOSEST])
+def test_device_outputs_equal_host_outputs(ctx, semantics):
+    """Requests whose outputs are device buffers (traced in place) give the
+    same bytes as requests with host outputs, in a mixed wave and alone."""
+    reqs, scenes = mix_requests()
+    reqs = reqs[:14]
+    bufs = []
+    with sq.Server(0, semantics=semantics) as srv:
+        host_ids = [srv.submit(k, tris=t, camera=c, rays=r) for (k, t, c, r) in reqs]
+        host = [srv.wait(i)[1] for i in host_ids]
+        dev_ids = []
+        for (k, t, c, r) in reqs:
+            nray = len(r) if r is not None else c.width * c.height * c.spp
+            npix = nray if r is not None else c.width * c.height
+            b = {"hit_id": ctx.alloc(4 * nray), "hit_t": ctx.alloc(4 * nray), "counts": ctx.alloc(4 * npix),
+                 "rgb": ctx.alloc(3 * npix) if r is None else None}
+            bufs.append((b, nray, npix))
+            dev_ids.append(srv.submit(k, camera=c, rays=r, device_out=b))
+        for i in dev_ids:
+            st, out = srv.wait(i)
+            assert st["state"] == sq.REQ_DONE and out["hit_id"] is None
+    for (b, nray, npix), h, (k, t, c, r) in zip(bufs, host, reqs):
+        assert same(ctx.to_host(b["hit_id"], 4 * nray, np.uint32), h["hit_id"]), k
+        assert same(ctx.to_host(b["hit_t"], 4 * nray, np.float32), h["hit_t"]), k
+        assert same(ctx.to_host(b["counts"], 4 * npix, np.uint32), h["counts"]), k
+        if r is None:
+            assert same(ctx.to_host(b["rgb"], 3 * npix, np.uint8), h["rgb"].reshape(-1)), k
+        for p in b.values():
+            if p is not None:
+                ctx.free(p)
