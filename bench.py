@@ -471,6 +471,36 @@ def batched_section(sq, ctx, steps, warmup):
         plain_all()
     ctx.sync()
     plain_wall = (time.perf_counter() - t0) / steps
+    # binned SAH: the same scenes built with the recursive builder (bin count
+    # autotuned among 8/16/32) and traced, against the plain per-scene calls
+    log("batched: binned SAH")
+    batch.set_cameras(cams, outputs=out_mask)
+    stune = batch.autotune("recursive", repeats=1)
+    nb = stune["num_bins"]
+    batch.run("recursive", num_bins=nb)
+    sruns, t0 = [], time.perf_counter()
+    for _ in range(steps):
+        batch.run("recursive", num_bins=nb)
+        sruns.append(batch.times()["run_ms"])
+    sah_wall = (time.perf_counter() - t0) / steps
+
+    def plain_sah(check=None):
+        for i, sc in enumerate(resident):
+            bvh = ctx.build(sc, "recursive", num_bins=nb)
+            ctx.trace_camera_device(bvh, cams[i], d_id, d_t, d_cnt, d_rgb)
+            if check is not None and i in check:
+                check[i] = ctx.to_host(d_cnt, cams[i].width * cams[i].height * 4, np.uint32)
+            bvh.free()
+
+    scheck = {i: None for i in range(BATCH_SCENES)}
+    plain_sah(scheck)
+    sah_parity = all(np.array_equal(scheck[i], batch.download(i, hits=False, rgb=False)["counts"]) for i in scheck)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plain_sah()
+    ctx.sync()
+    plain_sah_wall = (time.perf_counter() - t0) / steps
     for p in (d_id, d_t, d_cnt, d_rgb):
         ctx.free(p)
     for sc in resident:
@@ -493,6 +523,16 @@ def batched_section(sq, ctx, steps, warmup):
                                        "against the eager batched trace time"),
         "counts_equal_plain": bool(parity), "checked_scenes": len(check),
         "autotune": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tune.items()},
+        "sah": {
+            "workload": "the same 256 scenes built with binned SAH (recursive builder, bin count autotuned among "
+                        "8/16/32; eager build, graph trace) against 256 plain sq_build + sq_trace_camera calls with "
+                        "that bin count",
+            "num_bins": nb, "batched_ms": round(float(np.median(sruns)), 2), "batched_wall_ms": round(sah_wall * 1e3, 2),
+            "batched_mrays_s": round(nrays / sah_wall / 1e6, 1), "plain_wall_ms": round(plain_sah_wall * 1e3, 2),
+            "plain_mrays_s": round(nrays / plain_sah_wall / 1e6, 1),
+            "speedup_vs_plain": round(plain_sah_wall / sah_wall, 2), "counts_equal_plain": bool(sah_parity),
+            "autotune": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in stune.items()},
+        },
     }
 
 

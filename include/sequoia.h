@@ -329,12 +329,16 @@ int sq_batch_download(sq_ctx* ctx, const sq_batch* b, uint32_t scene, const sq_t
 int sq_batch_times(const sq_batch* b, float* build_ms, float* trace_ms, float* run_ms);
 
 /* Autotuner: times candidate radix digit widths (4..8) and subtree
- * thresholds (512..2048) for sq_batch_build, and the traversal kernel
+ * thresholds (512..2048) for a Morton sq_batch_build, or bin counts 8, 16
+ * and 32 for a binned-SAH one (recursive / horizontal: the bin count shapes
+ * the trees, so each candidate is built and traced with the default trace
+ * and the fastest build + trace kept), and the traversal kernel
  * (per-lane rays with stack slots 8..32, launch shapes of 8 to 24 waves
  * per CU in workgroups of 64 or 128 and 2 or 4 child lists per step, or wave
  * packets for exhaustive semantics) for sq_batch_trace, on this batch (which must have cameras
- * set), keeps the fastest in the batch and reports them. None of them
- * changes any result. */
+ * set), keeps the fastest in the batch and reports them. Apart from the bin
+ * count, none of them changes any result; with it, results equal the
+ * single-scene builds with that bin count. */
 typedef struct {
     uint32_t radix_digit_bits, subtree_threshold, stack_slots, block_size;
     float build_ms, trace_ms;           /* best times */
@@ -343,6 +347,7 @@ typedef struct {
     uint32_t kernel;                    /* SQ_KERNEL_RAYS or SQ_KERNEL_PACKETS */
     uint32_t blocks_per_cu;             /* 0 = the default launch shape */
     uint32_t lists_per_step;            /* per-ray kernel: 2 or 4 */
+    uint32_t num_bins;                  /* binned-SAH batches: the chosen bin count; 0 otherwise */
 } sq_tune_result;
 int sq_batch_autotune(sq_ctx* ctx, sq_batch* b, sq_builder kind, uint32_t repeats, sq_tune_result* out);
 

@@ -73,7 +73,8 @@ class TuneResult(C.Structure):
     _fields_ = [("radix_digit_bits", C.c_uint32), ("subtree_threshold", C.c_uint32), ("stack_slots", C.c_uint32),
                 ("block_size", C.c_uint32), ("build_ms", C.c_float), ("trace_ms", C.c_float),
                 ("default_build_ms", C.c_float), ("default_trace_ms", C.c_float), ("candidates_tried", C.c_uint32),
-                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lists_per_step", C.c_uint32)]
+                ("kernel", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lists_per_step", C.c_uint32),
+                ("num_bins", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -449,11 +450,11 @@ class Batch:
             pass
 
     @staticmethod
-    def _bp(kind, digit_bits, subtree_threshold, layout, morton_bits=0):
-        return BuildParams(morton_bits, digit_bits, 0, 0, layout, 0, subtree_threshold, 0)
+    def _bp(kind, digit_bits, subtree_threshold, layout, morton_bits=0, num_bins=0):
+        return BuildParams(morton_bits, digit_bits, num_bins, 0, layout, 0, subtree_threshold, 0)
 
-    def build(self, kind="morton63", digit_bits=0, subtree_threshold=0, layout=LAYOUT_REFERENCE):
-        p = self._bp(kind, digit_bits, subtree_threshold, layout)
+    def build(self, kind="morton63", digit_bits=0, subtree_threshold=0, layout=LAYOUT_REFERENCE, num_bins=0):
+        p = self._bp(kind, digit_bits, subtree_threshold, layout, num_bins=num_bins)
         _check(lib().sq_batch_build(self.ctx.h, self.h, BUILDERS[kind], C.byref(p)))
 
     def bvh(self, i):
@@ -473,8 +474,8 @@ class Batch:
     def trace(self):
         _check(lib().sq_batch_trace(self.ctx.h, self.h))
 
-    def run(self, kind="morton63", digit_bits=0, subtree_threshold=0, layout=LAYOUT_REFERENCE):
-        p = self._bp(kind, digit_bits, subtree_threshold, layout)
+    def run(self, kind="morton63", digit_bits=0, subtree_threshold=0, layout=LAYOUT_REFERENCE, num_bins=0):
+        p = self._bp(kind, digit_bits, subtree_threshold, layout, num_bins=num_bins)
         _check(lib().sq_batch_run(self.ctx.h, self.h, BUILDERS[kind], C.byref(p)))
 
     def times(self):

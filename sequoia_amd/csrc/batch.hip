@@ -176,7 +176,9 @@ struct sq_batch {
     std::vector<sq_bvh> bvhs;
     // build configuration of the last build / graph
     sq_builder kind = SQ_BUILDER_MORTON63;
-    uint32_t bits = 21, digit = 8, layout = SQ_LAYOUT_REFERENCE, threshold = 0;
+    uint32_t bits = 21, digit = 8, layout = SQ_LAYOUT_REFERENCE, threshold = 0, num_bins = 8;
+    bool sah = false;          // binned-SAH trees (built eagerly, outside the graph)
+    uint32_t sah_depth = 0;    // depth bound of the last SAH build
     bool built = false;
     // trace configuration
     bool cams_set = false;
@@ -252,15 +254,57 @@ void drop_graph(sq_batch* b) {
     b->graph_valid = false;
 }
 
+bool sah_kind(sq_builder kind) { return kind == SQ_BUILDER_RECURSIVE || kind == SQ_BUILDER_HORIZONTAL; }
+
 int morton_bits(sq_builder kind, const sq_build_params* p, uint32_t* bits) {
     uint32_t d = kind == SQ_BUILDER_MORTON ? 10 : kind == SQ_BUILDER_MORTON2 ? 20 : kind == SQ_BUILDER_MORTON63 ? 21 : 0;
-    if (!d) SQ_RET(SQ_ERR_UNSUPPORTED, "batches build with the Morton builders only (got %d)", (int)kind);
+    if (!d) SQ_RET(SQ_ERR_UNSUPPORTED, "batches build with the Morton and binned-SAH builders (got %d)", (int)kind);
     *bits = p && p->morton_bits_per_axis ? p->morton_bits_per_axis : d;
     if (*bits < 1 || *bits > 21) SQ_RET(SQ_ERR_INVALID, "morton bits per axis %u outside 1..21", *bits);
     return SQ_OK;
 }
 
+int upload(sq_ctx* c, void* dst, const void* src, size_t bytes);
+
+// Roots of the chosen builder (Morton trees at slot 0, binned-SAH trees at
+// 2n - 1), written into the trace job table too when cameras are set.
+int set_roots(sq_batch* b) {
+    bool changed = false;
+    for (uint32_t s = 0; s < b->nscenes; s++) {
+        const uint32_t r = b->sah ? 2 * b->counts[s] - 1 : 0;
+        b->bvhs[s].root = r;
+        if (b->cams_set && b->jobs[s].root != r) {
+            b->jobs[s].root = r;
+            changed = true;
+        }
+    }
+    if (changed) SQ_TRY(upload(b->ctx, b->d_jobs, b->jobs.data(), b->jobs.size() * sizeof(TraceArgs)));
+    return SQ_OK;
+}
+
+int configure_build_kind(sq_batch* b, sq_builder kind, const sq_build_params* p);
+
 int configure_build(sq_batch* b, sq_builder kind, const sq_build_params* p) {
+    SQ_TRY(configure_build_kind(b, kind, p));
+    return set_roots(b);
+}
+
+int configure_build_kind(sq_batch* b, sq_builder kind, const sq_build_params* p) {
+    if (sah_kind(kind)) {
+        // binned SAH (recursive; the horizontal builder makes the same tree)
+        const uint32_t nb = p && p->num_bins ? p->num_bins : 8;
+        if (nb < 2 || nb > 64) SQ_RET(SQ_ERR_INVALID, "num_bins %u outside 2..64", nb);
+        const uint32_t layout = p ? p->layout : SQ_LAYOUT_REFERENCE;
+        if (layout > SQ_LAYOUT_FIXED) SQ_RET(SQ_ERR_INVALID, "layout %u", layout);
+        if (!b->sah || nb != b->num_bins || layout != b->layout || kind != b->kind) drop_graph(b);
+        b->kind = kind;
+        b->sah = true;
+        b->num_bins = nb;
+        b->layout = layout;
+        return SQ_OK;
+    }
+    if (b->sah) drop_graph(b);
+    b->sah = false;
     uint32_t bits = 0;
     SQ_TRY(morton_bits(kind, p, &bits));
     uint32_t digit = p && p->radix_digit_bits ? p->radix_digit_bits : b->digit;
@@ -334,7 +378,19 @@ int enqueue_build(sq_ctx* c, sq_batch* b) {
 
 int plan_trace(sq_ctx* c, sq_batch* b, TracePlan* L) {
     if (!b->pk_grid) b->pk_grid = trace_packet_grid(c, true);  // outside any graph capture
-    return trace_plan(c, &b->tparams, morton_depth_bound(b->bits, b->max_n), false, L);
+    uint32_t depth = morton_depth_bound(b->bits, b->max_n);
+    if (b->sah && b->sah_depth > depth) depth = b->sah_depth;
+    return trace_plan(c, &b->tparams, depth, false, L);
+}
+
+// A binned-SAH build of every scene (eager), with the scenes' roots and depths.
+int build_sah(sq_ctx* c, sq_batch* b) {
+    SQ_TRY(launch_fill_u32_on(c->stream, c->num_cus, b->d_nodes, 0u, (size_t)2 * b->total * 8));
+    uint32_t depth = 0;
+    SQ_TRY(sah_build_batch(c, b->d_tris, b->total, b->d_off, b->nscenes, b->num_bins, b->layout, b->d_nodes, &depth));
+    b->sah_depth = depth;
+    for (uint32_t s = 0; s < b->nscenes; s++) b->bvhs[s].depth = depth;
+    return SQ_OK;
 }
 
 // Enqueues the batched trace (cameras, outputs and spill already set up).
@@ -506,9 +562,13 @@ int sq_batch_build(sq_ctx* c, sq_batch* b, sq_builder kind, const sq_build_param
     SQ_CHECK_HIP(hipSetDevice(c->device));
     SQ_TRY(configure_build(b, kind, p));
     SQ_CHECK_HIP(hipEventRecord(b->ev[0], c->stream));
-    SQ_TRY(enqueue_build(c, b));
+    if (b->sah) {
+        SQ_TRY(build_sah(c, b));
+    } else {
+        SQ_TRY(enqueue_build(c, b));
+    }
     SQ_CHECK_HIP(hipEventRecord(b->ev[1], c->stream));
-    SQ_TRY(finish_depths(c, b));
+    if (!b->sah) SQ_TRY(finish_depths(c, b));
     b->build_ms = elapsed(b);
     b->built = true;
     return SQ_OK;
@@ -574,7 +634,7 @@ int sq_batch_set_cameras(sq_ctx* c, sq_batch* b, const sq_camera* cams, const sq
         a = TraceArgs{};
         a.nodes = b->bvhs[s].d_nodes;
         a.tris = b->scenes[s].d_tris;
-        a.root = 0;
+        a.root = b->bvhs[s].root;
         a.node_count = b->bvhs[s].capacity;
         a.tri_count = b->scenes[s].n;
         a.camera = 1;
@@ -628,10 +688,19 @@ int sq_batch_run(sq_ctx* c, sq_batch* b, sq_builder kind, const sq_build_params*
     TracePlan L;
     SQ_TRY(plan_trace(c, b, &L));
     SQ_TRY(ensure_spill(b, L));
+    if (b->sah) {
+        // binned SAH waits on the host per level: the build runs eagerly and
+        // the graph holds the trace alone
+        SQ_CHECK_HIP(hipEventRecord(b->ev[0], c->stream));
+        SQ_TRY(build_sah(c, b));
+        b->built = true;
+        SQ_TRY(plan_trace(c, b, &L));
+        SQ_TRY(ensure_spill(b, L));
+    }
     if (!b->graph_valid) {
         drop_graph(b);
         SQ_CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_build(c, b);
+        int rc = b->sah ? SQ_OK : enqueue_build(c, b);
         if (rc == SQ_OK) rc = enqueue_trace(c, b, L);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -648,11 +717,11 @@ int sq_batch_run(sq_ctx* c, sq_batch* b, sq_builder kind, const sq_build_params*
         }
         b->graph_valid = true;
     }
-    SQ_CHECK_HIP(hipEventRecord(b->ev[0], c->stream));
+    if (!b->sah) SQ_CHECK_HIP(hipEventRecord(b->ev[0], c->stream));
     SQ_CHECK_HIP(hipGraphLaunch(b->exec, c->stream));
     SQ_CHECK_HIP(hipEventRecord(b->ev[1], c->stream));
     b->built = true;
-    SQ_TRY(finish_depths(c, b));
+    if (!b->sah) SQ_TRY(finish_depths(c, b));
     SQ_TRY(check_trace(c, b));
     b->run_ms = elapsed(b);
     return SQ_OK;
@@ -714,19 +783,44 @@ int sq_batch_autotune(sq_ctx* c, sq_batch* b, sq_builder kind, uint32_t repeats,
         out->candidates_tried++;
         return SQ_OK;
     };
-    const uint32_t digits[] = {8, 7, 6, 5, 4};
-    const uint32_t thresholds[] = {2048, 1024, 512};
     float best_b = 1e30f;
-    uint32_t bd = 8, bt = 2048;
-    for (uint32_t d : digits)
-        for (uint32_t t : thresholds) {
-            float ms = 0;
-            SQ_TRY(time_build(d, t, &ms));
-            if (d == 8 && t == 2048) out->default_build_ms = ms;
-            if (ms < best_b) { best_b = ms; bd = d; bt = t; }
+    uint32_t bd = 8, bt = 2048, bn = 0;
+    if (sah_kind(kind)) {
+        // the bin count shapes the trees: each candidate is built and traced
+        // with the batch's trace parameters and the fastest sum kept
+        float best_sum = 1e30f;
+        for (uint32_t nb : {8u, 16u, 32u}) {
+            bp.num_bins = nb;
+            bp.layout = b->layout;
+            float bms = 1e30f, tms = 1e30f;
+            for (uint32_t r = 0; r < repeats + 1; r++) {
+                SQ_TRY(sq_batch_build(c, b, kind, &bp));
+                SQ_TRY(sq_batch_trace(c, b));
+                if (r > 0) {
+                    bms = std::min(bms, b->build_ms);
+                    tms = std::min(tms, b->trace_ms);
+                }
+            }
+            out->candidates_tried++;
+            if (nb == 8) out->default_build_ms = bms;
+            if (bms + tms < best_sum) { best_sum = bms + tms; best_b = bms; bn = nb; }
         }
-    bp.radix_digit_bits = bd;
-    bp.subtree_threshold = bt;
+        bp.num_bins = bn;
+        bd = 0;
+        bt = 0;
+    } else {
+        const uint32_t digits[] = {8, 7, 6, 5, 4};
+        const uint32_t thresholds[] = {2048, 1024, 512};
+        for (uint32_t d : digits)
+            for (uint32_t t : thresholds) {
+                float ms = 0;
+                SQ_TRY(time_build(d, t, &ms));
+                if (d == 8 && t == 2048) out->default_build_ms = ms;
+                if (ms < best_b) { best_b = ms; bd = d; bt = t; }
+            }
+        bp.radix_digit_bits = bd;
+        bp.subtree_threshold = bt;
+    }
     SQ_TRY(sq_batch_build(c, b, kind, &bp));
     const sq_trace_params base = b->tparams;
     const std::vector<sq_camera> cams = b->cams;
@@ -784,6 +878,7 @@ int sq_batch_autotune(sq_ctx* c, sq_batch* b, sq_builder kind, uint32_t repeats,
     out->lists_per_step = bl4;
     out->radix_digit_bits = bd;
     out->subtree_threshold = bt;
+    out->num_bins = bn;
     out->stack_slots = bs;
     out->block_size = bb;
     out->blocks_per_cu = bw;

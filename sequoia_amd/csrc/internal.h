@@ -209,6 +209,9 @@ uint32_t morton_depth_bound(uint32_t bits, uint32_t n);
 int launch_fill_u32_on(hipStream_t stream, int num_cus, void* dst, uint32_t value, size_t words);
 // Host memcpy split over up to 16 threads for large copies.
 void par_memcpy(void* dst, const void* src, size_t bytes);
+// Binned SAH over the scenes of a batch (see sah.hip); eager (waits per level).
+int sah_build_batch(sq_ctx* c, const sq_triangle* tris, uint32_t total, const uint32_t* d_off, uint32_t nscenes,
+                    uint32_t nb, uint32_t layout, sq_node* d_nodes, uint32_t* depth);
 int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_params* p, sq_builder kind,
               sq_node* d_nodes, uint32_t* root, uint32_t* depth);
 int refit_tree(sq_ctx* c, sq_bvh* bvh);

@@ -88,6 +88,12 @@ struct SahArgs {
     uint32_t* part_bins;    // per chunk: nb * kBinWords words
     uint32_t* chunk_cnt;    // per chunk: 2 words
     uint8_t* bin_of;        // bin of the primitive at each position of a large task
+    // batched builds: the scenes' first primitives (nseg + 1 ascending
+    // offsets). Scene s owns positions [off[s], off[s+1]) and node slots
+    // [2 off[s], 2 off[s+1]); its tree is built with batch-wide slots and
+    // primitive ids, which k_batch_sah_local then makes scene-local.
+    const uint32_t* seg_off;
+    uint32_t nseg;
 };
 
 typedef uint32_t BinsLds[kSahWaves][kMaxBins][kBinWords];
@@ -929,6 +935,121 @@ __global__ void k_sah_root(SahArgs a, uint32_t n, const uint32_t* red) {
     }
 }
 
+// ---- batched binned SAH (recursive builder over many scenes) ----
+// Per-primitive boxes and ids as k_sah_setup, and each scene's primitive and
+// centroid boxes: one workgroup per 4096-primitive tile reduces its part of
+// every scene it overlaps (red: 12 ordered-int words per scene).
+constexpr uint32_t kSbTile = 4096;
+
+__global__ void k_batch_sah_red_init(uint32_t* red, uint32_t nscenes, uint32_t* counters) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 12 * nscenes; i += gridDim.x * blockDim.x)
+        red[i] = (((i % 12) / 3) % 2 == 0) ? 0xFFFFFFFFu : 0u;
+    if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ uint32_t seg_of_pos(const uint32_t* off, uint32_t nseg, uint32_t t) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_batch_sah_setup(const sq_triangle* __restrict__ tris,
+                                                         const uint32_t* __restrict__ off, uint32_t nscenes,
+                                                         uint32_t total, float4* __restrict__ boxes,
+                                                         uint32_t* __restrict__ ids, uint32_t* __restrict__ red) {
+    __shared__ uint32_t s_red[12][4];
+    const uint32_t t0 = blockIdx.x * kSbTile;
+    const uint32_t t1 = (total - t0) < kSbTile ? total : t0 + kSbTile;
+    const int wave = threadIdx.x >> 6;
+    for (uint32_t s = seg_of_pos(off, nscenes, t0); s < nscenes && off[s] < t1; s++) {
+        const uint32_t b = off[s] > t0 ? off[s] : t0, e = off[s + 1] < t1 ? off[s + 1] : t1;
+        uint32_t lo[6], hi[6];
+        for (int k = 0; k < 6; k++) { lo[k] = 0xFFFFFFFFu; hi[k] = 0u; }
+        for (uint32_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+            float v[9];
+            sqd::load_tri(tris, i, v);
+            sqd::Box3 bx = sqd::tri_box(v);
+            boxes[2 * i] = make_float4(bx.mn[0], bx.mn[1], bx.mn[2], 0.f);
+            boxes[2 * i + 1] = make_float4(bx.mx[0], bx.mx[1], bx.mx[2], 0.f);
+            ids[i] = i;
+            for (int k = 0; k < 3; k++) {
+                float c = (bx.mn[k] + bx.mx[k]) * 0.5f;
+                uint32_t a0 = sqd::f2ord(bx.mn[k]), a1 = sqd::f2ord(bx.mx[k]), a2 = sqd::f2ord(c);
+                lo[k] = a0 < lo[k] ? a0 : lo[k];
+                hi[k] = a1 > hi[k] ? a1 : hi[k];
+                lo[3 + k] = a2 < lo[3 + k] ? a2 : lo[3 + k];
+                hi[3 + k] = a2 > hi[3 + k] ? a2 : hi[3 + k];
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1)
+            for (int k = 0; k < 6; k++) {
+                uint32_t x = __shfl_xor(lo[k], o, 64), y = __shfl_xor(hi[k], o, 64);
+                lo[k] = x < lo[k] ? x : lo[k];
+                hi[k] = y > hi[k] ? y : hi[k];
+            }
+        if ((threadIdx.x & 63) == 0) {
+            for (int k = 0; k < 3; k++) {
+                s_red[k][wave] = lo[k];
+                s_red[3 + k][wave] = hi[k];
+                s_red[6 + k][wave] = lo[3 + k];
+                s_red[9 + k][wave] = hi[3 + k];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 12) {
+            const int k = threadIdx.x;
+            const bool is_min = (k / 3) % 2 == 0;
+            uint32_t r = s_red[k][0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+                const uint32_t x = s_red[k][w];
+                r = is_min ? (x < r ? x : r) : (x > r ? x : r);
+            }
+            if (is_min) atomicMin(&red[12 * s + k], r);
+            else atomicMax(&red[12 * s + k], r);
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per scene: the root of build_hierarchy (builder.cpp:1587-1612)
+// with the scene's slots and positions, as k_sah_root does for one scene.
+__global__ void k_batch_sah_root(SahArgs a, const uint32_t* red) {
+    const uint32_t s = blockIdx.x, base = a.seg_off[s], n = a.seg_off[s + 1] - base;
+    float pmn[3], pmx[3], cb[6];
+    decode_box(red + 12 * s, pmn, pmx);
+    decode_box(red + 12 * s + 6, cb, cb + 3);
+    const uint32_t nb = 2 * base, ri = nb + 2 * n - 1;
+    if (n <= 8 || box_sa(cb, cb + 3, a.sa_x2) <= 0.0f) {
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) write_leaf_box(a, nb + i, a.ids_in[base + i]);
+        if (threadIdx.x == 0) write_inner_box(a, ri, pmn, pmx, nb, n);
+        return;
+    }
+    if (threadIdx.x == 0) {
+        write_inner_box(a, ri, pmn, pmx, nb, 2);
+        SahTask t;
+        t.b = base; t.e = base + n; t.free = nb; t.self = ri;
+        for (int k = 0; k < 6; k++) t.cb[k] = cb[k];
+        push_task(a, t);
+    }
+}
+
+// Makes a batched build's trees scene-local: the node at batch-wide slot i
+// belongs to scene s with 2 off[s] <= i < 2 off[s+1]; an interior node's
+// child slot loses 2 off[s], a leaf's primitive id off[s]. Empty slots stay.
+__global__ void k_batch_sah_local(sq_node* nodes, const uint32_t* __restrict__ off, uint32_t nscenes, uint32_t slots) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += gridDim.x * blockDim.x) {
+        uint32_t* w = (uint32_t*)(nodes + i);
+        const uint32_t cf = w[3];
+        if ((cf & 0xFFFFu) == 0) continue;  // empty slot
+        const uint32_t s = seg_of_pos(off, nscenes, i >> 1);
+        w[7] -= (cf >> 16) & 1u ? off[s] : 2 * off[s];
+    }
+}
+
 // ---- grid builder ----
 __device__ __forceinline__ uint32_t spread3(uint32_t x) {
     uint32_t r = 0;
@@ -1040,52 +1161,41 @@ __global__ __launch_bounds__(256) void k_grid_init(SahArgs a, uint32_t cells, ui
     }
 }
 
-int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_params* p, sq_builder kind,
-              sq_node* d_nodes, uint32_t* root, uint32_t* depth) {
-    const uint32_t nb = p && p->num_bins ? p->num_bins : 8;
-    if (nb < 2 || nb > (uint32_t)kMaxBins) SQ_RET(SQ_ERR_INVALID, "num_bins %u outside 2..%d", nb, kMaxBins);
-    const bool grid = kind == SQ_BUILDER_GRID || kind == SQ_BUILDER_GRID_SSE || kind == SQ_BUILDER_GRID_AVX;
-    const uint32_t dim = p && p->grid_dim ? p->grid_dim : 4;
-    if (grid && (dim < 1 || dim > 8 || (dim & (dim - 1)))) SQ_RET(SQ_ERR_INVALID, "grid_dim %u", dim);
-    const uint32_t cells = dim * dim * dim;
-    SahArgs a{};
-    a.nodes = d_nodes;
-    a.cap = 2 * n + (grid ? 2 * cells : 0);
-    a.nb = nb;
-    a.part = (kind == SQ_BUILDER_GRID_SSE || kind == SQ_BUILDER_GRID_AVX) ? 1u : 0u;
-    a.sa_x2 = (kind == SQ_BUILDER_GRID_SSE || kind == SQ_BUILDER_GRID_AVX) ? 0u : 1u;
-    a.layout = p ? p->layout : SQ_LAYOUT_REFERENCE;
+// Queue and workspace capacities of a build over n primitives.
+struct SahCaps {
+    size_t qcap, bigcap, chunkcap;
+};
 
+// Sizes the context's workspace for n primitives (cells: grid cells, 0 for
+// the recursive builder) and points the arguments at it.
+static int sah_prepare(sq_ctx* c, uint32_t n, uint32_t cells, uint32_t nb, SahArgs& a, SahCaps& k) {
     // A task that is split has at least 9 primitives (a large one more than
     // kBigTask), and the tasks of one level own disjoint ranges.
-    const size_t qcap = n / 9 + 2 + cells, bigcap = n / kBigTask + 2 + cells;
-    const size_t chunkcap = n / kChunkMin + bigcap + 1;  // sum of ceil(count / kChunkMin) over a level
-    a.small_cap = (uint32_t)qcap;
-    a.big_cap = (uint32_t)bigcap;
-    a.sub_cap = (uint32_t)qcap;
+    k.qcap = n / 9 + 2 + cells;
+    k.bigcap = n / kBigTask + 2 + cells;
+    k.chunkcap = n / kChunkMin + k.bigcap + 1;  // sum of ceil(count / kChunkMin) over a level
+    a.small_cap = (uint32_t)k.qcap;
+    a.big_cap = (uint32_t)k.bigcap;
+    a.sub_cap = (uint32_t)k.qcap;
     SQ_TRY(c->aux[3].ensure((size_t)n * 32));             // boxes
     SQ_TRY(c->vals[0].ensure((size_t)n * 4));              // ids ping
     SQ_TRY(c->vals[1].ensure((size_t)n * 4));              // ids pong
     SQ_TRY(c->keys[0].ensure((size_t)n * 4));
     SQ_TRY(c->keys[1].ensure((size_t)n * 4));
     SQ_TRY(c->scratch.ensure((size_t)n * 4 + 256));
-    SQ_TRY(c->tasks[0].ensure(qcap * sizeof(SahTask)));
-    SQ_TRY(c->tasks[1].ensure(qcap * sizeof(SahTask)));
-    SQ_TRY(c->tasks[2].ensure(qcap * sizeof(SahTask)));
-    SQ_TRY(c->sah[0].ensure(bigcap * sizeof(SahTask)));
-    SQ_TRY(c->sah[1].ensure(bigcap * sizeof(SahTask)));
-    SQ_TRY(c->sah[2].ensure(bigcap * sizeof(SahPlan)));
-    SQ_TRY(c->sah[3].ensure(chunkcap * nb * kBinWords * 4));
-    SQ_TRY(c->sah[4].ensure(chunkcap * 8));
+    SQ_TRY(c->tasks[0].ensure(k.qcap * sizeof(SahTask)));
+    SQ_TRY(c->tasks[1].ensure(k.qcap * sizeof(SahTask)));
+    SQ_TRY(c->tasks[2].ensure(k.qcap * sizeof(SahTask)));
+    SQ_TRY(c->sah[0].ensure(k.bigcap * sizeof(SahTask)));
+    SQ_TRY(c->sah[1].ensure(k.bigcap * sizeof(SahTask)));
+    SQ_TRY(c->sah[2].ensure(k.bigcap * sizeof(SahPlan)));
+    SQ_TRY(c->sah[3].ensure(k.chunkcap * nb * kBinWords * 4));
+    SQ_TRY(c->sah[4].ensure(k.chunkcap * 8));
     SQ_TRY(c->sah[5].ensure((size_t)n + 256));
-    SQ_TRY(c->aux[0].ensure(64 * 4));
     SQ_TRY(c->counters.ensure(64));
     SQ_TRY(c->aux[2].ensure((size_t)cells * kBinWords * 4 + 64));
-    float4* boxes = c->aux[3].as<float4>();
-    uint32_t* red = c->aux[0].as<uint32_t>();
-    uint32_t* counters = c->counters.as<uint32_t>();
-    a.boxes = boxes;
-    a.counters = counters;
+    a.boxes = c->aux[3].as<float4>();
+    a.counters = c->counters.as<uint32_t>();
     a.scratch = c->scratch.as<uint32_t>();
     a.plans = c->sah[2].as<SahPlan>();
     a.part_bins = c->sah[3].as<uint32_t>();
@@ -1097,36 +1207,13 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     a.ids_buf[0] = c->vals[0].as<uint32_t>();
     a.ids_buf[1] = c->vals[1].as<uint32_t>();
     a.child_level = 0;
+    return SQ_OK;
+}
 
-    phase_begin(c, grid ? PH_GRID : PH_SAH);
-    hipLaunchKernelGGL(k_sah_red_init, dim3(1), dim3(64), 0, c->stream, red, counters);
-    uint32_t blocks = (n + 255) / 256, capb = (uint32_t)c->num_cus * 8;
-    if (blocks > capb) blocks = capb;
-    hipLaunchKernelGGL(k_sah_setup, dim3(blocks), dim3(256), 0, c->stream, tris, n, boxes, c->vals[0].as<uint32_t>(), red);
-    int in = 0;
-    if (grid) {
-        uint32_t* cellbins = c->aux[2].as<uint32_t>();
-        hipLaunchKernelGGL(k_grid_bins_init, dim3(1), dim3(256), 0, c->stream, cellbins, cells);
-        const uint32_t cblocks = blocks < (uint32_t)c->num_cus * 2 ? blocks : (uint32_t)c->num_cus * 2;
-        hipLaunchKernelGGL(k_grid_cells, dim3(cblocks), dim3(256), 0, c->stream, boxes, n, dim, red,
-                           c->keys[0].as<uint32_t>(), c->vals[0].as<uint32_t>(), cellbins);
-        int slot = 0;
-        uint32_t kbits = 0;
-        while ((1u << kbits) < cells) kbits++;
-        SQ_TRY(radix_sort(c, n, kbits ? kbits : 1, 8, false, 0, &slot));
-        in = slot;
-        a.ids_in = c->vals[in].as<uint32_t>();
-        a.out_buf = (uint32_t)in;
-        hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(256), 0, c->stream, a, cells,
-                           kind == SQ_BUILDER_GRID ? 0u : 1u, cellbins, red);
-        *root = 0;
-    } else {
-        a.ids_in = c->vals[0].as<uint32_t>();
-        a.out_buf = 0;
-        hipLaunchKernelGGL(k_sah_root, dim3(1), dim3(256), 0, c->stream, a, n, red);
-        *root = 2 * n - 1;
-    }
-    SQ_CHECK_HIP(hipGetLastError());
+// The level loop over the queued tasks (ids in buffer `in`), then the
+// LDS subtrees; *levels: split levels of the deepest path.
+static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* levels_out) {
+    uint32_t* counters = a.counters;
     SQ_TRY(pinned_ensure(c, 64));
     uint32_t* h = (uint32_t*)c->pinned;
     uint32_t level = 0;
@@ -1140,7 +1227,7 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
         const uint32_t nbig = (uint32_t)(packed >> kBigShift);
         const uint64_t nchunks = packed & ((1ull << kBigShift) - 1);
         if (!nsmall && !nbig) break;
-        if (nsmall > qcap || nbig > bigcap || nchunks > chunkcap)
+        if (nsmall > k.qcap || nbig > k.bigcap || nchunks > k.chunkcap)
             SQ_RET(SQ_ERR_OVERFLOW, "sah: level %u holds %u + %u tasks, %llu chunks", level, nsmall, nbig,
                    (unsigned long long)nchunks);
         hipLaunchKernelGGL(k_sah_counters_reset, dim3(1), dim3(64), 0, c->stream, counters);
@@ -1174,7 +1261,7 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
     const uint32_t nsub = h[6];
     uint32_t levels = level;
     if (nsub) {
-        if (nsub > qcap) SQ_RET(SQ_ERR_OVERFLOW, "sah: %u subtree tasks", nsub);
+        if (nsub > k.qcap) SQ_RET(SQ_ERR_OVERFLOW, "sah: %u subtree tasks", nsub);
         hipLaunchKernelGGL(k_sah_subtree, dim3(nsub), dim3(64), 0, c->stream, a, nsub);
         SQ_CHECK_HIP(hipGetLastError());
         SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 32, hipMemcpyDeviceToHost, c->stream));
@@ -1182,8 +1269,106 @@ int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_par
         if (h[4]) SQ_RET(SQ_ERR_OVERFLOW, "sah: subtree stack overflow");
         levels = h[5] > levels ? h[5] : levels;
     }
+    *levels_out = levels;
+    return SQ_OK;
+}
+
+int sah_build(sq_ctx* c, const sq_triangle* tris, uint32_t n, const sq_build_params* p, sq_builder kind,
+              sq_node* d_nodes, uint32_t* root, uint32_t* depth) {
+    const uint32_t nb = p && p->num_bins ? p->num_bins : 8;
+    if (nb < 2 || nb > (uint32_t)kMaxBins) SQ_RET(SQ_ERR_INVALID, "num_bins %u outside 2..%d", nb, kMaxBins);
+    const bool grid = kind == SQ_BUILDER_GRID || kind == SQ_BUILDER_GRID_SSE || kind == SQ_BUILDER_GRID_AVX;
+    const uint32_t dim = p && p->grid_dim ? p->grid_dim : 4;
+    if (grid && (dim < 1 || dim > 8 || (dim & (dim - 1)))) SQ_RET(SQ_ERR_INVALID, "grid_dim %u", dim);
+    const uint32_t cells = dim * dim * dim;
+    SahArgs a{};
+    a.nodes = d_nodes;
+    a.cap = 2 * n + (grid ? 2 * cells : 0);
+    a.nb = nb;
+    a.part = (kind == SQ_BUILDER_GRID_SSE || kind == SQ_BUILDER_GRID_AVX) ? 1u : 0u;
+    a.sa_x2 = (kind == SQ_BUILDER_GRID_SSE || kind == SQ_BUILDER_GRID_AVX) ? 0u : 1u;
+    a.layout = p ? p->layout : SQ_LAYOUT_REFERENCE;
+    SahCaps k;
+    SQ_TRY(sah_prepare(c, n, grid ? cells : 0u, nb, a, k));
+    SQ_TRY(c->aux[0].ensure(64 * 4));
+    uint32_t* red = c->aux[0].as<uint32_t>();
+    uint32_t* counters = a.counters;
+    float4* boxes = (float4*)a.boxes;
+
+    phase_begin(c, grid ? PH_GRID : PH_SAH);
+    hipLaunchKernelGGL(k_sah_red_init, dim3(1), dim3(64), 0, c->stream, red, counters);
+    uint32_t blocks = (n + 255) / 256, capb = (uint32_t)c->num_cus * 8;
+    if (blocks > capb) blocks = capb;
+    hipLaunchKernelGGL(k_sah_setup, dim3(blocks), dim3(256), 0, c->stream, tris, n, boxes, c->vals[0].as<uint32_t>(), red);
+    int in = 0;
+    if (grid) {
+        uint32_t* cellbins = c->aux[2].as<uint32_t>();
+        hipLaunchKernelGGL(k_grid_bins_init, dim3(1), dim3(256), 0, c->stream, cellbins, cells);
+        const uint32_t cblocks = blocks < (uint32_t)c->num_cus * 2 ? blocks : (uint32_t)c->num_cus * 2;
+        hipLaunchKernelGGL(k_grid_cells, dim3(cblocks), dim3(256), 0, c->stream, boxes, n, dim, red,
+                           c->keys[0].as<uint32_t>(), c->vals[0].as<uint32_t>(), cellbins);
+        int slot = 0;
+        uint32_t kbits = 0;
+        while ((1u << kbits) < cells) kbits++;
+        SQ_TRY(radix_sort(c, n, kbits ? kbits : 1, 8, false, 0, &slot));
+        in = slot;
+        a.ids_in = c->vals[in].as<uint32_t>();
+        a.out_buf = (uint32_t)in;
+        hipLaunchKernelGGL(k_grid_init, dim3(1), dim3(256), 0, c->stream, a, cells,
+                           kind == SQ_BUILDER_GRID ? 0u : 1u, cellbins, red);
+        *root = 0;
+    } else {
+        a.ids_in = c->vals[0].as<uint32_t>();
+        a.out_buf = 0;
+        hipLaunchKernelGGL(k_sah_root, dim3(1), dim3(256), 0, c->stream, a, n, red);
+        *root = 2 * n - 1;
+    }
+    SQ_CHECK_HIP(hipGetLastError());
+    uint32_t levels = 0;
+    SQ_TRY(sah_levels(c, a, in, k, &levels));
     phase_end(c, grid ? PH_GRID : PH_SAH);
     *depth = levels + 2 + (grid ? 7 : 0);
+    return SQ_OK;
+}
+
+// Binned SAH (the recursive builder, nb bins) over the scenes of a batch:
+// scene s holds primitives [off[s], off[s+1]) of `tris` and its tree goes to
+// node slots [2 off[s], 2 off[s+1]) of d_nodes, root at 2 off[s+1] - 1
+// (scene-local 2n - 1), child slots and leaf ids scene-local. Every scene's
+// tasks share the level launches. *depth bounds every scene's tree depth.
+int sah_build_batch(sq_ctx* c, const sq_triangle* tris, uint32_t total, const uint32_t* d_off, uint32_t nscenes,
+                    uint32_t nb, uint32_t layout, sq_node* d_nodes, uint32_t* depth) {
+    if (nb < 2 || nb > (uint32_t)kMaxBins) SQ_RET(SQ_ERR_INVALID, "num_bins %u outside 2..%d", nb, kMaxBins);
+    SahArgs a{};
+    a.nodes = d_nodes;
+    a.cap = 2 * total;
+    a.nb = nb;
+    a.part = 0;
+    a.sa_x2 = 1;
+    a.layout = layout;
+    a.seg_off = d_off;
+    a.nseg = nscenes;
+    SahCaps k;
+    // every scene may add its root task and a few more to a level
+    SQ_TRY(sah_prepare(c, total + 16 * nscenes, 0, nb, a, k));
+    SQ_TRY(c->aux[0].ensure((size_t)12 * nscenes * 4 + 256));
+    uint32_t* red = c->aux[0].as<uint32_t>();
+    const uint32_t ib = (12 * nscenes + 255) / 256;
+    hipLaunchKernelGGL(k_batch_sah_red_init, dim3(ib), dim3(256), 0, c->stream, red, nscenes, a.counters);
+    const uint32_t tiles = (total + kSbTile - 1) / kSbTile;
+    hipLaunchKernelGGL(k_batch_sah_setup, dim3(tiles), dim3(256), 0, c->stream, tris, d_off, nscenes, total,
+                       (float4*)a.boxes, c->vals[0].as<uint32_t>(), red);
+    a.ids_in = c->vals[0].as<uint32_t>();
+    a.out_buf = 0;
+    hipLaunchKernelGGL(k_batch_sah_root, dim3(nscenes), dim3(256), 0, c->stream, a, red);
+    SQ_CHECK_HIP(hipGetLastError());
+    uint32_t levels = 0;
+    SQ_TRY(sah_levels(c, a, 0, k, &levels));
+    const uint32_t slots = 2 * total, gb = (slots + 255) / 256 < (uint32_t)c->num_cus * 8 ? (slots + 255) / 256
+                                                                                           : (uint32_t)c->num_cus * 8;
+    hipLaunchKernelGGL(k_batch_sah_local, dim3(gb), dim3(256), 0, c->stream, d_nodes, d_off, nscenes, slots);
+    SQ_CHECK_HIP(hipGetLastError());
+    *depth = levels + 2;
     return SQ_OK;
 }
 

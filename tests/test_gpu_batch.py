@@ -195,7 +195,9 @@ def test_batch_rejects_bad_input(ctx):
         sq.Batch(ctx, [sq.gen_soup(10, 1, 0.01), np.zeros((0, 9), np.float32)])
     b = sq.Batch(ctx, [sq.gen_soup(100, 1, 0.01)])
     with pytest.raises(sq.SqError):
-        b.build("recursive")
+        b.build("grid")  # batches build with the Morton and binned-SAH (recursive / horizontal) builders
+    with pytest.raises(sq.SqError):
+        b.build("recursive", num_bins=65)
     with pytest.raises(sq.SqError):
         b.trace()
     b.free()
@@ -219,3 +221,78 @@ def test_batch_large_scenes(ctx):
         assert b.bvh(i).info()[2] == depth, i
         assert_same(b.download(i), plain_trace(ctx, t, cams[i], "morton63", sq.TRACE_EXHAUSTIVE), ("large", i))
     b.free()
+
+
+def plain_sah_tree(ctx, tris, kind, nb):
+    s = ctx.upload(tris)
+    bv = ctx.build(s, kind, num_bins=nb)
+    nodes, root = bv.download(), bv.root
+    bv.free()
+    s.free()
+    return nodes, root
+
+
+@pytest.mark.parametrize("kind,nb", [("recursive", 8), ("recursive", 16), ("recursive", 32), ("horizontal", 8)])
+def test_batch_sah_trees_equal_plain(ctx, mixed, kind, nb):
+    """Binned SAH over every scene of the batch at once: each scene's Node1[]
+    and root equal its single-scene build with the same bin count."""
+    scenes, b = mixed
+    say("batch sah", kind, nb)
+    b.build(kind, num_bins=nb)
+    for i, t in enumerate(scenes):
+        want, root = plain_sah_tree(ctx, t, kind, nb)
+        got = b.bvh(i)
+        assert got.root == root, (i, got.root, root)
+        nodes = got.download()
+        assert len(nodes) == len(want)
+        assert len(nodes_equal(nodes, want)) == 0, (kind, nb, i, nodes_equal(nodes, want)[:5])
+    b.build("morton63")
+
+
+def test_batch_sah_matches_oracle(ctx, mixed):
+    scenes, b = mixed
+    b.build("recursive", num_bins=16)
+    for i in (2, 4, 7):
+        want, root = ob.build_sah(scenes[i], 16)
+        assert b.bvh(i).root == root
+        assert len(nodes_equal(b.bvh(i).download(), want)) == 0, i
+    b.build("morton63")
+
+
+@pytest.mark.parametrize("semantics", [sq.TRACE_EXHAUSTIVE, sq.TRACE_CLOSEST])
+def test_batch_sah_trace_equals_plain(ctx, mixed, semantics):
+    """Traces of the batched SAH trees (eagerly and through run(): eager
+    build, graph trace), then back to Morton trees in the same batch."""
+    scenes, b = mixed
+    cams = cameras_for(scenes)
+    b.set_cameras(cams, semantics=semantics)
+    for rep in range(2):
+        if rep == 0:
+            b.build("recursive", num_bins=16)
+            b.trace()
+        else:
+            b.run("recursive", num_bins=16)
+        for i, t in enumerate(scenes):
+            s = ctx.upload(t)
+            bv = ctx.build(s, "recursive", num_bins=16)
+            want = ctx.trace_camera(bv, cams[i], semantics=semantics)
+            bv.free()
+            s.free()
+            assert_same(b.download(i), want, ("sah", rep, i))
+    b.run("morton63")
+    for i, t in enumerate(scenes):
+        assert_same(b.download(i), plain_trace(ctx, t, cams[i], "morton63", semantics), ("back to morton", i))
+
+
+def test_batch_sah_autotune_bins(ctx, mixed):
+    scenes, b = mixed
+    cams = cameras_for(scenes)
+    b.set_cameras(cams)
+    r = b.autotune("recursive", repeats=1)
+    say("sah autotune", r)
+    assert r["num_bins"] in (8, 16, 32) and r["candidates_tried"] == 3 + 18
+    b.run("recursive", num_bins=r["num_bins"])
+    for i, t in enumerate(scenes):
+        want, root = plain_sah_tree(ctx, t, "recursive", r["num_bins"])
+        assert len(nodes_equal(b.bvh(i).download(), want)) == 0, i
+    b.build("morton63")
