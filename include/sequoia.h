@@ -69,7 +69,9 @@ void* sq_ctx_stream(sq_ctx* ctx);
  * HIP events recorded on the context stream. total_build_ms and trace_ms are
  * always measured; the build's phase times (bounds_ms .. grid_ms) only after
  * sq_ctx_set_phase_timing(ctx, 1), since every phase boundary's event leaves
- * the device idle for about 10 us. */
+ * the device idle for about 10 us. total_build_ms spans the build's device
+ * work from its first kernel to the copy of its result record to the host
+ * (the host's wait for that copy is not part of it). */
 typedef struct {
     float bounds_ms, morton_ms, sort_ms, emit_ms, refit_ms, sah_ms, grid_ms, total_build_ms;
     float trace_ms;

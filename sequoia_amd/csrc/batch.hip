@@ -120,7 +120,9 @@ __global__ __launch_bounds__(256) void k_batch_morton(const sq_triangle* __restr
 
 int enqueue_batch_build(sq_ctx* c, hipStream_t stream, const BatchBuild& j) {
     const bool key64 = j.bits > 10;
-    SQ_TRY(launch_fill_u32_on(stream, c->num_cus, j.nodes, 0u, (size_t)2 * j.total * 8));
+    // the emission (with refit) writes every node slot of every scene, unused
+    // ones as zeros, so the node array takes no fill
+    if (c->poison_nodes) SQ_TRY(launch_fill_u32_on(stream, c->num_cus, j.nodes, 0xFFFFFFFFu, (size_t)2 * j.total * 8));
     const uint32_t ib = (6 * j.nscenes + 255) / 256;
     hipLaunchKernelGGL(k_batch_bounds_init, dim3(ib), dim3(256), 0, stream, j.d_ord, j.nscenes);
     const uint32_t tiles = (j.total + kBbTile - 1) / kBbTile;

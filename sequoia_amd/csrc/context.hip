@@ -147,6 +147,7 @@ int sq_ctx_create(int device, sq_ctx** out) {
     sq_ctx* c = new sq_ctx();
     c->device = device;
     if (const char* e = std::getenv("SEQUOIA_TEST_BARRIER_SPINS")) c->barrier_spins = (uint32_t)std::strtoul(e, nullptr, 10);
+    if (const char* e = std::getenv("SEQUOIA_TEST_POISON_NODES")) c->poison_nodes = std::strtoul(e, nullptr, 10) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -381,7 +382,13 @@ int sq_build(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_build_param
     (void)hipEventCreateWithFlags(&t0, hipEventReleaseToDevice);
     (void)hipEventCreateWithFlags(&t1, hipEventReleaseToDevice);
     (void)hipEventRecord(t0, c->stream);
-    int rc = launch_fill_u32(c, b->d_nodes, 0u, (size_t)cap * 8);
+    c->build_end = t1;
+    c->build_end_recorded = false;
+    // Morton builds with refit write every slot of the array (unused ones as
+    // zeros); the other builders and links-only Morton builds start from zeros
+    const bool writes_all = morton && !(p && p->no_refit);
+    int rc = writes_all ? (c->poison_nodes ? launch_fill_u32(c, b->d_nodes, 0xFFFFFFFFu, (size_t)cap * 8) : SQ_OK)
+                        : launch_fill_u32(c, b->d_nodes, 0u, (size_t)cap * 8);
     if (rc == SQ_OK) {
         switch (kind) {
             case SQ_BUILDER_MORTON:
@@ -395,7 +402,8 @@ int sq_build(sq_ctx* c, const sq_scene* s, sq_builder kind, const sq_build_param
             default: rc = SQ_ERR_INVALID; set_error("sq_build: unknown builder %d", (int)kind); break;
         }
     }
-    (void)hipEventRecord(t1, c->stream);
+    if (!c->build_end_recorded) (void)hipEventRecord(t1, c->stream);
+    c->build_end = nullptr;
     if (rc == SQ_OK) rc = phases_collect(c);
     if (rc == SQ_OK) {
         float ms = 0;

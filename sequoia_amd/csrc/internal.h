@@ -91,6 +91,15 @@ struct sq_ctx {
     void* pinned = nullptr;  // small pinned host staging area
     size_t pinned_bytes = 0;
     bool phase_timing = false;       // build phases timed separately (sq_ctx_set_phase_timing)
+    // End-of-build event of the running sq_build: a builder that waits for a
+    // result copy records it right after queueing that copy (the build's last
+    // device work), so the build time excludes the host's wake-up after the wait.
+    hipEvent_t build_end = nullptr;
+    bool build_end_recorded = false;
+    // Test hook (SEQUOIA_TEST_POISON_NODES at creation): node arrays of
+    // Morton builds that skip the zero fill are first filled with all-ones
+    // words, so a slot the emission fails to write shows up in the tree.
+    bool poison_nodes = false;
     uint32_t* h_sort_err = nullptr;  // pinned: the last single-pass sort's error word
     bool sort_err_pending = false;
 };
@@ -253,6 +262,14 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Orders one wave's LDS accesses around it (the wave's LDS operations
+// execute in program order; this keeps the compiler from moving any across).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
 __device__ __forceinline__ void load_tri(const sq_triangle* __restrict__ tris, uint32_t i, float v[9]) {
     const float* p = (const float*)(tris + i);
 #pragma unroll
@@ -282,6 +299,11 @@ __device__ __forceinline__ void store_node(sq_node* nodes, uint32_t i, const Box
     float4* p = (float4*)(nodes + i);
     p[0] = make_float4(b.mn[0], b.mn[1], b.mn[2], __uint_as_float((count & 0xFFFFu) | (flags << 16)));
     p[1] = make_float4(b.mx[0], b.mx[1], b.mx[2], __uint_as_float(child));
+}
+__device__ __forceinline__ void store_zero_node(sq_node* nodes, uint32_t i) {
+    float4* p = (float4*)(nodes + i);
+    p[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    p[1] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 __device__ __forceinline__ void store_node_links(sq_node* nodes, uint32_t i, uint32_t child, uint32_t count,
                                                  uint32_t flags) {

@@ -10,6 +10,12 @@
 // including the right-singleton quirk (layout REFERENCE) or with the
 // singleton given the second pair slot (layout FIXED).
 //
+// A range of c >= 2 primitives owns the 2(c-1) slots from its free index; a
+// fan of m leaves uses m of its 2(m-1) and leaves the rest unused, and slot
+// 2n-1 of a scene's 2n is never used. With refit on, the emission writes
+// every used slot whole and zeroes every unused one, so the node array needs
+// no fill beforehand (without refit the links-only stores rely on one).
+//
 // Boxes (no reference counterpart, the reference writes none): leaf = the
 // triangle's bounds; interior = union of its child list in list order.
 //
@@ -69,6 +75,9 @@ __device__ __forceinline__ void write_leaf(const EmitArgs& a, uint32_t slot, uin
 // issued before any result is used (one thread writes the whole fan, so
 // fetching leaf by leaf would wait for each in turn).
 __device__ __forceinline__ void write_fan(const EmitArgs& a, uint32_t f, uint32_t b, uint32_t m, sqd::Box3* u) {
+    // the fan's unused slots f+m .. f+2m-3
+    for (uint32_t i = f + m; i < f + 2 * m - 2; i++)
+        if (i < a.cap) sqd::store_zero_node(a.nodes, i);
     uint32_t gt[7];
 #pragma unroll
     for (uint32_t i = 0; i < 7; i++) gt[i] = i < m ? a.idx[b + i] : 0u;
@@ -295,6 +304,8 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = kTopThreads / 64;
     if (threadIdx.x == 0) {
         s_ncur = 0; s_nnext = 0; s_ntop = 0; s_cur = 0;
+        // the scene's unused slots: n+1 .. 2n-1 behind a root fan, else 2n-1
+        for (uint32_t i = d.n < 8 ? d.n + 1 : 2 * d.n - 1; i < 2 * d.n; i++) sqd::store_zero_node(e.nodes, i);
         if (d.n < 8) {
             sqd::Box3 u, lb;
             for (uint32_t i = 0; i < d.n; i++) {
@@ -412,6 +423,8 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top_grid(LbvhArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = kTopThreads / 64;
     const uint32_t gw = blockIdx.x * nwaves + wave, total_waves = gridDim.x * nwaves;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // the scene's unused slots: n+1 .. 2n-1 behind a root fan, else 2n-1
+        for (uint32_t i = d.n < 8 ? d.n + 1 : 2 * d.n - 1; i < 2 * d.n; i++) sqd::store_zero_node(e.nodes, i);
         if (d.n < 8) {
             sqd::Box3 u, lb;
             for (uint32_t i = 0; i < d.n; i++) {
@@ -478,7 +491,7 @@ struct SmallTask {
 // child slot (a range of c primitives places all its nodes and leaves in the
 // 2(c-1) slots from its root's `free`): a leaf (sorted position), a fan node
 // (first child slot, count), or a split node (written from the task list).
-constexpr uint32_t kSlotLeaf = 1u << 31, kSlotFan = 1u << 30;
+constexpr uint32_t kSlotLeaf = 1u << 31, kSlotFan = 1u << 30, kSlotSplit = 1u << 29;
 constexpr int kSmallSlots = 2 * kMaxThreshold;
 
 // Persistent: one workgroup finishes one queued range at a time, entirely in
@@ -488,7 +501,8 @@ constexpr int kSmallSlots = 2 * kMaxThreshold;
 //   2. every leaf's triangle is fetched at once and its node written; the
 //      leaf boxes go to an LDS box per slot (in the space the codes held);
 //   3. fan nodes, then the split nodes deepest level first, take the union of
-//      their child slots' LDS boxes in slot order, and are written.
+//      their child slots' LDS boxes in slot order, and are written; slots
+//      that hold nothing (the unused tail of each fan) are zeroed.
 // Only phase 2 reads global memory after the codes, and every node is
 // written once, whole.
 //
@@ -580,6 +594,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
                         const uint32_t k = atomicAdd(&s_total, 1u);
                         if (k < kMaxThreshold) s_tasks[k] = SmallTask{cb | ((cb + cn) << 16), side_free[sd], side_slot[sd]};
                         else s_err = 1;
+                        if (cs < nslots) s_slot[cs] = kSlotSplit;
                     }
                 }
             }
@@ -623,6 +638,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
         // 3. fans, then split nodes deepest first: unions of child slots in slot order
         for (uint32_t sl = threadIdx.x; sl < nslots; sl += kSmallThreads) {
             const uint32_t tag = s_slot[sl];
+            if (tag == 0 && sbase + sl < e.cap) sqd::store_zero_node(e.nodes, sbase + sl);  // unused slot
             if (!(tag & kSlotFan)) continue;
             const uint32_t c0 = tag & 0xFFFFu, cn = (tag >> 16) & 0x3FFFu, slot = sbase + sl;
             if (c0 + cn > nslots) continue;  // never tagged so (checked at emission)
@@ -821,6 +837,12 @@ int lbvh_emit_batch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint
                        clamp_threshold(threshold), d_nodes, ws, d_depth_out, false, false);
 }
 
+static void mark_build_end(sq_ctx* c) {
+    if (!c->build_end) return;
+    (void)hipEventRecord(c->build_end, c->stream);
+    c->build_end_recorded = true;
+}
+
 int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_codes, const uint32_t* d_idx, bool key64,
               uint32_t code_bits, uint32_t layout, bool refit, uint32_t threshold, sq_node* d_nodes, uint32_t* depth) {
     const uint32_t T = clamp_threshold(threshold);
@@ -856,6 +878,7 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
                        d_nodes, c->aux[1].p, d_depth, true, true, &hd, pt));
     uint32_t* h = (uint32_t*)c->pinned;
     SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 68, hipMemcpyDeviceToHost, c->stream));
+    mark_build_end(c);
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
     c->last.top_retries = 0;
     if (h[2] == 4u) {
@@ -866,6 +889,7 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
         SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit,
                            T, d_nodes, c->aux[1].p, d_depth, false, false, &hd, pt));
         SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 68, hipMemcpyDeviceToHost, c->stream));
+        mark_build_end(c);
         SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
         c->last.top_retries = 1;
     }

@@ -19,8 +19,8 @@ constexpr int kBoundsBlock = 256;
 // block that takes the last ticket (ord6[8], zeroed by k_bounds_init) folds
 // all partials into ord6[0..5]. Atomics on the six result words from every
 // block serialise at L2 and cost about 50 us whatever the triangle count.
-// VEC: the triangle array is 16-byte aligned and read as four triangles per
-// thread and step (nine 16-byte loads); otherwise one triangle at a time.
+// VEC: the triangle array is 16-byte aligned and read in chunks of 256
+// triangles per wave (below); otherwise one triangle per thread and step.
 template <bool VEC>
 __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle* __restrict__ tris, uint32_t n,
                                                                 uint32_t* __restrict__ ord6) {
@@ -35,26 +35,43 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle
             hi[k] = c > hi[k] ? c : hi[k];
         }
     };
-    const uint32_t quads = VEC ? n / 4 : 0;
-    const float4* q4 = (const float4*)tris;
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += gridDim.x * blockDim.x) {
-        float v[36];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t done = 0;  // triangles covered by the chunked path
+    if (VEC) {
+        // A wave reads 256 consecutive triangles (576 float4) with coalesced
+        // loads, passes them through LDS, and each lane then takes four whole
+        // triangles from it: every load instruction covers 1 KB of contiguous
+        // memory instead of 64 triangles 144 B apart.
+        __shared__ float4 s_chunk[kBoundsBlock / 64][9 * 64];
+        float4* sc = s_chunk[wave];
+        const float4* q4 = (const float4*)tris;
+        const uint32_t chunks = n / 256, waves = gridDim.x * (kBoundsBlock / 64);
+        for (uint32_t ch = blockIdx.x * (kBoundsBlock / 64) + wave; ch < chunks; ch += waves) {
+            float4 r[9];
 #pragma unroll
-        for (int j = 0; j < 9; j++) {
-            const float4 f = q4[(size_t)q * 9 + j];
-            v[4 * j] = f.x; v[4 * j + 1] = f.y; v[4 * j + 2] = f.z; v[4 * j + 3] = f.w;
+            for (int j = 0; j < 9; j++) r[j] = q4[(size_t)ch * 576 + j * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < 9; j++) sc[j * 64 + lane] = r[j];
+            sqd::wave_lds_sync();
+            float v[36];
+#pragma unroll
+            for (int j = 0; j < 9; j++) {
+                const float4 f = sc[9 * lane + j];
+                v[4 * j] = f.x; v[4 * j + 1] = f.y; v[4 * j + 2] = f.z; v[4 * j + 3] = f.w;
+            }
+            sqd::wave_lds_sync();  // the next chunk overwrites what this one read
+#pragma unroll
+            for (int t = 0; t < 4; t++) take(v + 9 * t);
         }
-#pragma unroll
-        for (int t = 0; t < 4; t++) take(v + 9 * t);
+        done = chunks * 256;
     }
-    for (uint32_t i = 4 * quads + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = done + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float v[9];
         sqd::load_tri(tris, i, v);
         take(v);
     }
     __shared__ uint32_t s[6][kBoundsBlock / 64];
     __shared__ uint32_t s_last;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     auto block_reduce = [&]() {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {

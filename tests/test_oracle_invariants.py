@@ -184,3 +184,37 @@ def test_heat_byte():
     assert o.sqo_heat_byte(0, 1) == 0 and o.sqo_heat_byte(200, 1) == 255 and o.sqo_heat_byte(1000, 1) == 255
     assert o.sqo_heat_byte(100, 1) == int(np.float32(0.5) * np.float32(255))
     assert o.sqo_heat_byte(400, 4) == o.sqo_heat_byte(100, 1)
+
+
+def unused_slots(nodes):
+    """Slots of a Morton tree that hold nothing, as the builders predict them:
+    the last m - 2 of the 2(m - 1) slots a fan of m leaves reserves, and the
+    array's last slot (2n slots hold at most 2n - 1 nodes)."""
+    n2 = len(nodes)
+    fan = (nodes["flags"] & 1) == 0
+    fan &= nodes["count"] >= 3
+    out = set()
+    for i in np.nonzero(fan)[0]:
+        c, m = int(nodes["child"][i]), int(nodes["count"][i])
+        out.update(range(c + m, c + 2 * m - 2))
+    if n2 >= 4:
+        out.add(n2 - 1)
+    return out
+
+
+@pytest.mark.parametrize("layout", [ob.LAYOUT_REFERENCE, ob.LAYOUT_FIXED])
+def test_unused_slots_are_fan_tails(layout):
+    """The slots the reference's Morton emission never writes (all-zero in
+    its memset array: a written leaf or interior node has a non-zero count)
+    are exactly the fan tails and the last slot. The GPU emission writes
+    zeros to those slots instead of filling the array first."""
+    rng = np.random.default_rng(17)
+    scenes = [rng.random((n, 9), dtype=np.float32) for n in (1, 2, 3, 5, 7, 8, 9, 15, 16, 100, 2049)]
+    scenes += [GOLDEN_SCENES[k]() for k in ("ico1280", "soup2000", "clus3000")]
+    scenes.append(np.tile(np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0]], np.float32), (500, 1)))
+    for tris in scenes:
+        for bits in (10, 21):
+            nodes, _, _, _ = ob.build_morton(tris, bits, layout=layout)
+            raw = nodes.view(np.uint8).reshape(len(nodes), -1)
+            zero = set(np.nonzero(~raw.any(axis=1))[0].tolist())
+            assert zero == unused_slots(nodes), (len(tris), bits)
