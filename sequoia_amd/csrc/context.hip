@@ -166,6 +166,15 @@ int sq_ctx_create(int device, sq_ctx** out) {
     // through them) and the allocation-free entry points' scratch address
     // on this device: both set up here, so later calls need not allocate.
     int rc = pinned_ensure(c, 4096);
+    if (rc == SQ_OK) {
+        // the bounds kernel's ticket starts at zero and every launch leaves it so
+        e = hipMalloc((void**)&c->d_bounds_ticket, 256);
+        if (e == hipSuccess) e = hipMemset(c->d_bounds_ticket, 0, 256);
+        if (e != hipSuccess) {
+            set_error("bounds ticket: %s", hipGetErrorString(e));
+            rc = SQ_ERR_HIP;
+        }
+    }
     if (rc == SQ_OK) rc = noalloc_setup(device);
     if (rc != SQ_OK) {
         sq_ctx_destroy(c);
@@ -190,6 +199,7 @@ void sq_ctx_destroy(sq_ctx* c) {
         (void)hipEventDestroy(c->ev_end[i]);
     }
     if (c->pinned) (void)hipHostFree(c->pinned);
+    if (c->d_bounds_ticket) (void)hipFree(c->d_bounds_ticket);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -89,6 +89,7 @@ struct sq_ctx {
     // at context creation, so tests can force the fallback top phase
     uint32_t barrier_spins = 0;
     void* pinned = nullptr;  // small pinned host staging area
+    uint32_t* d_bounds_ticket = nullptr;  // scene-bounds last-block ticket (zero between launches)
     size_t pinned_bytes = 0;
     bool phase_timing = false;       // build phases timed separately (sq_ctx_set_phase_timing)
     // End-of-build event of the running sq_build: a builder that waits for a

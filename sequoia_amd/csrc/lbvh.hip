@@ -170,9 +170,15 @@ struct PrefixTable {
 // A wave fills its lanes' entry ranges together, 64 entries per store, so a
 // long range of empty top bits (the inside of a closed surface) costs no one
 // lane thousands of stores.
+// The same launch zeroes the emission counters (`zero`, `zero_words`) and
+// writes the build's one scene descriptor, which would otherwise take a
+// launch of their own.
 template <typename K>
 __global__ void k_prefix_table(const K* __restrict__ codes, uint32_t n, uint32_t pshift, uint32_t entries,
-                               uint32_t* __restrict__ table) {
+                               uint32_t* __restrict__ table, uint32_t* __restrict__ zero, uint32_t zero_words,
+                               SceneDesc* desc, SceneDesc value) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < zero_words; i += gridDim.x * blockDim.x) zero[i] = 0;
+    if (desc && blockIdx.x == 0 && threadIdx.x == 0) *desc = value;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t wave0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u), wstride = gridDim.x * blockDim.x;
     for (uint32_t wb = wave0; wb <= n; wb += wstride) {  // wave-uniform loop over positions 0..n
@@ -277,6 +283,8 @@ struct LbvhArgs {
                                // counts [kMaxLevels + 2], then per-level barrier words
     PrefixTable pt;            // one-scene builds: split positions of the top bits
     uint32_t spin_limit;       // k_lbvh_top_grid: spins before a barrier wait gives up
+    uint32_t* host_rec;        // one-scene builds: pinned host record written by k_lbvh_finish
+                               // (words 0-2 counters, 8 depth, 16 top levels), else null
 };
 
 __device__ __forceinline__ EmitArgs scene_emit(const LbvhArgs& a, const SceneDesc& d) {
@@ -680,7 +688,17 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_finish(LbvhArgs a) {
     const uint32_t levels = a.scene_levels[scene];
     const bool failed = a.counters[2] != 0 || levels >= (uint32_t)kMaxLevels;
     // depth 0 marks a tree whose emission ran out of workspace (see counters[2])
-    if (threadIdx.x == 0 && a.depth_out) a.depth_out[scene] = failed ? 0u : levels + a.scene_small[scene] + 2;
+    const uint32_t depth = failed ? 0u : levels + a.scene_small[scene] + 2;
+    if (threadIdx.x == 0 && a.depth_out) a.depth_out[scene] = depth;
+    if (threadIdx.x == 0 && a.host_rec && scene == 0) {
+        // the host reads these once the stream is idle: no copy command for them
+        a.host_rec[0] = a.counters[0];
+        a.host_rec[1] = a.counters[1];
+        a.host_rec[2] = a.counters[2];
+        a.host_rec[8] = depth;
+        a.host_rec[16] = levels;
+        __threadfence_system();
+    }
     if (!a.refit || failed) return;
     sq_node* nodes = a.nodes + d.node_base;
     const uint32_t cap = 2 * d.n;
@@ -768,7 +786,8 @@ size_t lbvh_batch_workspace(uint32_t total, uint32_t nscenes, uint32_t threshold
 static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total, const void* d_codes,
                        const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
                        bool refit, uint32_t T, sq_node* d_nodes, void* ws, uint32_t* d_depth, bool split_refit_phase,
-                       bool top_grid, const SceneDesc* write_desc = nullptr, PrefixTable pt = PrefixTable{}) {
+                       bool top_grid, const SceneDesc* write_desc = nullptr, PrefixTable pt = PrefixTable{},
+                       uint32_t* table_out = nullptr, uint32_t* host_rec = nullptr) {
     WsLayout w = ws_layout(total, nscenes, T);
     char* base = (char*)ws;
     LbvhArgs a;
@@ -793,11 +812,26 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     a.topm = (uint32_t*)(base + w.topm);
     a.pt = pt;
     a.spin_limit = c->barrier_spins ? c->barrier_spins : kBarrierSpins;
+    a.host_rec = host_rec;
     const uint32_t zero_words = (uint32_t)((w.topm - w.counters) / 4) + 2 * (kMaxLevels + 2);
-    uint32_t zb = (zero_words + 255) / 256;
-    zb = zb > 64 ? 64 : zb;
-    hipLaunchKernelGGL(k_zero_counters, dim3(zb), dim3(256), 0, stream, a.counters, zero_words,
-                       write_desc ? const_cast<SceneDesc*>(d_scenes) : nullptr, write_desc ? *write_desc : SceneDesc{});
+    SceneDesc* desc = write_desc ? const_cast<SceneDesc*>(d_scenes) : nullptr;
+    const SceneDesc desc_value = write_desc ? *write_desc : SceneDesc{};
+    if (table_out) {
+        // one scene: the split table's launch also zeroes the counters
+        uint32_t tb = (total + 1 + 255) / 256;
+        const uint32_t cap = (uint32_t)c->num_cus * 8;
+        if (tb > cap) tb = cap;
+        if (key64)
+            hipLaunchKernelGGL(k_prefix_table<uint64_t>, dim3(tb), dim3(256), 0, stream, (const uint64_t*)d_codes,
+                               total, pt.pshift, pt.entries, table_out, a.counters, zero_words, desc, desc_value);
+        else
+            hipLaunchKernelGGL(k_prefix_table<uint32_t>, dim3(tb), dim3(256), 0, stream, (const uint32_t*)d_codes,
+                               total, pt.pshift, pt.entries, table_out, a.counters, zero_words, desc, desc_value);
+    } else {
+        uint32_t zb = (zero_words + 255) / 256;
+        zb = zb > 64 ? 64 : zb;
+        hipLaunchKernelGGL(k_zero_counters, dim3(zb), dim3(256), 0, stream, a.counters, zero_words, desc, desc_value);
+    }
     bool grid_done = false;
     if (nscenes == 1 && top_grid) {
         // One scene: its top levels over a few workgroups (enough waves for a
@@ -837,6 +871,8 @@ int lbvh_emit_batch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint
                        clamp_threshold(threshold), d_nodes, ws, d_depth_out, false, false);
 }
 
+constexpr uint32_t kRecUnwritten = 0xFFFFFFFFu;  // error bits are 1, 2 and 4 only
+
 static void mark_build_end(sq_ctx* c) {
     if (!c->build_end) return;
     (void)hipEventRecord(c->build_end, c->stream);
@@ -860,24 +896,14 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
     char* base = c->aux[1].as<char>();
     uint32_t* d_table = (uint32_t*)(base + w.total + 256);
     pt.table = d_table;
-    {
-        uint32_t tb = (n + 1 + 255) / 256;
-        const uint32_t cap = (uint32_t)c->num_cus * 8;
-        if (tb > cap) tb = cap;
-        if (key64)
-            hipLaunchKernelGGL(k_prefix_table<uint64_t>, dim3(tb), dim3(256), 0, c->stream, (const uint64_t*)d_codes,
-                               n, pt.pshift, pt.entries, d_table);
-        else
-            hipLaunchKernelGGL(k_prefix_table<uint32_t>, dim3(tb), dim3(256), 0, c->stream, (const uint32_t*)d_codes,
-                               n, pt.pshift, pt.entries, d_table);
-    }
-    // the depth goes to a spare counter word, so that one copy brings back the
-    // counters (words 0-2), the depth (word 8) and the scene's top levels (word 16)
+    // the depth goes to a spare counter word; k_lbvh_finish writes the counters
+    // (words 0-2), the depth (word 8) and the scene's top levels (word 16) to
+    // the pinned record h, whose error word starts at a value it never takes
     uint32_t* d_depth = (uint32_t*)(base + w.counters) + 8;
-    SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit, T,
-                       d_nodes, c->aux[1].p, d_depth, true, true, &hd, pt));
     uint32_t* h = (uint32_t*)c->pinned;
-    SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 68, hipMemcpyDeviceToHost, c->stream));
+    h[2] = kRecUnwritten;
+    SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit, T,
+                       d_nodes, c->aux[1].p, d_depth, true, true, &hd, pt, d_table, h));
     mark_build_end(c);
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
     c->last.top_retries = 0;
@@ -886,14 +912,15 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
         // held CUs the grid needed): clear the nodes and run the emission again
         // with the one-workgroup top phase, which waits for nothing
         SQ_TRY(launch_fill_u32(c, d_nodes, 0u, (size_t)2 * n * 8));
+        h[2] = kRecUnwritten;
         SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit,
-                           T, d_nodes, c->aux[1].p, d_depth, false, false, &hd, pt));
-        SQ_CHECK_HIP(hipMemcpyAsync(h, base + w.counters, 68, hipMemcpyDeviceToHost, c->stream));
+                           T, d_nodes, c->aux[1].p, d_depth, false, false, &hd, pt, d_table, h));
         mark_build_end(c);
         SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
         c->last.top_retries = 1;
     }
     const uint32_t nsmall = h[0], top_levels = h[16];
+    if (h[2] == kRecUnwritten) SQ_RET(SQ_ERR_HIP, "lbvh: the emission's result record was not written");
     if (h[2] & 4u) SQ_RET(SQ_ERR_HIP, "lbvh: a top-phase barrier wait exceeded its spin limit");
     if (h[2]) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: emission workspace overflow");
     if (top_levels >= (uint32_t)kMaxLevels) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: more than %d top levels", kMaxLevels);

@@ -16,14 +16,14 @@ constexpr int kBoundsBlock = 256;
 
 // Block-reduces the six box extremes (ordered uints: min x,y,z then max
 // x,y,z). Each block writes its six values to ord6[16 + 6 * block + k]; the
-// block that takes the last ticket (ord6[8], zeroed by k_bounds_init) folds
-// all partials into ord6[0..5]. Atomics on the six result words from every
+// block that takes the last ticket (a context word that is zero between
+// launches: the last block resets it) folds all partials into ord6[0..5]. Atomics on the six result words from every
 // block serialise at L2 and cost about 50 us whatever the triangle count.
 // VEC: the triangle array is 16-byte aligned and read in chunks of 256
 // triangles per wave (below); otherwise one triangle per thread and step.
 template <bool VEC>
 __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle* __restrict__ tris, uint32_t n,
-                                                                uint32_t* __restrict__ ord6) {
+                                                                uint32_t* __restrict__ ord6, uint32_t* ticket) {
     uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
     uint32_t hi[3] = {0u, 0u, 0u};
     auto take = [&](const float* v) {
@@ -96,9 +96,10 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle
         __threadfence();
     }
     __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&ord6[8], 1u) == gridDim.x - 1 ? 1u : 0u;
+    if (threadIdx.x == 0) s_last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1u : 0u;
     __syncthreads();
     if (!s_last) return;
+    if (threadIdx.x == 0) *ticket = 0u;  // every other block has taken its ticket: ready for the next launch
     __threadfence();
     // last block: fold every block's partial. The fence after the ticket
     // (agent-scope acquire) invalidates this CU's L1, so plain loads read the
@@ -123,10 +124,6 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const sq_triangle
     }
 }
 
-__global__ void k_bounds_init(uint32_t* ord6) {
-    if (threadIdx.x == 0) ord6[8] = 0u;
-}
-
 // Two workgroups per CU keep enough loads in flight to stream the triangles
 // and leave the last block few partials to fold.
 static uint32_t bounds_blocks(const sq_ctx* c, uint32_t n) {
@@ -139,13 +136,12 @@ static uint32_t bounds_blocks(const sq_ctx* c, uint32_t n) {
 size_t scene_bounds_bytes(const sq_ctx* c, uint32_t n) { return (16 + 6 * (size_t)bounds_blocks(c, n)) * 4; }
 
 int scene_bounds(sq_ctx* c, const sq_triangle* tris, uint32_t n, uint32_t* d_ord6) {
-    hipLaunchKernelGGL(k_bounds_init, dim3(1), dim3(64), 0, c->stream, d_ord6);
     if (((uintptr_t)tris & 15u) == 0)
         hipLaunchKernelGGL(k_scene_bounds<true>, dim3(bounds_blocks(c, n)), dim3(kBoundsBlock), 0, c->stream, tris, n,
-                           d_ord6);
+                           d_ord6, c->d_bounds_ticket);
     else
         hipLaunchKernelGGL(k_scene_bounds<false>, dim3(bounds_blocks(c, n)), dim3(kBoundsBlock), 0, c->stream, tris,
-                           n, d_ord6);
+                           n, d_ord6, c->d_bounds_ticket);
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
 }
