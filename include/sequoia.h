@@ -77,7 +77,7 @@ typedef struct {
     float trace_ms;
     uint64_t node_tests, tri_tests; /* filled when the trace was asked to count work */
     uint32_t sort_passes, tree_depth, top_levels, small_tasks;
-    uint32_t top_retries; /* 1: the Morton build's many-workgroup top phase gave up waiting and ran again */
+    uint32_t top_retries; /* always 0: no build phase waits on other workgroups or retries */
 } sq_timings;
 int sq_get_timings(sq_ctx* ctx, sq_timings* out);
 /* 1: time each build phase separately from now on; 0 (the default): not. */

@@ -146,7 +146,6 @@ int sq_ctx_create(int device, sq_ctx** out) {
     SQ_CHECK_HIP(hipSetDevice(device));
     sq_ctx* c = new sq_ctx();
     c->device = device;
-    if (const char* e = std::getenv("SEQUOIA_TEST_BARRIER_SPINS")) c->barrier_spins = (uint32_t)std::strtoul(e, nullptr, 10);
     if (const char* e = std::getenv("SEQUOIA_TEST_POISON_NODES")) c->poison_nodes = std::strtoul(e, nullptr, 10) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;

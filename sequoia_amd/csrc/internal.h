@@ -84,10 +84,6 @@ struct sq_ctx {
     hipEvent_t ev_end[sq::PH_COUNT] = {};
     bool phase_used[sq::PH_COUNT] = {};
     sq_timings last = {};
-    // spins before a grid-barrier wait of the many-workgroup LBVH top phase
-    // gives up (0: the default of seconds); SEQUOIA_TEST_BARRIER_SPINS sets it
-    // at context creation, so tests can force the fallback top phase
-    uint32_t barrier_spins = 0;
     void* pinned = nullptr;  // small pinned host staging area
     uint32_t* d_bounds_ticket = nullptr;  // scene-bounds last-block ticket (zero between launches)
     size_t pinned_bytes = 0;

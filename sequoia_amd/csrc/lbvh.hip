@@ -23,8 +23,9 @@
 // (scene s: sorted range [seg_begin, seg_begin + n), nodes at node_base,
 // leaf child = triangle index - tri_base):
 //   top    one workgroup per scene splits every range larger than T level
-//          by level (one wave per range, 65-ary search on the sorted codes)
-//          and queues ranges of 8..T primitives,
+//          by level, one range per thread (the split from a table of the
+//          codes' top bits, else an 8-ary search on the sorted codes), and
+//          queues ranges of 8..T primitives,
 //   small  persistent workgroups take queued ranges of every scene: codes in
 //          LDS, the reference's binary search per thread, level by level,
 //          then refit bottom-up,
@@ -103,7 +104,7 @@ __device__ __forceinline__ void write_fan(const EmitArgs& a, uint32_t f, uint32_
 // Writes the two children of range [b, e) split after `split`; hands the
 // child ranges that still need splitting (count >= 8) to the callback.
 template <typename Push>
-__device__ void emit_children(const EmitArgs& a, uint32_t b, uint32_t e, uint32_t f, uint32_t split, Push push) {
+__device__ __forceinline__ void emit_children(const EmitArgs& a, uint32_t b, uint32_t e, uint32_t f, uint32_t split, Push push) {
     const uint32_t count = e - b;
     const uint32_t L = split + 1, R = count - L;
     const uint32_t nn = (L != 1) + (R != 1);
@@ -203,48 +204,6 @@ __global__ void k_prefix_table(const K* __restrict__ codes, uint32_t n, uint32_t
     }
 }
 
-// find_split_morton as a 65-ary search by one wave: the largest s < count-1
-// whose code shares more leading bits with the first code than the last does.
-// When the highest bit in which the range's first and last codes differ lies
-// within the table's top bits, the split is read from the table instead: it
-// is one before the first position whose code has that bit set above the
-// common prefix.
-template <typename K>
-__device__ uint32_t wave_find_split(const K* __restrict__ codes, uint32_t b, uint32_t e, int lane,
-                                    const PrefixTable& pt) {
-    const uint32_t count = e - b;
-    const K first = codes[b], last = codes[e - 1];
-    if (first == last) return count >> 1;
-    const int cp = clz_k<K>(first ^ last);
-    const int h = (int)(8 * sizeof(K)) - 1 - cp;  // highest differing bit
-    if (pt.table && h >= (int)pt.pshift) {
-        const K x = ((first >> h) | (K)1) << h;
-        const uint64_t p = (uint64_t)(x >> pt.pshift);
-        if (p < pt.entries) {
-            const uint32_t lb = pt.table[p];
-            if (lb > b && lb < e) return lb - 1 - b;
-        }
-    }
-    uint32_t lo = 0, hi = count - 1;
-    while (hi - lo > 1) {
-        uint32_t span = hi - lo;
-        uint32_t p = lo + (uint32_t)(((uint64_t)(lane + 1) * span) / 65u);
-        bool valid = p > lo && p < hi;
-        bool pred = valid && clz_k<K>(first ^ codes[b + p]) > cp;
-        uint32_t nlo = pred ? p : lo;
-        uint32_t nhi = (valid && !pred) ? p : hi;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            uint32_t x = __shfl_xor(nlo, off, 64), y = __shfl_xor(nhi, off, 64);
-            nlo = x > nlo ? x : nlo;
-            nhi = y < nhi ? y : nhi;
-        }
-        lo = nlo;
-        hi = nhi;
-    }
-    return lo;
-}
-
 // find_split_morton (builder.cpp:2281-2314), literal, on LDS codes.
 template <typename K>
 __device__ uint32_t find_split_ref(const K* codes, uint32_t count) {
@@ -279,10 +238,7 @@ struct LbvhArgs {
     uint32_t* depth_out;       // per scene: tree depth (root = 1), may be null
     uint32_t* counters;        // [0] small-task count, [1] max small-range levels, [2] error flag
     uint32_t small_cap;        // entries of `small`
-    uint32_t* topm;            // one-scene top phase over many workgroups: per-level range
-                               // counts [kMaxLevels + 2], then per-level barrier words
     PrefixTable pt;            // one-scene builds: split positions of the top bits
-    uint32_t spin_limit;       // k_lbvh_top_grid: spins before a barrier wait gives up
     uint32_t* host_rec;        // one-scene builds: pinned host record written by k_lbvh_finish
                                // (words 0-2 counters, 8 depth, 16 top levels), else null
 };
@@ -297,21 +253,76 @@ __device__ __forceinline__ void push_small(const LbvhArgs& a, const Task& t) {
     else atomicOr(&a.counters[2], 1u);
 }
 
-// One workgroup per scene: splits every range larger than T level by level.
+// find_split_morton by one thread: the table when the highest differing bit
+// lies within its top bits, else an 8-ary search (seven independent loads per
+// step). The predicate "shares more leading bits with the first code than
+// the last does" holds on a prefix of the sorted range, so the boundary the
+// search finds is the reference's binary-search result.
 template <typename K>
-__global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
-    __shared__ uint32_t s_ncur, s_nnext, s_ntop, s_cur;
+__device__ uint32_t thread_find_split(const K* __restrict__ codes, uint32_t b, uint32_t e, const PrefixTable& pt) {
+    const uint32_t count = e - b;
+    const K first = codes[b], last = codes[e - 1];
+    if (first == last) return count >> 1;
+    const int cp = clz_k<K>(first ^ last);
+    const int h = (int)(8 * sizeof(K)) - 1 - cp;
+    if (pt.table && h >= (int)pt.pshift) {
+        const K x = ((first >> h) | (K)1) << h;
+        const uint64_t p = (uint64_t)(x >> pt.pshift);
+        if (p < pt.entries) {
+            const uint32_t lb = pt.table[p];
+            if (lb > b && lb < e) return lb - 1 - b;
+        }
+    }
+    uint32_t lo = 0, hi = count - 1;  // the predicate holds at lo and fails at hi
+    while (hi - lo > 1) {
+        const uint32_t span = hi - lo;
+        uint32_t pr[7];
+        K c[7];
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            pr[j] = lo + (uint32_t)(((uint64_t)(j + 1) * span) >> 3);
+            c[j] = codes[b + pr[j]];
+        }
+        uint32_t nlo = lo, nhi = hi;
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            if (pr[j] <= lo || pr[j] >= hi) continue;
+            if (clz_k<K>(first ^ c[j]) > cp) nlo = pr[j] > nlo ? pr[j] : nlo;
+            else nhi = pr[j] < nhi ? pr[j] : nhi;
+        }
+        lo = nlo;
+        hi = nhi;
+    }
+    return lo;
+}
+
+// Top phase with one range per thread: one workgroup per scene splits the
+// ranges larger than T level by level, a level's ranges spread over the
+// workgroup's threads (up to kTopThreads at once), the queue of the next
+// level in LDS (in the scene's global queue when a level may hold more than
+// kLaneQ ranges). The small ranges a chunk of threads produces take their
+// places in the small queue with one global atomic per chunk. Slots, writes
+// and the top list (level by level, in queue order) follow the reference's
+// recursion; only the order in which a level's ranges are split differs, and
+// node slots do not depend on it.
+constexpr uint32_t kLaneQ = 2048;
+
+template <typename K>
+__global__ __launch_bounds__(kTopThreads) void k_lbvh_top_lanes(LbvhArgs a) {
+    __shared__ Task s_q[2][kLaneQ];
+    __shared__ uint32_t s_ncur, s_nnext, s_nsmall, s_small_base;
     const uint32_t scene = blockIdx.x;
     const SceneDesc d = a.scenes[scene];
     const EmitArgs e = scene_emit(a, d);
     const K* codes = (const K*)a.codes;
-    Task* q[2] = {a.qa + d.q_base, a.qb + d.q_base};
     const uint32_t qcap = d.n / a.threshold + 4;  // ranges > T in one level are disjoint
+    const bool lds_q = qcap <= kLaneQ;
+    Task* const q0 = lds_q ? s_q[0] : a.qa + d.q_base;
+    Task* const q1 = lds_q ? s_q[1] : a.qb + d.q_base;
     uint32_t* top_list = a.top_list + d.top_base;
     uint32_t* level_off = a.level_off + d.lvl_base;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = kTopThreads / 64;
     if (threadIdx.x == 0) {
-        s_ncur = 0; s_nnext = 0; s_ntop = 0; s_cur = 0;
+        s_ncur = 0; s_nnext = 0; s_nsmall = 0; s_small_base = 0;
         // the scene's unused slots: n+1 .. 2n-1 behind a root fan, else 2n-1
         for (uint32_t i = d.n < 8 ? d.n + 1 : 2 * d.n - 1; i < 2 * d.n; i++) sqd::store_zero_node(e.nodes, i);
         if (d.n < 8) {
@@ -327,163 +338,69 @@ __global__ __launch_bounds__(kTopThreads) void k_lbvh_top(LbvhArgs a) {
         } else {
             sqd::store_node_links(e.nodes, 0, 1, 2, 0);
             Task t{d.seg_begin, d.seg_begin + d.n, 1, 0, scene};
-            if (d.n > a.threshold) { q[0][0] = t; s_ncur = 1; }
+            if (d.n > a.threshold) { q0[0] = t; s_ncur = 1; }
             else push_small(a, t);
         }
     }
     __syncthreads();
-    uint32_t level = 0;
+    uint32_t level = 0, top_off = 0, cur = 0;
     while (true) {
         const uint32_t ncur = s_ncur;
-        if (ncur == 0) break;
-        Task* cur = q[s_cur];
-        Task* nxt = q[s_cur ^ 1];
-        if (threadIdx.x == 0 && level < kMaxLevels) level_off[level] = s_ntop;
-        __syncthreads();
-        for (uint32_t t = wave; t < ncur; t += nwaves) {
-            Task tk = cur[t];
-            uint32_t split = wave_find_split<K>(codes, tk.begin, tk.end, lane, a.pt);
-            if (lane == 0) {
+        if (ncur == 0 || level >= (uint32_t)kMaxLevels) break;
+        if (threadIdx.x == 0) level_off[level] = top_off;
+        for (uint32_t c0 = 0; c0 < ncur; c0 += kTopThreads) {
+            const uint32_t t = c0 + threadIdx.x;
+            Task sm0{}, sm1{};
+            uint32_t nsm = 0, li = 0;
+            if (t < ncur) {
+                const Task tk = (cur ? q1 : q0)[t];
+                const uint32_t split = thread_find_split<K>(codes, tk.begin, tk.end, a.pt);
+                Task* const nxt = cur ? q0 : q1;
                 emit_children(e, tk.begin, tk.end, tk.free, split,
                               [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
-                                  Task c{b, en, f, self, scene};
+                                  const Task c{b, en, f, self, scene};
                                   if (en - b > a.threshold) {
                                       const uint32_t k = atomicAdd(&s_nnext, 1u);
                                       if (k < qcap) nxt[k] = c;
                                       else atomicOr(&a.counters[2], 1u);
                                   } else {
-                                      push_small(a, c);
+                                      if (nsm == 0) sm0 = c; else sm1 = c;
+                                      nsm++;
                                   }
                               });
-                const uint32_t k = atomicAdd(&s_ntop, 1u);
-                if (k < d.top_cap) top_list[k] = tk.self;
+                if (top_off + t < d.top_cap) top_list[top_off + t] = tk.self;
+                else atomicOr(&a.counters[2], 1u);
+                if (nsm) li = atomicAdd(&s_nsmall, nsm);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                s_small_base = s_nsmall ? atomicAdd(&a.counters[0], s_nsmall) : 0u;
+                s_nsmall = 0;
+            }
+            __syncthreads();
+            const uint32_t k = s_small_base + li;
+            if (nsm >= 1) {
+                if (k < a.small_cap) a.small[k] = sm0;
+                else atomicOr(&a.counters[2], 1u);
+            }
+            if (nsm >= 2) {
+                if (k + 1 < a.small_cap) a.small[k + 1] = sm1;
                 else atomicOr(&a.counters[2], 1u);
             }
         }
-        __syncthreads();
+        // every push of this level is in (the chunk barriers ordered them)
         if (threadIdx.x == 0) {
             s_ncur = s_nnext;
             s_nnext = 0;
-            s_cur ^= 1;
         }
+        top_off += ncur;
+        cur ^= 1;
         level++;
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        if (level < kMaxLevels) level_off[level] = s_ntop;
+        if (level < kMaxLevels) level_off[level] = top_off;
         a.scene_levels[scene] = level;
-    }
-}
-
-// One scene's top phase spread over gridDim.x workgroups (all resident: the
-// grid is small): the ranges of a level are dealt to every wave of the grid,
-// and the workgroups meet at a barrier (an arrival counter per level) before
-// the next level. The ranges, node slots and writes are those of
-// k_lbvh_top; only the order in which ranges of a level are split differs
-// (node slots do not depend on it). A barrier wait is bounded: past
-// LbvhArgs::spin_limit spins (by default seconds: long enough to wait out
-// other streams' kernels that hold the CUs) the workgroup raises the error
-// flag (counters[2] bit 4) and stops, so it never reads a level whose tasks
-// may not all be written, and the launch always ends; the host then runs the
-// top phase again with k_lbvh_top, which needs no barrier.
-constexpr uint32_t kBarrierSpins = 1u << 26;
-
-// false: the wait gave up (the same answer for every thread of the workgroup)
-__device__ __forceinline__ bool grid_barrier(uint32_t* word, uint32_t* err, uint32_t spin_limit) {
-    __shared__ uint32_t s_ok;
-    // every wave waits for its own stores to be acknowledged (L1 writes
-    // through to this XCD's L2) before the workgroup meets
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // One agent-scope fence per workgroup: it writes this XCD's L2 back
-        // (and invalidates it) for the whole workgroup; a fence per thread
-        // would repeat that a thousand times.
-        __threadfence();
-        atomicAdd(word, 1u);
-        uint32_t spins = 0, ok = 1;
-        while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < gridDim.x) {
-            if (++spins > spin_limit) {
-                atomicOr(err, 4u);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        __threadfence();  // acquire: invalidates this CU's L1 and this XCD's L2
-        s_ok = ok;
-    }
-    __syncthreads();
-    return s_ok != 0;
-}
-
-template <typename K>
-__global__ __launch_bounds__(kTopThreads) void k_lbvh_top_grid(LbvhArgs a) {
-    const SceneDesc d = a.scenes[0];
-    const EmitArgs e = scene_emit(a, d);
-    const K* codes = (const K*)a.codes;
-    Task* q[2] = {a.qa + d.q_base, a.qb + d.q_base};
-    const uint32_t qcap = d.n / a.threshold + 4;  // ranges > T in one level are disjoint
-    uint32_t* top_list = a.top_list + d.top_base;
-    uint32_t* level_off = a.level_off + d.lvl_base;
-    uint32_t* cnt = a.topm;
-    uint32_t* arrive = a.topm + kMaxLevels + 2;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = kTopThreads / 64;
-    const uint32_t gw = blockIdx.x * nwaves + wave, total_waves = gridDim.x * nwaves;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        // the scene's unused slots: n+1 .. 2n-1 behind a root fan, else 2n-1
-        for (uint32_t i = d.n < 8 ? d.n + 1 : 2 * d.n - 1; i < 2 * d.n; i++) sqd::store_zero_node(e.nodes, i);
-        if (d.n < 8) {
-            sqd::Box3 u, lb;
-            for (uint32_t i = 0; i < d.n; i++) {
-                write_leaf(e, 1 + i, a.idx[d.seg_begin + i], &lb);
-                if (i == 0) u = lb; else sqd::box_grow(u, lb);
-            }
-            if (d.n > 0) {
-                if (a.refit) sqd::store_node(e.nodes, 0, u, 1, d.n, 0);
-                else sqd::store_node_links(e.nodes, 0, 1, d.n, 0);
-            }
-        } else {
-            sqd::store_node_links(e.nodes, 0, 1, 2, 0);
-            Task t{d.seg_begin, d.seg_begin + d.n, 1, 0, 0};
-            if (d.n > a.threshold) { q[0][0] = t; cnt[0] = 1; }
-            else push_small(a, t);
-        }
-    }
-    if (!grid_barrier(&arrive[0], &a.counters[2], a.spin_limit)) return;
-    uint32_t level = 0, off = 0;
-    while (level < (uint32_t)kMaxLevels) {
-        const uint32_t ncur = __hip_atomic_load(&cnt[level], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (ncur == 0) break;
-        if (blockIdx.x == 0 && threadIdx.x == 0) level_off[level] = off;
-        const Task* cur = q[level & 1];
-        Task* nxt = q[(level + 1) & 1];
-        for (uint32_t t = gw; t < ncur; t += total_waves) {
-            Task tk = cur[t];
-            uint32_t split = wave_find_split<K>(codes, tk.begin, tk.end, lane, a.pt);
-            if (lane == 0) {
-                emit_children(e, tk.begin, tk.end, tk.free, split,
-                              [&](uint32_t b, uint32_t en, uint32_t f, uint32_t self) {
-                                  Task c{b, en, f, self, 0};
-                                  if (en - b > a.threshold) {
-                                      const uint32_t k = atomicAdd(&cnt[level + 1], 1u);
-                                      if (k < qcap) nxt[k] = c;
-                                      else atomicOr(&a.counters[2], 1u);
-                                  } else {
-                                      push_small(a, c);
-                                  }
-                              });
-                if (off + t < d.top_cap) top_list[off + t] = tk.self;
-                else atomicOr(&a.counters[2], 1u);
-            }
-        }
-        off += ncur;
-        if (!grid_barrier(&arrive[level + 1], &a.counters[2], a.spin_limit)) return;
-        level++;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        if (level < kMaxLevels) level_off[level] = off;
-        a.scene_levels[0] = level;
     }
 }
 
@@ -749,7 +666,7 @@ void lbvh_fill_descs(SceneDesc* d, const uint32_t* counts, uint32_t nscenes, uin
 }
 
 struct WsLayout {
-    size_t qa, qb, small, top, lvl, counters, scene_levels, scene_small, topm, total;
+    size_t qa, qb, small, top, lvl, counters, scene_levels, scene_small, zero_end, total;
     size_t small_cap;
 };
 static WsLayout ws_layout(uint32_t total, uint32_t nscenes, uint32_t T) {
@@ -765,15 +682,14 @@ static WsLayout ws_layout(uint32_t total, uint32_t nscenes, uint32_t T) {
     w.small_cap = sm;
     w.top = take(tp * 4);
     w.lvl = take((size_t)nscenes * (kMaxLevels + 2) * 4);
-    // counters, scene_levels, scene_small and topm are contiguous (zeroed together)
+    // counters, scene_levels and scene_small are contiguous (zeroed together)
     w.counters = off;
     off += 64;
     w.scene_levels = off;
     off += (size_t)nscenes * 4;
     w.scene_small = off;
     off += (size_t)nscenes * 4;
-    w.topm = off;
-    off += 2 * ((size_t)kMaxLevels + 2) * 4;
+    w.zero_end = off;
     off = (off + 255) & ~(size_t)255;
     w.total = off;
     return w;
@@ -786,7 +702,7 @@ size_t lbvh_batch_workspace(uint32_t total, uint32_t nscenes, uint32_t threshold
 static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total, const void* d_codes,
                        const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
                        bool refit, uint32_t T, sq_node* d_nodes, void* ws, uint32_t* d_depth, bool split_refit_phase,
-                       bool top_grid, const SceneDesc* write_desc = nullptr, PrefixTable pt = PrefixTable{},
+                       const SceneDesc* write_desc = nullptr, PrefixTable pt = PrefixTable{},
                        uint32_t* table_out = nullptr, uint32_t* host_rec = nullptr) {
     WsLayout w = ws_layout(total, nscenes, T);
     char* base = (char*)ws;
@@ -809,11 +725,9 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     a.depth_out = d_depth;
     a.counters = (uint32_t*)(base + w.counters);
     a.small_cap = (uint32_t)w.small_cap;
-    a.topm = (uint32_t*)(base + w.topm);
     a.pt = pt;
-    a.spin_limit = c->barrier_spins ? c->barrier_spins : kBarrierSpins;
     a.host_rec = host_rec;
-    const uint32_t zero_words = (uint32_t)((w.topm - w.counters) / 4) + 2 * (kMaxLevels + 2);
+    const uint32_t zero_words = (uint32_t)((w.zero_end - w.counters) / 4);
     SceneDesc* desc = write_desc ? const_cast<SceneDesc*>(d_scenes) : nullptr;
     const SceneDesc desc_value = write_desc ? *write_desc : SceneDesc{};
     if (table_out) {
@@ -832,23 +746,8 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
         zb = zb > 64 ? 64 : zb;
         hipLaunchKernelGGL(k_zero_counters, dim3(zb), dim3(256), 0, stream, a.counters, zero_words, desc, desc_value);
     }
-    bool grid_done = false;
-    if (nscenes == 1 && top_grid) {
-        // One scene: its top levels over a few workgroups (enough waves for a
-        // level of n / T ranges, at most one workgroup per 8 CUs, so the grid
-        // always fits the device at once; work of other streams can delay a
-        // workgroup's start, which the bounded barrier waits out).
-        uint32_t wgs = (uint32_t)(total / T / (kTopThreads / 64)) + 1;
-        const uint32_t lim = (uint32_t)c->num_cus / 8 > 1 ? (uint32_t)c->num_cus / 8 : 1;
-        if (wgs > lim) wgs = lim;
-        if (key64) hipLaunchKernelGGL(k_lbvh_top_grid<uint64_t>, dim3(wgs), dim3(kTopThreads), 0, stream, a);
-        else hipLaunchKernelGGL(k_lbvh_top_grid<uint32_t>, dim3(wgs), dim3(kTopThreads), 0, stream, a);
-        grid_done = true;
-    }
-    if (!grid_done) {
-        if (key64) hipLaunchKernelGGL(k_lbvh_top<uint64_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
-        else hipLaunchKernelGGL(k_lbvh_top<uint32_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
-    }
+    if (key64) hipLaunchKernelGGL(k_lbvh_top_lanes<uint64_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
+    else hipLaunchKernelGGL(k_lbvh_top_lanes<uint32_t>, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
     uint32_t max_tasks = (uint32_t)(total / 8 + nscenes);
     uint32_t grid = (uint32_t)c->num_cus * 8;
     if (grid > max_tasks) grid = max_tasks;
@@ -865,13 +764,11 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
 int lbvh_emit_batch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, uint32_t total, const void* d_codes,
                     const uint32_t* d_idx, bool key64, const SceneDesc* d_scenes, uint32_t nscenes, uint32_t layout,
                     uint32_t threshold, sq_node* d_nodes, void* ws, uint32_t* d_depth_out) {
-    // batched and served builds may share the device with other streams'
-    // work, so they keep the one-workgroup-per-scene top phase (no barrier)
     return lbvh_launch(c, stream, tris, total, d_codes, d_idx, key64, d_scenes, nscenes, layout, true,
-                       clamp_threshold(threshold), d_nodes, ws, d_depth_out, false, false);
+                       clamp_threshold(threshold), d_nodes, ws, d_depth_out, false);
 }
 
-constexpr uint32_t kRecUnwritten = 0xFFFFFFFFu;  // error bits are 1, 2 and 4 only
+constexpr uint32_t kRecUnwritten = 0xFFFFFFFFu;  // the error word is 0 or 1
 
 static void mark_build_end(sq_ctx* c) {
     if (!c->build_end) return;
@@ -903,25 +800,12 @@ int lbvh_emit(sq_ctx* c, const sq_triangle* tris, uint32_t n, const void* d_code
     uint32_t* h = (uint32_t*)c->pinned;
     h[2] = kRecUnwritten;
     SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit, T,
-                       d_nodes, c->aux[1].p, d_depth, true, true, &hd, pt, d_table, h));
+                       d_nodes, c->aux[1].p, d_depth, true, &hd, pt, d_table, h));
     mark_build_end(c);
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
     c->last.top_retries = 0;
-    if (h[2] == 4u) {
-        // a barrier wait of the many-workgroup top phase gave up (other work
-        // held CUs the grid needed): clear the nodes and run the emission again
-        // with the one-workgroup top phase, which waits for nothing
-        SQ_TRY(launch_fill_u32(c, d_nodes, 0u, (size_t)2 * n * 8));
-        h[2] = kRecUnwritten;
-        SQ_TRY(lbvh_launch(c, c->stream, tris, n, d_codes, d_idx, key64, c->aux[2].as<SceneDesc>(), 1, layout, refit,
-                           T, d_nodes, c->aux[1].p, d_depth, false, false, &hd, pt, d_table, h));
-        mark_build_end(c);
-        SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
-        c->last.top_retries = 1;
-    }
     const uint32_t nsmall = h[0], top_levels = h[16];
     if (h[2] == kRecUnwritten) SQ_RET(SQ_ERR_HIP, "lbvh: the emission's result record was not written");
-    if (h[2] & 4u) SQ_RET(SQ_ERR_HIP, "lbvh: a top-phase barrier wait exceeded its spin limit");
     if (h[2]) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: emission workspace overflow");
     if (top_levels >= (uint32_t)kMaxLevels) SQ_RET(SQ_ERR_OVERFLOW, "lbvh: more than %d top levels", kMaxLevels);
     c->last.top_levels = top_levels;

@@ -201,32 +201,37 @@ def test_phase_timing_opt_in(ctx):
         ctx.set_phase_timing(False)
 
 
-def test_top_phase_falls_back_when_a_barrier_gives_up():
-    """With the grid barrier's spin limit cut to one spin (the test hook read
-    at context creation) the many-workgroup top phase gives up waiting; the
-    build runs the emission again with the one-workgroup top phase and the
-    tree still equals the oracle's."""
-    import os
+def test_top_phase_global_queue_and_search():
+    """The top phase's lane-per-range kernel keeps a level's ranges in LDS up
+    to 2048 of them, else in the scene's global queue, and takes a level in
+    chunks of 1024 threads; splits come from the top-bits table or, when the
+    range's codes agree on all of those bits, from the 8-ary search. A 1M soup
+    with a top-phase threshold of 64 (up to ~15k ranges a level) and a dense
+    cluster inside a wide scene (every top range within one top-bits cell)
+    both equal the oracle's trees."""
     tris = sq.gen_soup(1_000_000, 2, 0.01)
-    os.environ["SEQUOIA_TEST_BARRIER_SPINS"] = "1"
+    c = sq.Context(0)
     try:
-        c = sq.Context(0)
-    finally:
-        del os.environ["SEQUOIA_TEST_BARRIER_SPINS"]
-    try:
-        s = c.upload(tris)
-        b = c.build(s, "morton63")
-        t = c.timings()
-        say("top retries", t["top_retries"], "depth", t["tree_depth"])
-        assert t["top_retries"] == 1
-        got = b.download()
-        want, root, _, _ = ob.build_morton(tris, 21)
-        assert b.root == root and len(nodes_equal(got, want)) == 0
-        b.free()
-        b = c.build(s, "morton")  # 30-bit codes take the same path
-        want, _, _, _ = ob.build_morton(tris, 10)
-        assert len(nodes_equal(b.download(), want)) == 0
-        b.free()
-        s.free()
+        for kind, bits in (("morton63", 21), ("morton", 10)):
+            say("1M soup, threshold 64,", kind)
+            s = c.upload(tris)
+            b = c.build(s, kind, subtree_threshold=64)
+            assert c.timings()["top_levels"] > 10
+            want, root, _, _ = ob.build_morton(tris, bits)
+            assert b.root == root and len(nodes_equal(b.download(), want)) == 0
+            b.free()
+            s.free()
+        rng = np.random.default_rng(41)
+        dense = (0.5 + 1e-4 * rng.random((200_000, 9))).astype(np.float32)
+        wide = np.array([[0, 0, 0, 1e-3, 0, 0, 0, 1e-3, 0], [1, 1, 1, 1, 1, 1 - 1e-3, 1, 1, 1 - 1e-3]], np.float32)
+        clustered = np.concatenate([dense[:100_000], wide, dense[100_000:]])
+        for threshold in (0, 64):
+            say("dense cluster, threshold", threshold)
+            s = c.upload(clustered)
+            b = c.build(s, "morton63", subtree_threshold=threshold)
+            want, root, _, _ = ob.build_morton(clustered, 21)
+            assert b.root == root and len(nodes_equal(b.download(), want)) == 0
+            b.free()
+            s.free()
     finally:
         c.close()
