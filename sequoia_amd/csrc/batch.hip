@@ -136,7 +136,7 @@ int enqueue_batch_build(sq_ctx* c, hipStream_t stream, const BatchBuild& j) {
         hipLaunchKernelGGL(k_batch_morton<false>, dim3(mb), dim3(256), 0, stream, j.tris, j.d_off, j.nscenes, j.total,
                            j.bits, j.d_ord, j.keys[0], j.vals[0]);
     SQ_CHECK_HIP(hipGetLastError());
-    SortBufs sb{stream, {j.keys[0], j.keys[1]}, {j.vals[0], j.vals[1]}, j.hist};
+    SortBufs sb{stream, {j.keys[0], j.keys[1]}, {j.vals[0], j.vals[1]}, j.hist, j.os, j.os_err, c->num_cus};
     int slot = 0;
     uint32_t passes = 0;
     SQ_TRY(radix_sort_bufs(sb, j.total, 3 * j.bits, j.digit, key64, j.d_off, j.nscenes, 0, &slot, &passes));
@@ -170,6 +170,9 @@ struct sq_batch {
     uint32_t* d_vals[2] = {nullptr, nullptr};
     uint32_t* d_hist = nullptr;
     size_t hist_bytes = 0;
+    uint32_t* d_os = nullptr;        // single-pass sort status area (sort_onesweep_bytes)
+    size_t os_bytes = 0;
+    uint32_t* h_sort_err = nullptr;  // pinned: set by a failed look-back wait of the last build
     void* d_ws = nullptr;
     size_t ws_bytes = 0;
     uint32_t* d_depth = nullptr;
@@ -336,6 +339,22 @@ int configure_build_kind(sq_batch* b, sq_builder kind, const sq_build_params* p)
             SQ_TRY(balloc_t(b, hb / 4 + 1, &b->d_hist));
             b->hist_bytes = hb;
         }
+        // key passes as single-pass kernels while prefix counts fit their 30-bit status words
+        if (b->total < (1u << 30)) {
+            const size_t ob = sort_onesweep_bytes(b->total, 3 * bits, digit);
+            if (ob > b->os_bytes) {
+                bfree(b, b->d_os);
+                b->d_os = nullptr;
+                SQ_TRY(balloc_t(b, ob / 4 + 1, &b->d_os));
+                b->os_bytes = ob;
+            }
+            if (!b->h_sort_err) {
+                void* hp = nullptr;
+                SQ_CHECK_HIP(hipHostMalloc(&hp, 64, hipHostMallocDefault));
+                b->h_sort_err = (uint32_t*)hp;
+                *b->h_sort_err = 0;
+            }
+        }
     }
     const size_t kb = bits > 10 ? 8 : 4;
     if (!b->d_keys[0]) {
@@ -375,6 +394,10 @@ int enqueue_build(sq_ctx* c, sq_batch* b) {
     j.digit = b->digit;
     j.layout = b->layout;
     j.threshold = b->threshold;
+    if (b->d_os && b->os_bytes >= sort_onesweep_bytes(b->total, 3 * b->bits, b->digit)) {
+        j.os = b->d_os;
+        j.os_err = b->h_sort_err;
+    }
     return enqueue_batch_build(c, c->stream, j);
 }
 
@@ -439,6 +462,10 @@ int finish_depths(sq_ctx* c, sq_batch* b) {
     std::vector<uint32_t> depth(b->nscenes);
     SQ_CHECK_HIP(hipMemcpyAsync(depth.data(), b->d_depth, 4ull * b->nscenes, hipMemcpyDeviceToHost, c->stream));
     SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+    if (b->h_sort_err && *b->h_sort_err) {
+        *b->h_sort_err = 0;
+        SQ_RET(SQ_ERR_HIP, "batch build: a radix-sort look-back wait exceeded its spin limit");
+    }
     for (uint32_t s = 0; s < b->nscenes; s++) b->bvhs[s].depth = depth[s];
     for (uint32_t s = 0; s < b->nscenes; s++)
         if (depth[s] == 0) SQ_RET(SQ_ERR_OVERFLOW, "batch build: emission workspace overflow (scene %u)", s);
@@ -554,6 +581,7 @@ void sq_batch_free(sq_batch* b) {
     for (void* p : b->allocs) (void)hipFree(p);
     for (auto& ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
+    if (b->h_sort_err) (void)hipHostFree(b->h_sort_err);
     delete b;
 }
 

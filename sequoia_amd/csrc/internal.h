@@ -207,6 +207,10 @@ struct BatchBuild {
     uint32_t* d_depth;
     sq_node* nodes;
     uint32_t bits, digit, layout, threshold;
+    // non-null: the key passes run as single-pass look-back kernels in this
+    // area (sort_onesweep_bytes), a failed wait sets *os_err (pinned host word)
+    uint32_t* os = nullptr;
+    uint32_t* os_err = nullptr;
 };
 int enqueue_batch_build(sq_ctx* c, hipStream_t stream, const BatchBuild& j);
 // Upper bound of a Morton tree's depth: at most 3*bits splits on a code bit,
