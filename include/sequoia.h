@@ -124,7 +124,7 @@ typedef struct {
     uint32_t grid_dim;             /* 0 = 4 (reference GRID_DIM); power of two <= 8 */
     uint32_t layout;               /* SQ_LAYOUT_* */
     uint32_t no_refit;             /* Morton builders: 1 = leave node boxes zero like the reference */
-    uint32_t subtree_threshold;    /* 0 = default; largest range one workgroup finishes alone */
+    uint32_t subtree_threshold;    /* 0 = default (1024); largest range one workgroup finishes alone (8..2048) */
     uint32_t reserved;
 } sq_build_params;
 

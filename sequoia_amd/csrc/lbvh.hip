@@ -37,7 +37,7 @@ namespace sq {
 
 constexpr int kTopThreads = 1024;
 constexpr int kSmallThreads = 512;
-constexpr uint32_t kDefaultThreshold = 2048;
+constexpr uint32_t kDefaultThreshold = 1024;  // 2048 takes half the LDS occupancy (see k_lbvh_small)
 constexpr uint32_t kMaxThreshold = 2048;
 constexpr int kMaxLevels = 160;
 
@@ -417,7 +417,6 @@ struct SmallTask {
 // 2(c-1) slots from its root's `free`): a leaf (sorted position), a fan node
 // (first child slot, count), or a split node (written from the task list).
 constexpr uint32_t kSlotLeaf = 1u << 31, kSlotFan = 1u << 30, kSlotSplit = 1u << 29;
-constexpr int kSmallSlots = 2 * kMaxThreshold;
 
 // Persistent: one workgroup finishes one queued range at a time, entirely in
 // LDS until its nodes are written:
@@ -437,14 +436,15 @@ constexpr int kSmallSlots = 2 * kMaxThreshold;
 // so T entries always suffice; the level count is at most the number of
 // code bits plus log2(T). Both are still checked, and a violation raises the
 // launch's error flag instead of writing out of bounds.
-template <typename K>
-__global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
+template <typename K, uint32_t TM, int NT>
+__global__ __launch_bounds__(NT) void k_lbvh_small(LbvhArgs a) {
+    constexpr int kSlots = 2 * TM;
     // the codes (emission) and the slot boxes (phases 2-3) share one area
-    constexpr size_t kCodeBytes = sizeof(K) * kMaxThreshold, kBoxBytes = sizeof(sqd::Box3) * kSmallSlots;
+    constexpr size_t kCodeBytes = sizeof(K) * TM, kBoxBytes = sizeof(sqd::Box3) * kSlots;
     __shared__ __align__(16) unsigned char s_area[kCodeBytes > kBoxBytes ? kCodeBytes : kBoxBytes];
-    __shared__ uint32_t s_slot[kSmallSlots];
-    __shared__ uint16_t s_leafslot[kMaxThreshold];  // sorted position -> its leaf's slot (range-relative)
-    __shared__ SmallTask s_tasks[kMaxThreshold];
+    __shared__ uint32_t s_slot[kSlots];
+    __shared__ uint16_t s_leafslot[TM];  // sorted position -> its leaf's slot (range-relative)
+    __shared__ SmallTask s_tasks[TM];
     __shared__ uint32_t s_lvl[kMaxLevels + 1];
     __shared__ uint32_t s_total, s_err;
     K* s_codes = (K*)s_area;
@@ -457,27 +457,27 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
         const SceneDesc d = a.scenes[root.scene];
         const EmitArgs e = scene_emit(a, d);
         const uint32_t base = root.begin, count = root.end - root.begin;
-        if (count > kMaxThreshold || count < 8) {  // never queued so; flag and skip
+        if (count > TM || count < 8) {  // never queued so; flag and skip
             if (threadIdx.x == 0) atomicOr(&a.counters[2], 1u);
             continue;
         }
         const uint32_t sbase = root.free, nslots = 2 * (count - 1);
-        // this thread's leaves are positions threadIdx.x + j * kSmallThreads: their
+        // this thread's leaves are positions threadIdx.x + j * NT: their
         // codes and triangle ids are loaded together (branch-free: positions past
         // the range load the range's last entry)
-        constexpr int kLeafPer = (kMaxThreshold + kSmallThreads - 1) / kSmallThreads;
+        constexpr int kLeafPer = (TM + NT - 1) / NT;
         uint32_t lt[kLeafPer];
         K ck[kLeafPer];
 #pragma unroll
         for (int j = 0; j < kLeafPer; j++) {
-            const uint32_t i = threadIdx.x + j * kSmallThreads, ic = i < count ? i : count - 1;
+            const uint32_t i = threadIdx.x + j * NT, ic = i < count ? i : count - 1;
             ck[j] = codes[base + ic];
             lt[j] = a.idx[base + ic];
         }
 #pragma unroll
         for (int j = 0; j < kLeafPer; j++)
-            if (threadIdx.x + j * kSmallThreads < count) s_codes[threadIdx.x + j * kSmallThreads] = ck[j];
-        for (uint32_t i = threadIdx.x; i < nslots; i += kSmallThreads) s_slot[i] = 0;
+            if (threadIdx.x + j * NT < count) s_codes[threadIdx.x + j * NT] = ck[j];
+        for (uint32_t i = threadIdx.x; i < nslots; i += NT) s_slot[i] = 0;
         if (threadIdx.x == 0) {
             s_tasks[0] = SmallTask{count << 16, root.free, root.self};
             s_total = 1;
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
                 if (nlev < (uint32_t)kMaxLevels) s_lvl[nlev] = lb;
                 else s_err = 1;
             }
-            for (uint32_t i = lb + threadIdx.x; i < le; i += kSmallThreads) {
+            for (uint32_t i = lb + threadIdx.x; i < le; i += NT) {
                 const SmallTask tk = s_tasks[i];
                 const uint32_t tb = tk.be & 0xFFFFu, te = tk.be >> 16, f = tk.free;
                 const uint32_t cnt = te - tb, L = find_split_ref<K>(s_codes + tb, cnt) + 1, R = cnt - L;
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
                         if (cn > 1) s_slot[cs] = kSlotFan | (cn << 16) | cf;
                     } else {
                         const uint32_t k = atomicAdd(&s_total, 1u);
-                        if (k < kMaxThreshold) s_tasks[k] = SmallTask{cb | ((cb + cn) << 16), side_free[sd], side_slot[sd]};
+                        if (k < TM) s_tasks[k] = SmallTask{cb | ((cb + cn) << 16), side_free[sd], side_slot[sd]};
                         else s_err = 1;
                         if (cs < nslots) s_slot[cs] = kSlotSplit;
                     }
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
             }
             sqd::lds_barrier();
             lb = le;
-            le = s_total < kMaxThreshold ? s_total : kMaxThreshold;
+            le = s_total < TM ? s_total : TM;
             nlev++;
             const bool stop = s_err != 0;
             sqd::lds_barrier();
@@ -548,7 +548,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
         }
 #pragma unroll
         for (int j = 0; j < kLeafPer; j++) {
-            const uint32_t i = threadIdx.x + j * kSmallThreads;
+            const uint32_t i = threadIdx.x + j * NT;
             if (i >= count) break;
             const uint32_t sl = s_leafslot[i], slot = sbase + sl;
             if (a.refit) {
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
         }
         sqd::lds_barrier();
         // 3. fans, then split nodes deepest first: unions of child slots in slot order
-        for (uint32_t sl = threadIdx.x; sl < nslots; sl += kSmallThreads) {
+        for (uint32_t sl = threadIdx.x; sl < nslots; sl += NT) {
             const uint32_t tag = s_slot[sl];
             if (tag == 0 && sbase + sl < e.cap) sqd::store_zero_node(e.nodes, sbase + sl);  // unused slot
             if (!(tag & kSlotFan)) continue;
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(kSmallThreads) void k_lbvh_small(LbvhArgs a) {
         }
         sqd::lds_barrier();
         for (int lv = (int)nlev - 1; lv >= 0; lv--) {
-            for (uint32_t i = s_lvl[lv] + threadIdx.x; i < s_lvl[lv + 1]; i += kSmallThreads) {
+            for (uint32_t i = s_lvl[lv] + threadIdx.x; i < s_lvl[lv + 1]; i += NT) {
                 const SmallTask tk = s_tasks[i];
                 const uint32_t c0 = tk.free - sbase;
                 if (c0 + 1 >= nslots || (i > 0 && tk.self - sbase >= nslots)) continue;  // never queued so
@@ -752,8 +752,15 @@ static int lbvh_launch(sq_ctx* c, hipStream_t stream, const sq_triangle* tris, u
     uint32_t grid = (uint32_t)c->num_cus * 8;
     if (grid > max_tasks) grid = max_tasks;
     if (grid == 0) grid = 1;
-    if (key64) hipLaunchKernelGGL(k_lbvh_small<uint64_t>, dim3(grid), dim3(kSmallThreads), 0, stream, a);
-    else hipLaunchKernelGGL(k_lbvh_small<uint32_t>, dim3(grid), dim3(kSmallThreads), 0, stream, a);
+    // LDS sized for the threshold: up to 1024 primitives per range, two
+    // workgroups fit a CU
+    if (T <= 1024) {
+        if (key64) hipLaunchKernelGGL((k_lbvh_small<uint64_t, 1024, kSmallThreads>), dim3(grid), dim3(kSmallThreads), 0, stream, a);
+        else hipLaunchKernelGGL((k_lbvh_small<uint32_t, 1024, kSmallThreads>), dim3(grid), dim3(kSmallThreads), 0, stream, a);
+    } else {
+        if (key64) hipLaunchKernelGGL((k_lbvh_small<uint64_t, kMaxThreshold, kSmallThreads>), dim3(grid), dim3(kSmallThreads), 0, stream, a);
+        else hipLaunchKernelGGL((k_lbvh_small<uint32_t, kMaxThreshold, kSmallThreads>), dim3(grid), dim3(kSmallThreads), 0, stream, a);
+    }
     if (refit && split_refit_phase) { phase_end(c, PH_EMIT); phase_begin(c, PH_REFIT); }
     hipLaunchKernelGGL(k_lbvh_finish, dim3(nscenes), dim3(kTopThreads), 0, stream, a);
     if (refit && split_refit_phase) phase_end(c, PH_REFIT);  // the emit phase ended before the refit
