@@ -537,7 +537,7 @@ def batched_section(sq, ctx, steps, warmup):
 
 
 SERVED_REQUESTS = 240
-SERVED_RATES = (100.0, 300.0, 1000.0)
+SERVED_RATES = (100.0, 300.0, 1000.0, 10000.0)
 
 
 def served_workload(sq):

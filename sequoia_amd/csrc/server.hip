@@ -660,6 +660,7 @@ int sq_server_create(int device, const sq_server_config* cfg, sq_server** out) {
     lim.max_tris = c.max_wave_tris;
     lim.cost_ratio = 2;
     lim.min_wave_cost = 2u << 20;
+    lim.group_backlog = 32;
     s->adm = std::make_unique<Admission>(c.budget_bytes, lim);
     sq_trace_params tp{};
     tp.semantics = c.semantics;

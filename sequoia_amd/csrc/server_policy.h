@@ -146,6 +146,11 @@ struct WaveLimits {
     // wave and a large one runs in a wave of its own.
     uint32_t cost_ratio = 0;
     uint64_t min_wave_cost = 0;
+    // Size grouping applies while at most this many requests are queued
+    // (0: always). Past it the queue is a backlog, and waves take every
+    // request that fits the limits: fewer, larger waves raise throughput when
+    // latency is already set by the queue.
+    uint32_t group_backlog = 0;
 };
 
 inline uint64_t request_cost(const PolicyRequest& r) { return r.rays + (r.build ? 2 * r.tris : 0); }
@@ -190,6 +195,7 @@ class Admission {
         std::vector<Decision> out;
         uint32_t nreq = 0;
         uint64_t rays = 0, tris = 0, bytes = 0, cost = 0, cap = 0;
+        const bool group = lim_.cost_ratio && (lim_.group_backlog == 0 || pending.size() <= lim_.group_backlog);
         for (auto it = pending.begin(); it != pending.end();) {
             PolicyRequest& r = *it;
             if (r.cancelled) { out.push_back({r.id, Verdict::Cancelled}); it = pending.erase(it); continue; }
@@ -216,7 +222,7 @@ class Admission {
             }
             if (nreq >= lim_.max_requests || (nreq && (rays + r.rays > lim_.max_rays || tris + r.tris > lim_.max_tris)))
                 break;
-            if (lim_.cost_ratio && nreq && cost + request_cost(r) > cap) {
+            if (group && nreq && cost + request_cost(r) > cap) {
                 ++it;  // another size class: a later wave
                 continue;
             }

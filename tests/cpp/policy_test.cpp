@@ -204,6 +204,25 @@ static void test_size_grouping() {
     q2.push_back(build(2, 2, 100, 100, 1000000, 500000));
     d = all.form_wave(q2, 0, 1, &ev);
     CHECK(d.size() == 2 && q2.empty());
+    // a backlog longer than group_backlog: grouping is off for that wave
+    WaveLimits lb = lim;
+    lb.group_backlog = 3;
+    Admission busy(1 << 30, lb);
+    std::deque<PolicyRequest> q3;
+    q3.push_back(build(1, 1, 100, 100, 1000, 2000));
+    q3.push_back(build(2, 2, 100, 100, 1000000, 500000));
+    q3.push_back(build(3, 3, 100, 100, 1000, 3000));
+    q3.push_back(build(4, 4, 100, 100, 1000, 3000));
+    d = busy.form_wave(q3, 0, 1, &ev);
+    CHECK(d.size() == 4 && q3.empty());
+    // at or below it, grouping holds
+    Admission calm(1 << 30, lb);
+    std::deque<PolicyRequest> q4;
+    q4.push_back(build(1, 1, 100, 100, 1000, 2000));
+    q4.push_back(build(2, 2, 100, 100, 1000000, 500000));
+    q4.push_back(build(3, 3, 100, 100, 1000, 3000));
+    d = calm.form_wave(q4, 0, 1, &ev);
+    CHECK(d.size() == 2 && d[0].id == 1 && d[1].id == 3);
 }
 
 int main() {
