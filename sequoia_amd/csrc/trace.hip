@@ -676,8 +676,8 @@ constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode (fewer for
 // work at different times.
 template <bool CLOSEST, int FMT, int WL>
 __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, uint32_t lane, Stack& st,
-                                           uint32_t* s_cnt, uint32_t* s_stage, uint32_t& tests_total,
-                                           uint32_t& tris_total, uint32_t& faults, uint32_t* job_fail) {
+                                           uint32_t* s_cnt, uint32_t& tests_total, uint32_t& tris_total,
+                                           uint32_t& faults, uint32_t* job_fail) {
     uint32_t tx = 0, ty = 0, tile_rays;
     const uint32_t ts = a.tile_shift ? a.tile_shift : 3u;  // tile edge 1 << ts pixels
     const uint32_t tpix = 1u << (2 * ts);                  // pixels per tile (at most 64)
@@ -693,8 +693,6 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
         size_t base = (size_t)tile * rt;
         tile_rays = (uint32_t)((a.nrays - base) < rt ? (a.nrays - base) : rt);
     }
-    // a camera tile's hit ids and t go through LDS and leave in memory order
-    const bool stage = s_stage && a.camera && tile_rays <= kStageRays && (a.hit_id || a.hit_t);
     __builtin_amdgcn_wave_barrier();
     uint32_t next = 0;
     bool active = false;
@@ -756,34 +754,11 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
                     ri = (size_t)tile * (a.ray_tile ? a.ray_tile : kRayTile) + my_ray;
                     if (a.counts) a.counts[ri] = L.tests;
                 }
-                if (stage) {
-                    s_stage[my_ray] = L.best_id;
-                    s_stage[kStageRays + my_ray] = __float_as_uint(L.best_t);
-                } else {
-                    if (a.hit_id) a.hit_id[ri] = L.best_id;
-                    if (a.hit_t) a.hit_t[ri] = L.best_t;
-                }
+                if (a.hit_id) a.hit_id[ri] = L.best_id;
+                if (a.hit_t) a.hit_t[ri] = L.best_t;
                 active = false;
             }
         }
-    }
-    if (stage) {
-        // tile ray r = pixel-in-tile + sample * tpix; memory order is (row,
-        // column, sample), so q walks that order and each row's rays are one
-        // contiguous run
-        sqd::wave_lds_sync();
-        for (uint32_t q = lane; q < tile_rays; q += 64) {
-            const uint32_t pix = q / a.spp, smp = q - pix * a.spp;
-            const uint32_t pxl = pix & ((1u << ts) - 1u), pyl = pix >> ts;
-            const uint32_t px = (tx << ts) + pxl, py = (ty << ts) + pyl;
-            if (px < a.w && py < a.h) {
-                const uint32_t r = pix + smp * tpix;
-                const size_t ri = ((size_t)py * a.w + px) * a.spp + smp;
-                if (a.hit_id) a.hit_id[ri] = s_stage[r];
-                if (a.hit_t) a.hit_t[ri] = __uint_as_float(s_stage[kStageRays + r]);
-            }
-        }
-        sqd::wave_lds_sync();
     }
     if (a.camera) {
         __builtin_amdgcn_wave_barrier();
@@ -815,11 +790,6 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
     st.lds = s_dyn + threadIdx.x;
     st.lds_t = CLOSEST ? (float*)(s_dyn + a.slots * blockDim.x) + threadIdx.x : nullptr;
     uint32_t* s_cnt = s_dyn + a.slots * blockDim.x * (CLOSEST ? 2 : 1) + wave * 64;
-    // (closest hits over Node2 trees keep direct stores: the staging's
-    // registers cost that kernel more than the stores it saves)
-    uint32_t* s_stage = (a.stage && !(FMT == 2 && CLOSEST))
-                            ? s_dyn + a.slots * blockDim.x * (CLOSEST ? 2 : 1) + blockDim.x + wave * (2 * kStageRays)
-                            : nullptr;
     const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     st.cap = a.spill_cap;
     st.spill = a.spill + gtid * (size_t)a.spill_cap * (CLOSEST ? 2 : 1);
@@ -850,10 +820,10 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
                 else hi = mid;
             }
             const TraceArgs J = jobs[lo];
-            trace_tile<CLOSEST, FMT, WL>(J, tile - jt[lo], lane, st, s_cnt, s_stage, tests_total, tris_total, faults,
+            trace_tile<CLOSEST, FMT, WL>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, faults,
                                      a.job_fail ? a.job_fail + lo : nullptr);
         } else {
-            trace_tile<CLOSEST, FMT, WL>(a, tile, lane, st, s_cnt, s_stage, tests_total, tris_total, faults, nullptr);
+            trace_tile<CLOSEST, FMT, WL>(a, tile, lane, st, s_cnt, tests_total, tris_total, faults, nullptr);
         }
     }
     if (faults) atomicOr(&a.work[1], faults);
@@ -905,32 +875,26 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     L.slots = pow2 > 64 ? 64 : pow2;
     L.block = p && p->block_size ? p->block_size : 64;
     if (L.block != 64 && L.block != 128) SQ_RET(SQ_ERR_INVALID, "trace block size %u (64 or 128)", L.block);
-    // LDS per thread: stack slots (and their distances for closest hits), one
-    // count word, and 2 x kStageRays / 64 words of output staging per wave
-    const size_t lds_plain = ((size_t)L.slots * (L.closest ? 2 : 1) + 1) * L.block * 4;
-    const size_t lds_staged = lds_plain + (size_t)(2 * kStageRays / 64) * L.block * 4;
-    auto occupancy = [&](size_t lds) {
-        int n = 0;
-        if (node2 && L.closest && L.lists == 1)
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<true, false, 2, 1>, L.block, lds);
-        else if (node2 && L.closest)
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<true, false, 2>, L.block, lds);
-        else if (L.closest && L.lists == 1)
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<true, false, 1, 1>, L.block, lds);
-        else if (node2 && L.lists == kWideLists)
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<false, false, 2, kWideLists>, L.block, lds);
-        else if (node2)
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<false, false, 2>, L.block, lds);
-        else if (L.closest)
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<true, false>, L.block, lds);
-        else if (L.lists == kWideLists)
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<false, false, 1, kWideLists>, L.block, lds);
-        else
-            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace<false, false>, L.block, lds);
-        return n;
-    };
-    int per_cu = occupancy(lds_plain);
-    const int per_cu_staged = occupancy(lds_staged);
+    L.lds_bytes = ((size_t)L.slots * (L.closest ? 2 : 1) + 1) * L.block * 4;
+    int per_cu = 0;
+    if (node2 && L.closest && L.lists == 1)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 2, 1>, L.block, L.lds_bytes);
+    else if (node2 && L.closest)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 2>, L.block, L.lds_bytes);
+    else if (L.closest && L.lists == 1)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 1, 1>, L.block, L.lds_bytes);
+    else if (node2 && L.lists == kWideLists)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 2, kWideLists>, L.block,
+                                                           L.lds_bytes);
+    else if (node2)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 2>, L.block, L.lds_bytes);
+    else if (L.closest)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false>, L.block, L.lds_bytes);
+    else if (L.lists == kWideLists)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false, 1, kWideLists>, L.block,
+                                                           L.lds_bytes);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<false, false>, L.block, L.lds_bytes);
     if (per_cu < 1) per_cu = 8;
     // Default waves per CU, measured on 1080p x 4 jittered over the 1M soup
     // (Morton63 tree; 4x4-pixel tiles): exhaustive Node1 steps run best at 12
@@ -953,9 +917,6 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
         if (p->blocks_per_cu > 64) SQ_RET(SQ_ERR_INVALID, "trace blocks_per_cu %u (1..64)", p->blocks_per_cu);
         per_cu = (int)p->blocks_per_cu;
     }
-    // output staging only where it leaves the launch's workgroups per CU as they are
-    L.stage = per_cu_staged >= per_cu && !(node2 && L.closest);
-    L.lds_bytes = L.stage ? lds_staged : lds_plain;
     L.grid = (uint32_t)c->num_cus * (uint32_t)per_cu;
     if (depth == 0) depth = 64;
     // stack entries a ray can need: WL x depth + 2 WL for the exhaustive step,
@@ -967,9 +928,7 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     return SQ_OK;
 }
 
-int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& args, bool multi) {
-    TraceArgs a = args;
-    a.stage = L.stage ? 1u : 0u;
+int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     if (multi) {
         if (L.closest && L.lists == 1)
@@ -992,9 +951,7 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
     return SQ_OK;
 }
 
-int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& args) {
-    TraceArgs a = args;
-    a.stage = L.stage ? 1u : 0u;
+int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     if (L.closest && L.lists == 1)
         hipLaunchKernelGGL((k_trace<true, false, 2, 1>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);

@@ -44,12 +44,7 @@ struct TraceArgs {
     float rb[6];
     uint32_t root_kind, root_count;
     const sq_plucker* plk;  // non-null: precomputed triangle test
-    // launch-level: camera tiles of at most kStageRays rays keep their hit ids
-    // and t in LDS and store them once per tile in memory order (the launch's
-    // LDS holds 2 x kStageRays words per wave past the stacks and counts)
-    uint32_t stage;
 };
-constexpr uint32_t kStageRays = 256;
 
 // Camera tile geometry of a job. `quad`: one camera traced by the per-ray
 // kernel with exhaustive semantics, which takes 4x4-pixel tiles once a tile
@@ -273,7 +268,6 @@ int trace_rays_device(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ra
 // Launch geometry and stack sizing of the traversal kernel.
 struct TracePlan {
     bool closest;
-    bool stage;      // per-tile output staging in LDS (when it costs no occupancy)
     uint32_t lists;  // child lists per exhaustive step (k_trace's WL)
     uint32_t slots, block, grid, spill_cap;
     size_t lds_bytes, spill_words;
