@@ -443,7 +443,9 @@ int enqueue_trace(sq_ctx* c, sq_batch* b, const TracePlan& L) {
     a.njobs = b->nscenes;
     a.job_fail = b->d_job_fail;
     SQ_TRY(launch_fill_u32_on(c->stream, c->num_cus, b->d_job_fail, 0u, b->nscenes));
-    return trace_launch_plan(c->stream, L, a, true);
+    bool narrow = !c->wide_fetch;
+    for (const TraceArgs& j : b->jobs) narrow = narrow && narrow_ok(j.node_count, j.tri_count);
+    return trace_launch_plan(c->stream, L, a, true, narrow);
 }
 
 int ensure_spill(sq_batch* b, const TracePlan& L) {

@@ -147,6 +147,7 @@ int sq_ctx_create(int device, sq_ctx** out) {
     sq_ctx* c = new sq_ctx();
     c->device = device;
     if (const char* e = std::getenv("SEQUOIA_TEST_POISON_NODES")) c->poison_nodes = std::strtoul(e, nullptr, 10) != 0;
+    if (const char* e = std::getenv("SEQUOIA_TEST_WIDE_FETCH")) c->wide_fetch = std::strtoul(e, nullptr, 10) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);

@@ -97,6 +97,10 @@ struct sq_ctx {
     // Morton builds that skip the zero fill are first filled with all-ones
     // words, so a slot the emission fails to write shows up in the tree.
     bool poison_nodes = false;
+    // Test hook (SEQUOIA_TEST_WIDE_FETCH at creation): traversal launches
+    // keep 64-bit fetch addressing even when every array is small enough for
+    // 32-bit offsets, so tests can compare the two forms.
+    bool wide_fetch = false;
     uint32_t* h_sort_err = nullptr;  // pinned: the last single-pass sort's error word
     bool sort_err_pending = false;
 };

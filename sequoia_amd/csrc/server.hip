@@ -462,7 +462,9 @@ int launch_wave(sq_server* s, Slot& sl, const std::vector<uint64_t>& ids, uint64
     a.njobs = (uint32_t)rq.size();
     a.job_fail = (uint32_t*)(outd + o_fail);
     SQ_TRY(launch_fill_u32_on(st, s->ctx->num_cus, a.job_fail, 0u, rq.size()));
-    SQ_TRY(trace_launch_plan(st, s->plan, a, true));
+    bool narrow = !s->ctx->wide_fetch;
+    for (const TraceArgs& j : jobs) narrow = narrow && narrow_ok(j.node_count, j.tri_count);
+    SQ_TRY(trace_launch_plan(st, s->plan, a, true, narrow));
     SQ_CHECK_HIP(hipMemcpyAsync(sl.pin_out, outd, out_bytes, hipMemcpyDeviceToHost, st));
     SQ_CHECK_HIP(hipEventRecord(sl.ev_done, st));
     guard.armed = false;

@@ -51,7 +51,8 @@ __device__ __forceinline__ void lane_begin(LaneRay& L, const RayF& r, uint32_t r
     st.sp = 0;
 }
 
-__device__ __forceinline__ void leaf_tris(const TraceArgs& a, const NodeV& n, LaneRay& L, uint32_t& tri_tests,
+template <class TA>
+__device__ __forceinline__ void leaf_tris(const TA& a, const NodeV& n, LaneRay& L, uint32_t& tri_tests,
                                           bool& bad, uint32_t j0 = 0) {
     for (uint32_t j = j0; j < n.count; j++) {
         const uint32_t tid = n.child + j;
@@ -63,7 +64,7 @@ __device__ __forceinline__ void leaf_tris(const TraceArgs& a, const NodeV& n, La
         if (a.plk) {
             m = plucker_test(a.plk, tid, L.r, t);
         } else {
-            sqd::load_tri(a.tris, tid, v);
+            fetch_tri(a, tid, v);
             m = tri_test(v, L.r, t);
         }
         if (m && t > L.r.tmin && t < L.r.tmax && (t < L.best_t || (t == L.best_t && tid < L.best_id))) {
@@ -80,8 +81,8 @@ __device__ __forceinline__ void leaf_tris(const TraceArgs& a, const NodeV& n, La
 // every hit leaf is fetched before any is tested, so the fetches are in flight
 // together. The reference's leaves hold one triangle; the rest of a longer
 // leaf is tested after the others.
-template <int N>
-__device__ __forceinline__ void leaf_batch(const TraceArgs& a, const NodeV* nd, const bool* hit, LaneRay& L,
+template <int N, class TA>
+__device__ __forceinline__ void leaf_batch(const TA& a, const NodeV* nd, const bool* hit, LaneRay& L,
                                            uint32_t& tri_tests, bool& bad) {
     if (a.plk) {
 #pragma unroll
@@ -112,7 +113,7 @@ __device__ __forceinline__ void leaf_batch(const TraceArgs& a, const NodeV* nd, 
     float tv[N][9];
 #pragma unroll
     for (int k = 0; k < N; k++)
-        if ((uint32_t)k < c) sqd::load_tri(a.tris, ids[k], tv[k]);
+        if ((uint32_t)k < c) fetch_tri(a, ids[k], tv[k]);
 #pragma unroll
     for (int k = 0; k < N; k++)
         if ((uint32_t)k < c) {
@@ -134,8 +135,8 @@ __device__ __forceinline__ void leaf_batch(const TraceArgs& a, const NodeV* nd, 
 
 // One step: pop a list if the current one is exhausted, then test up to two
 // of its nodes. Returns false when the ray is finished.
-template <bool CLOSEST>
-__device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+template <bool CLOSEST, class TA>
+__device__ __forceinline__ bool lane_step(const TA& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
     while (L.cnt == 0) {
         if (st.sp == 0) return false;
@@ -147,7 +148,7 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
             L.cnt = (e & 15u) + 1;
         } else {
             if ((e >> 4) >= a.node_count) { st.fault |= kFaultNode; return false; }
-            NodeV p = load_node(a.nodes, e >> 4);
+            NodeV p = fetch_node(a, e >> 4);
             L.first = p.child;
             L.cnt = p.count;
         }
@@ -155,8 +156,8 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
     const uint32_t i0 = L.first;
     const bool two = L.cnt >= 2;
     if (i0 >= a.node_count || (two && i0 + 1 >= a.node_count)) { st.fault |= kFaultNode; return false; }
-    const NodeV n0 = load_node(a.nodes, i0);
-    const NodeV n1 = load_node(a.nodes, i0 + (two ? 1u : 0u));  // unconditional: no branch between fetches
+    const NodeV n0 = fetch_node(a, i0);
+    const NodeV n1 = fetch_node(a, i0 + (two ? 1u : 0u));  // unconditional: no branch between fetches
     L.first += two ? 2 : 1;
     L.cnt -= two ? 2 : 1;
     L.tests += two ? 2 : 1;
@@ -210,8 +211,8 @@ __device__ __forceinline__ bool lane_step(const TraceArgs& a, LaneRay& L, Stack&
 // 7.07 ms).
 constexpr int kWideLists = 4, kListLists = 2;
 
-template <int WL>
-__device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+template <int WL, class TA>
+__device__ __forceinline__ bool lane_step_wide(const TA& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     while (L.cnt == 0) {
         if (st.sp == 0) return false;
         float te = 0.f;
@@ -221,7 +222,7 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
             L.cnt = (e & 15u) + 1;
         } else {
             if ((e >> 4) >= a.node_count) { st.fault |= kFaultNode; return false; }
-            const NodeV p = load_node(a.nodes, e >> 4);
+            const NodeV p = fetch_node(a, e >> 4);
             L.first = p.child;
             L.cnt = p.count;
         }
@@ -253,8 +254,8 @@ __device__ __forceinline__ bool lane_step_wide(const TraceArgs& a, LaneRay& L, S
     NodeV n[WL][2];
 #pragma unroll
     for (int g = 0; g < WL; g++) {
-        n[g][0] = load_node(a.nodes, idx[g]);
-        n[g][1] = load_node(a.nodes, idx[g] + (num[g] == 2 ? 1u : 0u));
+        n[g][0] = fetch_node(a, idx[g]);
+        n[g][1] = fetch_node(a, idx[g] + (num[g] == 2 ? 1u : 0u));
     }
     bool h[WL][2];
     float f;
@@ -304,7 +305,8 @@ __device__ __forceinline__ void cswap(float& fa, uint32_t& ea, float& fb, uint32
     ea = te;
 }
 
-__device__ __forceinline__ bool lane_step_cwide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+template <class TA>
+__device__ __forceinline__ bool lane_step_cwide(const TA& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
     while (L.cnt == 0) {
         if (st.sp == 0) return false;
@@ -316,7 +318,7 @@ __device__ __forceinline__ bool lane_step_cwide(const TraceArgs& a, LaneRay& L, 
             L.cnt = (e & 15u) + 1;
         } else {
             if ((e >> 4) >= a.node_count) { st.fault |= kFaultNode; return false; }
-            const NodeV p = load_node(a.nodes, e >> 4);
+            const NodeV p = fetch_node(a, e >> 4);
             L.first = p.child;
             L.cnt = p.count;
         }
@@ -348,8 +350,8 @@ __device__ __forceinline__ bool lane_step_cwide(const TraceArgs& a, LaneRay& L, 
     NodeV n[2][2];
 #pragma unroll
     for (int g = 0; g < 2; g++) {
-        n[g][0] = load_node(a.nodes, idx[g]);
-        n[g][1] = load_node(a.nodes, idx[g] + (num[g] == 2 ? 1u : 0u));
+        n[g][0] = fetch_node(a, idx[g]);
+        n[g][1] = fetch_node(a, idx[g] + (num[g] == 2 ? 1u : 0u));
     }
     bool h[2][2];
     float f[2][2];
@@ -392,13 +394,14 @@ __device__ __forceinline__ bool lane_step_cwide(const TraceArgs& a, LaneRay& L, 
 }
 
 // Node2 trees: triangles of a leaf group; hit ids are original indices.
-__device__ __forceinline__ void leaf_tris2(const TraceArgs& a, uint32_t first, uint32_t count, LaneRay& L,
+template <class TA>
+__device__ __forceinline__ void leaf_tris2(const TA& a, uint32_t first, uint32_t count, LaneRay& L,
                                            uint32_t& tri_tests, bool& bad) {
     for (uint32_t j = 0; j < count; j++) {
         const uint32_t k = first + j;
         if (k >= a.tri_count) { bad = true; break; }
         float v[9];
-        sqd::load_tri(a.tris, k, v);
+        fetch_tri(a, k, v);
         float t;
         tri_tests++;
         if (tri_test(v, L.r, t) && t > L.r.tmin && t < L.r.tmax && t <= L.best_t) {
@@ -445,8 +448,8 @@ __device__ __forceinline__ Node2V load_node2(const sq_node2* nodes, uint32_t e) 
 // Leaf groups of a Node2 step (first triangle first[j], cnt[j] triangles,
 // hit[j]): the lane's hit groups are compacted, the first triangle of each is
 // fetched before any is tested, and the rest of longer groups follow.
-template <int N>
-__device__ __forceinline__ void leaf_batch2(const TraceArgs& a, const uint32_t* first, const uint32_t* cnt,
+template <int N, class TA>
+__device__ __forceinline__ void leaf_batch2(const TA& a, const uint32_t* first, const uint32_t* cnt,
                                             const bool* hit, LaneRay& L, uint32_t& tri_tests, bool& bad) {
     bool lf[N];
     uint32_t pos[N], c = 0;
@@ -471,7 +474,7 @@ __device__ __forceinline__ void leaf_batch2(const TraceArgs& a, const uint32_t* 
     float tv[N][9];
 #pragma unroll
     for (int k = 0; k < N; k++)
-        if ((uint32_t)k < c) sqd::load_tri(a.tris, ids[k], tv[k]);
+        if ((uint32_t)k < c) fetch_tri(a, ids[k], tv[k]);
 #pragma unroll
     for (int k = 0; k < N; k++)
         if ((uint32_t)k < c) {
@@ -495,8 +498,8 @@ __device__ __forceinline__ void leaf_batch2(const TraceArgs& a, const uint32_t* 
 // Exhaustive Node2 step: up to WL Node2s (independent 32-byte fetches) per
 // step, the current one and the next ones on the stack; a step pops at most
 // WL entries and pushes at most 2 WL, so at most WL per level stay behind.
-template <int WL>
-__device__ __forceinline__ bool lane_step2_wide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+template <int WL, class TA>
+__device__ __forceinline__ bool lane_step2_wide(const TA& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     float te = 0.f;
     if (st.sp == 0) return false;
     uint32_t e[WL];
@@ -548,7 +551,8 @@ __device__ __forceinline__ bool lane_step2_wide(const TraceArgs& a, LaneRay& L, 
 // Node2 trees, closest semantics, camera rays: two Node2s per step (four
 // child boxes, both fetches issued together), leaf groups tested first, then
 // the surviving children pushed farthest first, as in lane_step_cwide.
-__device__ __forceinline__ bool lane_step2_cwide(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+template <class TA>
+__device__ __forceinline__ bool lane_step2_cwide(const TA& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
     uint32_t e[2];
     uint32_t got = 0;
@@ -602,8 +606,8 @@ __device__ __forceinline__ bool lane_step2_cwide(const TraceArgs& a, LaneRay& L,
     return st.fault == 0;
 }
 
-template <bool CLOSEST, int WL>
-__device__ __forceinline__ bool lane_step2(const TraceArgs& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+template <bool CLOSEST, int WL, class TA>
+__device__ __forceinline__ bool lane_step2(const TA& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
     if (L.cnt) {
         L.cnt = 0;
         L.tests += 1;
@@ -674,8 +678,8 @@ constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode (fewer for
 // camera looks at the scene-box centre, so the costly tiles come first and
 // the cheap outer rows fill the end of the launch, when waves run out of
 // work at different times.
-template <bool CLOSEST, int FMT, int WL>
-__device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, uint32_t lane, Stack& st,
+template <bool CLOSEST, int FMT, int WL, class TA>
+__device__ __forceinline__ void trace_tile(const TA& a, uint32_t tile, uint32_t lane, Stack& st,
                                            uint32_t* s_cnt, uint32_t& tests_total, uint32_t& tris_total,
                                            uint32_t& faults, uint32_t* job_fail) {
     uint32_t tx = 0, ty = 0, tile_rays;
@@ -779,8 +783,9 @@ __device__ __forceinline__ void trace_tile(const TraceArgs& a, uint32_t tile, ui
 // MULTI: the launch covers a.njobs jobs (tree + rays + outputs each); tile t
 // of the launch is tile t - job_tiles[j] of the job j with job_tiles[j] <= t
 // < job_tiles[j + 1]. Waves steal tiles from one counter across all jobs.
-template <bool CLOSEST, bool MULTI, int FMT = 1, int WL = kListLists>
+template <bool CLOSEST, bool MULTI, int FMT = 1, int WL = kListLists, bool NARROW = false>
 __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
+    using TA = typename std::conditional<NARROW, TraceArgsN, TraceArgs>::type;
     extern __shared__ uint32_t s_dyn[];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     Stack st;
@@ -819,11 +824,13 @@ __global__ __launch_bounds__(128) void k_trace(TraceArgs a) {
                 if (jt[mid] <= tile) lo = mid;
                 else hi = mid;
             }
-            const TraceArgs J = jobs[lo];
+            TA J;
+            static_cast<TraceArgs&>(J) = jobs[lo];
             trace_tile<CLOSEST, FMT, WL>(J, tile - jt[lo], lane, st, s_cnt, tests_total, tris_total, faults,
-                                     a.job_fail ? a.job_fail + lo : nullptr);
+                                         a.job_fail ? a.job_fail + lo : nullptr);
         } else {
-            trace_tile<CLOSEST, FMT, WL>(a, tile, lane, st, s_cnt, tests_total, tris_total, faults, nullptr);
+            trace_tile<CLOSEST, FMT, WL>(static_cast<const TA&>(a), tile, lane, st, s_cnt, tests_total, tris_total,
+                                         faults, nullptr);
         }
     }
     if (faults) atomicOr(&a.work[1], faults);
@@ -928,24 +935,28 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     return SQ_OK;
 }
 
-int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi) {
+template <bool CLOSEST, bool MULTI, int FMT, int WL>
+static void launch_trace(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool narrow) {
+    if (narrow)
+        hipLaunchKernelGGL((k_trace<CLOSEST, MULTI, FMT, WL, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream,
+                           a);
+    else
+        hipLaunchKernelGGL((k_trace<CLOSEST, MULTI, FMT, WL, false>), dim3(L.grid), dim3(L.block), L.lds_bytes,
+                           stream, a);
+}
+
+int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi, bool narrow) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     if (multi) {
-        if (L.closest && L.lists == 1)
-            hipLaunchKernelGGL((k_trace<true, true, 1, 1>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-        else if (L.closest) hipLaunchKernelGGL((k_trace<true, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-        else if (L.lists == kWideLists)
-            hipLaunchKernelGGL((k_trace<false, true, 1, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
-                               stream, a);
-        else hipLaunchKernelGGL((k_trace<false, true>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        if (L.closest && L.lists == 1) launch_trace<true, true, 1, 1>(stream, L, a, narrow);
+        else if (L.closest) launch_trace<true, true, 1, kListLists>(stream, L, a, narrow);
+        else if (L.lists == kWideLists) launch_trace<false, true, 1, kWideLists>(stream, L, a, narrow);
+        else launch_trace<false, true, 1, kListLists>(stream, L, a, narrow);
     } else {
-        if (L.closest && L.lists == 1)
-            hipLaunchKernelGGL((k_trace<true, false, 1, 1>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-        else if (L.closest) hipLaunchKernelGGL((k_trace<true, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-        else if (L.lists == kWideLists)
-            hipLaunchKernelGGL((k_trace<false, false, 1, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes,
-                               stream, a);
-        else hipLaunchKernelGGL((k_trace<false, false>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+        if (L.closest && L.lists == 1) launch_trace<true, false, 1, 1>(stream, L, a, narrow);
+        else if (L.closest) launch_trace<true, false, 1, kListLists>(stream, L, a, narrow);
+        else if (L.lists == kWideLists) launch_trace<false, false, 1, kWideLists>(stream, L, a, narrow);
+        else launch_trace<false, false, 1, kListLists>(stream, L, a, narrow);
     }
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
@@ -953,13 +964,11 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
 
 int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
-    if (L.closest && L.lists == 1)
-        hipLaunchKernelGGL((k_trace<true, false, 2, 1>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-    else if (L.closest) hipLaunchKernelGGL((k_trace<true, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
-    else if (L.lists == kWideLists)
-        hipLaunchKernelGGL((k_trace<false, false, 2, kWideLists>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream,
-                           a);
-    else hipLaunchKernelGGL((k_trace<false, false, 2>), dim3(L.grid), dim3(L.block), L.lds_bytes, stream, a);
+    // Node2 trees keep 64-bit addressing (their nodes come through load_node2)
+    if (L.closest && L.lists == 1) launch_trace<true, false, 2, 1>(stream, L, a, false);
+    else if (L.closest) launch_trace<true, false, 2, kListLists>(stream, L, a, false);
+    else if (L.lists == kWideLists) launch_trace<false, false, 2, kWideLists>(stream, L, a, false);
+    else launch_trace<false, false, 2, kListLists>(stream, L, a, false);
     SQ_CHECK_HIP(hipGetLastError());
     return SQ_OK;
 }
@@ -1065,7 +1074,8 @@ static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, con
     a.slots = L.slots;
     a.spill_cap = L.spill_cap;
     phase_begin(c, PH_TRACE);
-    int rc = node2 ? trace_launch_plan2(c->stream, L, a) : trace_launch_plan(c->stream, L, a, false);
+    int rc = node2 ? trace_launch_plan2(c->stream, L, a)
+                   : trace_launch_plan(c->stream, L, a, false, !c->wide_fetch && narrow_ok(a.node_count, a.tri_count));
     phase_end(c, PH_TRACE);
     return rc;
 }

@@ -4,6 +4,8 @@
 #pragma once
 #include "internal.h"
 
+#include <type_traits>
+
 namespace sq {
 
 struct TraceArgs {
@@ -165,6 +167,23 @@ __device__ __forceinline__ NodeV load_node(const sq_node* __restrict__ nodes, ui
     return n;
 }
 
+// Launch arguments whose node array and triangle array each span less than
+// 4 GiB (every index a fetch uses is bounds-checked first): the traversal
+// then addresses them with 32-bit byte offsets from the scalar base pointer
+// instead of 64-bit per-lane address arithmetic.
+struct TraceArgsN : TraceArgs {};
+inline bool narrow_ok(uint64_t node_count, uint64_t tri_count) {
+    return node_count * 32 < (1ull << 32) && tri_count * 36 < (1ull << 32);
+}
+__device__ __forceinline__ NodeV fetch_node(const TraceArgs& a, uint32_t i) { return load_node(a.nodes, i); }
+__device__ __forceinline__ NodeV fetch_node(const TraceArgsN& a, uint32_t i) {
+    return load_node((const sq_node*)((const char*)a.nodes + (i << 5)), 0);
+}
+__device__ __forceinline__ void fetch_tri(const TraceArgs& a, uint32_t i, float v[9]) { sqd::load_tri(a.tris, i, v); }
+__device__ __forceinline__ void fetch_tri(const TraceArgsN& a, uint32_t i, float v[9]) {
+    sqd::load_tri((const sq_triangle*)((const char*)a.tris + i * 36u), 0, v);
+}
+
 // Slab test with the reference's operation order and selects. When every
 // reciprocal direction component is finite no NaN can arise (|1/d| >= 1, so
 // (b - o) * inv is finite or +-inf), and IEEE min/max then differ from the
@@ -278,7 +297,8 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
 // SQ_ERR_OVERFLOW with a message naming the fault bits.
 int trace_fault(uint32_t bits, const char* what);
 // Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).
-int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi);
+// narrow: every job's node and triangle arrays span less than 4 GiB (narrow_ok).
+int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a, bool multi, bool narrow);
 // Same over a Node2 tree (a.nodes2, a.perm, root fields).
 int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a);
 

@@ -252,3 +252,47 @@ def test_packets_equal_rays_at_full_size(ctx, kind):
     for k in ("hit_id", "hit_t", "counts", "rgb"):
         assert np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8), np.ascontiguousarray(b[k]).view(np.uint8)), k
         assert np.array_equal(np.ascontiguousarray(a[k]).view(np.uint8), np.ascontiguousarray(c[k]).view(np.uint8)), k
+
+
+def test_wide_and_narrow_fetches_agree(ctx):
+    """Traversal launches address node and triangle arrays below 4 GiB with
+    32-bit offsets; a context created with SEQUOIA_TEST_WIDE_FETCH keeps the
+    64-bit form every launch over larger arrays takes. Both give the same
+    bytes: single camera (exhaustive and closest), ray lists and a batch."""
+    import os
+    tris = sq.gen_soup(200_000, 5, 0.01)
+    cam = sq.pinhole_camera(tris, 320, 180, 60.0, 3.0, spp=4, jitter=1, seed=5)
+    rays = sq.gen_secondary_rays(tris, 20000, seed=3, tmin=1e-4)
+    os.environ["SEQUOIA_TEST_WIDE_FETCH"] = "1"
+    try:
+        wide = sq.Context(0)
+    finally:
+        del os.environ["SEQUOIA_TEST_WIDE_FETCH"]
+    scenes = [sq.gen_icosphere(1280, 1, 0.1), sq.gen_soup(3000, 7, 0.01), tris]
+    cams = [sq.pinhole_camera(t, 64, 48, 60.0, 3.0, spp=2, jitter=1, seed=9) for t in scenes]
+    try:
+        outs = []
+        for c in (ctx, wide):
+            s = c.upload(tris)
+            b = c.build(s, "morton63")
+            o = {"ex": c.trace_camera(b, cam), "cl": c.trace_camera(b, cam, semantics=sq.TRACE_CLOSEST),
+                 "rays": c.trace_rays(b, rays)}
+            b.free()
+            s.free()
+            bt = sq.Batch(c, scenes)
+            bt.set_cameras(cams)
+            bt.run("morton63")
+            o["batch"] = [bt.download(i) for i in range(len(scenes))]
+            bt.free()
+            outs.append(o)
+        n, w = outs
+        for k in ("ex", "cl", "rays"):
+            for f in ("hit_id", "hit_t", "counts"):
+                assert np.array_equal(np.ascontiguousarray(n[k][f]).view(np.uint8),
+                                      np.ascontiguousarray(w[k][f]).view(np.uint8)), (k, f)
+        for i in range(len(scenes)):
+            for f in ("hit_id", "hit_t", "counts", "rgb"):
+                assert np.array_equal(np.ascontiguousarray(n["batch"][i][f]).view(np.uint8),
+                                      np.ascontiguousarray(w["batch"][i][f]).view(np.uint8)), (i, f)
+    finally:
+        wide.close()
