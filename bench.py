@@ -376,7 +376,7 @@ def incoherent_section(sq, ctx, reps=3):
         out["closest_node_tests_per_ray"] = round(t["node_tests"] / INC_RAYS, 1)
         out["closest_tri_tests_per_ray"] = round(t["tri_tests"] / INC_RAYS, 1)
         alg = 32.0 * t["node_tests"] + 36.0 * t["tri_tests"]
-        out["closest_roofline"] = dict(_rl(alg, t["trace_ms"]), kernel="k_trace<true, false, 1, 1>",
+        out["closest_roofline"] = dict(_rl(alg, t["trace_ms"]), kernel="k_trace<true, false, 1, 1, true>",
                                        formula="32 B per node test + 36 B per triangle test (counted on the device "
                                                "in the same launch)")
         out["hit_fraction"] = round(float((r["hit_id"] != sq.MISS).mean()), 4)
@@ -517,7 +517,7 @@ def batched_section(sq, ctx, steps, warmup):
         "plain_wall_ms": round(plain_wall * 1e3, 2),
         "plain_mrays_s": round(nrays / plain_wall / 1e6, 1),
         "speedup_vs_plain": round(plain_wall / batch_wall, 2),
-        "trace_roofline": dict(_rl(batch_alg, eager["trace_ms"]), kernel="k_trace<false, true, 1, 2>",
+        "trace_roofline": dict(_rl(batch_alg, eager["trace_ms"]), kernel="k_trace<false, true, 1, 2, true>",
                                node_tests=int(work["node_tests"]), tri_tests=int(work["tri_tests"]),
                                formula="32 B per node test + 36 B per triangle test over all 256 scenes, "
                                        "against the eager batched trace time"),
@@ -858,7 +858,7 @@ def main():
         "heatmap_mean_tests_per_pixel": round(float(heat.mean()) / SPP, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_note,
-                     "kernel": "k_trace<false, false, 1, 2>", "algorithmic_bytes_per_launch": alg_bytes},
+                     "kernel": "k_trace<false, false, 1, 2, true>", "algorithmic_bytes_per_launch": alg_bytes},
     }
     # the secondary sections never take the headline line down with them
     if not a.no_build_roofline:
