@@ -26,6 +26,13 @@ NODE2_DTYPE = np.dtype([("c", "<f4", 3), ("w0", "<u4"), ("w1", "<u4"), ("q", "u1
 ROOT2_DTYPE = np.dtype([("min", "<f4", 3), ("max", "<f4", 3), ("kind", "<u4"), ("child", "<u4"), ("count", "<u4"),
                         ("pad", "<u4")])
 assert NODE2_DTYPE.itemsize == 32 and ROOT2_DTYPE.itemsize == 40
+ROOTK_DTYPE = ROOT2_DTYPE  # wide trees: kind 1 leaf (child, count), 2 node (child, used slots)
+
+
+def nodek_dtype(k):
+    """Node<K> (aabb.h:26-32): min x/y/z[K], flags[K], max x/y/z[K], child[K]."""
+    return np.dtype([("min_x", "<f4", k), ("min_y", "<f4", k), ("min_z", "<f4", k), ("flags", "<u4", k),
+                     ("max_x", "<f4", k), ("max_y", "<f4", k), ("max_z", "<f4", k), ("child", "<u4", k)])
 
 LAYOUT_REFERENCE, LAYOUT_FIXED = 0, 1
 PART_HOARE, PART_STABLE = 0, 1
@@ -89,6 +96,8 @@ def oracle():
             "sqo_node2_convert": (C.c_int, [V, U, U, V, U, U, V, U, V, V, V, U, V, V]),
             "sqo_node2_decode": (None, [V, V, V, V, V]),
             "sqo_node2_trace": (None, [V, V, V, V, V, U, U, V, V, V]),
+            "sqo_nodek_convert": (C.c_int, [V, U, U, U, U, V, U, V, V, V]),
+            "sqo_nodek_trace": (None, [V, U, V, V, V, U, U, V, V, V]),
             "sqo_plucker_precompute": (None, [V, U, V]),
             "sqo_plucker_test": (C.c_int, [V, V, V]),
             "sqo_trace_rays_plucker": (None, [V, U, V, V, U, V, V, V]),
@@ -365,6 +374,38 @@ def node2_trace(n2, root, tris2, perm, rays, closest=False):
     counts, hid, ht = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.float32)
     oracle().sqo_node2_trace(_p(n2), _p(r), _p(tris2), _p(perm), _p(rays), n, 1 if closest else 0, _p(counts),
                              _p(hid), _p(ht))
+    return {"counts": counts, "hit_id": hid, "hit_t": ht}
+
+
+def nodek_convert(nodes, root, k, expand=True):
+    """Node1 tree -> (Node<k> array, root record, levels)."""
+    nodes = np.ascontiguousarray(nodes, NODE_DTYPE)
+    dt = nodek_dtype(k)
+    cap = max(1, len(nodes) // 2)
+    while True:
+        out = np.zeros(cap, dt)
+        nn, lv = C.c_uint32(), C.c_uint32()
+        r = np.zeros(1, ROOTK_DTYPE)
+        rc = oracle().sqo_nodek_convert(_p(nodes), len(nodes), root, k, 1 if expand else 0, _p(out), cap,
+                                        C.byref(nn), C.byref(lv), _p(r))
+        if rc == -4:
+            cap *= 2
+        elif rc != 0:
+            raise ValueError("wide-node conversion failed: %d" % rc)
+        else:
+            return out[:nn.value].copy(), r[0], lv.value
+
+
+def nodek_trace(wide, root, tris, rays, closest=False):
+    k = wide.dtype["child"].shape[0]
+    wide = np.ascontiguousarray(wide)
+    r = np.array([root], ROOTK_DTYPE)
+    tris = tris32(tris)
+    rays = np.ascontiguousarray(rays, RAY_DTYPE)
+    n = len(rays)
+    counts, hid, ht = np.empty(n, np.uint32), np.empty(n, np.uint32), np.empty(n, np.float32)
+    oracle().sqo_nodek_trace(_p(wide), k, _p(r), _p(tris), _p(rays), n, 1 if closest else 0, _p(counts), _p(hid),
+                             _p(ht))
     return {"counts": counts, "hit_id": hid, "hit_t": ht}
 
 

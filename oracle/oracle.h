@@ -117,6 +117,23 @@ void sqo_node2_trace(const sqo_node2* nodes, const sqo_node2_root* root, const s
                      const sqo_ray* rays, uint32_t nrays, uint32_t closest, uint32_t* counts, uint32_t* hit_id,
                      float* hit_t);
 
+/* ---- Wide nodes (Node<K>, aabb.h:26-32; meaning defined in nodek.cpp) ----
+ * A Node<K> is 8 K 32-bit words: min x[K], min y[K], min z[K], flags[K],
+ * max x[K], max y[K], max z[K], child[K]. K = 2, 4 or 8. */
+typedef struct { float min[3], max[3]; uint32_t kind, child, count, pad; } sqo_nodek_root;
+/* kind: 1 = the root is a leaf (child = first triangle, count), 2 = wide node `child` with `count` used slots,
+ * 0 = an interior root with an empty list.
+ * expand: 1 = pull grandchildren up (largest half surface area first) while they fit, 0 = one Node1 child
+ * list per wide node (lists longer than K still become K runs). Returns 0, or < 0: -1 malformed tree,
+ * -3 bad K, -4 node_cap too small. */
+int sqo_nodek_convert(const sqo_node* nodes, uint32_t node_count, uint32_t root, uint32_t K, uint32_t expand,
+                      uint32_t* out, uint32_t node_cap, uint32_t* out_nodes, uint32_t* out_levels,
+                      sqo_nodek_root* root_out);
+/* counts: box tests per ray (root + one per used slot of each visited node); hits use the Node1 tree's ids. */
+void sqo_nodek_trace(const uint32_t* nodes, uint32_t K, const sqo_nodek_root* root, const sqo_tri* tris,
+                     const sqo_ray* rays, uint32_t nrays, uint32_t closest, uint32_t* counts, uint32_t* hit_id,
+                     float* hit_t);
+
 /* ---- Precomputed-triangle (PluckerTriangle, aabb.h:182-195) test; plucker.cpp ---- */
 typedef struct {
     float nu, nv, np, pu, pv;

@@ -1,5 +1,5 @@
 // CPU run of every oracle entry point (oracle/oracle.cpp, node2.cpp,
-// plucker.cpp) on small seeded scenes, built with AddressSanitizer and
+// nodek.cpp, plucker.cpp) on small seeded scenes, built with AddressSanitizer and
 // UndefinedBehaviorSanitizer by tests/test_sanitizers.py: each Morton, SAH
 // and grid variant is built, then traced exhaustively, closest-hit, per
 // camera, per packet and bundle, through the quantised two-child nodes and
@@ -130,6 +130,27 @@ static void trace_all(const std::vector<sqo_node>& nodes, uint32_t root, const s
         sqo_node2_trace(n2.data(), &r2, tris2.data(), perm.data(), r.data(), nr, 0, cnt3.data(), hid3.data(),
                         ht3.data());
         break;
+    }
+
+    // wide nodes, K = 2, 4, 8, collapsed or not: capacity grows until it fits
+    for (uint32_t K : {2u, 4u, 8u}) {
+        for (uint32_t expand : {0u, 1u}) {
+            uint32_t wcap = 1 + (uint32_t)nodes.size() / 4;
+            for (int attempt = 0; attempt < 12; attempt++) {
+                std::vector<uint32_t> wide((size_t)wcap * 8 * K);
+                uint32_t nn = 0, levels = 0;
+                sqo_nodek_root rk;
+                const int rc = sqo_nodek_convert(nodes.data(), (uint32_t)nodes.size(), root, K, expand, wide.data(),
+                                                 wcap, &nn, &levels, &rk);
+                if (rc == -4) { wcap *= 2; continue; }
+                CHECK(rc == 0, "%s nodek K=%u convert %d", tag, K, rc);
+                sqo_nodek_trace(wide.data(), K, &rk, tris.data(), r.data(), nr, 1, cnt3.data(), hid3.data(),
+                                ht3.data());
+                sqo_nodek_trace(wide.data(), K, &rk, tris.data(), r.data(), nr, 0, cnt3.data(), hid3.data(),
+                                ht3.data());
+                break;
+            }
+        }
     }
 
     std::vector<sqo_plucker> pl(n);

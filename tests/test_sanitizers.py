@@ -53,6 +53,6 @@ def test_oracle_under_sanitizers(tmp_path):
     od = os.path.join(ROOT, "oracle")
     _run(["g++", "-std=c++17", "-Wall", "-Werror", "-Wno-unused-parameter", "-fopenmp", "-ffp-contract=off", *SAN,
           "-I", od, os.path.join(ROOT, "tests", "cpp", "oracle_asan.cpp"),
-          *(os.path.join(od, f) for f in ("oracle.cpp", "node2.cpp", "plucker.cpp")), "-o", exe])
+          *(os.path.join(od, f) for f in ("oracle.cpp", "node2.cpp", "nodek.cpp", "plucker.cpp")), "-o", exe])
     out = _run([exe], timeout=300)
     assert "trees built and traced" in out, out
