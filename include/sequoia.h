@@ -286,6 +286,56 @@ int sq_trace2_camera(sq_ctx* ctx, const sq_bvh2* t, const sq_camera* cam, const 
 int sq_trace2_rays(sq_ctx* ctx, const sq_bvh2* t, const sq_ray* rays, uint32_t n, const sq_trace_params* params,
                    const sq_trace_out* out, int io_on_device);
 
+/* ---- wide nodes (Node<K>, aabb.h:26-32) ----
+ * The reference declares template <size_t K> struct Node (vec3f<K> min,
+ * simdi<K> flags, vec3f<K> max, simdi<K> child: structure-of-arrays, 32 K
+ * bytes) and never uses it; here it is a traversal format converted from any
+ * Node1 tree, for K = 4 (128 B) and K = 8 (256 B). Slot j of a node is one
+ * child: a Node1 node (its box as stored there) or a run of a longer Node1
+ * list (the union box); flags[j] is in Node1's count | flags << 16 order: a
+ * leaf slot copies the Node1 leaf (bit 16 set, count triangles from child[j],
+ * the Node1 tree's triangle ids), an interior slot names wide node child[j]
+ * and holds how many of its slots are used (used slots first; 0 with child
+ * 0xFFFFFFFF for a Node1 node with an empty list). Unused slots: min +inf,
+ * max -inf, flags 0, child 0xFFFFFFFF. Conversion pulls grandchildren into
+ * free slots, largest half surface area first (SQ_BVHK_NO_COLLAPSE: one Node1
+ * list per node, so a tree without lists longer than K is traced with exactly
+ * the Node1 tree's box tests and counts); nodes are numbered breadth first.
+ * Counts are box tests of this tree: the root box, then one per used slot of
+ * each visited node. */
+typedef struct {
+    float min_x[4], min_y[4], min_z[4];
+    uint32_t flags[4];
+    float max_x[4], max_y[4], max_z[4];
+    uint32_t child[4];
+} sq_node4;
+typedef struct {
+    float min_x[8], min_y[8], min_z[8];
+    uint32_t flags[8];
+    float max_x[8], max_y[8], max_z[8];
+    uint32_t child[8];
+} sq_node8;
+typedef struct {
+    float min[3], max[3];   /* root box, tested as stored */
+    uint32_t kind;          /* 0: empty list, 1: leaf (child = first triangle, count), 2: node `child`, count used slots */
+    uint32_t child, count, reserved;
+} sq_nodek_root;
+typedef struct sq_bvhk sq_bvhk;
+enum { SQ_BVHK_NO_COLLAPSE = 1 };
+/* Converts a device Node1 tree into Node<k> (k = 4 or 8) on the device.
+ * flags: 0 or SQ_BVHK_NO_COLLAPSE. The tree keeps using the Node1 tree's
+ * scene (its triangles are not copied). */
+int sq_bvhk_create(sq_ctx* ctx, const sq_bvh* bvh, uint32_t k, uint32_t flags, sq_bvhk** out);
+int sq_bvhk_info(const sq_bvhk* t, uint32_t* k, uint32_t* nodes, uint32_t* levels, sq_nodek_root* root);
+/* Host copy of the nodes (32 k bytes each; node_cap in nodes). */
+int sq_bvhk_download(const sq_bvhk* t, void* nodes, uint32_t node_cap);
+void sq_bvhk_free(sq_bvhk* t);
+/* As sq_trace_camera / sq_trace_rays (ray heatmaps, per-ray kernel). */
+int sq_tracek_camera(sq_ctx* ctx, const sq_bvhk* t, const sq_camera* cam, const sq_trace_params* params,
+                     const sq_trace_out* out, int out_on_device);
+int sq_tracek_rays(sq_ctx* ctx, const sq_bvhk* t, const sq_ray* rays, uint32_t n, const sq_trace_params* params,
+                   const sq_trace_out* out, int io_on_device);
+
 /* ---- batches: many scenes built and traced together ----
  * No counterpart in the reference, which builds and traces one Hierarchy per
  * call (Builder::build_hierarchy_morton, builder.cpp:2474-2508; trace,

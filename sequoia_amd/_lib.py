@@ -26,6 +26,7 @@ TRACE_EXHAUSTIVE, TRACE_CLOSEST = 0, 1
 HEATMAP_RAY, HEATMAP_PACKET, HEATMAP_BUNDLE = 0, 1, 2
 KERNEL_AUTO, KERNEL_RAYS, KERNEL_PACKETS = 0, 1, 2
 MISS = 0xFFFFFFFF
+BVHK_NO_COLLAPSE = 1
 
 NODE_DTYPE = np.dtype([("min", "<f4", 3), ("count", "<u2"), ("flags", "<u2"), ("max", "<f4", 3), ("child", "<u4")])
 RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("dir", "<f4", 3), ("tmax", "<f4")])
@@ -191,6 +192,12 @@ _SIGS = {
     "sq_bvh2_free": (None, [_P]),
     "sq_trace2_camera": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
     "sq_trace2_rays": (C.c_int, [_P, _P, _P, _U32, C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
+    "sq_bvhk_create": (C.c_int, [_P, _P, _U32, _U32, C.POINTER(_P)]),
+    "sq_bvhk_info": (C.c_int, [_P, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(_U32), _P]),
+    "sq_bvhk_download": (C.c_int, [_P, _P, _U32]),
+    "sq_bvhk_free": (None, [_P]),
+    "sq_tracek_camera": (C.c_int, [_P, _P, C.POINTER(Camera), C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
+    "sq_tracek_rays": (C.c_int, [_P, _P, _P, _U32, C.POINTER(TraceParams), C.POINTER(TraceOut), C.c_int]),
     "sq_load_obj": (C.c_int, [C.c_char_p, C.POINTER(_P), C.POINTER(_U32), C.POINTER(_U32)]),
     "sq_host_free": (None, [_P]),
     "sq_bmp_size": (C.c_size_t, [_U32, _U32]),
@@ -376,6 +383,52 @@ class Bvh:
 class Node2Root(C.Structure):
     _fields_ = [("min", C.c_float * 3), ("max", C.c_float * 3), ("kind", C.c_uint32), ("child", C.c_uint32),
                 ("count", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+def nodek_dtype(k):
+    """Node<K> (aabb.h:26-32): min x/y/z[K], flags[K], max x/y/z[K], child[K]."""
+    return np.dtype([("min_x", "<f4", k), ("min_y", "<f4", k), ("min_z", "<f4", k), ("flags", "<u4", k),
+                     ("max_x", "<f4", k), ("max_y", "<f4", k), ("max_z", "<f4", k), ("child", "<u4", k)])
+
+
+class BvhK:
+    """Wide (Node<K>, K = 4 or 8) tree converted from a Node1 tree; its leaves
+    index the Node1 tree's scene."""
+
+    def __init__(self, ctx, handle, scene):
+        self.ctx, self.h, self.scene = ctx, handle, scene
+
+    def info(self):
+        k, n, lv, r = _U32(), _U32(), _U32(), Node2Root()
+        _check(lib().sq_bvhk_info(self.h, C.byref(k), C.byref(n), C.byref(lv), C.byref(r)))
+        return {"k": k.value, "nodes": n.value, "levels": lv.value, "root": r}
+
+    def download(self):
+        """(nodes, root record)."""
+        i = self.info()
+        nodes = np.empty(i["nodes"], nodek_dtype(i["k"]))
+        _check(lib().sq_bvhk_download(self.h, _ptr(nodes), len(nodes)))
+        return nodes, i["root"]
+
+    def free(self):
+        if self.h:
+            lib().sq_bvhk_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _trace_fns(bvh):
+    """(camera, rays) entry points for a Node1, Node2 or Node<K> tree."""
+    if isinstance(bvh, Bvh2):
+        return lib().sq_trace2_camera, lib().sq_trace2_rays
+    if isinstance(bvh, BvhK):
+        return lib().sq_tracek_camera, lib().sq_tracek_rays
+    return lib().sq_trace_camera, lib().sq_trace_rays
 
 
 class Bvh2:
@@ -662,7 +715,7 @@ class Context:
         """Trace with outputs already in device memory (no host copies)."""
         out = TraceOut(hit_id, hit_t, counts, rgb)
         p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work, kernel)
-        f = lib().sq_trace2_camera if isinstance(bvh, Bvh2) else lib().sq_trace_camera
+        f = _trace_fns(bvh)[0]
         _check(f(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 1))
 
     def trace_rays_device(self, bvh, rays, n, hit_id=None, hit_t=None, counts=None, semantics=TRACE_EXHAUSTIVE,
@@ -674,7 +727,7 @@ class Context:
         if noalloc:
             _check(lib().sq_trace_rays_noalloc(self.h, bvh.h, rays, n, C.byref(p), C.byref(out)))
         else:
-            _check(lib().sq_trace_rays(self.h, bvh.h, rays, n, C.byref(p), C.byref(out), 1))
+            _check(_trace_fns(bvh)[1](self.h, bvh.h, rays, n, C.byref(p), C.byref(out), 1))
 
     def upload(self, tris):
         tris = as_tris(tris)
@@ -727,14 +780,14 @@ class Context:
         rgb = np.empty((cam.height, cam.width, 3), np.uint8) if want_rgb else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), _ptr(rgb))
         p = self._params(semantics, heatmap, width, stack_slots, count_work, kernel, tri_test, lists_per_step)
-        f = lib().sq_trace2_camera if isinstance(bvh, Bvh2) else lib().sq_trace_camera
+        f = _trace_fns(bvh)[0]
         _check(f(self.h, bvh.h, C.byref(cam), C.byref(p), C.byref(out), 0))
         res.update(hit_id=hid, hit_t=ht, counts=cnt, rgb=rgb)
         return res
 
     def trace_rays(self, bvh, rays, semantics=TRACE_EXHAUSTIVE, want_counts=True, stack_slots=0, count_work=False,
                    tri_test=0, lists_per_step=0):
-        """Ray list through a Node1 tree (Bvh) or a Node2 tree (Bvh2)."""
+        """Ray list through a Node1 (Bvh), Node2 (Bvh2) or Node<K> (BvhK) tree."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         n = len(rays)
         hid = np.empty(n, np.uint32)
@@ -742,7 +795,7 @@ class Context:
         cnt = np.empty(n, np.uint32) if want_counts else None
         out = TraceOut(_ptr(hid), _ptr(ht), _ptr(cnt), None)
         p = self._params(semantics, HEATMAP_RAY, 0, stack_slots, count_work, 0, tri_test, lists_per_step)
-        f = lib().sq_trace2_rays if isinstance(bvh, Bvh2) else lib().sq_trace_rays
+        f = _trace_fns(bvh)[1]
         _check(f(self.h, bvh.h, _ptr(rays), n, C.byref(p), C.byref(out), 0))
         return {"hit_id": hid, "hit_t": ht, "counts": cnt}
 
@@ -752,6 +805,12 @@ class Context:
         _check(lib().sq_bvh_plucker(self.h, bvh.h, C.byref(p)))
         n = lib().sq_scene_count(bvh.scene.h)
         return self.to_host(p.value, n * PLUCKER_DTYPE.itemsize, np.uint8).view(PLUCKER_DTYPE)
+
+    def nodek(self, bvh, k=4, collapse=True):
+        """Converts a Node1 tree into a Node<k> tree (k = 4 or 8) on the device."""
+        h = C.c_void_p()
+        _check(lib().sq_bvhk_create(self.h, bvh.h, k, 0 if collapse else BVHK_NO_COLLAPSE, C.byref(h)))
+        return BvhK(self, h, bvh.scene)
 
     def node2(self, bvh, max_leaf=0):
         """Converts a Node1 tree into a Node2 tree on the device."""

@@ -139,6 +139,16 @@ struct sq_bvh2 {
     sq_node2_root root{};
 };
 
+// A Node<K> tree (nodek.hip): K = 4 or 8, 8 K words per node, root record,
+// breadth-first levels; leaves index the source tree's scene.
+struct sq_bvhk {
+    sq_ctx* ctx = nullptr;
+    const sq_scene* scene = nullptr;
+    uint32_t* d_nodes = nullptr;
+    uint32_t k = 0, nodes = 0, levels = 0;
+    sq_nodek_root root{};
+};
+
 namespace sq {
 
 // Phase timing helpers (events recorded on the context stream).

@@ -312,8 +312,8 @@ __global__ __launch_bounds__(kN2Block) void k_n2_emit(N2Args a) {
 }
 
 // Grows a device array to `want` elements of `esz` bytes, keeping the first
-// `keep` elements.
-static int grow_copy(hipStream_t stream, void** p, size_t* cap, size_t want, size_t keep, size_t esz) {
+// `keep` elements (also used by the wide-node conversion).
+int grow_copy(hipStream_t stream, void** p, size_t* cap, size_t want, size_t keep, size_t esz) {
     if (want <= *cap) return SQ_OK;
     size_t n = *cap + *cap / 2;
     if (n < want) n = want;

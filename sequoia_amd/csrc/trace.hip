@@ -668,6 +668,136 @@ __device__ __forceinline__ bool lane_step2(const TA& a, LaneRay& L, Stack& st, u
     return st.fault == 0;
 }
 
+// Node<K> trees (K = 4, 8): a node is K slots in structure-of-arrays order
+// (min x/y/z, words, max x/y/z, children: K / 4 16-byte loads per field), so
+// one fetch brings K boxes. Stack entries: node << 3 | (used slots - 1).
+template <int K>
+struct NodeKV {
+    float mn[3][K], mx[3][K];
+    uint32_t w[K], child[K];
+};
+
+template <int K>
+__device__ __forceinline__ void load_nodek(const uint32_t* __restrict__ nodes, uint32_t i, NodeKV<K>& n) {
+    constexpr int Q = K / 4;
+    const float4* p = (const float4*)(nodes + (size_t)i * (8 * K));
+#pragma unroll
+    for (int f = 0; f < 8; f++)
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const float4 v = p[f * Q + q];
+            const float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int j = 4 * q + t;
+                if (f < 3) n.mn[f][j] = e[t];
+                else if (f == 3) n.w[j] = __float_as_uint(e[t]);
+                else if (f < 7) n.mx[f - 4][j] = e[t];
+                else n.child[j] = __float_as_uint(e[t]);
+            }
+        }
+}
+
+// Sorting networks, largest distance first (Batcher's odd-even merge sort).
+template <int N>
+__device__ __forceinline__ void sort_desc(float* f, uint32_t* e) {
+    static_assert(N == 4 || N == 8, "4 or 8 entries");
+    if constexpr (N == 4) {
+        cswap(f[0], e[0], f[1], e[1]);
+        cswap(f[2], e[2], f[3], e[3]);
+        cswap(f[0], e[0], f[2], e[2]);
+        cswap(f[1], e[1], f[3], e[3]);
+        cswap(f[1], e[1], f[2], e[2]);
+    } else {
+        constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                    {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+        for (int c = 0; c < 19; c++) cswap(f[net[c][0]], e[net[c][0]], f[net[c][1]], e[net[c][1]]);
+    }
+}
+
+// One Node<K> per step: the root box on the first step, then the used slots
+// of the popped node; hit leaves are tested (compacted, first triangles
+// fetched together), hit interior slots pushed (closest: culled against the
+// best hit after the leaves, farthest first). A step pops one entry and
+// pushes at most K: (K - 1) x depth + K entries bound the stack.
+template <bool CLOSEST, int K, class TA>
+__device__ __forceinline__ bool lane_stepk(const TA& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+    if (L.cnt) {
+        L.cnt = 0;
+        L.tests += 1;
+        NodeV rb;
+        for (int k = 0; k < 3; k++) { rb.mn[k] = a.rb[k]; rb.mx[k] = a.rb[3 + k]; }
+        float f;
+        if (!slab(rb, L.r, L.ix, L.iy, L.iz, L.fast, f)) return false;
+        if (a.root_kind == 1) {
+            NodeV lf;
+            lf.child = a.root;
+            lf.count = a.root_count;
+            lf.leaf = 1;
+            bool bad = false;
+            leaf_tris(a, lf, L, tri_tests, bad);
+            if (bad) st.fault |= kFaultTri;
+            return false;
+        }
+        if (a.root_kind != 2 || a.root_count == 0) return false;
+        push<CLOSEST>(st, (a.root << 3) | (a.root_count - 1), f);
+        return st.fault == 0;
+    }
+    const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
+    uint32_t e = 0;
+    while (true) {
+        if (st.sp == 0) return false;
+        float te = 0.f;
+        e = pop<CLOSEST>(st, te);
+        if (CLOSEST && te > cull) continue;
+        break;
+    }
+    const uint32_t idx = e >> 3, used = (e & 7u) + 1;
+    if (idx >= a.node_count) { st.fault |= kFaultNode; return false; }
+    NodeKV<K> n;
+    load_nodek<K>(a.nodesk, idx, n);
+    L.tests += used;
+    NodeV nd[K];
+    bool h[K];
+    float f[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        NodeV b;
+        for (int k = 0; k < 3; k++) { b.mn[k] = n.mn[k][j]; b.mx[k] = n.mx[k][j]; }
+        f[j] = 0.f;
+        h[j] = (uint32_t)j < used && slab(b, L.r, L.ix, L.iy, L.iz, L.fast, f[j]);
+        if (CLOSEST) h[j] = h[j] && !(f[j] > cull);
+        nd[j].leaf = (n.w[j] >> 16) & 1u;
+        nd[j].count = n.w[j] & 0xFFFFu;
+        nd[j].child = n.child[j];
+    }
+    bool bad = false;
+    leaf_batch<K>(a, nd, h, L, tri_tests, bad);
+    if (bad) { st.fault |= kFaultTri; return false; }
+    if (!CLOSEST) {
+#pragma unroll
+        for (int j = 0; j < K; j++)
+            if (h[j] && !nd[j].leaf && nd[j].count) push<false>(st, (nd[j].child << 3) | (nd[j].count - 1), 0.f);
+        return st.fault == 0;
+    }
+    const float cull2 = L.best_t + L.best_t * (1.0f / 1048576.0f);
+    const float inf = __int_as_float(0x7f800000);
+    float fk[K];
+    uint32_t ek[K];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        const bool c = h[j] && !nd[j].leaf && nd[j].count && !(f[j] > cull2);
+        fk[j] = c ? f[j] : -inf;  // absent entries sort last and are not pushed
+        ek[j] = c ? ((nd[j].child << 3) | (nd[j].count - 1)) : 0u;
+    }
+    sort_desc<K>(fk, ek);
+#pragma unroll
+    for (int j = 0; j < K; j++)
+        if (fk[j] != -inf) push<true>(st, ek[j], fk[j]);
+    return st.fault == 0;
+}
+
 constexpr uint32_t kRayTile = 256;  // rays per tile in ray-list mode (fewer for short lists)
 
 // One tile of job J: the wave's lanes take the tile's rays one after
@@ -736,10 +866,16 @@ __device__ __forceinline__ void trace_tile(const TA& a, uint32_t tile, uint32_t 
             continue;
         }
         if (active) {
-            const bool more = FMT == 2 ? lane_step2<CLOSEST, WL>(a, L, st, tris_total)
-                                       : (CLOSEST ? (WL >= 2 && a.camera ? lane_step_cwide(a, L, st, tris_total)
-                                                                         : lane_step<CLOSEST>(a, L, st, tris_total))
-                                                  : lane_step_wide<WL>(a, L, st, tris_total));
+            bool more;
+            if constexpr (FMT >= 4) {
+                more = lane_stepk<CLOSEST, FMT>(a, L, st, tris_total);
+            } else if constexpr (FMT == 2) {
+                more = lane_step2<CLOSEST, WL>(a, L, st, tris_total);
+            } else {
+                more = CLOSEST ? (WL >= 2 && a.camera ? lane_step_cwide(a, L, st, tris_total)
+                                                      : lane_step<CLOSEST>(a, L, st, tris_total))
+                               : lane_step_wide<WL>(a, L, st, tris_total);
+            }
             if (!more) {
                 // ray finished
                 if (st.fault) {
@@ -860,9 +996,10 @@ int trace_fault(uint32_t bits, const char* what) {
 
 constexpr uint32_t kCameraWavesPerCu = 12, kCameraClosestWavesPerCu = 12, kWavesPerCu = 16;
 
-int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out, bool node2,
+int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out, int fmt,
                bool ray_list) {
     TracePlan L;
+    const bool node2 = fmt == 2, wide = fmt >= 4;
     L.closest = p && p->semantics == SQ_TRACE_CLOSEST;
     // closest hits: camera rays take two lists per step (134 VGPRs, three
     // waves per SIMD), ray lists one. A launch of one ray list uses the
@@ -876,7 +1013,8 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
             SQ_RET(SQ_ERR_INVALID, "trace lists_per_step %u (2 or 4)", p->lists_per_step);
         if (!L.closest) L.lists = p->lists_per_step;
     }
-    uint32_t slots = p && p->stack_slots ? p->stack_slots : (L.lists == kWideLists ? 32 : 16);
+    if (wide) L.lists = 1;  // one Node<K> per step
+    uint32_t slots = p && p->stack_slots ? p->stack_slots : (L.lists == kWideLists || wide ? 32 : 16);
     uint32_t pow2 = 1;
     while (pow2 < slots) pow2 <<= 1;
     L.slots = pow2 > 64 ? 64 : pow2;
@@ -884,7 +1022,13 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     if (L.block != 64 && L.block != 128) SQ_RET(SQ_ERR_INVALID, "trace block size %u (64 or 128)", L.block);
     L.lds_bytes = ((size_t)L.slots * (L.closest ? 2 : 1) + 1) * L.block * 4;
     int per_cu = 0;
-    if (node2 && L.closest && L.lists == 1)
+    if (fmt == 4)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, L.closest ? k_trace<true, false, 4, 1> : k_trace<false, false, 4, 1>, L.block, L.lds_bytes);
+    else if (fmt == 8)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, L.closest ? k_trace<true, false, 8, 1> : k_trace<false, false, 8, 1>, L.block, L.lds_bytes);
+    else if (node2 && L.closest && L.lists == 1)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 2, 1>, L.block, L.lds_bytes);
     else if (node2 && L.closest)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<true, false, 2>, L.block, L.lds_bytes);
@@ -916,8 +1060,8 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     // (256 scenes: 30 ms at 16 against 49 ms at 8) want 16.
     const uint32_t waves = !coherent ? kWavesPerCu
                            : L.closest ? kCameraClosestWavesPerCu
-                           : node2     ? kWavesPerCu
-                                       : kCameraWavesPerCu;
+                           : node2 || wide ? kWavesPerCu
+                                           : kCameraWavesPerCu;
     const int target = (int)(waves * 64 / L.block);
     if (per_cu > target) per_cu = target;
     if (p && p->blocks_per_cu) {
@@ -927,8 +1071,8 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     L.grid = (uint32_t)c->num_cus * (uint32_t)per_cu;
     if (depth == 0) depth = 64;
     // stack entries a ray can need: WL x depth + 2 WL for the exhaustive step,
-    // depth + 2 for the closest step
-    uint32_t cap = (L.closest ? 2 : L.lists) * depth + 16;
+    // depth + 2 for the closest step, (K - 1) x depth + K for Node<K> steps
+    uint32_t cap = wide ? (uint32_t)(fmt - 1) * depth + (uint32_t)fmt + 16 : (L.closest ? 2 : L.lists) * depth + 16;
     L.spill_cap = cap > L.slots ? cap - L.slots + 8 : 8;
     L.spill_words = (size_t)L.grid * L.block * L.spill_cap * (L.closest ? 2 : 1);
     *out = L;
@@ -962,6 +1106,19 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
     return SQ_OK;
 }
 
+static int trace_launch_plank(hipStream_t stream, const TracePlan& L, const TraceArgs& a, int fmt) {
+    hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
+    if (fmt == 4) {
+        if (L.closest) launch_trace<true, false, 4, 1>(stream, L, a, false);
+        else launch_trace<false, false, 4, 1>(stream, L, a, false);
+    } else {
+        if (L.closest) launch_trace<true, false, 8, 1>(stream, L, a, false);
+        else launch_trace<false, false, 8, 1>(stream, L, a, false);
+    }
+    SQ_CHECK_HIP(hipGetLastError());
+    return SQ_OK;
+}
+
 int trace_launch_plan2(hipStream_t stream, const TracePlan& L, const TraceArgs& a) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
     // Node2 trees keep 64-bit addressing (their nodes come through load_node2)
@@ -980,8 +1137,35 @@ bool use_packets(const sq_trace_params* p) {
 }
 
 // Tree fields of the launch arguments (Node1 tree or Node2 tree).
-static int target_args(const sq_bvh* b, const sq_bvh2* t2, const sq_trace_params* p, TraceArgs& a) {
+// The tree a trace runs over: a Node1 tree, a Node2 tree or a Node<K> tree.
+struct Target {
+    const sq_bvh* b = nullptr;
+    const sq_bvh2* t2 = nullptr;
+    const sq_bvhk* tk = nullptr;
+    int fmt() const { return b ? 1 : t2 ? 2 : (int)tk->k; }
+};
+
+static int target_args(const Target& t, const sq_trace_params* p, TraceArgs& a) {
+    const sq_bvh* b = t.b;
+    const sq_bvh2* t2 = t.t2;
     if (p && p->tri_test > 1) SQ_RET(SQ_ERR_INVALID, "trace: tri_test %u", p->tri_test);
+    if (t.tk) {
+        const sq_bvhk* k = t.tk;
+        if (p && p->tri_test) SQ_RET(SQ_ERR_UNSUPPORTED, "trace: Node<K> trees use the reference triangle test");
+        if (!k->scene) SQ_RET(SQ_ERR_INVALID, "trace: the tree has no scene");
+        a.nodesk = k->d_nodes;
+        a.tris = k->scene->d_tris;
+        a.root = k->root.child;
+        a.root_kind = k->root.kind;
+        a.root_count = k->root.count;
+        for (int j = 0; j < 3; j++) {
+            a.rb[j] = k->root.min[j];
+            a.rb[3 + j] = k->root.max[j];
+        }
+        a.node_count = k->nodes;
+        a.tri_count = k->scene->n;
+        return SQ_OK;
+    }
     if (b) {
         if (!b->scene) SQ_RET(SQ_ERR_INVALID, "trace: the tree has no scene");
         if (b->capacity > kMaxTraceNodes)
@@ -1013,11 +1197,13 @@ static int target_args(const sq_bvh* b, const sq_bvh2* t2, const sq_trace_params
     return SQ_OK;
 }
 
-static uint32_t target_depth(const sq_bvh* b, const sq_bvh2* t2) { return b ? b->depth : t2->levels + 2; }
+static uint32_t target_depth(const Target& t) {
+    return t.b ? t.b->depth : t.t2 ? t.t2->levels + 2 : t.tk->levels + 2;
+}
 
-static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, const sq_trace_params* p,
+static int trace_launch(sq_ctx* c, uint32_t depth, int fmt, TraceArgs& a, const sq_trace_params* p,
                         const TraceScratch* ext = nullptr) {
-    if (!node2 && a.camera && use_packets(p)) {
+    if (fmt == 1 && a.camera && use_packets(p)) {
         if (ext) SQ_RET(SQ_ERR_UNSUPPORTED, "trace: wave packets need the context workspace");
         SQ_TRY(c->counters.ensure(64));
         SQ_TRY(c->stats.ensure(64));
@@ -1030,7 +1216,7 @@ static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, con
         return rc;
     }
     TracePlan L;
-    SQ_TRY(trace_plan(c, p, depth, a.camera != 0u, &L, node2, a.camera == 0u));
+    SQ_TRY(trace_plan(c, p, depth, a.camera != 0u, &L, fmt, a.camera == 0u));
     if (ext) {
         // Caller scratch: counters first, then per-lane spill areas. A
         // persistent launch stays correct with fewer workgroups (tiles are
@@ -1074,8 +1260,9 @@ static int trace_launch(sq_ctx* c, uint32_t depth, bool node2, TraceArgs& a, con
     a.slots = L.slots;
     a.spill_cap = L.spill_cap;
     phase_begin(c, PH_TRACE);
-    int rc = node2 ? trace_launch_plan2(c->stream, L, a)
-                   : trace_launch_plan(c->stream, L, a, false, !c->wide_fetch && narrow_ok(a.node_count, a.tri_count));
+    int rc = fmt >= 4   ? trace_launch_plank(c->stream, L, a, fmt)
+             : fmt == 2 ? trace_launch_plan2(c->stream, L, a)
+                        : trace_launch_plan(c->stream, L, a, false, !c->wide_fetch && narrow_ok(a.node_count, a.tri_count));
     phase_end(c, PH_TRACE);
     return rc;
 }
@@ -1098,18 +1285,26 @@ static int trace_finish(sq_ctx* c, const sq_trace_params* p, const TraceArgs& a)
     return SQ_OK;
 }
 
-int trace_rays_device(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray* rays, uint32_t n,
-                      const sq_trace_params* p, const sq_trace_out* out, const TraceScratch* ext) {
+static int rays_device(sq_ctx* c, const Target& t, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
+                       const sq_trace_out* out, const TraceScratch* ext) {
     SQ_CHECK_HIP(hipSetDevice(c->device));
     phases_reset(c);
     TraceArgs a{};
-    SQ_TRY(target_args(b, t2, p, a));
+    SQ_TRY(target_args(t, p, a));
     a.camera = 0;
     a.nrays = n;
     a.rays = rays;
     a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts;
-    SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p, ext));
+    SQ_TRY(trace_launch(c, target_depth(t), t.fmt(), a, p, ext));
     return trace_finish(c, p, a);
+}
+
+int trace_rays_device(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray* rays, uint32_t n,
+                      const sq_trace_params* p, const sq_trace_out* out, const TraceScratch* ext) {
+    Target t;
+    t.b = b;
+    t.t2 = t2;
+    return rays_device(c, t, rays, n, p, out, ext);
 }
 
 }  // namespace sq
@@ -1118,24 +1313,26 @@ using namespace sq;
 
 extern "C" {
 
-static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_camera* cam,
-                       const sq_trace_params* p, const sq_trace_out* out, int out_on_device) {
-    if (!c || !(b || t2) || !cam || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: NULL argument");
+static int camera_impl(sq_ctx* c, const Target& t, const sq_camera* cam, const sq_trace_params* p,
+                       const sq_trace_out* out, int out_on_device) {
+    const sq_bvh* b = t.b;
+    const bool other = t.t2 || t.tk;  // a Node2 or Node<K> tree
+    if (!c || !(b || other) || !cam || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: NULL argument");
     if (cam->width == 0 || cam->height == 0 || cam->spp == 0) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: empty image");
     if (p && p->heatmap > SQ_HEATMAP_BUNDLE) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: heatmap mode %u", p->heatmap);
     if (p && p->kernel > 2) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: kernel %u", p->kernel);
     const bool heat = p && p->heatmap != SQ_HEATMAP_RAY;
     if (heat && (cam->spp != 1 || out->hit_id || out->hit_t))
         SQ_RET(SQ_ERR_UNSUPPORTED, "packet/bundle heatmaps take one sample per pixel and produce counts/rgb only");
-    if (t2 && (heat || (p && p->kernel == SQ_KERNEL_PACKETS)))
-        SQ_RET(SQ_ERR_UNSUPPORTED, "Node2 trees are traced by the per-ray kernel with ray heatmaps only");
+    if (other && (heat || (p && p->kernel == SQ_KERNEL_PACKETS)))
+        SQ_RET(SQ_ERR_UNSUPPORTED, "Node2 and Node<K> trees are traced by the per-ray kernel with ray heatmaps only");
     if (p && p->tri_test && (heat || p->kernel == SQ_KERNEL_PACKETS))
         SQ_RET(SQ_ERR_UNSUPPORTED, "precomputed triangles are tested by the per-ray kernel only");
     SQ_CHECK_HIP(hipSetDevice(c->device));
     phases_reset(c);
     const size_t npix = (size_t)cam->width * cam->height, nray = npix * cam->spp;
     TraceArgs a{};
-    SQ_TRY(target_args(b, t2, p, a));
+    SQ_TRY(target_args(t, p, a));
     a.camera = 1;
     a.ox = cam->origin.x; a.oy = cam->origin.y; a.oz = cam->origin.z;
     a.tanh = cam->tan_half_fov;
@@ -1147,12 +1344,12 @@ static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_c
     // tiles, whose ~2,500 costly tiles fell on 2,048 waves (half the waves
     // idle for the last fifth of the launch).
     const bool exhaustive = !p || p->semantics == SQ_TRACE_EXHAUSTIVE;
-    if (camera_tiling(cam->width, cam->height, cam->spp, exhaustive && (t2 || !use_packets(p)), a) >= (1ull << 31))
+    if (camera_tiling(cam->width, cam->height, cam->spp, exhaustive && (other || !use_packets(p)), a) >= (1ull << 31))
         SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: %ux%u image has too many tiles", cam->width, cam->height);
     if (out_on_device) {
         a.hit_id = out->hit_id; a.hit_t = out->hit_t; a.counts = out->counts; a.rgb = out->rgb;
         if (heat) SQ_TRY(heat_launch(c, b, a, p));
-        else SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
+        else SQ_TRY(trace_launch(c, target_depth(t), t.fmt(), a, p));
         return trace_finish(c, heat ? nullptr : p, a);
     }
     // Host outputs: stage through workspace buffers.
@@ -1166,7 +1363,7 @@ static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_c
     a.counts = out->counts ? (uint32_t*)(base + off_c) : nullptr;
     a.rgb = out->rgb ? (uint8_t*)(base + off_rgb) : nullptr;
     if (heat) SQ_TRY(heat_launch(c, b, a, p));
-    else SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
+    else SQ_TRY(trace_launch(c, target_depth(t), t.fmt(), a, p));
     if (out->hit_id) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_id, a.hit_id, nray * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->hit_t) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_t, a.hit_t, nray * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->counts) SQ_CHECK_HIP(hipMemcpyAsync(out->counts, a.counts, npix * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1174,16 +1371,16 @@ static int camera_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_c
     return trace_finish(c, heat ? nullptr : p, a);
 }
 
-static int rays_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray* rays, uint32_t n,
-                     const sq_trace_params* p, const sq_trace_out* out, int io_on_device) {
-    if (!c || !(b || t2) || !rays || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_rays: NULL argument");
+static int rays_impl(sq_ctx* c, const Target& t, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
+                     const sq_trace_out* out, int io_on_device) {
+    if (!c || !(t.b || t.t2 || t.tk) || !rays || !out) SQ_RET(SQ_ERR_INVALID, "sq_trace_rays: NULL argument");
     if (p && p->heatmap != SQ_HEATMAP_RAY) SQ_RET(SQ_ERR_UNSUPPORTED, "sq_trace_rays: heatmap mode %u", p->heatmap);
     if (n == 0) return SQ_OK;
-    if (io_on_device) return trace_rays_device(c, b, t2, rays, n, p, out, nullptr);
+    if (io_on_device) return rays_device(c, t, rays, n, p, out, nullptr);
     SQ_CHECK_HIP(hipSetDevice(c->device));
     phases_reset(c);
     TraceArgs a{};
-    SQ_TRY(target_args(b, t2, p, a));
+    SQ_TRY(target_args(t, p, a));
     a.camera = 0;
     a.nrays = n;
     const size_t nr = n;
@@ -1195,7 +1392,7 @@ static int rays_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray
     a.hit_id = out->hit_id ? (uint32_t*)(base + off_id) : nullptr;
     a.hit_t = out->hit_t ? (float*)(base + off_t) : nullptr;
     a.counts = out->counts ? (uint32_t*)(base + off_c) : nullptr;
-    SQ_TRY(trace_launch(c, target_depth(b, t2), t2 != nullptr, a, p));
+    SQ_TRY(trace_launch(c, target_depth(t), t.fmt(), a, p));
     if (out->hit_id) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_id, a.hit_id, nr * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->hit_t) SQ_CHECK_HIP(hipMemcpyAsync(out->hit_t, a.hit_t, nr * 4, hipMemcpyDeviceToHost, c->stream));
     if (out->counts) SQ_CHECK_HIP(hipMemcpyAsync(out->counts, a.counts, nr * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1205,25 +1402,49 @@ static int rays_impl(sq_ctx* c, const sq_bvh* b, const sq_bvh2* t2, const sq_ray
 int sq_trace_camera(sq_ctx* c, const sq_bvh* b, const sq_camera* cam, const sq_trace_params* p,
                     const sq_trace_out* out, int out_on_device) {
     if (!b) SQ_RET(SQ_ERR_INVALID, "sq_trace_camera: NULL tree");
-    return camera_impl(c, b, nullptr, cam, p, out, out_on_device);
+    Target t;
+    t.b = b;
+    return camera_impl(c, t, cam, p, out, out_on_device);
 }
 
 int sq_trace_rays(sq_ctx* c, const sq_bvh* b, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
                   const sq_trace_out* out, int io_on_device) {
     if (!b) SQ_RET(SQ_ERR_INVALID, "sq_trace_rays: NULL tree");
-    return rays_impl(c, b, nullptr, rays, n, p, out, io_on_device);
+    Target t;
+    t.b = b;
+    return rays_impl(c, t, rays, n, p, out, io_on_device);
 }
 
 int sq_trace2_camera(sq_ctx* c, const sq_bvh2* t, const sq_camera* cam, const sq_trace_params* p,
                      const sq_trace_out* out, int out_on_device) {
     if (!t) SQ_RET(SQ_ERR_INVALID, "sq_trace2_camera: NULL tree");
-    return camera_impl(c, nullptr, t, cam, p, out, out_on_device);
+    Target g;
+    g.t2 = t;
+    return camera_impl(c, g, cam, p, out, out_on_device);
 }
 
 int sq_trace2_rays(sq_ctx* c, const sq_bvh2* t, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
                    const sq_trace_out* out, int io_on_device) {
     if (!t) SQ_RET(SQ_ERR_INVALID, "sq_trace2_rays: NULL tree");
-    return rays_impl(c, nullptr, t, rays, n, p, out, io_on_device);
+    Target g;
+    g.t2 = t;
+    return rays_impl(c, g, rays, n, p, out, io_on_device);
+}
+
+int sq_tracek_camera(sq_ctx* c, const sq_bvhk* t, const sq_camera* cam, const sq_trace_params* p,
+                     const sq_trace_out* out, int out_on_device) {
+    if (!t) SQ_RET(SQ_ERR_INVALID, "sq_tracek_camera: NULL tree");
+    Target g;
+    g.tk = t;
+    return camera_impl(c, g, cam, p, out, out_on_device);
+}
+
+int sq_tracek_rays(sq_ctx* c, const sq_bvhk* t, const sq_ray* rays, uint32_t n, const sq_trace_params* p,
+                   const sq_trace_out* out, int io_on_device) {
+    if (!t) SQ_RET(SQ_ERR_INVALID, "sq_tracek_rays: NULL tree");
+    Target g;
+    g.tk = t;
+    return rays_impl(c, g, rays, n, p, out, io_on_device);
 }
 
 }  // extern "C"

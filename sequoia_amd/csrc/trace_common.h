@@ -46,6 +46,10 @@ struct TraceArgs {
     float rb[6];
     uint32_t root_kind, root_count;
     const sq_plucker* plk;  // non-null: precomputed triangle test
+    // Node<K> trees (K = 4, 8): 8 K words per node; root fields as for Node2
+    // (root_kind 0: nothing below the root box, 1: leaf at `root`, root_count
+    // triangles, 2: node `root` with root_count used slots)
+    const uint32_t* nodesk;
 };
 
 // Camera tile geometry of a job. `quad`: one camera traced by the per-ray
@@ -292,8 +296,9 @@ struct TracePlan {
     size_t lds_bytes, spill_words;
 };
 // coherent: one camera's primary rays (a smaller default launch).
+// fmt: 1 Node1, 2 Node2, 4 / 8 Node<K>.
 int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coherent, TracePlan* out,
-               bool node2 = false, bool ray_list = false);
+               int fmt = 1, bool ray_list = false);
 // SQ_ERR_OVERFLOW with a message naming the fault bits.
 int trace_fault(uint32_t bits, const char* what);
 // Zeroes a.work / a.stats and launches k_trace (multi: a.jobs / a.job_tiles).

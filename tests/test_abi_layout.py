@@ -16,9 +16,10 @@ PAIRS = {
     "sq_camera": L.Camera, "sq_build_params": L.BuildParams, "sq_trace_params": L.TraceParams,
     "sq_trace_out": L.TraceOut, "sq_tune_result": L.TuneResult, "sq_server_config": L.ServerConfig,
     "sq_request": L.Request, "sq_request_status": L.RequestStatus, "sq_server_stats": L.ServerStats,
-    "sq_timings": L.Timings, "sq_node2_root": L.Node2Root,
+    "sq_timings": L.Timings, "sq_node2_root": L.Node2Root, "sq_nodek_root": L.Node2Root,
 }
-ARRAYS = {"sq_node": L.NODE_DTYPE, "sq_ray": L.RAY_DTYPE, "sq_node2": L.NODE2_DTYPE, "sq_plucker": L.PLUCKER_DTYPE}
+ARRAYS = {"sq_node": L.NODE_DTYPE, "sq_ray": L.RAY_DTYPE, "sq_node2": L.NODE2_DTYPE, "sq_plucker": L.PLUCKER_DTYPE,
+          "sq_node4": L.nodek_dtype(4), "sq_node8": L.nodek_dtype(8)}
 
 
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs a C compiler")
@@ -43,6 +44,32 @@ def test_struct_layouts_match_header(tmp_path):
             assert int(out["%s.%s" % (cname, f)]) == getattr(cls, f).offset, (cname, f)
     for cname, dt in ARRAYS.items():
         assert int(out[cname]) == np.dtype(dt).itemsize, cname
+
+
+REF_SRC = "/root/reference/src"
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or not os.path.isdir(REF_SRC),
+                    reason="needs g++ and the reference's headers")
+def test_wide_node_layout_matches_reference(tmp_path):
+    """sq_node4 / sq_node8 have the field offsets of the reference's
+    template <size_t K> struct Node (aabb.h:26-32) for K = 4 and 8."""
+    src = tmp_path / "nodek.cpp"
+    src.write_text("\n".join([
+        '#include <cstdio>', '#include <cstddef>', '#include "aabb.h"', "int main() {",
+        *['std::printf("%d %zu %zu %zu %zu %zu\\n", {k}, sizeof(Node<{k}>), offsetof(Node<{k}>, min), '
+          'offsetof(Node<{k}>, flags), offsetof(Node<{k}>, max), offsetof(Node<{k}>, child));'.format(k=k)
+          for k in (4, 8)],
+        "return 0; }"]))
+    exe = tmp_path / "nodek"
+    subprocess.run(["g++", "-std=c++17", "-march=x86-64-v3", "-w", "-I", REF_SRC, str(src), "-o", str(exe)],
+                   check=True)
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        k, size, omin, oflags, omax, ochild = map(int, line.split())
+        dt = L.nodek_dtype(k)
+        assert size == dt.itemsize
+        assert (omin, oflags, omax, ochild) == (dt.fields["min_x"][1], dt.fields["flags"][1], dt.fields["max_x"][1],
+                                                dt.fields["child"][1])
 
 
 REF_MAIN_OBJ = os.path.join(ROOT, "oracle", "_ref", "main_cli.o")
