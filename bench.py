@@ -313,32 +313,49 @@ def sah_section(sq, ctx, scene, cam, nrays, reps=3):
 
 
 def closest_section(sq, ctx, scene, cam, nrays, reps=3):
-    """Closest hits of the headline frame through the Morton63 tree (Node1)
-    and through its quantised two-child conversion (Node2), device outputs;
-    the Node2 conversion time is reported with it."""
-    log("closest hits: Node1 and Node2")
+    """Closest hits of the headline frame through the Morton63 tree (Node1),
+    its quantised two-child conversion (Node2) and its wide conversions
+    (Node<4>, Node<8>), device outputs; each conversion's time is reported
+    with it, and the wide trees' exhaustive frame (their own box-test counts)
+    beside it."""
+    log("closest hits: Node1, Node2, Node<4>, Node<8>")
     b = ctx.build(scene, "morton63")
-    t0 = time.perf_counter()
-    t2 = ctx.node2(b)
-    conv_ms = (time.perf_counter() - t0) * 1e3
+    trees, conv = [(b, "node1")], {}
     d_id, d_t = ctx.alloc(nrays * 4), ctx.alloc(nrays * 4)
     out = {"workload": "headline frame, closest-hit semantics"}
     try:
-        for tree, name in ((b, "node1"), (t2, "node2")):
+        for name, make in (("node2", lambda: ctx.node2(b)), ("node4", lambda: ctx.nodek(b, 4)),
+                           ("node8", lambda: ctx.nodek(b, 8))):
+            ctx.sync()
+            t0 = time.perf_counter()
+            t = make()
+            ctx.sync()
+            conv[name] = (time.perf_counter() - t0) * 1e3
+            trees.append((t, name))
+
+        def timed(tree, sem):
             ms = []
             for r in range(reps + 1):
-                ctx.trace_camera_device(tree, cam, d_id, d_t, None, None, semantics=sq.TRACE_CLOSEST)
+                ctx.trace_camera_device(tree, cam, d_id, d_t, None, None, semantics=sem)
                 if r:
                     ms.append(ctx.timings()["trace_ms"])
-            m = float(np.median(ms))
+            return float(np.median(ms))
+
+        for tree, name in trees:
+            m = timed(tree, sq.TRACE_CLOSEST)
             out[name + "_trace_ms"] = round(m, 3)
             out[name + "_mrays_s"] = round(nrays / (m / 1e3) / 1e6, 1)
-        out["node2_convert_ms"] = round(conv_ms, 2)
-        out["node2_nodes"] = t2.info()["nodes"]
+            if name != "node1":
+                out[name + "_convert_ms"] = round(conv[name], 2)
+                out[name + "_nodes"] = tree.info()["nodes"]
+            if name in ("node4", "node8"):
+                m = timed(tree, sq.TRACE_EXHAUSTIVE)
+                out[name + "_exhaustive_trace_ms"] = round(m, 3)
     finally:
         ctx.free(d_id)
         ctx.free(d_t)
-        t2.free()
+        for tree, name in trees[1:]:
+            tree.free()
         b.free()
     return out
 
@@ -380,6 +397,25 @@ def incoherent_section(sq, ctx, reps=3):
                                        formula="32 B per node test + 36 B per triangle test (counted on the device "
                                                "in the same launch)")
         out["hit_fraction"] = round(float((r["hit_id"] != sq.MISS).mean()), 4)
+        # the same rays through the tree's Node<4> conversion (hits compared)
+        ctx.sync()
+        t0 = time.perf_counter()
+        t4 = ctx.nodek(b, 4)
+        ctx.sync()
+        out["node4_convert_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+        log("incoherent: closest-hit trace, Node<4>")
+        r4 = ctx.trace_rays(t4, rays, semantics=sq.TRACE_CLOSEST, count_work=True)
+        t = ctx.timings()
+        out["node4_closest_trace_ms"] = round(t["trace_ms"], 1)
+        out["node4_closest_mrays_s"] = round(INC_RAYS / (t["trace_ms"] / 1e3) / 1e6, 3)
+        out["node4_box_tests_per_ray"] = round(t["node_tests"] / INC_RAYS, 1)
+        out["node4_tri_tests_per_ray"] = round(t["tri_tests"] / INC_RAYS, 1)
+        alg4 = 32.0 * t["node_tests"] + 36.0 * t["tri_tests"]
+        out["node4_closest_roofline"] = dict(_rl(alg4, t["trace_ms"]), kernel="k_trace<true, false, 4, 1>",
+                                             formula="32 B per box test (a Node<4> is 4 boxes in 128 B) + 36 B "
+                                                     "per triangle test")
+        out["node4_hits_differ_from_node1"] = int((r4["hit_id"] != r["hit_id"]).sum())
+        t4.free()
         sub = rays[:INC_SUB]
         ctx.trace_rays(b, sub, count_work=True)
         t = ctx.timings()

@@ -184,7 +184,8 @@ typedef struct {
                                exhaustive Node2 steps; 16 otherwise; at most what fits); else 1..64 */
     uint32_t tri_test;    /* 0: reference Moller-Trumbore; 1: precomputed triangles (sq_bvh_plucker) */
     uint32_t lists_per_step; /* per-ray kernel, exhaustive semantics: child lists (Node1) or Node2s
-                                tested per step, 0 = default (2), else 2 or 4 */
+                                tested per step, 0 = default (2), else 2 or 4; Node<K> trees (both
+                                semantics): nodes per step, 0 = default, 1, or 2 for K = 4 */
 } sq_trace_params;
 
 /* Outputs (any may be NULL). hit_id/hit_t: one per ray (SQ_MISS / +inf on

@@ -716,13 +716,17 @@ __device__ __forceinline__ void sort_desc(float* f, uint32_t* e) {
     }
 }
 
-// One Node<K> per step: the root box on the first step, then the used slots
-// of the popped node; hit leaves are tested (compacted, first triangles
-// fetched together), hit interior slots pushed (closest: culled against the
-// best hit after the leaves, farthest first). A step pops one entry and
-// pushes at most K: (K - 1) x depth + K entries bound the stack.
-template <bool CLOSEST, int K, class TA>
+// Node<K> steps: the root box on the first step, then WL nodes per step
+// (the entries on top of the stack; closest: entries beyond the best hit are
+// dropped as they are popped), their fetches issued together; the used slots
+// are tested, hit leaves tested (compacted, first triangles fetched
+// together), hit interior slots pushed (closest: culled against the best hit
+// after the leaves, farthest first). A step pops WL entries and pushes at
+// most WL K: WL (K - 1) x depth + WL K entries bound the stack.
+template <bool CLOSEST, int K, int WL, class TA>
 __device__ __forceinline__ bool lane_stepk(const TA& a, LaneRay& L, Stack& st, uint32_t& tri_tests) {
+    static_assert(WL == 1 || (WL == 2 && K == 4), "one node per step, or two Node<4>s");
+    constexpr int N = K * WL;
     if (L.cnt) {
         L.cnt = 0;
         L.tests += 1;
@@ -745,56 +749,68 @@ __device__ __forceinline__ bool lane_stepk(const TA& a, LaneRay& L, Stack& st, u
         return st.fault == 0;
     }
     const float cull = L.best_t + L.best_t * (1.0f / 1048576.0f);
-    uint32_t e = 0;
-    while (true) {
-        if (st.sp == 0) return false;
+    uint32_t e[WL];
+    int got = 0;
+    while (got < WL && st.sp > 0) {
         float te = 0.f;
-        e = pop<CLOSEST>(st, te);
+        const uint32_t x = pop<CLOSEST>(st, te);
         if (CLOSEST && te > cull) continue;
-        break;
+        e[got++] = x;
     }
-    const uint32_t idx = e >> 3, used = (e & 7u) + 1;
-    if (idx >= a.node_count) { st.fault |= kFaultNode; return false; }
-    NodeKV<K> n;
-    load_nodek<K>(a.nodesk, idx, n);
-    L.tests += used;
-    NodeV nd[K];
-    bool h[K];
-    float f[K];
+    if (got == 0) return false;
+    uint32_t idx[WL], used[WL];
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        NodeV b;
-        for (int k = 0; k < 3; k++) { b.mn[k] = n.mn[k][j]; b.mx[k] = n.mx[k][j]; }
-        f[j] = 0.f;
-        h[j] = (uint32_t)j < used && slab(b, L.r, L.ix, L.iy, L.iz, L.fast, f[j]);
-        if (CLOSEST) h[j] = h[j] && !(f[j] > cull);
-        nd[j].leaf = (n.w[j] >> 16) & 1u;
-        nd[j].count = n.w[j] & 0xFFFFu;
-        nd[j].child = n.child[j];
+    for (int g = 0; g < WL; g++) {
+        const bool have = g < got;
+        idx[g] = have ? e[g] >> 3 : idx[0];  // in-bounds repeat, never tested
+        used[g] = have ? (e[g] & 7u) + 1 : 0u;
+        if (idx[g] >= a.node_count) { st.fault |= kFaultNode; return false; }
+    }
+    NodeKV<K> n[WL];
+#pragma unroll
+    for (int g = 0; g < WL; g++) load_nodek<K>(a.nodesk, idx[g], n[g]);
+    NodeV nd[N];
+    bool h[N];
+    float f[N];
+#pragma unroll
+    for (int g = 0; g < WL; g++) {
+        L.tests += used[g];
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            NodeV b;
+            for (int k = 0; k < 3; k++) { b.mn[k] = n[g].mn[k][j]; b.mx[k] = n[g].mx[k][j]; }
+            const int q = g * K + j;
+            f[q] = 0.f;
+            h[q] = (uint32_t)j < used[g] && slab(b, L.r, L.ix, L.iy, L.iz, L.fast, f[q]);
+            if (CLOSEST) h[q] = h[q] && !(f[q] > cull);
+            nd[q].leaf = (n[g].w[j] >> 16) & 1u;
+            nd[q].count = n[g].w[j] & 0xFFFFu;
+            nd[q].child = n[g].child[j];
+        }
     }
     bool bad = false;
-    leaf_batch<K>(a, nd, h, L, tri_tests, bad);
+    leaf_batch<N>(a, nd, h, L, tri_tests, bad);
     if (bad) { st.fault |= kFaultTri; return false; }
     if (!CLOSEST) {
 #pragma unroll
-        for (int j = 0; j < K; j++)
-            if (h[j] && !nd[j].leaf && nd[j].count) push<false>(st, (nd[j].child << 3) | (nd[j].count - 1), 0.f);
+        for (int q = 0; q < N; q++)
+            if (h[q] && !nd[q].leaf && nd[q].count) push<false>(st, (nd[q].child << 3) | (nd[q].count - 1), 0.f);
         return st.fault == 0;
     }
     const float cull2 = L.best_t + L.best_t * (1.0f / 1048576.0f);
     const float inf = __int_as_float(0x7f800000);
-    float fk[K];
-    uint32_t ek[K];
+    float fk[N];
+    uint32_t ek[N];
 #pragma unroll
-    for (int j = 0; j < K; j++) {
-        const bool c = h[j] && !nd[j].leaf && nd[j].count && !(f[j] > cull2);
-        fk[j] = c ? f[j] : -inf;  // absent entries sort last and are not pushed
-        ek[j] = c ? ((nd[j].child << 3) | (nd[j].count - 1)) : 0u;
+    for (int q = 0; q < N; q++) {
+        const bool c = h[q] && !nd[q].leaf && nd[q].count && !(f[q] > cull2);
+        fk[q] = c ? f[q] : -inf;  // absent entries sort last and are not pushed
+        ek[q] = c ? ((nd[q].child << 3) | (nd[q].count - 1)) : 0u;
     }
-    sort_desc<K>(fk, ek);
+    sort_desc<N>(fk, ek);
 #pragma unroll
-    for (int j = 0; j < K; j++)
-        if (fk[j] != -inf) push<true>(st, ek[j], fk[j]);
+    for (int q = 0; q < N; q++)
+        if (fk[q] != -inf) push<true>(st, ek[q], fk[q]);
     return st.fault == 0;
 }
 
@@ -868,7 +884,7 @@ __device__ __forceinline__ void trace_tile(const TA& a, uint32_t tile, uint32_t 
         if (active) {
             bool more;
             if constexpr (FMT >= 4) {
-                more = lane_stepk<CLOSEST, FMT>(a, L, st, tris_total);
+                more = lane_stepk<CLOSEST, FMT, WL>(a, L, st, tris_total);
             } else if constexpr (FMT == 2) {
                 more = lane_step2<CLOSEST, WL>(a, L, st, tris_total);
             } else {
@@ -1008,13 +1024,27 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     // many-job launches use the two-list kernel, which steps each job's rays
     // the way a launch of that job alone would.
     L.lists = L.closest && ray_list ? 1 : kListLists;
-    if (p && p->lists_per_step) {
+    if (wide) {
+        // Node<K> trees: nodes per step (two only for K = 4)
+        L.lists = 1;
+        if (p && p->lists_per_step) {
+            if (p->lists_per_step > (fmt == 4 ? 2u : 1u))
+                SQ_RET(SQ_ERR_INVALID, "trace lists_per_step %u (Node<%d>: 1%s)", p->lists_per_step, fmt,
+                       fmt == 4 ? " or 2" : "");
+            L.lists = p->lists_per_step;
+        }
+    } else if (p && p->lists_per_step) {
         if (p->lists_per_step != kListLists && p->lists_per_step != kWideLists)
             SQ_RET(SQ_ERR_INVALID, "trace lists_per_step %u (2 or 4)", p->lists_per_step);
         if (!L.closest) L.lists = p->lists_per_step;
     }
-    if (wide) L.lists = 1;  // one Node<K> per step
-    uint32_t slots = p && p->stack_slots ? p->stack_slots : (L.lists == kWideLists || wide ? 32 : 16);
+    // Node<K> closest hits of camera rays take 16 LDS entries per lane, other
+    // Node<K> launches 32 (1M incoherent closest rays on 10M clustered
+    // triangles, Node<4>: 614 ms at 32, 675 at 16)
+    uint32_t slots = p && p->stack_slots    ? p->stack_slots
+                     : wide                 ? (L.closest && coherent ? 16 : 32)
+                     : L.lists == kWideLists ? 32
+                                             : 16;
     uint32_t pow2 = 1;
     while (pow2 < slots) pow2 <<= 1;
     L.slots = pow2 > 64 ? 64 : pow2;
@@ -1022,7 +1052,10 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     if (L.block != 64 && L.block != 128) SQ_RET(SQ_ERR_INVALID, "trace block size %u (64 or 128)", L.block);
     L.lds_bytes = ((size_t)L.slots * (L.closest ? 2 : 1) + 1) * L.block * 4;
     int per_cu = 0;
-    if (fmt == 4)
+    if (fmt == 4 && L.lists == 2)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, L.closest ? k_trace<true, false, 4, 2> : k_trace<false, false, 4, 2>, L.block, L.lds_bytes);
+    else if (fmt == 4)
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, L.closest ? k_trace<true, false, 4, 1> : k_trace<false, false, 4, 1>, L.block, L.lds_bytes);
     else if (fmt == 8)
@@ -1071,8 +1104,9 @@ int trace_plan(sq_ctx* c, const sq_trace_params* p, uint32_t depth, bool coheren
     L.grid = (uint32_t)c->num_cus * (uint32_t)per_cu;
     if (depth == 0) depth = 64;
     // stack entries a ray can need: WL x depth + 2 WL for the exhaustive step,
-    // depth + 2 for the closest step, (K - 1) x depth + K for Node<K> steps
-    uint32_t cap = wide ? (uint32_t)(fmt - 1) * depth + (uint32_t)fmt + 16 : (L.closest ? 2 : L.lists) * depth + 16;
+    // depth + 2 for the closest step, WL ((K - 1) x depth + K) for Node<K> steps
+    uint32_t cap = wide ? L.lists * ((uint32_t)(fmt - 1) * depth + (uint32_t)fmt) + 16
+                        : (L.closest ? 2 : L.lists) * depth + 16;
     L.spill_cap = cap > L.slots ? cap - L.slots + 8 : 8;
     L.spill_words = (size_t)L.grid * L.block * L.spill_cap * (L.closest ? 2 : 1);
     *out = L;
@@ -1108,7 +1142,10 @@ int trace_launch_plan(hipStream_t stream, const TracePlan& L, const TraceArgs& a
 
 static int trace_launch_plank(hipStream_t stream, const TracePlan& L, const TraceArgs& a, int fmt) {
     hipLaunchKernelGGL(k_trace_init, dim3(1), dim3(64), 0, stream, a.work, a.stats);
-    if (fmt == 4) {
+    if (fmt == 4 && L.lists == 2) {
+        if (L.closest) launch_trace<true, false, 4, 2>(stream, L, a, false);
+        else launch_trace<false, false, 4, 2>(stream, L, a, false);
+    } else if (fmt == 4) {
         if (L.closest) launch_trace<true, false, 4, 1>(stream, L, a, false);
         else launch_trace<false, false, 4, 1>(stream, L, a, false);
     } else {

@@ -129,6 +129,36 @@ def test_nodek_large_soup(ctx, k):
     tk.free(); b.free(); s.free()
 
 
+@pytest.mark.parametrize("kind", ["morton63", "recursive"])
+def test_two_nodes_per_step(ctx, kind):
+    """Node<4> steps over two nodes give the one-node results: exhaustive
+    counts, hits and heatmaps, closest hits."""
+    say("nodek two per step", kind)
+    tris = GOLDEN_SCENES["ico69632"]()
+    s = ctx.upload(tris)
+    b = ctx.build(s, kind)
+    tk = ctx.nodek(b, 4)
+    cam = sq.pinhole_camera(tris, 200, 150, 60.0, 3.0, spp=4, jitter=1, seed=3)
+    for sem in (sq.TRACE_EXHAUSTIVE, sq.TRACE_CLOSEST):
+        one = ctx.trace_camera(tk, cam, semantics=sem, lists_per_step=1)
+        two = ctx.trace_camera(tk, cam, semantics=sem, lists_per_step=2)
+        assert np.array_equal(one["hit_id"], two["hit_id"])
+        assert np.array_equal(one["hit_t"].view(np.uint32), two["hit_t"].view(np.uint32))
+        if sem == sq.TRACE_EXHAUSTIVE:
+            assert np.array_equal(one["counts"], two["counts"])
+            assert np.array_equal(one["rgb"], two["rgb"])
+    rays = sq.gen_secondary_rays(tris, 20000, seed=4, tmin=1e-4)
+    one = ctx.trace_rays(tk, rays, semantics=sq.TRACE_CLOSEST, lists_per_step=1)
+    two = ctx.trace_rays(tk, rays, semantics=sq.TRACE_CLOSEST, lists_per_step=2)
+    assert np.array_equal(one["hit_id"], two["hit_id"])
+    t8 = ctx.nodek(b, 8)
+    with pytest.raises(sq.SqError):
+        ctx.trace_rays(t8, rays, lists_per_step=2)
+    with pytest.raises(sq.SqError):
+        ctx.trace_rays(tk, rays, lists_per_step=3)
+    t8.free(); tk.free(); b.free(); s.free()
+
+
 def test_nodek_rejects_unsupported(ctx):
     tris = GOLDEN_SCENES["ico1280"]()
     s = ctx.upload(tris)
