@@ -46,7 +46,6 @@ constexpr int kBigShift = 40;  // large-queue counter: (tasks << 40) | chunks
 // Tasks of at most kSubtree primitives are queued once and finished, their
 // whole subtree, by one wave in LDS after the level loop.
 constexpr uint32_t kSubtree = 256;
-constexpr int kSubStack = 16;
 
 struct SahTask {
     uint32_t b, e, free, self;
@@ -191,6 +190,107 @@ __device__ __forceinline__ uint32_t bin_combine(uint32_t f, uint32_t v, uint32_t
     return x > v ? x : v;
 }
 
+// Inclusive scan over lanes 0 .. nb-1 (lanes at or above nb hold the
+// identity and may end with anything) with DPP: row shifts by 1, 2, 4, 8
+// within each 16-lane row, then the row broadcasts of lanes 15 and 31 (the
+// GFX9 scan sequence), each only while the bins reach that far.
+// Lanes without a source lane combine with the operation's own identity
+// (0 for sums and maxima, all ones for minima of the unsigned images), which
+// leaves their value unchanged and lets the compiler fuse the lane move into
+// the min / max / add.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_step(uint32_t v, uint32_t f) {
+    const int id = (f != 0 && (f - 1) % 6 < 3) ? -1 : 0;
+    const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, CTRL, ROWS, 0xF, false);
+    return bin_combine(f, v, y);
+}
+
+__device__ __forceinline__ uint32_t wave_scan_bins(uint32_t v, uint32_t f, uint32_t nb) {
+    v = dpp_step<0x111, 0xF>(v, f);
+    if (nb > 2) v = dpp_step<0x112, 0xF>(v, f);
+    if (nb > 4) v = dpp_step<0x114, 0xF>(v, f);
+    if (nb > 8) v = dpp_step<0x118, 0xF>(v, f);
+    if (nb > 16) v = dpp_step<0x142, 0xA>(v, f);
+    if (nb > 32) v = dpp_step<0x143, 0xC>(v, f);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t lane_of(uint32_t v, uint32_t lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+// Full-wave inclusive scan (every lane active); returns lane 63's total.
+__device__ __forceinline__ uint32_t wave_total_bin(uint32_t v, uint32_t f) {
+    v = dpp_step<0x111, 0xF>(v, f);
+    v = dpp_step<0x112, 0xF>(v, f);
+    v = dpp_step<0x114, 0xF>(v, f);
+    v = dpp_step<0x118, 0xF>(v, f);
+    v = dpp_step<0x142, 0xA>(v, f);
+    v = dpp_step<0x143, 0xC>(v, f);
+    return lane_of(v, 63);
+}
+
+// Adds each lane's primitive (act: the lane has one; bin bi; w: its 13 bin
+// words, count 1 first) to the bins B. Called by every lane of the wave.
+// Coherent primitives share bins, and LDS atomics on one address serialise:
+// so, up to three times, the lanes that share the bin of the first lane not
+// yet added are combined by a wave reduction and added by one lane (when they
+// are 8 or more); the others use LDS atomics. The bins' operations (sum,
+// min, max of ordered-integer images) make the result independent of the
+// order and grouping.
+__device__ __forceinline__ void bin_add(uint32_t (*B)[kBinWords], bool act, uint32_t bi, const uint32_t* w) {
+    const uint32_t lane = __lane_id();
+    uint64_t rem = __ballot(act);
+    bool done = false;
+    for (int it = 0; it < 3 && rem; it++) {
+        const uint32_t first = (uint32_t)__ffsll((long long)rem) - 1u;
+        const uint32_t lead = lane_of(bi, first);
+        const bool in = act && !done && bi == lead;
+        const uint64_t m = __ballot(in);
+        rem &= ~m;
+        if (__popcll(m) < 8) continue;
+        uint32_t tot[kBinWords];
+#pragma unroll
+        for (uint32_t f = 0; f < (uint32_t)kBinWords; f++) tot[f] = wave_total_bin(in ? w[f] : bin_identity(f), f);
+        if (lane == first) {
+            uint32_t* W = B[lead];
+            atomicAdd(&W[0], tot[0]);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                atomicMin(&W[1 + k], tot[1 + k]);
+                atomicMax(&W[4 + k], tot[4 + k]);
+                atomicMin(&W[7 + k], tot[7 + k]);
+                atomicMax(&W[10 + k], tot[10 + k]);
+            }
+        }
+        done = done || in;
+    }
+    if (act && !done) {
+        uint32_t* W = B[bi];
+        atomicAdd(&W[0], 1u);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&W[1 + k], w[1 + k]);
+            atomicMax(&W[4 + k], w[4 + k]);
+            atomicMin(&W[7 + k], w[7 + k]);
+            atomicMax(&W[10 + k], w[10 + k]);
+        }
+    }
+}
+
+// The 13 bin words of one primitive: count 1, box min, box max, centroid
+// (as min and as max), ordered-integer images.
+__device__ __forceinline__ void prim_words(const float* mn, const float* mx, const float* c, uint32_t* w) {
+    w[0] = 1u;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        w[1 + k] = sqd::f2ord(mn[k]);
+        w[4 + k] = sqd::f2ord(mx[k]);
+        w[7 + k] = sqd::f2ord(c[k]);
+        w[10 + k] = sqd::f2ord(c[k]);
+    }
+}
+
 __device__ __forceinline__ void bins_clear(BinsLds& s, uint32_t nb) {
     for (uint32_t i = threadIdx.x; i < (uint32_t)kSahWaves * nb * kBinWords; i += kSahThreads) {
         uint32_t w = i / (nb * kBinWords), r = i % (nb * kBinWords);
@@ -204,24 +304,23 @@ __device__ __forceinline__ void bin_range(const SahArgs& a, uint32_t lo, uint32_
                                           BinsLds& s, bool keep) {
     const int wave = threadIdx.x >> 6;
     const uint32_t nb = a.nb;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSahThreads) {
-        uint32_t id = a.ids_in[i];
+    // every thread of the block runs the same number of iterations (bin_add
+    // needs whole waves)
+    for (uint32_t base = lo; base < hi; base += kSahThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const bool act = i < hi;
         float mn[3], mx[3], c[3];
-        prim_box(a.boxes, id, mn, mx);
+        uint32_t bi = 0, w[kBinWords];
+        if (act) {
+            prim_box(a.boxes, a.ids_in[i], mn, mx);
 #pragma unroll
-        for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
-        int bi = (int)(k1 * (sel3(c, axis) - cmin));
-        bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
-        if (keep) a.bin_of[i] = (uint8_t)bi;
-        uint32_t* B = s[wave][bi];
-        atomicAdd(&B[0], 1u);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            atomicMin(&B[1 + k], sqd::f2ord(mn[k]));
-            atomicMax(&B[4 + k], sqd::f2ord(mx[k]));
-            atomicMin(&B[7 + k], sqd::f2ord(c[k]));
-            atomicMax(&B[10 + k], sqd::f2ord(c[k]));
+            for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
+            int b = (int)(k1 * (sel3(c, axis) - cmin));
+            bi = (uint32_t)(b < 0 ? 0 : (b >= (int)nb ? (int)nb - 1 : b));
+            if (keep) a.bin_of[i] = (uint8_t)bi;
+            prim_words(mn, mx, c, w);
         }
+        bin_add(s[wave], act, bi, w);
     }
 }
 
@@ -317,39 +416,26 @@ __device__ void emit_children(const SahArgs& a, const SahTask& t, const SahPlan&
 
 // ---- small tasks: one wave per task ----
 // Lane-parallel choose_plane: lane i holds bin i; inclusive prefix unions give
-// the left side of plane i, suffix unions from lane i+1 its right side (exact
-// on the ordered-integer images). The sequential scan from plane nb-2 down to
-// 0 with a strict < keeps the smallest score and, among equal scores, the
-// highest plane; scores that are NaN or not below FLT_MAX never win, and when
-// none wins plane 0 is taken with the same boxes as the scan's fallback.
-// Inclusive scans over the wave (fixed distances: the shuffles unroll).
-__device__ __forceinline__ uint32_t wave_scan_up(uint32_t v, uint32_t f, uint32_t lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(v, off, 64);
-        if (lane >= (uint32_t)off) v = bin_combine(f, v, y);
-    }
-    return v;
-}
-
-__device__ __forceinline__ uint32_t wave_scan_down(uint32_t v, uint32_t f, uint32_t lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_down(v, off, 64);
-        if (lane + off < 64) v = bin_combine(f, v, y);
-    }
-    return v;
-}
-
+// the left side of plane i, and inclusive prefix unions over the bins in
+// reverse order (lane i holding bin nb - 1 - i) give the right side of plane
+// i at lane nb - 2 - i (exact on the ordered-integer images). The sequential
+// scan from plane nb-2 down to 0 with a strict < keeps the smallest score
+// and, among equal scores, the highest plane; scores that are NaN or not
+// below FLT_MAX never win, and when none wins plane 0 is taken with the same
+// boxes as the scan's fallback.
 __device__ __forceinline__ void choose_plane_wave(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
                                                   uint32_t lane, SahPlan& P) {
-    uint32_t pre[kBinWords], rgt[kBinWords];
+    uint32_t pre[kBinWords], suf[kBinWords];
 #pragma unroll
     for (uint32_t f = 0; f < (uint32_t)kBinWords; f++) {
-        const uint32_t v = lane < nb ? bins[lane][f] : bin_identity(f);
-        pre[f] = wave_scan_up(v, f, lane);
-        rgt[f] = __shfl_down(wave_scan_down(v, f, lane), 1, 64);
+        pre[f] = wave_scan_bins(lane < nb ? bins[lane][f] : bin_identity(f), f, nb);
+        suf[f] = wave_scan_bins(lane < nb ? bins[nb - 1 - lane][f] : bin_identity(f), f, nb);
     }
+    const bool valid_lane = lane + 2 <= nb;
+    const uint32_t src = valid_lane ? nb - 2 - lane : lane;
+    uint32_t rgt[7];
+#pragma unroll
+    for (int f = 0; f < 7; f++) rgt[f] = (uint32_t)__shfl((int)suf[f], (int)src, 64);
     float lp[6], rp[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -358,24 +444,71 @@ __device__ __forceinline__ void choose_plane_wave(const uint32_t (*bins)[kBinWor
     }
     const float score = __fmaf_rn(box_sa(lp, lp + 3, sa_x2), (float)(int)pre[0],
                                   box_sa(rp, rp + 3, sa_x2) * (float)(int)rgt[0]);
-    const bool valid = lane + 2 <= nb && score < 3.402823466e+38f;
+    const bool valid = valid_lane && score < 3.402823466e+38f;
     float best = valid ? score : 3.402823466e+38f;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, off, 64));
     const uint64_t win = __ballot(valid && score == best);
     const uint32_t plane = win ? 63u - (uint32_t)__clzll((long long)win) : 0u;
+    const uint32_t rl = nb - 2 - plane;
     P.plane = plane;
-    P.m = __shfl(pre[0], plane, 64);
+    P.m = lane_of(pre[0], plane);
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        P.lp[k] = sqd::ord2f(__shfl(pre[1 + k], plane, 64));
-        P.lc[k] = sqd::ord2f(__shfl(pre[7 + k], plane, 64));
-        P.rp[k] = sqd::ord2f(__shfl(rgt[1 + k], plane, 64));
-        P.rc[k] = sqd::ord2f(__shfl(rgt[7 + k], plane, 64));
+        P.lp[k] = sqd::ord2f(lane_of(pre[1 + k], plane));
+        P.lc[k] = sqd::ord2f(lane_of(pre[7 + k], plane));
+        P.rp[k] = sqd::ord2f(lane_of(suf[1 + k], rl));
+        P.rc[k] = sqd::ord2f(lane_of(suf[7 + k], rl));
     }
 }
 
-// Makes a wave's LDS and global writes visible to its other lanes.
+// choose_plane_wave for G = 64 / W groups of W (16 or 32) lanes at once,
+// group g holding its bins in rows g W .. g W + nb - 1 (nb <= W): the scans
+// and the minimum stay inside each group, whose plane is then read with
+// per-lane shuffles. Lanes of groups without a range (gv false) compute
+// nothing that is used.
+__device__ __forceinline__ void choose_plane_groups(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
+                                                    uint32_t W, bool gv, SahPlan& P) {
+    const uint32_t lane = __lane_id(), g = lane / W, gl = lane % W, row = g * W;
+    uint32_t pre[kBinWords], suf[kBinWords];
+#pragma unroll
+    for (uint32_t f = 0; f < (uint32_t)kBinWords; f++) {
+        pre[f] = wave_scan_bins(gl < nb ? bins[row + gl][f] : bin_identity(f), f, nb);
+        suf[f] = wave_scan_bins(gl < nb ? bins[row + nb - 1 - gl][f] : bin_identity(f), f, nb);
+    }
+    const bool valid_lane = gv && gl + 2 <= nb;
+    const uint32_t src = row + (valid_lane ? nb - 2 - gl : gl);
+    uint32_t rgt[7];
+#pragma unroll
+    for (int f = 0; f < 7; f++) rgt[f] = (uint32_t)__shfl((int)suf[f], (int)src, 64);
+    float lp[6], rp[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        lp[k] = sqd::ord2f(pre[1 + k]);
+        rp[k] = sqd::ord2f(rgt[1 + k]);
+    }
+    const float score = __fmaf_rn(box_sa(lp, lp + 3, sa_x2), (float)(int)pre[0],
+                                  box_sa(rp, rp + 3, sa_x2) * (float)(int)rgt[0]);
+    const bool valid = valid_lane && score < 3.402823466e+38f;
+    float best = valid ? score : 3.402823466e+38f;
+    for (uint32_t off = W >> 1; off > 0; off >>= 1) best = fminf(best, __shfl_xor(best, (int)off, 64));
+    const uint64_t win = (__ballot(valid && score == best) >> row) & (W == 64 ? ~0ull : (1ull << W) - 1ull);
+    const uint32_t plane = win ? 63u - (uint32_t)__clzll((long long)win) : 0u;
+    const uint32_t pl = row + plane, pr = row + (nb - 2 - plane);
+    P.plane = plane;
+    P.m = (uint32_t)__shfl((int)pre[0], (int)pl, 64);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        P.lp[k] = sqd::ord2f((uint32_t)__shfl((int)pre[1 + k], (int)pl, 64));
+        P.lc[k] = sqd::ord2f((uint32_t)__shfl((int)pre[7 + k], (int)pl, 64));
+        P.rp[k] = sqd::ord2f((uint32_t)__shfl((int)suf[1 + k], (int)pr, 64));
+        P.rc[k] = sqd::ord2f((uint32_t)__shfl((int)suf[7 + k], (int)pr, 64));
+    }
+}
+
+// Makes a wave's LDS and global writes visible to its other lanes (waits for
+// the wave's global stores). Where other lanes read only LDS, the kernels
+// use the LDS-only form (sqd::wave_lds_sync), which leaves stores in flight.
 __device__ __forceinline__ void wave_sync() {
     __threadfence_block();
     __builtin_amdgcn_wave_barrier();
@@ -394,27 +527,24 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
     uint32_t(*B)[kBinWords] = s_bins[wave];
 
     for (uint32_t i = lane; i < nb * kBinWords; i += 64) B[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
-    wave_sync();
-    for (uint32_t i = b + lane; i < e; i += 64) {
-        const uint32_t id = a.ids_in[i];
+    sqd::wave_lds_sync();  // LDS bins; each lane reads back only its own bin_of
+    for (uint32_t base = b; base < e; base += 64) {
+        const uint32_t i = base + lane;
+        const bool act = i < e;
         float mn[3], mx[3], c[3];
-        prim_box(a.boxes, id, mn, mx);
+        uint32_t bi = 0, w[kBinWords];
+        if (act) {
+            prim_box(a.boxes, a.ids_in[i], mn, mx);
 #pragma unroll
-        for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
-        int bi = (int)(k1 * (sel3(c, axis) - cmin));
-        bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
-        a.bin_of[i] = (uint8_t)bi;
-        uint32_t* W = B[bi];
-        atomicAdd(&W[0], 1u);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            atomicMin(&W[1 + k], sqd::f2ord(mn[k]));
-            atomicMax(&W[4 + k], sqd::f2ord(mx[k]));
-            atomicMin(&W[7 + k], sqd::f2ord(c[k]));
-            atomicMax(&W[10 + k], sqd::f2ord(c[k]));
+            for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
+            int bb = (int)(k1 * (sel3(c, axis) - cmin));
+            bi = (uint32_t)(bb < 0 ? 0 : (bb >= (int)nb ? (int)nb - 1 : bb));
+            a.bin_of[i] = (uint8_t)bi;
+            prim_words(mn, mx, c, w);
         }
+        bin_add(B, act, bi, w);
     }
-    wave_sync();
+    sqd::wave_lds_sync();  // LDS bins; each lane reads back only its own bin_of
     SahPlan P;
     choose_plane_wave(B, nb, a.sa_x2, lane, P);
     const uint32_t plane = P.plane, m = P.m;
@@ -476,13 +606,22 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
 // The range's ids and boxes are loaded once; the splits permute a list of
 // local slots (Hoare in place by the same pairing, stable through a staging
 // list), so every split sees the order the level kernels would give it, and
-// node slots follow from counts alone. Pending ranges sit on a small LDS stack
-// (the larger child is pushed first, so the smaller is split next and the
-// stack never holds more than log2(kSubtree / 9) + 2 ranges).
+// node slots follow from counts alone. The kernel never reads back what it
+// stores to global memory, so its synchronisation orders LDS only (node
+// stores stay in flight across splits). Pending ranges sit on an LDS stack
+// (the larger child is pushed first); the ranges on it are disjoint and each
+// holds more than 8 primitives, so it never holds more than kSubtree / 9.
+// A step splits the range on top of the stack with the whole wave or, when
+// it and the ranges below it are small, several at once: 2 ranges of at most
+// 32 primitives on 32-lane groups (at most 32 bins), 4 of at most 16 on
+// 16-lane groups (at most 16 bins). Each group bins, chooses its plane (the
+// DPP scans stay inside 16-lane rows and 32-lane halves) and partitions its
+// own range; the splits are independent, so the tree is the same.
 struct SubEntry {
     uint32_t lo, hi, free, self, level;
     float cb[6];
 };
+constexpr int kSubStackCap = (int)(kSubtree / 9) + 1;
 
 __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
     __shared__ float s_box[kSubtree][6];
@@ -491,7 +630,7 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
     __shared__ uint16_t s_tmp[kSubtree];
     __shared__ uint8_t s_bin[kSubtree];
     __shared__ uint32_t s_bins[kMaxBins][kBinWords];
-    __shared__ SubEntry s_stack[kSubStack];
+    __shared__ SubEntry s_stack[kSubStackCap];
     const uint32_t lane = threadIdx.x;
     if (blockIdx.x >= nsub) return;
     const SahTask T = a.sub[blockIdx.x];
@@ -517,51 +656,70 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
         s_stack[0] = E;
     }
     uint32_t sp = 1, deepest = 0;
-    const uint64_t lt = (1ull << lane) - 1;
-    wave_sync();
+    sqd::wave_lds_sync();
     while (sp > 0) {
-        sp--;
-        const SubEntry E = s_stack[sp];
-        deepest = E.level + 1 > deepest ? E.level + 1 : deepest;
+        // group width: the smallest of 16 / 32 / 64 lanes that holds the top
+        // range and its bins; then as many ranges as groups, from the top
+        const uint32_t top_cnt = s_stack[sp - 1].hi - s_stack[sp - 1].lo;
+        const uint32_t W = (top_cnt <= 16 && nb <= 16) ? 16u : (top_cnt <= 32 && nb <= 32) ? 32u : 64u;
+        uint32_t G = 1;
+        while (G < 64 / W && G < sp) {
+            const SubEntry& x = s_stack[sp - 1 - G];
+            if (x.hi - x.lo > W) break;
+            G++;
+        }
+        const uint32_t g = lane / W, gl = lane % W;
+        const bool gv = g < G;
+        const SubEntry E = s_stack[sp - 1 - (gv ? g : 0u)];
+        sp -= G;
+        deepest = gv && E.level + 1 > deepest ? E.level + 1 : deepest;
         const uint32_t lo = E.lo, hi = E.hi, cnt = hi - lo;
+        const uint32_t row = g * W;  // this group's bins: rows row .. row + nb - 1
         const int axis = choose_axis(E.cb);
         const float k1 = bin_k1(nb, E.cb, axis);
         const float cmin = sel3(E.cb, axis);
-        for (uint32_t i = lane; i < nb * kBinWords; i += 64) s_bins[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
-        wave_sync();
-        for (uint32_t j = lo + lane; j < hi; j += 64) {
-            const uint32_t sl = s_ord[j];
+        const uint64_t gmask = W == 64 ? ~0ull : ((1ull << W) - 1ull) << row;
+        const uint64_t lt = ((1ull << lane) - 1ull) & gmask;
+        if (gv)
+            for (uint32_t i = gl; i < nb * kBinWords; i += W)
+                s_bins[row + i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
+        sqd::wave_lds_sync();
+        const uint32_t iters = W == 64 ? (cnt + 63) / 64 : 1u;  // uniform: one range when W = 64
+        for (uint32_t it = 0; it < iters; it++) {
+            const uint32_t j = lo + it * W + gl;
+            const bool act = gv && it * W + gl < cnt;
             float mn[3], mx[3], c[3];
+            uint32_t bi = 0, w[kBinWords];
+            if (act) {
+                const uint32_t sl = s_ord[j];
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                mn[k] = s_box[sl][k];
-                mx[k] = s_box[sl][3 + k];
-                c[k] = (mn[k] + mx[k]) * 0.5f;
+                for (int k = 0; k < 3; k++) {
+                    mn[k] = s_box[sl][k];
+                    mx[k] = s_box[sl][3 + k];
+                    c[k] = (mn[k] + mx[k]) * 0.5f;
+                }
+                int bb = (int)(k1 * (sel3(c, axis) - cmin));
+                bi = (uint32_t)(bb < 0 ? 0 : (bb >= (int)nb ? (int)nb - 1 : bb));
+                s_bin[j] = (uint8_t)bi;
+                prim_words(mn, mx, c, w);
             }
-            int bi = (int)(k1 * (sel3(c, axis) - cmin));
-            bi = bi < 0 ? 0 : (bi >= (int)nb ? (int)nb - 1 : bi);
-            s_bin[j] = (uint8_t)bi;
-            uint32_t* W = s_bins[bi];
-            atomicAdd(&W[0], 1u);
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                atomicMin(&W[1 + k], sqd::f2ord(mn[k]));
-                atomicMax(&W[4 + k], sqd::f2ord(mx[k]));
-                atomicMin(&W[7 + k], sqd::f2ord(c[k]));
-                atomicMax(&W[10 + k], sqd::f2ord(c[k]));
-            }
+            bin_add(s_bins, act, row + bi, w);
         }
-        wave_sync();
+        sqd::wave_lds_sync();
         SahPlan P;
-        choose_plane_wave(s_bins, nb, a.sa_x2, lane, P);
+        if (W == 64) {
+            choose_plane_wave(s_bins, nb, a.sa_x2, lane, P);
+        } else {
+            choose_plane_groups(s_bins, nb, a.sa_x2, W, gv, P);
+        }
         const uint32_t plane = P.plane, m = P.m;
         if (a.part == 1) {
             uint32_t run_l = 0;
-            for (uint32_t base = lo; base < hi; base += 64) {
-                const uint32_t j = base + lane;
-                const bool valid = j < hi;
+            for (uint32_t it = 0; it < iters; it++) {
+                const uint32_t j = lo + it * W + gl;
+                const bool valid = gv && it * W + gl < cnt;
                 const bool isl = valid && s_bin[j] <= plane;
-                const uint64_t ml = __ballot(isl);
+                const uint64_t ml = __ballot(isl) & gmask;
                 if (valid) {
                     const uint32_t rank_l = run_l + (uint32_t)__popcll(ml & lt);
                     const uint32_t pos = isl ? rank_l : m + ((j - lo) - rank_l);
@@ -569,17 +727,18 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
                 }
                 run_l += (uint32_t)__popcll(ml);
             }
-            wave_sync();
-            for (uint32_t j = lo + lane; j < hi; j += 64) s_ord[j] = s_tmp[j];
+            sqd::wave_lds_sync();
+            if (gv)
+                for (uint32_t j = lo + gl; j < hi; j += W) s_ord[j] = s_tmp[j];
         } else {
             uint32_t run_r = 0, run_l = 0;
-            for (uint32_t base = lo; base < hi; base += 64) {
-                const uint32_t j = base + lane;
-                const bool valid = j < hi;
+            for (uint32_t it = 0; it < iters; it++) {
+                const uint32_t j = lo + it * W + gl;
+                const bool valid = gv && it * W + gl < cnt;
                 const bool right = valid && s_bin[j] > plane;
                 const bool mis_r = valid && right && j < lo + m;
                 const bool mis_l = valid && !right && j >= lo + m;
-                const uint64_t mr = __ballot(mis_r), ml = __ballot(mis_l);
+                const uint64_t mr = __ballot(mis_r) & gmask, ml = __ballot(mis_l) & gmask;
                 if (mis_r) {
                     const uint32_t r = run_r + (uint32_t)__popcll(mr & lt);
                     if (r < cnt) s_tmp[lo + r] = (uint16_t)j;
@@ -590,9 +749,9 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
                 run_r += (uint32_t)__popcll(mr);
                 run_l += (uint32_t)__popcll(ml);
             }
-            wave_sync();
-            const uint32_t k = run_r;  // == run_l
-            for (uint32_t r = lane; r < k; r += 64) {
+            sqd::wave_lds_sync();
+            const uint32_t k = gv ? run_r : 0u;  // == run_l
+            for (uint32_t r = gl; r < k; r += W) {
                 const uint32_t pr = s_tmp[lo + r], pl = s_tmp[lo + m + (k - 1 - r)];
                 if (pr - lo >= cnt || pl - lo >= cnt) continue;
                 const uint16_t x = s_ord[pr], y = s_ord[pl];
@@ -600,7 +759,7 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
                 s_ord[pl] = x;
             }
         }
-        wave_sync();
+        sqd::wave_lds_sync();
 
         // children: the slots of emit_children
         const uint32_t L = m, R = cnt - m, f = E.free;
@@ -615,30 +774,38 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
         }
         const bool left_leaf = L <= 8 || box_sa(P.lc, P.lc + 3, a.sa_x2) <= 0.0f;
         const bool right_leaf = R <= 8 || box_sa(P.rc, P.rc + 3, a.sa_x2) <= 0.0f;
-        for (uint32_t i = lane; i < cnt; i += 64) {
-            const bool is_left = i < L;
-            if (is_left ? !left_leaf : !right_leaf) continue;
-            const uint32_t slot = is_left ? left_free + i : right_free + (i - L);
-            if (slot >= a.cap) continue;
-            const uint32_t sl = s_ord[lo + i];
-            sqd::Box3 bx{{s_box[sl][0], s_box[sl][1], s_box[sl][2]}, {s_box[sl][3], s_box[sl][4], s_box[sl][5]}};
-            sqd::store_node(a.nodes, slot, bx, s_id[sl], 1, 1);
-        }
-        if (lane == 0) {
+        if (gv)
+            for (uint32_t i = gl; i < cnt; i += W) {
+                const bool is_left = i < L;
+                if (is_left ? !left_leaf : !right_leaf) continue;
+                const uint32_t slot = is_left ? left_free + i : right_free + (i - L);
+                if (slot >= a.cap) continue;
+                const uint32_t sl = s_ord[lo + i];
+                sqd::Box3 bx{{s_box[sl][0], s_box[sl][1], s_box[sl][2]}, {s_box[sl][3], s_box[sl][4], s_box[sl][5]}};
+                sqd::store_node(a.nodes, slot, bx, s_id[sl], 1, 1);
+            }
+        if (gv && gl == 0) {
             if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2);
             if (R != 1) write_inner_box(a, right_index, P.rp, P.rp + 3, right_free, right_leaf ? R : 2);
         }
-        // larger child first: the smaller one is popped next (entries built
-        // in registers and written to fixed positions: an array indexed by a
-        // running count would live in scratch memory)
-        const bool left_first = L >= R;
-        const bool push_l = !left_leaf, push_r = !right_leaf;
+        // children to the stack, groups in order, each group's larger child
+        // first (entries built in registers and written to computed
+        // positions: an array indexed by a running count would live in
+        // scratch memory)
+        const bool push_l = gv && !left_leaf, push_r = gv && !right_leaf;
         const uint32_t nc = (push_l ? 1u : 0u) + (push_r ? 1u : 0u);
-        if (sp + nc > (uint32_t)kSubStack) {
+        const uint64_t lead = __ballot(gv && gl == 0);
+        const uint64_t one = __ballot(gv && gl == 0 && nc >= 1), two = __ballot(gv && gl == 0 && nc == 2);
+        const uint64_t before = (1ull << row) - 1ull;
+        const uint32_t base = sp + (uint32_t)__popcll(one & before) + (uint32_t)__popcll(two & before);
+        const uint32_t total = (uint32_t)__popcll(one) + (uint32_t)__popcll(two);
+        (void)lead;
+        if (sp + total > (uint32_t)kSubStackCap) {  // cannot happen: disjoint ranges of more than 8
             if (lane == 0) atomicOr(&a.counters[4], 2u);
             break;
         }
-        if (lane == 0) {
+        if (gv && gl == 0 && nc) {
+            const bool left_first = L >= R;
             SubEntry cl, cr;
             cl.lo = lo; cl.hi = lo + L; cl.free = left_free; cl.self = left_index; cl.level = E.level + 1;
             cr.lo = lo + L; cr.hi = hi; cr.free = right_free; cr.self = right_index; cr.level = E.level + 1;
@@ -646,13 +813,17 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
                 cl.cb[k] = P.lc[k];
                 cr.cb[k] = P.rc[k];
             }
-            const uint32_t pos_l = sp + (left_first ? 0u : (push_r ? 1u : 0u));
-            const uint32_t pos_r = sp + (left_first ? (push_l ? 1u : 0u) : 0u);
+            const uint32_t pos_l = base + (left_first ? 0u : (push_r ? 1u : 0u));
+            const uint32_t pos_r = base + (left_first ? (push_l ? 1u : 0u) : 0u);
             if (push_l) s_stack[pos_l] = cl;
             if (push_r) s_stack[pos_r] = cr;
         }
-        sp += nc;
-        wave_sync();
+        sp += total;
+        sqd::wave_lds_sync();
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t o = (uint32_t)__shfl_xor((int)deepest, off, 64);
+        deepest = o > deepest ? o : deepest;
     }
     if (lane == 0) atomicMax(&a.counters[5], deepest);
 }
