@@ -205,14 +205,27 @@ __device__ __forceinline__ uint32_t dpp_step(uint32_t v, uint32_t f) {
     return bin_combine(f, v, y);
 }
 
-__device__ __forceinline__ uint32_t wave_scan_bins(uint32_t v, uint32_t f, uint32_t nb) {
-    v = dpp_step<0x111, 0xF>(v, f);
-    if (nb > 2) v = dpp_step<0x112, 0xF>(v, f);
-    if (nb > 4) v = dpp_step<0x114, 0xF>(v, f);
-    if (nb > 8) v = dpp_step<0x118, 0xF>(v, f);
-    if (nb > 16) v = dpp_step<0x142, 0xA>(v, f);
-    if (nb > 32) v = dpp_step<0x143, 0xC>(v, f);
-    return v;
+// NBC: the bin count rounded up to a power of two (the steps it needs are
+// fixed at compile time: no branch between the steps). Both arrays (13 bin
+// words each) take each step together, so consecutive DPP reads never wait
+// on the instruction just before them.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_step_all(uint32_t* a, uint32_t* b) {
+#pragma unroll
+    for (uint32_t f = 0; f < (uint32_t)kBinWords; f++) {
+        a[f] = dpp_step<CTRL, ROWS>(a[f], f);
+        b[f] = dpp_step<CTRL, ROWS>(b[f], f);
+    }
+}
+
+template <int NBC>
+__device__ __forceinline__ void wave_scan_bins2(uint32_t* a, uint32_t* b) {
+    dpp_step_all<0x111, 0xF>(a, b);
+    if constexpr (NBC > 2) dpp_step_all<0x112, 0xF>(a, b);
+    if constexpr (NBC > 4) dpp_step_all<0x114, 0xF>(a, b);
+    if constexpr (NBC > 8) dpp_step_all<0x118, 0xF>(a, b);
+    if constexpr (NBC > 16) dpp_step_all<0x142, 0xA>(a, b);
+    if constexpr (NBC > 32) dpp_step_all<0x143, 0xC>(a, b);
 }
 
 __device__ __forceinline__ uint32_t lane_of(uint32_t v, uint32_t lane) {
@@ -423,14 +436,16 @@ __device__ void emit_children(const SahArgs& a, const SahTask& t, const SahPlan&
 // and, among equal scores, the highest plane; scores that are NaN or not
 // below FLT_MAX never win, and when none wins plane 0 is taken with the same
 // boxes as the scan's fallback.
-__device__ __forceinline__ void choose_plane_wave(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
-                                                  uint32_t lane, SahPlan& P) {
+template <int NBC>
+__device__ __forceinline__ void choose_plane_wave_t(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
+                                                    uint32_t lane, SahPlan& P) {
     uint32_t pre[kBinWords], suf[kBinWords];
 #pragma unroll
     for (uint32_t f = 0; f < (uint32_t)kBinWords; f++) {
-        pre[f] = wave_scan_bins(lane < nb ? bins[lane][f] : bin_identity(f), f, nb);
-        suf[f] = wave_scan_bins(lane < nb ? bins[nb - 1 - lane][f] : bin_identity(f), f, nb);
+        pre[f] = lane < nb ? bins[lane][f] : bin_identity(f);
+        suf[f] = lane < nb ? bins[nb - 1 - lane][f] : bin_identity(f);
     }
+    wave_scan_bins2<NBC>(pre, suf);
     const bool valid_lane = lane + 2 <= nb;
     const uint32_t src = valid_lane ? nb - 2 - lane : lane;
     uint32_t rgt[7];
@@ -467,15 +482,17 @@ __device__ __forceinline__ void choose_plane_wave(const uint32_t (*bins)[kBinWor
 // and the minimum stay inside each group, whose plane is then read with
 // per-lane shuffles. Lanes of groups without a range (gv false) compute
 // nothing that is used.
-__device__ __forceinline__ void choose_plane_groups(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
-                                                    uint32_t W, bool gv, SahPlan& P) {
+template <int NBC>
+__device__ __forceinline__ void choose_plane_groups_t(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
+                                                      uint32_t W, bool gv, SahPlan& P) {
     const uint32_t lane = __lane_id(), g = lane / W, gl = lane % W, row = g * W;
     uint32_t pre[kBinWords], suf[kBinWords];
 #pragma unroll
     for (uint32_t f = 0; f < (uint32_t)kBinWords; f++) {
-        pre[f] = wave_scan_bins(gl < nb ? bins[row + gl][f] : bin_identity(f), f, nb);
-        suf[f] = wave_scan_bins(gl < nb ? bins[row + nb - 1 - gl][f] : bin_identity(f), f, nb);
+        pre[f] = gl < nb ? bins[row + gl][f] : bin_identity(f);
+        suf[f] = gl < nb ? bins[row + nb - 1 - gl][f] : bin_identity(f);
     }
+    wave_scan_bins2<NBC>(pre, suf);
     const bool valid_lane = gv && gl + 2 <= nb;
     const uint32_t src = row + (valid_lane ? nb - 2 - gl : gl);
     uint32_t rgt[7];
@@ -504,6 +521,27 @@ __device__ __forceinline__ void choose_plane_groups(const uint32_t (*bins)[kBinW
         P.rp[k] = sqd::ord2f((uint32_t)__shfl((int)suf[1 + k], (int)pr, 64));
         P.rc[k] = sqd::ord2f((uint32_t)__shfl((int)suf[7 + k], (int)pr, 64));
     }
+}
+
+// Dispatch on the bin count rounded up to a power of two (2 .. 64).
+__device__ __forceinline__ void choose_plane_wave(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
+                                                  uint32_t lane, SahPlan& P) {
+    if (nb <= 2) choose_plane_wave_t<2>(bins, nb, sa_x2, lane, P);
+    else if (nb <= 4) choose_plane_wave_t<4>(bins, nb, sa_x2, lane, P);
+    else if (nb <= 8) choose_plane_wave_t<8>(bins, nb, sa_x2, lane, P);
+    else if (nb <= 16) choose_plane_wave_t<16>(bins, nb, sa_x2, lane, P);
+    else if (nb <= 32) choose_plane_wave_t<32>(bins, nb, sa_x2, lane, P);
+    else choose_plane_wave_t<64>(bins, nb, sa_x2, lane, P);
+}
+
+// Groups hold at most W <= 32 bins.
+__device__ __forceinline__ void choose_plane_groups(const uint32_t (*bins)[kBinWords], uint32_t nb, uint32_t sa_x2,
+                                                    uint32_t W, bool gv, SahPlan& P) {
+    if (nb <= 2) choose_plane_groups_t<2>(bins, nb, sa_x2, W, gv, P);
+    else if (nb <= 4) choose_plane_groups_t<4>(bins, nb, sa_x2, W, gv, P);
+    else if (nb <= 8) choose_plane_groups_t<8>(bins, nb, sa_x2, W, gv, P);
+    else if (nb <= 16) choose_plane_groups_t<16>(bins, nb, sa_x2, W, gv, P);
+    else choose_plane_groups_t<32>(bins, nb, sa_x2, W, gv, P);
 }
 
 // Makes a wave's LDS and global writes visible to its other lanes (waits for
