@@ -319,21 +319,30 @@ __device__ __forceinline__ void bin_range(const SahArgs& a, uint32_t lo, uint32_
     const uint32_t nb = a.nb;
     // every thread of the block runs the same number of iterations (bin_add
     // needs whole waves)
-    for (uint32_t base = lo; base < hi; base += kSahThreads) {
-        const uint32_t i = base + threadIdx.x;
-        const bool act = i < hi;
-        float mn[3], mx[3], c[3];
-        uint32_t bi = 0, w[kBinWords];
-        if (act) {
-            prim_box(a.boxes, a.ids_in[i], mn, mx);
+    for (uint32_t base = lo; base < hi; base += 4 * kSahThreads) {
+        uint32_t idv[4];
 #pragma unroll
-            for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
-            int b = (int)(k1 * (sel3(c, axis) - cmin));
-            bi = (uint32_t)(b < 0 ? 0 : (b >= (int)nb ? (int)nb - 1 : b));
-            if (keep) a.bin_of[i] = (uint8_t)bi;
-            prim_words(mn, mx, c, w);
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = base + u * kSahThreads + threadIdx.x;
+            idv[u] = a.ids_in[i < hi ? i : lo];
         }
-        bin_add(s[wave], act, bi, w);
+        float mn[4][3], mx[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++) prim_box(a.boxes, idv[u], mn[u], mx[u]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = base + u * kSahThreads + threadIdx.x;
+            const bool act = i < hi;
+            float c[3];
+            uint32_t w[kBinWords];
+#pragma unroll
+            for (int k = 0; k < 3; k++) c[k] = (mn[u][k] + mx[u][k]) * 0.5f;
+            int b = (int)(k1 * (sel3(c, axis) - cmin));
+            const uint32_t bi = (uint32_t)(b < 0 ? 0 : (b >= (int)nb ? (int)nb - 1 : b));
+            if (act && keep) a.bin_of[i] = (uint8_t)bi;
+            prim_words(mn[u], mx[u], c, w);
+            bin_add(s[wave], act, bi, w);
+        }
     }
 }
 
@@ -566,21 +575,32 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
 
     for (uint32_t i = lane; i < nb * kBinWords; i += 64) B[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
     sqd::wave_lds_sync();  // LDS bins; each lane reads back only its own bin_of
-    for (uint32_t base = b; base < e; base += 64) {
-        const uint32_t i = base + lane;
-        const bool act = i < e;
-        float mn[3], mx[3], c[3];
-        uint32_t bi = 0, w[kBinWords];
-        if (act) {
-            prim_box(a.boxes, a.ids_in[i], mn, mx);
+    // four positions per lane per step, their loads issued together (a
+    // position is binned and partitioned by the same lane: (i - b) mod 64)
+    for (uint32_t base = b; base < e; base += 4 * 64) {
+        uint32_t idv[4];
 #pragma unroll
-            for (int k = 0; k < 3; k++) c[k] = (mn[k] + mx[k]) * 0.5f;
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = base + u * 64 + lane;
+            idv[u] = a.ids_in[i < e ? i : b];
+        }
+        float mn[4][3], mx[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; u++) prim_box(a.boxes, idv[u], mn[u], mx[u]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = base + u * 64 + lane;
+            const bool act = i < e;
+            float c[3];
+            uint32_t bi = 0, w[kBinWords];
+#pragma unroll
+            for (int k = 0; k < 3; k++) c[k] = (mn[u][k] + mx[u][k]) * 0.5f;
             int bb = (int)(k1 * (sel3(c, axis) - cmin));
             bi = (uint32_t)(bb < 0 ? 0 : (bb >= (int)nb ? (int)nb - 1 : bb));
-            a.bin_of[i] = (uint8_t)bi;
-            prim_words(mn, mx, c, w);
+            if (act) a.bin_of[i] = (uint8_t)bi;
+            prim_words(mn[u], mx[u], c, w);
+            bin_add(B, act, bi, w);
         }
-        bin_add(B, act, bi, w);
     }
     sqd::wave_lds_sync();  // LDS bins; each lane reads back only its own bin_of
     SahPlan P;
@@ -591,40 +611,58 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
     const uint64_t lt = (1ull << lane) - 1;
     if (a.part == 1) {
         uint32_t run_l = 0;
-        for (uint32_t base = b; base < e; base += 64) {
-            const uint32_t i = base + lane;
-            const bool valid = i < e;
-            const uint32_t id = valid ? a.ids_in[i] : 0u;
-            const bool isl = valid && a.bin_of[i] <= plane;
-            const uint64_t ml = __ballot(isl);
-            if (valid) {
-                const uint32_t rank_l = run_l + (uint32_t)__popcll(ml & lt);
-                const uint32_t pos = isl ? b + rank_l : b + m + ((i - b) - rank_l);
-                if (pos < e) a.ids_out[pos] = id;
+        for (uint32_t base0 = b; base0 < e; base0 += 4 * 64) {
+            uint32_t idv[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t i = base0 + u * 64 + lane, ic = i < e ? i : b;
+                idv[u] = a.ids_in[ic];
+                bv[u] = a.bin_of[ic];
             }
-            run_l += (uint32_t)__popcll(ml);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t i = base0 + u * 64 + lane;
+                const bool valid = i < e;
+                const bool isl = valid && bv[u] <= plane;
+                const uint64_t ml = __ballot(isl);
+                if (valid) {
+                    const uint32_t rank_l = run_l + (uint32_t)__popcll(ml & lt);
+                    const uint32_t pos = isl ? b + rank_l : b + m + ((i - b) - rank_l);
+                    if (pos < e) a.ids_out[pos] = idv[u];
+                }
+                run_l += (uint32_t)__popcll(ml);
+            }
         }
     } else {
         uint32_t run_r = 0, run_l = 0;
-        for (uint32_t base = b; base < e; base += 64) {
-            const uint32_t i = base + lane;
-            const bool valid = i < e;
-            const uint32_t id = valid ? a.ids_in[i] : 0u;
-            const bool right = valid && a.bin_of[i] > plane;
-            const bool mis_r = valid && right && i < b + m;
-            const bool mis_l = valid && !right && i >= b + m;
-            const uint64_t mr = __ballot(mis_r), ml = __ballot(mis_l);
-            if (mis_r) {
-                const uint32_t r = run_r + (uint32_t)__popcll(mr & lt);
-                if (b + r < e) a.scratch[b + r] = i;
-            } else if (mis_l) {
-                const uint32_t r = run_l + (uint32_t)__popcll(ml & lt);
-                if (b + m + r < e) a.scratch[b + m + r] = i;
-            } else if (valid) {
-                a.ids_out[i] = id;
+        for (uint32_t base0 = b; base0 < e; base0 += 4 * 64) {
+            uint32_t idv[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t i = base0 + u * 64 + lane, ic = i < e ? i : b;
+                idv[u] = a.ids_in[ic];
+                bv[u] = a.bin_of[ic];
             }
-            run_r += (uint32_t)__popcll(mr);
-            run_l += (uint32_t)__popcll(ml);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint32_t i = base0 + u * 64 + lane;
+                const bool valid = i < e;
+                const bool right = valid && bv[u] > plane;
+                const bool mis_r = valid && right && i < b + m;
+                const bool mis_l = valid && !right && i >= b + m;
+                const uint64_t mr = __ballot(mis_r), ml = __ballot(mis_l);
+                if (mis_r) {
+                    const uint32_t r = run_r + (uint32_t)__popcll(mr & lt);
+                    if (b + r < e) a.scratch[b + r] = i;
+                } else if (mis_l) {
+                    const uint32_t r = run_l + (uint32_t)__popcll(ml & lt);
+                    if (b + m + r < e) a.scratch[b + m + r] = i;
+                } else if (valid) {
+                    a.ids_out[i] = idv[u];
+                }
+                run_r += (uint32_t)__popcll(mr);
+                run_l += (uint32_t)__popcll(ml);
+            }
         }
         wave_sync();
         const uint32_t k = run_r;  // == run_l
@@ -675,17 +713,29 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
     const uint32_t n = T.e - T.b, nb = a.nb;
     if (n > kSubtree) return;  // never queued (push_task), kept as a guard of the LDS arrays
     const uint32_t* ids = a.ids_buf[T.chunk0 & 1];
-    for (uint32_t j = lane; j < n; j += 64) {
-        const uint32_t id = ids[T.b + j];
-        float mn[3], mx[3];
-        prim_box(a.boxes, id, mn, mx);
-        s_id[j] = id;
+    {
+        // all loads of the range first (at most kSubtree / 64 per lane)
+        uint32_t idv[kSubtree / 64];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            s_box[j][k] = mn[k];
-            s_box[j][3 + k] = mx[k];
+        for (uint32_t u = 0; u < kSubtree / 64; u++) {
+            const uint32_t j = u * 64 + lane;
+            idv[u] = ids[T.b + (j < n ? j : 0u)];
         }
-        s_ord[j] = (uint16_t)j;
+        float mn[kSubtree / 64][3], mx[kSubtree / 64][3];
+#pragma unroll
+        for (uint32_t u = 0; u < kSubtree / 64; u++) prim_box(a.boxes, idv[u], mn[u], mx[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < kSubtree / 64; u++) {
+            const uint32_t j = u * 64 + lane;
+            if (j >= n) continue;
+            s_id[j] = idv[u];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                s_box[j][k] = mn[u][k];
+                s_box[j][3 + k] = mx[u][k];
+            }
+            s_ord[j] = (uint16_t)j;
+        }
     }
     if (lane == 0) {
         SubEntry E;
