@@ -689,8 +689,8 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
 // holds more than 8 primitives, so it never holds more than kSubtree / 9.
 // A step splits the range on top of the stack with the whole wave or, when
 // it and the ranges below it are small, several at once: 2 ranges of at most
-// 32 primitives on 32-lane groups (at most 32 bins), 4 of at most 16 on
-// 16-lane groups (at most 16 bins). Each group bins, chooses its plane (the
+// 128 primitives on 32-lane groups (at most 32 bins), 4 of at most 64 on
+// 16-lane groups (at most 16 bins), each lane taking up to four positions. Each group bins, chooses its plane (the
 // DPP scans stay inside 16-lane rows and 32-lane halves) and partitions its
 // own range; the splits are independent, so the tree is the same.
 struct SubEntry {
@@ -698,6 +698,8 @@ struct SubEntry {
     float cb[6];
 };
 constexpr int kSubStackCap = (int)(kSubtree / 9) + 1;
+// Ranges of up to kGroupSpan W primitives share a step on W-lane groups.
+constexpr uint32_t kGroupSpan = 4;
 
 __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
     __shared__ float s_box[kSubtree][6];
@@ -749,11 +751,13 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
         // group width: the smallest of 16 / 32 / 64 lanes that holds the top
         // range and its bins; then as many ranges as groups, from the top
         const uint32_t top_cnt = s_stack[sp - 1].hi - s_stack[sp - 1].lo;
-        const uint32_t W = (top_cnt <= 16 && nb <= 16) ? 16u : (top_cnt <= 32 && nb <= 32) ? 32u : 64u;
-        uint32_t G = 1;
+        const uint32_t W = (top_cnt <= kGroupSpan * 16 && nb <= 16) ? 16u : (top_cnt <= kGroupSpan * 32 && nb <= 32) ? 32u : 64u;
+        uint32_t G = 1, iters = (top_cnt + W - 1) / W;
         while (G < 64 / W && G < sp) {
             const SubEntry& x = s_stack[sp - 1 - G];
-            if (x.hi - x.lo > W) break;
+            const uint32_t c = x.hi - x.lo;
+            if (c > kGroupSpan * W) break;
+            iters = (c + W - 1) / W > iters ? (c + W - 1) / W : iters;
             G++;
         }
         const uint32_t g = lane / W, gl = lane % W;
@@ -772,7 +776,6 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
             for (uint32_t i = gl; i < nb * kBinWords; i += W)
                 s_bins[row + i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
         sqd::wave_lds_sync();
-        const uint32_t iters = W == 64 ? (cnt + 63) / 64 : 1u;  // uniform: one range when W = 64
         for (uint32_t it = 0; it < iters; it++) {
             const uint32_t j = lo + it * W + gl;
             const bool act = gv && it * W + gl < cnt;
