@@ -703,7 +703,6 @@ constexpr uint32_t kGroupSpan = 4;
 
 __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
     __shared__ float s_box[kSubtree][6];
-    __shared__ uint32_t s_id[kSubtree];
     __shared__ uint16_t s_ord[kSubtree];
     __shared__ uint16_t s_tmp[kSubtree];
     __shared__ uint8_t s_bin[kSubtree];
@@ -730,7 +729,6 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
         for (uint32_t u = 0; u < kSubtree / 64; u++) {
             const uint32_t j = u * 64 + lane;
             if (j >= n) continue;
-            s_id[j] = idv[u];
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 s_box[j][k] = mn[u][k];
@@ -873,7 +871,8 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
                 if (slot >= a.cap) continue;
                 const uint32_t sl = s_ord[lo + i];
                 sqd::Box3 bx{{s_box[sl][0], s_box[sl][1], s_box[sl][2]}, {s_box[sl][3], s_box[sl][4], s_box[sl][5]}};
-                sqd::store_node(a.nodes, slot, bx, s_id[sl], 1, 1);
+                // local slot sl is position T.b + sl of the id buffer (not kept in LDS)
+                sqd::store_node(a.nodes, slot, bx, ids[T.b + sl], 1, 1);
             }
         if (gv && gl == 0) {
             if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2);
