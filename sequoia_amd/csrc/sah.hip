@@ -93,7 +93,8 @@ struct SahArgs {
     // batched builds: the scenes' first primitives (nseg + 1 ascending
     // offsets). Scene s owns positions [off[s], off[s+1]) and node slots
     // [2 off[s], 2 off[s+1]); its tree is built with batch-wide slots and
-    // primitive ids, which k_batch_sah_local then makes scene-local.
+    // primitive ids, and stored child links are made scene-local as they are
+    // written (leaf ids minus off[s], child slots minus 2 off[s]).
     const uint32_t* seg_off;
     uint32_t nseg;
 };
@@ -129,19 +130,38 @@ __device__ __forceinline__ void prim_box(const float4* boxes, uint32_t id, float
     mx[0] = b.x; mx[1] = b.y; mx[2] = b.z;
 }
 
-__device__ __forceinline__ void write_leaf_box(const SahArgs& a, uint32_t slot, uint32_t id) {
+// The scene of position t of a batch: off[s] <= t < off[s + 1].
+__device__ __forceinline__ uint32_t seg_of_pos(const uint32_t* off, uint32_t nseg, uint32_t t) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// First primitive of the scene that holds position t (0 outside batches).
+__device__ __forceinline__ uint32_t scene_base(const SahArgs& a, uint32_t t) {
+    return a.seg_off ? a.seg_off[seg_of_pos(a.seg_off, a.nseg, t)] : 0u;
+}
+
+// pb: the first primitive of the node's scene in a batched build (0 for one
+// scene): leaf ids and child slots are stored scene-local (id - pb, slot -
+// 2 pb) while nodes are written at their batch-wide slots.
+__device__ __forceinline__ void write_leaf_box(const SahArgs& a, uint32_t slot, uint32_t id, uint32_t pb) {
     if (slot >= a.cap) return;
     float mn[3], mx[3];
     prim_box(a.boxes, id, mn, mx);
     sqd::Box3 bx{{mn[0], mn[1], mn[2]}, {mx[0], mx[1], mx[2]}};
-    sqd::store_node(a.nodes, slot, bx, id, 1, 1);
+    sqd::store_node(a.nodes, slot, bx, id - pb, 1, 1);
 }
 
 __device__ __forceinline__ void write_inner_box(const SahArgs& a, uint32_t slot, const float* mn, const float* mx,
-                                                uint32_t child, uint32_t count) {
+                                                uint32_t child, uint32_t count, uint32_t pb) {
     if (slot >= a.cap) return;
     sqd::Box3 bx{{mn[0], mn[1], mn[2]}, {mx[0], mx[1], mx[2]}};
-    sqd::store_node(a.nodes, slot, bx, child, count, 0);
+    sqd::store_node(a.nodes, slot, bx, child - 2 * pb, count, 0);
 }
 
 // Exclusive block scan of one value per thread; returns prefix, sets *total.
@@ -414,13 +434,14 @@ __device__ void emit_children(const SahArgs& a, const SahTask& t, const SahPlan&
     }
     const bool left_leaf = L <= 8 || box_sa(P.lc, P.lc + 3, a.sa_x2) <= 0.0f;
     const bool right_leaf = R <= 8 || box_sa(P.rc, P.rc + 3, a.sa_x2) <= 0.0f;
+    const uint32_t pb = scene_base(a, b);
     if (left_leaf)
-        for (uint32_t i = tid; i < L; i += nthr) write_leaf_box(a, left_free + i, a.ids_out[b + i]);
+        for (uint32_t i = tid; i < L; i += nthr) write_leaf_box(a, left_free + i, a.ids_out[b + i], pb);
     if (right_leaf)
-        for (uint32_t i = tid; i < R; i += nthr) write_leaf_box(a, right_free + i, a.ids_out[b + L + i]);
+        for (uint32_t i = tid; i < R; i += nthr) write_leaf_box(a, right_free + i, a.ids_out[b + L + i], pb);
     if (tid == 0) {
-        if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2);
-        if (R != 1) write_inner_box(a, right_index, P.rp, P.rp + 3, right_free, right_leaf ? R : 2);
+        if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2, pb);
+        if (R != 1) write_inner_box(a, right_index, P.rp, P.rp + 3, right_free, right_leaf ? R : 2, pb);
         if (!left_leaf) {
             SahTask c;
             c.b = b; c.e = b + L; c.free = left_free; c.self = left_index;
@@ -714,6 +735,7 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
     const uint32_t n = T.e - T.b, nb = a.nb;
     if (n > kSubtree) return;  // never queued (push_task), kept as a guard of the LDS arrays
     const uint32_t* ids = a.ids_buf[T.chunk0 & 1];
+    const uint32_t pb = scene_base(a, T.b);  // the whole subtree lies in one scene
     {
         // all loads of the range first (at most kSubtree / 64 per lane)
         uint32_t idv[kSubtree / 64];
@@ -872,11 +894,11 @@ __global__ __launch_bounds__(64) void k_sah_subtree(SahArgs a, uint32_t nsub) {
                 const uint32_t sl = s_ord[lo + i];
                 sqd::Box3 bx{{s_box[sl][0], s_box[sl][1], s_box[sl][2]}, {s_box[sl][3], s_box[sl][4], s_box[sl][5]}};
                 // local slot sl is position T.b + sl of the id buffer (not kept in LDS)
-                sqd::store_node(a.nodes, slot, bx, ids[T.b + sl], 1, 1);
+                sqd::store_node(a.nodes, slot, bx, ids[T.b + sl] - pb, 1, 1);
             }
         if (gv && gl == 0) {
-            if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2);
-            if (R != 1) write_inner_box(a, right_index, P.rp, P.rp + 3, right_free, right_leaf ? R : 2);
+            if (L != 1) write_inner_box(a, left_index, P.lp, P.lp + 3, left_free, left_leaf ? L : 2, pb);
+            if (R != 1) write_inner_box(a, right_index, P.rp, P.rp + 3, right_free, right_leaf ? R : 2, pb);
         }
         // children to the stack, groups in order, each group's larger child
         // first (entries built in registers and written to computed
@@ -1186,12 +1208,12 @@ __global__ void k_sah_root(SahArgs a, uint32_t n, const uint32_t* red) {
     decode_box(red + 6, cb, cb + 3);
     const uint32_t ri = 2 * n - 1;
     if (n <= 8 || box_sa(cb, cb + 3, a.sa_x2) <= 0.0f) {
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) write_leaf_box(a, i, a.ids_in[i]);
-        if (threadIdx.x == 0) write_inner_box(a, ri, pmn, pmx, 0, n);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) write_leaf_box(a, i, a.ids_in[i], 0);
+        if (threadIdx.x == 0) write_inner_box(a, ri, pmn, pmx, 0, n, 0);
         return;
     }
     if (threadIdx.x == 0) {
-        write_inner_box(a, ri, pmn, pmx, 0, 2);
+        write_inner_box(a, ri, pmn, pmx, 0, 2, 0);
         SahTask t;
         t.b = 0; t.e = n; t.free = 0; t.self = ri;
         for (int k = 0; k < 6; k++) t.cb[k] = cb[k];
@@ -1211,15 +1233,6 @@ __global__ void k_batch_sah_red_init(uint32_t* red, uint32_t nscenes, uint32_t* 
     if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
 }
 
-__device__ __forceinline__ uint32_t seg_of_pos(const uint32_t* off, uint32_t nseg, uint32_t t) {
-    uint32_t lo = 0, hi = nseg;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off[mid] <= t) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 __global__ __launch_bounds__(256) void k_batch_sah_setup(const sq_triangle* __restrict__ tris,
                                                          const uint32_t* __restrict__ off, uint32_t nscenes,
@@ -1288,29 +1301,16 @@ __global__ void k_batch_sah_root(SahArgs a, const uint32_t* red) {
     decode_box(red + 12 * s + 6, cb, cb + 3);
     const uint32_t nb = 2 * base, ri = nb + 2 * n - 1;
     if (n <= 8 || box_sa(cb, cb + 3, a.sa_x2) <= 0.0f) {
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) write_leaf_box(a, nb + i, a.ids_in[base + i]);
-        if (threadIdx.x == 0) write_inner_box(a, ri, pmn, pmx, nb, n);
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) write_leaf_box(a, nb + i, a.ids_in[base + i], base);
+        if (threadIdx.x == 0) write_inner_box(a, ri, pmn, pmx, nb, n, base);
         return;
     }
     if (threadIdx.x == 0) {
-        write_inner_box(a, ri, pmn, pmx, nb, 2);
+        write_inner_box(a, ri, pmn, pmx, nb, 2, base);
         SahTask t;
         t.b = base; t.e = base + n; t.free = nb; t.self = ri;
         for (int k = 0; k < 6; k++) t.cb[k] = cb[k];
         push_task(a, t);
-    }
-}
-
-// Makes a batched build's trees scene-local: the node at batch-wide slot i
-// belongs to scene s with 2 off[s] <= i < 2 off[s+1]; an interior node's
-// child slot loses 2 off[s], a leaf's primitive id off[s]. Empty slots stay.
-__global__ void k_batch_sah_local(sq_node* nodes, const uint32_t* __restrict__ off, uint32_t nscenes, uint32_t slots) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += gridDim.x * blockDim.x) {
-        uint32_t* w = (uint32_t*)(nodes + i);
-        const uint32_t cf = w[3];
-        if ((cf & 0xFFFFu) == 0) continue;  // empty slot
-        const uint32_t s = seg_of_pos(off, nscenes, i >> 1);
-        w[7] -= (cf >> 16) & 1u ? off[s] : 2 * off[s];
     }
 }
 
@@ -1401,11 +1401,11 @@ __global__ __launch_bounds__(256) void k_grid_init(SahArgs a, uint32_t cells, ui
         const uint32_t free = 2 * cells + 2 * begin;
         const bool leaf = cnt <= 8 || box_sa(cb, cb + 3, a.sa_x2) <= 0.0f;
         if (leaf)
-            for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) write_leaf_box(a, free + j, a.ids_in[begin + j]);
+            for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) write_leaf_box(a, free + j, a.ids_in[begin + j], 0);
         if (threadIdx.x == 0) {
             const float* bmn = cell_tight ? p : pmn;
             const float* bmx = cell_tight ? p + 3 : pmx;
-            write_inner_box(a, cells - 1 + i, bmn, bmx, free, leaf ? cnt : 2);
+            write_inner_box(a, cells - 1 + i, bmn, bmx, free, leaf ? cnt : 2, 0);
             if (!leaf) {
                 SahTask t;
                 t.b = begin; t.e = begin + cnt; t.free = free; t.self = cells - 1 + i;
@@ -1646,10 +1646,6 @@ int sah_build_batch(sq_ctx* c, const sq_triangle* tris, uint32_t total, const ui
     SQ_CHECK_HIP(hipGetLastError());
     uint32_t levels = 0;
     SQ_TRY(sah_levels(c, a, 0, k, &levels));
-    const uint32_t slots = 2 * total, gb = (slots + 255) / 256 < (uint32_t)c->num_cus * 8 ? (slots + 255) / 256
-                                                                                           : (uint32_t)c->num_cus * 8;
-    hipLaunchKernelGGL(k_batch_sah_local, dim3(gb), dim3(256), 0, c->stream, d_nodes, d_off, nscenes, slots);
-    SQ_CHECK_HIP(hipGetLastError());
     *depth = levels + 2;
     return SQ_OK;
 }
