@@ -175,6 +175,15 @@ int sq_ctx_create(int device, sq_ctx** out) {
             rc = SQ_ERR_HIP;
         }
     }
+    if (rc == SQ_OK) {
+        e = hipHostMalloc((void**)&c->h_mail, 256, hipHostMallocCoherent);
+        if (e == hipSuccess) std::memset(c->h_mail, 0, 256);
+        else {
+            c->h_mail = nullptr;
+            set_error("level mailbox: %s", hipGetErrorString(e));
+            rc = SQ_ERR_HIP;
+        }
+    }
     if (rc == SQ_OK) rc = noalloc_setup(device);
     if (rc != SQ_OK) {
         sq_ctx_destroy(c);
@@ -194,6 +203,7 @@ void sq_ctx_destroy(sq_ctx* c) {
                     &c->sort_os})
         b->release();
     if (c->h_sort_err) (void)hipHostFree(c->h_sort_err);
+    if (c->h_mail) (void)hipHostFree(c->h_mail);
     for (int i = 0; i < PH_COUNT; i++) {
         (void)hipEventDestroy(c->ev_begin[i]);
         (void)hipEventDestroy(c->ev_end[i]);

@@ -102,6 +102,11 @@ struct sq_ctx {
     // 32-bit offsets, so tests can compare the two forms.
     bool wide_fetch = false;
     uint32_t* h_sort_err = nullptr;  // pinned: the last single-pass sort's error word
+    // pinned, coherent: a level loop's task counts posted by the device
+    // (words 0..7) with a sequence number (word 8), read without a copy or a
+    // stream synchronisation
+    uint32_t* h_mail = nullptr;
+    uint32_t mail_seq = 0;
     bool sort_err_pending = false;
 };
 

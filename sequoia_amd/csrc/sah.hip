@@ -1132,11 +1132,6 @@ __global__ __launch_bounds__(kSahThreads) void k_big_emit(SahArgs a) {
     emit_children(a, a.big[blockIdx.x], s_plan, threadIdx.x, kSahThreads);
 }
 
-__global__ void k_sah_counters_reset(uint32_t* counters) {
-    if (threadIdx.x == 0) counters[0] = 0;
-    if (threadIdx.x == 2 || threadIdx.x == 3) counters[threadIdx.x] = 0;
-}
-
 // Per-primitive boxes, identity ids and the scene/centroid boxes (setup_reduction,
 // builder.cpp:114-147). red[0..11]: p.min, p.max, c.min, c.max (ordered ints).
 __global__ __launch_bounds__(256) void k_sah_setup(const sq_triangle* __restrict__ tris, uint32_t n,
@@ -1491,15 +1486,52 @@ static int sah_prepare(sq_ctx* c, uint32_t n, uint32_t cells, uint32_t nb, SahAr
 
 // The level loop over the queued tasks (ids in buffer `in`), then the
 // LDS subtrees; *levels: split levels of the deepest path.
+// Posts the level's counters to the context's pinned mailbox: words 0..7,
+// then (after a system-scope fence) the sequence number in word 8; and
+// zeroes the next level's queue counters (words 0, 2, 3) for the launches
+// the host makes after reading them.
+__global__ void k_sah_post(uint32_t* __restrict__ counters, uint32_t* mail, uint32_t seq) {
+    const uint32_t v = threadIdx.x < 8 ? counters[threadIdx.x] : 0u;
+    if (threadIdx.x < 8) mail[threadIdx.x] = v;
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(mail + 8, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0 || threadIdx.x == 2 || threadIdx.x == 3) counters[threadIdx.x] = 0;
+}
+
+// Waits for the mailbox to carry `seq` and copies its eight counters to h.
+// The host polls the pinned word; every few hundred polls it asks whether
+// the stream is idle, which ends the wait (with an error if the post never
+// arrived) instead of spinning on a stream that has failed or finished.
+static int wait_mail(sq_ctx* c, uint32_t seq, uint32_t* h) {
+    for (uint32_t spin = 1;; spin++) {
+        if (__atomic_load_n(&c->h_mail[8], __ATOMIC_ACQUIRE) == seq) break;
+        if ((spin & 255u) == 0) {
+            const hipError_t e = hipStreamQuery(c->stream);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(&c->h_mail[8], __ATOMIC_ACQUIRE) == seq) break;
+                SQ_RET(SQ_ERR_HIP, "sah: a level's task counts were not posted");
+            }
+            if (e != hipErrorNotReady) SQ_CHECK_HIP(e);
+        }
+    }
+    for (int i = 0; i < 8; i++) h[i] = c->h_mail[i];
+    return SQ_OK;
+}
+
 static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* levels_out) {
     uint32_t* counters = a.counters;
     SQ_TRY(pinned_ensure(c, 64));
     uint32_t* h = (uint32_t*)c->pinned;
+    if (!c->h_mail) SQ_RET(SQ_ERR_HIP, "sah: the context has no level mailbox");
     uint32_t level = 0;
     int q = 0;
     for (;;) {
-        SQ_CHECK_HIP(hipMemcpyAsync(h, counters, 32, hipMemcpyDeviceToHost, c->stream));
-        SQ_CHECK_HIP(hipStreamSynchronize(c->stream));
+        // the level's task counts, posted by the device to pinned memory
+        const uint32_t seq = ++c->mail_seq;
+        hipLaunchKernelGGL(k_sah_post, dim3(1), dim3(64), 0, c->stream, counters, c->h_mail, seq);
+        SQ_CHECK_HIP(hipGetLastError());
+        SQ_TRY(wait_mail(c, seq, h));
         if (h[4]) SQ_RET(SQ_ERR_OVERFLOW, "sah: task queue overflow at level %u", level);
         const uint32_t nsmall = h[0];
         const uint64_t packed = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
@@ -1509,7 +1541,6 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
         if (nsmall > k.qcap || nbig > k.bigcap || nchunks > k.chunkcap)
             SQ_RET(SQ_ERR_OVERFLOW, "sah: level %u holds %u + %u tasks, %llu chunks", level, nsmall, nbig,
                    (unsigned long long)nchunks);
-        hipLaunchKernelGGL(k_sah_counters_reset, dim3(1), dim3(64), 0, c->stream, counters);
         a.tasks = c->tasks[q].as<SahTask>();
         a.next = c->tasks[q ^ 1].as<SahTask>();
         a.big = c->sah[q].as<SahTask>();
