@@ -46,6 +46,11 @@ constexpr int kBigShift = 40;  // large-queue counter: (tasks << 40) | chunks
 // Tasks of at most kSubtree primitives are queued once and finished, their
 // whole subtree, by one wave in LDS after the level loop.
 constexpr uint32_t kSubtree = 256;
+// counter words posted to the host's level mailbox (the sequence number
+// follows them), and levels enqueued ahead per batch once only small
+// tasks are left
+constexpr uint32_t kMailWords = 16;
+constexpr uint32_t kAheadLevels = 4;
 
 struct SahTask {
     uint32_t b, e, free, self;
@@ -82,7 +87,14 @@ struct SahArgs {
     uint32_t child_level;   // level of queued children
     const uint32_t* ids_buf[2];
     uint32_t* counters;     // [0] next small count, [2..3] next large (packed), [4] error flag,
-                            // [5] 1 + deepest subtree level, [6] subtree tasks
+                            // [5] 1 + deepest level split by a subtree or look-ahead small-task
+                            // wave, [6] subtree tasks, [8..10] look-ahead small counts
+    uint32_t* small_ctr;    // counter of the next level's small tasks (counters + 0, or 8..10)
+    // look-ahead levels (small tasks only, enqueued before the host has seen
+    // their counts): the word holding this level's task count, and the word
+    // to clear for the level after next; null otherwise
+    const uint32_t* small_n;
+    uint32_t* small_zero;
     const uint32_t* ids_in;
     uint32_t* ids_out;
     uint32_t* scratch;      // Hoare misplaced positions
@@ -396,7 +408,7 @@ __device__ void push_task(const SahArgs& a, SahTask c) {
         c.nchunks = nch;
         a.next_big[idx] = c;
     } else if (cnt > a.sub_max) {
-        const uint32_t idx = atomicAdd(&a.counters[0], 1u);
+        const uint32_t idx = atomicAdd(a.small_ctr, 1u);
         if (idx >= a.small_cap) {
             atomicOr(&a.counters[4], 1u);
             return;
@@ -582,17 +594,13 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-__global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t ntasks) {
-    __shared__ uint32_t s_bins[kSahWaves][kMaxBins][kBinWords];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint32_t ti = blockIdx.x * kSahWaves + wave;
-    if (ti >= ntasks) return;  // whole wave; the kernel has no workgroup barrier
-    const SahTask t = a.tasks[ti];
+// One wave splits small task t (bins in B) and emits its children.
+__device__ __forceinline__ void sah_wave_split(const SahArgs& a, const SahTask& t, uint32_t lane,
+                                               uint32_t (*B)[kBinWords]) {
     const uint32_t b = t.b, e = t.e, nb = a.nb;
     const int axis = choose_axis(t.cb);
     const float k1 = bin_k1(nb, t.cb, axis);
     const float cmin = sel3(t.cb, axis);
-    uint32_t(*B)[kBinWords] = s_bins[wave];
 
     for (uint32_t i = lane; i < nb * kBinWords; i += 64) B[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
     sqd::wave_lds_sync();  // LDS bins; each lane reads back only its own bin_of
@@ -696,6 +704,32 @@ __global__ __launch_bounds__(kSahThreads) void k_sah_wave(SahArgs a, uint32_t nt
     }
     wave_sync();
     emit_children(a, t, P, lane, 64);
+}
+
+// A level's small tasks, one wave each (waves stride over the tasks when the
+// grid is smaller). A look-ahead level reads its task count from the device
+// (at most small_cap are queued), clears the counter of the level after
+// next and records its depth.
+__global__ __launch_bounds__(kSahThreads, 8) void k_sah_wave(SahArgs a, uint32_t ntasks) {
+    __shared__ uint32_t s_bins[kSahWaves][kMaxBins][kBinWords];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (a.small_n) {
+        ntasks = *a.small_n;
+        ntasks = ntasks < a.small_cap ? ntasks : a.small_cap;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *a.small_zero = 0u;
+            if (ntasks) atomicMax(&a.counters[5], a.child_level);
+        }
+    }
+    // whole waves leave the loop together; the kernel has no workgroup barrier
+    for (uint32_t ti = blockIdx.x * kSahWaves + wave; ti < ntasks; ti += gridDim.x * kSahWaves)
+        sah_wave_split(a, a.tasks[ti], lane, s_bins[wave]);
+}
+
+// First look-ahead level: its task count n in counters[8], the next two
+// counters clear.
+__global__ void k_sah_ahead_init(uint32_t* counters, uint32_t n) {
+    if (threadIdx.x < 3) counters[8 + threadIdx.x] = threadIdx.x == 0 ? n : 0u;
 }
 
 // ---- subtree tasks: one wave builds the whole subtree of a range of at
@@ -1470,6 +1504,9 @@ static int sah_prepare(sq_ctx* c, uint32_t n, uint32_t cells, uint32_t nb, SahAr
     SQ_TRY(c->aux[2].ensure((size_t)cells * kBinWords * 4 + 64));
     a.boxes = c->aux[3].as<float4>();
     a.counters = c->counters.as<uint32_t>();
+    a.small_ctr = a.counters;
+    a.small_n = nullptr;
+    a.small_zero = nullptr;
     a.scratch = c->scratch.as<uint32_t>();
     a.plans = c->sah[2].as<SahPlan>();
     a.part_bins = c->sah[3].as<uint32_t>();
@@ -1491,47 +1528,97 @@ static int sah_prepare(sq_ctx* c, uint32_t n, uint32_t cells, uint32_t nb, SahAr
 // zeroes the next level's queue counters (words 0, 2, 3) for the launches
 // the host makes after reading them.
 __global__ void k_sah_post(uint32_t* __restrict__ counters, uint32_t* mail, uint32_t seq) {
-    const uint32_t v = threadIdx.x < 8 ? counters[threadIdx.x] : 0u;
-    if (threadIdx.x < 8) mail[threadIdx.x] = v;
+    const uint32_t v = threadIdx.x < kMailWords ? counters[threadIdx.x] : 0u;
+    if (threadIdx.x < kMailWords) mail[threadIdx.x] = v;
     __threadfence_system();
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(mail + 8, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (threadIdx.x == 0)
+        __hip_atomic_store(mail + kMailWords, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     if (threadIdx.x == 0 || threadIdx.x == 2 || threadIdx.x == 3) counters[threadIdx.x] = 0;
 }
 
-// Waits for the mailbox to carry `seq` and copies its eight counters to h.
+// Waits for the mailbox to carry `seq` and copies its counters to h.
 // The host polls the pinned word; every few hundred polls it asks whether
 // the stream is idle, which ends the wait (with an error if the post never
 // arrived) instead of spinning on a stream that has failed or finished.
 static int wait_mail(sq_ctx* c, uint32_t seq, uint32_t* h) {
     for (uint32_t spin = 1;; spin++) {
-        if (__atomic_load_n(&c->h_mail[8], __ATOMIC_ACQUIRE) == seq) break;
+        if (__atomic_load_n(&c->h_mail[kMailWords], __ATOMIC_ACQUIRE) == seq) break;
         if ((spin & 255u) == 0) {
             const hipError_t e = hipStreamQuery(c->stream);
             if (e == hipSuccess) {
-                if (__atomic_load_n(&c->h_mail[8], __ATOMIC_ACQUIRE) == seq) break;
+                if (__atomic_load_n(&c->h_mail[kMailWords], __ATOMIC_ACQUIRE) == seq) break;
                 SQ_RET(SQ_ERR_HIP, "sah: a level's task counts were not posted");
             }
             if (e != hipErrorNotReady) SQ_CHECK_HIP(e);
         }
     }
-    for (int i = 0; i < 8; i++) h[i] = c->h_mail[i];
+    for (uint32_t i = 0; i < kMailWords; i++) h[i] = c->h_mail[i];
+    return SQ_OK;
+}
+
+static int post_and_wait(sq_ctx* c, uint32_t* counters, uint32_t* h) {
+    const uint32_t seq = ++c->mail_seq;
+    hipLaunchKernelGGL(k_sah_post, dim3(1), dim3(64), 0, c->stream, counters, c->h_mail, seq);
+    SQ_CHECK_HIP(hipGetLastError());
+    return wait_mail(c, seq, h);
+}
+
+// Levels below the last large task hold small tasks only, so they are
+// enqueued kAheadLevels at a time without the host seeing their counts:
+// level j reads its count from counters[8 + j % 3], queues its children's
+// into 8 + (j + 1) % 3 and clears 8 + (j + 2) % 3 (read by level j - 1,
+// counted into by level j + 1, both ordered by the stream); a level's grid
+// covers at most twice the tasks of the level before (at most the wave
+// slots of the chip, the waves striding over the rest). The host checks the
+// counts after each batch. On entry `nsmall` tasks of level `level` are
+// queued in c->tasks[q] with their ids in buffer `in`.
+static int sah_ahead_levels(sq_ctx* c, SahArgs a, int in, int q, uint32_t level, uint32_t nsmall,
+                            const SahCaps& k, uint32_t* h) {
+    uint32_t* counters = a.counters;
+    hipLaunchKernelGGL(k_sah_ahead_init, dim3(1), dim3(64), 0, c->stream, counters, nsmall);
+    const uint64_t slots = (uint64_t)c->num_cus * 32;  // 8 waves per SIMD
+    uint64_t bound = nsmall;
+    for (uint32_t j = 0;; ) {
+        for (uint32_t i = 0; i < kAheadLevels; i++, j++) {
+            a.tasks = c->tasks[q].as<SahTask>();
+            a.next = c->tasks[q ^ 1].as<SahTask>();
+            a.ids_in = c->vals[in].as<uint32_t>();
+            a.ids_out = c->vals[in ^ 1].as<uint32_t>();
+            a.out_buf = (uint32_t)(in ^ 1);
+            a.child_level = level + 1;
+            a.small_n = counters + 8 + j % 3;
+            a.small_ctr = counters + 8 + (j + 1) % 3;
+            a.small_zero = counters + 8 + (j + 2) % 3;
+            uint64_t waves = bound < k.qcap ? bound : k.qcap;
+            waves = waves < slots ? waves : slots;
+            if (waves < 1) waves = 1;
+            hipLaunchKernelGGL(k_sah_wave, dim3((uint32_t)((waves + kSahWaves - 1) / kSahWaves)), dim3(kSahThreads), 0,
+                               c->stream, a, 0u);
+            bound = bound * 2 < k.qcap ? bound * 2 : k.qcap;
+            q ^= 1;
+            in ^= 1;
+            level++;
+        }
+        SQ_CHECK_HIP(hipGetLastError());
+        SQ_TRY(post_and_wait(c, counters, h));
+        if (h[4]) SQ_RET(SQ_ERR_OVERFLOW, "sah: task queue overflow at level %u", level);
+        if (h[8 + j % 3] == 0) break;  // the next level has no task
+        if (level > 4 * 1024 * 1024) SQ_RET(SQ_ERR_OVERFLOW, "sah: runaway recursion");
+    }
     return SQ_OK;
 }
 
 static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* levels_out) {
     uint32_t* counters = a.counters;
-    SQ_TRY(pinned_ensure(c, 64));
+    SQ_TRY(pinned_ensure(c, kMailWords * 4));
     uint32_t* h = (uint32_t*)c->pinned;
     if (!c->h_mail) SQ_RET(SQ_ERR_HIP, "sah: the context has no level mailbox");
     uint32_t level = 0;
     int q = 0;
     for (;;) {
         // the level's task counts, posted by the device to pinned memory
-        const uint32_t seq = ++c->mail_seq;
-        hipLaunchKernelGGL(k_sah_post, dim3(1), dim3(64), 0, c->stream, counters, c->h_mail, seq);
-        SQ_CHECK_HIP(hipGetLastError());
-        SQ_TRY(wait_mail(c, seq, h));
+        SQ_TRY(post_and_wait(c, counters, h));
         if (h[4]) SQ_RET(SQ_ERR_OVERFLOW, "sah: task queue overflow at level %u", level);
         const uint32_t nsmall = h[0];
         const uint64_t packed = (uint64_t)h[2] | ((uint64_t)h[3] << 32);
@@ -1541,6 +1628,10 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
         if (nsmall > k.qcap || nbig > k.bigcap || nchunks > k.chunkcap)
             SQ_RET(SQ_ERR_OVERFLOW, "sah: level %u holds %u + %u tasks, %llu chunks", level, nsmall, nbig,
                    (unsigned long long)nchunks);
+        if (!nbig) {
+            SQ_TRY(sah_ahead_levels(c, a, in, q, level, nsmall, k, h));
+            break;
+        }
         a.tasks = c->tasks[q].as<SahTask>();
         a.next = c->tasks[q ^ 1].as<SahTask>();
         a.big = c->sah[q].as<SahTask>();
@@ -1569,7 +1660,8 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
         if (level > 4 * 1024 * 1024) SQ_RET(SQ_ERR_OVERFLOW, "sah: runaway recursion");
     }
     const uint32_t nsub = h[6];
-    uint32_t levels = level;
+    // counters[5]: 1 + the deepest level the look-ahead levels split
+    uint32_t levels = level > h[5] ? level : h[5];
     if (nsub) {
         if (nsub > k.qcap) SQ_RET(SQ_ERR_OVERFLOW, "sah: %u subtree tasks", nsub);
         hipLaunchKernelGGL(k_sah_subtree, dim3(nsub), dim3(64), 0, c->stream, a, nsub);
