@@ -144,7 +144,68 @@ __device__ __forceinline__ void pk_walk(const TraceArgs& a, uint64_t m0, uint32_
     }
 }
 
-// All samples of one 8x8 tile of job `a`.
+// All samples of one 8x8 tile of job `a`, as packets of 4x4 pixels x 4
+// samples (spp a multiple of 4): the rays of one packet start within a few
+// pixels of each other, so their walks share more of the tree than the 64
+// pixels of one sample do. The per-pixel counts gather in LDS.
+__device__ __forceinline__ void pk_tile4(const TraceArgs& a, uint32_t tile, uint32_t lane, uint32_t* s_enc,
+                                         uint64_t* s_msk, uint32_t* s_cnt, uint32_t& tests_total,
+                                         uint32_t& tris_total, uint32_t& faults, uint32_t* job_fail) {
+    const uint32_t tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    s_cnt[lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t q = 0; q < 4; q++) {
+        const uint32_t col = ((q & 1u) << 2) + (lane & 3u), row = ((q >> 1) << 2) + ((lane >> 2) & 3u);
+        const uint32_t px = tx * 8 + col, py = ty * 8 + row;
+        const bool valid = px < a.w && py < a.h;
+        const uint64_t m0 = __ballot(valid);
+        if (!m0) continue;
+        for (uint32_t g = 0; g < a.spp; g += 4) {
+            const uint32_t smp = g + (lane >> 4);
+            PkLane L;
+            L.tests = 0;
+            L.tri_tests = 0;
+            L.fault = 0;
+            L.best_id = SQ_MISS;
+            L.best_t = __int_as_float(0x7f800000);
+            if (valid) {
+                L.r = camera_ray(a, px, py, smp);
+            } else {
+                L.r = RayF{0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f};
+            }
+            L.ix = 1.0f / L.r.dx; L.iy = 1.0f / L.r.dy; L.iz = 1.0f / L.r.dz;
+            L.fast = isfinite(L.ix) && isfinite(L.iy) && isfinite(L.iz) && isfinite(L.r.ox) && isfinite(L.r.oy) &&
+                     isfinite(L.r.oz);
+            pk_walk(a, m0, lane, s_enc, s_msk, L);
+            if (valid) {
+                const size_t ri = ((size_t)py * a.w + px) * a.spp + smp;
+                if (a.hit_id) a.hit_id[ri] = L.best_id;
+                if (a.hit_t) a.hit_t[ri] = L.best_t;
+                atomicAdd(&s_cnt[row * 8 + col], L.tests);
+                tests_total += L.tests;
+                tris_total += L.tri_tests;
+                if (L.fault) {
+                    faults |= L.fault;
+                    if (job_fail) atomicOr(job_fail, L.fault);
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t px = tx * 8 + (lane & 7), py = ty * 8 + (lane >> 3);
+    if (px < a.w && py < a.h) {
+        const uint32_t pix_tests = s_cnt[lane];
+        const size_t p = (size_t)py * a.w + px;
+        if (a.counts) a.counts[p] = pix_tests;
+        if (a.rgb) {
+            const uint8_t b = heat_byte(pix_tests, a.spp);
+            a.rgb[3 * p] = b; a.rgb[3 * p + 1] = b; a.rgb[3 * p + 2] = b;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// All samples of one 8x8 tile of job `a`, one sample of the 64 pixels per packet.
 __device__ __forceinline__ void pk_tile(const TraceArgs& a, uint32_t tile, uint32_t lane, uint32_t* s_enc,
                                         uint64_t* s_msk, uint32_t& tests_total, uint32_t& tris_total,
                                         uint32_t& faults, uint32_t* job_fail) {
@@ -196,6 +257,7 @@ template <bool MULTI>
 __global__ __launch_bounds__(64 * kPkWaves) void k_trace_pk(TraceArgs a) {
     __shared__ uint32_t s_enc[kPkWaves][kPkStack];
     __shared__ uint64_t s_msk[kPkWaves][kPkStack];
+    __shared__ uint32_t s_cnt[kPkWaves][64];
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t tests_total = 0, tris_total = 0, faults = 0;
     while (true) {
@@ -220,8 +282,14 @@ __global__ __launch_bounds__(64 * kPkWaves) void k_trace_pk(TraceArgs a) {
                 else hi = mid;
             }
             const TraceArgs J = jobs[lo];
-            pk_tile(J, tile - jt[lo], lane, s_enc[wave], s_msk[wave], tests_total, tris_total, faults,
-                    a.job_fail ? a.job_fail + lo : nullptr);
+            if ((J.spp & 3u) == 0)
+                pk_tile4(J, tile - jt[lo], lane, s_enc[wave], s_msk[wave], s_cnt[wave], tests_total, tris_total,
+                         faults, a.job_fail ? a.job_fail + lo : nullptr);
+            else
+                pk_tile(J, tile - jt[lo], lane, s_enc[wave], s_msk[wave], tests_total, tris_total, faults,
+                        a.job_fail ? a.job_fail + lo : nullptr);
+        } else if ((a.spp & 3u) == 0) {
+            pk_tile4(a, tile, lane, s_enc[wave], s_msk[wave], s_cnt[wave], tests_total, tris_total, faults, nullptr);
         } else {
             pk_tile(a, tile, lane, s_enc[wave], s_msk[wave], tests_total, tris_total, faults, nullptr);
         }
