@@ -726,6 +726,124 @@ __global__ __launch_bounds__(kSahThreads, 8) void k_sah_wave(SahArgs a, uint32_t
         sah_wave_split(a, a.tasks[ti], lane, s_bins[wave]);
 }
 
+// A level's small tasks with one workgroup each, for levels that hold too
+// few of them to fill the chip one wave per task (the top small levels of a
+// scene of some 10^4..10^5 primitives): the four waves bin interleaved
+// quarters of the range into their own LDS bins (bin_range, which records
+// each position's bin), wave 0 chooses the plane on the folded bins, and the
+// partition ranks positions block-wide, in position order, so the ids land
+// where k_sah_wave puts them (stable: left ranks; Hoare: the r-th misplaced
+// right position swaps with the r-th misplaced left position from the end).
+// Each thread partitions the positions it binned. Look-ahead levels read
+// their task count as k_sah_wave does.
+__global__ __launch_bounds__(kSahThreads) void k_sah_group(SahArgs a, uint32_t ntasks) {
+    __shared__ BinsLds s_bins;
+    __shared__ SahPlan s_plan;
+    __shared__ uint32_t s_w[2][2][kSahWaves];  // per-wave counts, double-buffered
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (a.small_n) {
+        ntasks = *a.small_n;
+        ntasks = ntasks < a.small_cap ? ntasks : a.small_cap;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *a.small_zero = 0u;
+            if (ntasks) atomicMax(&a.counters[5], a.child_level);
+        }
+    }
+    if (blockIdx.x >= ntasks) return;  // whole workgroup
+    const SahTask t = a.tasks[blockIdx.x];
+    const uint32_t b = t.b, e = t.e;
+    const int axis = choose_axis(t.cb);
+    const float k1 = bin_k1(a.nb, t.cb, axis);
+    bins_clear(s_bins, a.nb);
+    __syncthreads();
+    bin_range(a, b, e, axis, k1, sel3(t.cb, axis), s_bins, true);
+    __syncthreads();
+    bins_merge_waves(s_bins, a.nb);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        SahPlan P;
+        choose_plane_wave(s_bins[0], a.nb, a.sa_x2, threadIdx.x, P);
+        if (threadIdx.x == 0) s_plan = P;
+    }
+    __syncthreads();
+    const SahPlan P = s_plan;
+    const uint32_t plane = P.plane, m = P.m;
+    const uint64_t lt = (1ull << lane) - 1;
+    uint32_t run0 = 0, run1 = 0, it = 0;
+    for (uint32_t base = b; base < e; base += 4 * kSahThreads) {
+        uint32_t idv[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t i = base + u * kSahThreads + threadIdx.x, ic = i < e ? i : b;
+            idv[u] = a.ids_in[ic];
+            bv[u] = a.bin_of[ic];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++, it++) {
+            const uint32_t i = base + u * kSahThreads + threadIdx.x;
+            const bool valid = i < e;
+            // stable: q0 = left; Hoare: q0 = misplaced right, q1 = misplaced left
+            bool q0, q1 = false;
+            if (a.part == 1) {
+                q0 = valid && bv[u] <= plane;
+            } else {
+                const bool right = valid && bv[u] > plane;
+                q0 = right && i < b + m;
+                q1 = valid && !right && i >= b + m;
+            }
+            const uint64_t m0 = __ballot(q0), m1 = __ballot(q1);
+            uint32_t(*W)[kSahWaves] = s_w[it & 1];
+            if (lane == 0) {
+                W[0][wave] = (uint32_t)__popcll(m0);
+                W[1][wave] = (uint32_t)__popcll(m1);
+            }
+            __syncthreads();
+            uint32_t pre0 = 0, pre1 = 0, tot0 = 0, tot1 = 0;
+#pragma unroll
+            for (int w = 0; w < kSahWaves; w++) {
+                const uint32_t c0 = W[0][w], c1 = W[1][w];
+                pre0 += (uint32_t)w < wave ? c0 : 0u;
+                pre1 += (uint32_t)w < wave ? c1 : 0u;
+                tot0 += c0;
+                tot1 += c1;
+            }
+            if (a.part == 1) {
+                if (valid) {
+                    const uint32_t rank_l = run0 + pre0 + (uint32_t)__popcll(m0 & lt);
+                    const uint32_t pos = q0 ? b + rank_l : b + m + ((i - b) - rank_l);
+                    if (pos < e) a.ids_out[pos] = idv[u];
+                }
+            } else {
+                if (q0) {
+                    const uint32_t r = run0 + pre0 + (uint32_t)__popcll(m0 & lt);
+                    if (b + r < e) a.scratch[b + r] = i;
+                } else if (q1) {
+                    const uint32_t r = run1 + pre1 + (uint32_t)__popcll(m1 & lt);
+                    if (b + m + r < e) a.scratch[b + m + r] = i;
+                } else if (valid) {
+                    a.ids_out[i] = idv[u];
+                }
+            }
+            run0 += tot0;
+            run1 += tot1;
+        }
+    }
+    if (a.part == 0) {
+        __threadfence_block();
+        __syncthreads();  // every misplaced position recorded
+        const uint32_t k = run0;  // == run1
+        for (uint32_t r = threadIdx.x; r < k; r += kSahThreads) {
+            const uint32_t pr = a.scratch[b + r], pl = a.scratch[b + m + (k - 1 - r)];
+            if (pr - b >= e - b || pl - b >= e - b) continue;
+            a.ids_out[pr] = a.ids_in[pl];
+            a.ids_out[pl] = a.ids_in[pr];
+        }
+    }
+    __threadfence_block();
+    __syncthreads();  // the range's ids, for the leaves
+    emit_children(a, t, P, threadIdx.x, kSahThreads);
+}
+
 // First look-ahead level: its task count n in counters[8], the next two
 // counters clear.
 __global__ void k_sah_ahead_init(uint32_t* counters, uint32_t n) {
@@ -1578,6 +1696,7 @@ static int sah_ahead_levels(sq_ctx* c, SahArgs a, int in, int q, uint32_t level,
     uint32_t* counters = a.counters;
     hipLaunchKernelGGL(k_sah_ahead_init, dim3(1), dim3(64), 0, c->stream, counters, nsmall);
     const uint64_t slots = (uint64_t)c->num_cus * 32;  // 8 waves per SIMD
+    const uint64_t group_max = (uint64_t)c->num_cus * 2;
     uint64_t bound = nsmall;
     for (uint32_t j = 0;; ) {
         for (uint32_t i = 0; i < kAheadLevels; i++, j++) {
@@ -1591,10 +1710,14 @@ static int sah_ahead_levels(sq_ctx* c, SahArgs a, int in, int q, uint32_t level,
             a.small_ctr = counters + 8 + (j + 1) % 3;
             a.small_zero = counters + 8 + (j + 2) % 3;
             uint64_t waves = bound < k.qcap ? bound : k.qcap;
-            waves = waves < slots ? waves : slots;
             if (waves < 1) waves = 1;
-            hipLaunchKernelGGL(k_sah_wave, dim3((uint32_t)((waves + kSahWaves - 1) / kSahWaves)), dim3(kSahThreads), 0,
-                               c->stream, a, 0u);
+            if (waves < group_max) {
+                hipLaunchKernelGGL(k_sah_group, dim3((uint32_t)waves), dim3(kSahThreads), 0, c->stream, a, 0u);
+            } else {
+                waves = waves < slots ? waves : slots;
+                hipLaunchKernelGGL(k_sah_wave, dim3((uint32_t)((waves + kSahWaves - 1) / kSahWaves)), dim3(kSahThreads),
+                                   0, c->stream, a, 0u);
+            }
             bound = bound * 2 < k.qcap ? bound * 2 : k.qcap;
             q ^= 1;
             in ^= 1;
@@ -1616,6 +1739,8 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
     if (!c->h_mail) SQ_RET(SQ_ERR_HIP, "sah: the context has no level mailbox");
     uint32_t level = 0;
     int q = 0;
+    // levels with fewer small tasks than this take a workgroup per task
+    const uint32_t group_max = (uint32_t)c->num_cus * 2;
     for (;;) {
         // the level's task counts, posted by the device to pinned memory
         SQ_TRY(post_and_wait(c, counters, h));
@@ -1641,7 +1766,9 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
         a.ids_out = c->vals[in ^ 1].as<uint32_t>();
         a.out_buf = (uint32_t)(in ^ 1);
         a.child_level = level + 1;
-        if (nsmall)
+        if (nsmall && nsmall < group_max)
+            hipLaunchKernelGGL(k_sah_group, dim3(nsmall), dim3(kSahThreads), 0, c->stream, a, nsmall);
+        else if (nsmall)
             hipLaunchKernelGGL(k_sah_wave, dim3((nsmall + kSahWaves - 1) / kSahWaves), dim3(kSahThreads), 0, c->stream, a,
                                nsmall);
         if (nbig) {
