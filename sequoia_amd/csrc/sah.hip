@@ -1696,7 +1696,7 @@ static int sah_ahead_levels(sq_ctx* c, SahArgs a, int in, int q, uint32_t level,
     uint32_t* counters = a.counters;
     hipLaunchKernelGGL(k_sah_ahead_init, dim3(1), dim3(64), 0, c->stream, counters, nsmall);
     const uint64_t slots = (uint64_t)c->num_cus * 32;  // 8 waves per SIMD
-    const uint64_t group_max = (uint64_t)c->num_cus * 8;
+    const uint64_t group_max = (uint64_t)c->num_cus * 32;
     uint64_t bound = nsmall;
     for (uint32_t j = 0;; ) {
         for (uint32_t i = 0; i < kAheadLevels; i++, j++) {
@@ -1740,7 +1740,7 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
     uint32_t level = 0;
     int q = 0;
     // levels with fewer small tasks than this take a workgroup per task
-    const uint32_t group_max = (uint32_t)c->num_cus * 8;
+    const uint32_t group_max = (uint32_t)c->num_cus * 32;
     for (;;) {
         // the level's task counts, posted by the device to pinned memory
         SQ_TRY(post_and_wait(c, counters, h));
