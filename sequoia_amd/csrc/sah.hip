@@ -469,7 +469,7 @@ __device__ void emit_children(const SahArgs& a, const SahTask& t, const SahPlan&
     }
 }
 
-// ---- small tasks: one wave per task ----
+// ---- plane choice by one wave (small tasks, large-task folds) ----
 // Lane-parallel choose_plane: lane i holds bin i; inclusive prefix unions give
 // the left side of plane i, and inclusive prefix unions over the bins in
 // reverse order (lane i holding bin nb - 1 - i) give the right side of plane
@@ -585,157 +585,17 @@ __device__ __forceinline__ void choose_plane_groups(const uint32_t (*bins)[kBinW
     else if (nb <= 16) choose_plane_groups_t<16>(bins, nb, sa_x2, W, gv, P);
     else choose_plane_groups_t<32>(bins, nb, sa_x2, W, gv, P);
 }
-
-// Makes a wave's LDS and global writes visible to its other lanes (waits for
-// the wave's global stores). Where other lanes read only LDS, the kernels
-// use the LDS-only form (sqd::wave_lds_sync), which leaves stores in flight.
-__device__ __forceinline__ void wave_sync() {
-    __threadfence_block();
-    __builtin_amdgcn_wave_barrier();
-}
-
-// One wave splits small task t (bins in B) and emits its children.
-__device__ __forceinline__ void sah_wave_split(const SahArgs& a, const SahTask& t, uint32_t lane,
-                                               uint32_t (*B)[kBinWords]) {
-    const uint32_t b = t.b, e = t.e, nb = a.nb;
-    const int axis = choose_axis(t.cb);
-    const float k1 = bin_k1(nb, t.cb, axis);
-    const float cmin = sel3(t.cb, axis);
-
-    for (uint32_t i = lane; i < nb * kBinWords; i += 64) B[i / kBinWords][i % kBinWords] = bin_identity(i % kBinWords);
-    sqd::wave_lds_sync();  // LDS bins; each lane reads back only its own bin_of
-    // four positions per lane per step, their loads issued together (a
-    // position is binned and partitioned by the same lane: (i - b) mod 64)
-    for (uint32_t base = b; base < e; base += 4 * 64) {
-        uint32_t idv[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const uint32_t i = base + u * 64 + lane;
-            idv[u] = a.ids_in[i < e ? i : b];
-        }
-        float mn[4][3], mx[4][3];
-#pragma unroll
-        for (int u = 0; u < 4; u++) prim_box(a.boxes, idv[u], mn[u], mx[u]);
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const uint32_t i = base + u * 64 + lane;
-            const bool act = i < e;
-            float c[3];
-            uint32_t bi = 0, w[kBinWords];
-#pragma unroll
-            for (int k = 0; k < 3; k++) c[k] = (mn[u][k] + mx[u][k]) * 0.5f;
-            int bb = (int)(k1 * (sel3(c, axis) - cmin));
-            bi = (uint32_t)(bb < 0 ? 0 : (bb >= (int)nb ? (int)nb - 1 : bb));
-            if (act) a.bin_of[i] = (uint8_t)bi;
-            prim_words(mn[u], mx[u], c, w);
-            bin_add(B, act, bi, w);
-        }
-    }
-    sqd::wave_lds_sync();  // LDS bins; each lane reads back only its own bin_of
-    SahPlan P;
-    choose_plane_wave(B, nb, a.sa_x2, lane, P);
-    const uint32_t plane = P.plane, m = P.m;
-
-    // partition: each lane reads back the bins it wrote (same lane mapping)
-    const uint64_t lt = (1ull << lane) - 1;
-    if (a.part == 1) {
-        uint32_t run_l = 0;
-        for (uint32_t base0 = b; base0 < e; base0 += 4 * 64) {
-            uint32_t idv[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t i = base0 + u * 64 + lane, ic = i < e ? i : b;
-                idv[u] = a.ids_in[ic];
-                bv[u] = a.bin_of[ic];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t i = base0 + u * 64 + lane;
-                const bool valid = i < e;
-                const bool isl = valid && bv[u] <= plane;
-                const uint64_t ml = __ballot(isl);
-                if (valid) {
-                    const uint32_t rank_l = run_l + (uint32_t)__popcll(ml & lt);
-                    const uint32_t pos = isl ? b + rank_l : b + m + ((i - b) - rank_l);
-                    if (pos < e) a.ids_out[pos] = idv[u];
-                }
-                run_l += (uint32_t)__popcll(ml);
-            }
-        }
-    } else {
-        uint32_t run_r = 0, run_l = 0;
-        for (uint32_t base0 = b; base0 < e; base0 += 4 * 64) {
-            uint32_t idv[4], bv[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t i = base0 + u * 64 + lane, ic = i < e ? i : b;
-                idv[u] = a.ids_in[ic];
-                bv[u] = a.bin_of[ic];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint32_t i = base0 + u * 64 + lane;
-                const bool valid = i < e;
-                const bool right = valid && bv[u] > plane;
-                const bool mis_r = valid && right && i < b + m;
-                const bool mis_l = valid && !right && i >= b + m;
-                const uint64_t mr = __ballot(mis_r), ml = __ballot(mis_l);
-                if (mis_r) {
-                    const uint32_t r = run_r + (uint32_t)__popcll(mr & lt);
-                    if (b + r < e) a.scratch[b + r] = i;
-                } else if (mis_l) {
-                    const uint32_t r = run_l + (uint32_t)__popcll(ml & lt);
-                    if (b + m + r < e) a.scratch[b + m + r] = i;
-                } else if (valid) {
-                    a.ids_out[i] = idv[u];
-                }
-                run_r += (uint32_t)__popcll(mr);
-                run_l += (uint32_t)__popcll(ml);
-            }
-        }
-        wave_sync();
-        const uint32_t k = run_r;  // == run_l
-        for (uint32_t r = lane; r < k; r += 64) {
-            const uint32_t pr = a.scratch[b + r], pl = a.scratch[b + m + (k - 1 - r)];
-            if (pr - b >= e - b || pl - b >= e - b) continue;
-            a.ids_out[pr] = a.ids_in[pl];
-            a.ids_out[pl] = a.ids_in[pr];
-        }
-    }
-    wave_sync();
-    emit_children(a, t, P, lane, 64);
-}
-
-// A level's small tasks, one wave each (waves stride over the tasks when the
-// grid is smaller). A look-ahead level reads its task count from the device
-// (at most small_cap are queued), clears the counter of the level after
-// next and records its depth.
-__global__ __launch_bounds__(kSahThreads, 8) void k_sah_wave(SahArgs a, uint32_t ntasks) {
-    __shared__ uint32_t s_bins[kSahWaves][kMaxBins][kBinWords];
-    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (a.small_n) {
-        ntasks = *a.small_n;
-        ntasks = ntasks < a.small_cap ? ntasks : a.small_cap;
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            *a.small_zero = 0u;
-            if (ntasks) atomicMax(&a.counters[5], a.child_level);
-        }
-    }
-    // whole waves leave the loop together; the kernel has no workgroup barrier
-    for (uint32_t ti = blockIdx.x * kSahWaves + wave; ti < ntasks; ti += gridDim.x * kSahWaves)
-        sah_wave_split(a, a.tasks[ti], lane, s_bins[wave]);
-}
-
-// A level's small tasks with one workgroup each, for levels that hold too
-// few of them to fill the chip one wave per task (the top small levels of a
-// scene of some 10^4..10^5 primitives): the four waves bin interleaved
-// quarters of the range into their own LDS bins (bin_range, which records
-// each position's bin), wave 0 chooses the plane on the folded bins, and the
-// partition ranks positions block-wide, in position order, so the ids land
-// where k_sah_wave puts them (stable: left ranks; Hoare: the r-th misplaced
-// right position swaps with the r-th misplaced left position from the end).
-// Each thread partitions the positions it binned. Look-ahead levels read
-// their task count as k_sah_wave does.
+// A level's small tasks (sub_max + 1 .. kBigTask primitives), one workgroup
+// each: the four waves bin interleaved quarters of the range into their own
+// LDS bins (bin_range, which records each position's bin), wave 0 chooses
+// the plane on the folded bins, and the partition ranks positions
+// block-wide in position order (stable: left ranks; Hoare: the r-th
+// misplaced right position swaps with the r-th misplaced left position from
+// the end). Each thread partitions the positions it binned. One wave per
+// task measured slower at every scene size (a 2048-primitive range is then
+// a chain of eight dependent binning steps on one wave). A look-ahead level
+// reads its task count from the device (at most small_cap are queued),
+// clears the counter of the level after next and records its depth.
 __global__ __launch_bounds__(kSahThreads) void k_sah_group(SahArgs a, uint32_t ntasks) {
     __shared__ BinsLds s_bins;
     __shared__ SahPlan s_plan;
@@ -1687,16 +1547,14 @@ static int post_and_wait(sq_ctx* c, uint32_t* counters, uint32_t* h) {
 // level j reads its count from counters[8 + j % 3], queues its children's
 // into 8 + (j + 1) % 3 and clears 8 + (j + 2) % 3 (read by level j - 1,
 // counted into by level j + 1, both ordered by the stream); a level's grid
-// covers at most twice the tasks of the level before (at most the wave
-// slots of the chip, the waves striding over the rest). The host checks the
-// counts after each batch. On entry `nsmall` tasks of level `level` are
+// covers at most twice the tasks of the level before (its workgroups past
+// the level's count leave at once). The host checks the counts after each
+// batch. On entry `nsmall` tasks of level `level` are
 // queued in c->tasks[q] with their ids in buffer `in`.
 static int sah_ahead_levels(sq_ctx* c, SahArgs a, int in, int q, uint32_t level, uint32_t nsmall,
                             const SahCaps& k, uint32_t* h) {
     uint32_t* counters = a.counters;
     hipLaunchKernelGGL(k_sah_ahead_init, dim3(1), dim3(64), 0, c->stream, counters, nsmall);
-    const uint64_t slots = (uint64_t)c->num_cus * 32;  // 8 waves per SIMD
-    const uint64_t group_max = (uint64_t)c->num_cus * 32;
     uint64_t bound = nsmall;
     for (uint32_t j = 0;; ) {
         for (uint32_t i = 0; i < kAheadLevels; i++, j++) {
@@ -1709,15 +1567,9 @@ static int sah_ahead_levels(sq_ctx* c, SahArgs a, int in, int q, uint32_t level,
             a.small_n = counters + 8 + j % 3;
             a.small_ctr = counters + 8 + (j + 1) % 3;
             a.small_zero = counters + 8 + (j + 2) % 3;
-            uint64_t waves = bound < k.qcap ? bound : k.qcap;
-            if (waves < 1) waves = 1;
-            if (waves < group_max) {
-                hipLaunchKernelGGL(k_sah_group, dim3((uint32_t)waves), dim3(kSahThreads), 0, c->stream, a, 0u);
-            } else {
-                waves = waves < slots ? waves : slots;
-                hipLaunchKernelGGL(k_sah_wave, dim3((uint32_t)((waves + kSahWaves - 1) / kSahWaves)), dim3(kSahThreads),
-                                   0, c->stream, a, 0u);
-            }
+            uint64_t groups = bound < k.qcap ? bound : k.qcap;
+            if (groups < 1) groups = 1;
+            hipLaunchKernelGGL(k_sah_group, dim3((uint32_t)groups), dim3(kSahThreads), 0, c->stream, a, 0u);
             bound = bound * 2 < k.qcap ? bound * 2 : k.qcap;
             q ^= 1;
             in ^= 1;
@@ -1739,8 +1591,6 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
     if (!c->h_mail) SQ_RET(SQ_ERR_HIP, "sah: the context has no level mailbox");
     uint32_t level = 0;
     int q = 0;
-    // levels with fewer small tasks than this take a workgroup per task
-    const uint32_t group_max = (uint32_t)c->num_cus * 32;
     for (;;) {
         // the level's task counts, posted by the device to pinned memory
         SQ_TRY(post_and_wait(c, counters, h));
@@ -1766,11 +1616,7 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
         a.ids_out = c->vals[in ^ 1].as<uint32_t>();
         a.out_buf = (uint32_t)(in ^ 1);
         a.child_level = level + 1;
-        if (nsmall && nsmall < group_max)
-            hipLaunchKernelGGL(k_sah_group, dim3(nsmall), dim3(kSahThreads), 0, c->stream, a, nsmall);
-        else if (nsmall)
-            hipLaunchKernelGGL(k_sah_wave, dim3((nsmall + kSahWaves - 1) / kSahWaves), dim3(kSahThreads), 0, c->stream, a,
-                               nsmall);
+        if (nsmall) hipLaunchKernelGGL(k_sah_group, dim3(nsmall), dim3(kSahThreads), 0, c->stream, a, nsmall);
         if (nbig) {
             const dim3 gc((uint32_t)nchunks), gt(nbig), bt(kSahThreads);
             hipLaunchKernelGGL(k_big_bin, gc, bt, 0, c->stream, a);
