@@ -16,9 +16,10 @@
 //     recurse(), right-singleton quirk included (or fixed).
 //
 // GPU structure: tasks are processed level by level. A task of up to
-// kBigTask primitives gets one wave; a larger one is cut into chunks, one
-// workgroup each, whose bins are folded by a second kernel before the plane is
-// chosen, and whose partition ranks start at the counts of earlier chunks.
+// kBigTask primitives gets one workgroup; a larger one is cut into chunks,
+// one workgroup each, whose bins are folded by a second kernel before the
+// plane is chosen, and whose partition ranks start at the counts of earlier
+// chunks. Ranges of at most sub_max primitives finish in one wave's LDS.
 // Bins are reduced with LDS atomics on order-preserving integer images of the
 // floats (min/max are exact and order-free; of +0 and -0, -0 is the smaller),
 // the plane is chosen by lane-parallel prefix/suffix unions over the bins, and
@@ -390,7 +391,7 @@ __device__ __forceinline__ void bins_merge_waves(BinsLds& s, uint32_t nb) {
 
 // Queues a child task: above kBigTask primitives on the large queue with its
 // chunk range (one packed atomic hands out the queue slot and the chunks, so
-// chunk ranges ascend with queue slots), above kSubtree on the small queue,
+// chunk ranges ascend with queue slots), above sub_max on the small queue,
 // else on the subtree queue with the buffer holding its ids and its level.
 __device__ void push_task(const SahArgs& a, SahTask c) {
     const uint32_t cnt = c.e - c.b;
