@@ -993,34 +993,48 @@ __global__ __launch_bounds__(kSahThreads) void k_big_bin(SahArgs a) {
         a.part_bins[(size_t)chunk * words + w] = s_bins[0][w / kBinWords][w % kBinWords];
 }
 
-// Folds a task's chunk bins and chooses its plane. Each thread owns bin words
-// and folds them over all chunks in registers (a chunk's words are adjacent,
-// so the threads' loads of one chunk are coalesced); LDS atomics from every
-// (chunk, word) pair serialised on the task's few hundred words (33 us for
-// the 488 chunks of a 1M-primitive root at 32 bins, against 5 us this way).
-__global__ __launch_bounds__(kSahThreads) void k_big_plane(SahArgs a) {
+// Folds a task's chunk bins and chooses its plane. The block's threads split
+// the fold: thread (part p, word w) folds word w over chunks p, p + P, ...
+// (P = threads / words parts; a chunk's words are adjacent, so a part's
+// loads of one chunk are coalesced), sixteen chunks' words fetched before
+// any is folded; the parts are folded in LDS. The root of a 1M-primitive
+// scene has 488 chunks: 87 us with one thread per word and eight loads in
+// flight. LDS atomics from every (chunk, word) pair would serialise on the
+// task's few hundred words.
+constexpr int kPlaneThreads = 1024;
+
+__global__ __launch_bounds__(kPlaneThreads) void k_big_plane(SahArgs a) {
     __shared__ uint32_t s_red[kMaxBins][kBinWords];
+    __shared__ uint32_t s_part[kPlaneThreads];
     const SahTask t = a.big[blockIdx.x];
-    const uint32_t words = a.nb * kBinWords;
+    const uint32_t words = a.nb * kBinWords;  // at most kMaxBins * kBinWords = 832
+    const uint32_t parts = kPlaneThreads / words;
+    const uint32_t w = threadIdx.x % words, p = threadIdx.x / words;
     const uint32_t* src = a.part_bins + (size_t)t.chunk0 * words;
-    for (uint32_t w = threadIdx.x; w < words; w += kSahThreads) {
+    if (p < parts) {
         const uint32_t f = w % kBinWords;
         const uint32_t op = f == 0 ? 0u : ((f - 1) % 6 < 3 ? 1u : 2u);  // sum, min, max
         uint32_t acc = bin_identity(f);
-        // eight chunks' words fetched before any is folded
-        uint32_t c = 0;
-        for (; c + 8 <= t.nchunks; c += 8) {
-            uint32_t v[8];
+        uint32_t c = p;
+        for (; c + 15 * parts < t.nchunks; c += 16 * parts) {
+            uint32_t v[16];
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = src[(size_t)(c + k) * words + w];
+            for (int k = 0; k < 16; k++) v[k] = src[(size_t)(c + k * parts) * words + w];
 #pragma unroll
-            for (int k = 0; k < 8; k++) acc = op == 0 ? acc + v[k] : (op == 1 ? min(acc, v[k]) : max(acc, v[k]));
+            for (int k = 0; k < 16; k++) acc = op == 0 ? acc + v[k] : (op == 1 ? min(acc, v[k]) : max(acc, v[k]));
         }
-        for (; c < t.nchunks; c++) {
+        for (; c < t.nchunks; c += parts) {
             const uint32_t v = src[(size_t)c * words + w];
             acc = op == 0 ? acc + v : (op == 1 ? min(acc, v) : max(acc, v));
         }
-        s_red[w / kBinWords][f] = acc;
+        s_part[p * words + w] = acc;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < words; i += kPlaneThreads) {
+        const uint32_t f = i % kBinWords;
+        uint32_t acc = s_part[i];
+        for (uint32_t q = 1; q < parts; q++) acc = bin_combine(f, acc, s_part[q * words + i]);
+        s_red[i / kBinWords][f] = acc;
     }
     __syncthreads();
     if (threadIdx.x < 64) {
@@ -1621,7 +1635,7 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
         if (nbig) {
             const dim3 gc((uint32_t)nchunks), gt(nbig), bt(kSahThreads);
             hipLaunchKernelGGL(k_big_bin, gc, bt, 0, c->stream, a);
-            hipLaunchKernelGGL(k_big_plane, gt, bt, 0, c->stream, a);
+            hipLaunchKernelGGL(k_big_plane, gt, dim3(kPlaneThreads), 0, c->stream, a);
             hipLaunchKernelGGL(k_big_count, gc, bt, 0, c->stream, a);
             hipLaunchKernelGGL(k_big_scatter, gc, bt, 0, c->stream, a);
             if (a.part == 0) hipLaunchKernelGGL(k_big_swap, gc, bt, 0, c->stream, a);
