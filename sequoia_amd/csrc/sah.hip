@@ -1037,36 +1037,83 @@ __global__ __launch_bounds__(kPlaneThreads) void k_big_plane(SahArgs a) {
         s_red[i / kBinWords][f] = acc;
     }
     __syncthreads();
+    __shared__ SahPlan s_plan;
+    __shared__ uint32_t s_sum[2][kPlaneThreads / 64];
+    __shared__ uint32_t s_mid;
     if (threadIdx.x < 64) {
         SahPlan P;
         choose_plane_wave(s_red, a.nb, a.sa_x2, threadIdx.x, P);
-        if (threadIdx.x == 0) a.plans[blockIdx.x] = P;
+        if (threadIdx.x == 0) {
+            a.plans[blockIdx.x] = P;
+            s_plan = P;
+            s_mid = 0xFFFFFFFFu;
+        }
     }
-}
-
-// Per chunk: left count (stable) or misplaced right/left counts (Hoare).
-__global__ __launch_bounds__(kSahThreads) void k_big_count(SahArgs a) {
-    __shared__ uint32_t s_tmp[kSahWaves];
-    const uint32_t chunk = blockIdx.x, ti = big_task_of(a, chunk);
-    const SahTask t = a.big[ti];
-    const uint32_t plane = a.plans[ti].plane, mid = t.b + a.plans[ti].m;
-    uint32_t lo, hi;
-    chunk_range(t, chunk - t.chunk0, lo, hi);
-    uint32_t c0 = 0, c1 = 0;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kSahThreads) {
-        const bool right = a.bin_of[i] > plane;
-        if (a.part == 1) {
-            c0 += right ? 0u : 1u;
-        } else {
+    __syncthreads();
+    // Each chunk's partition counts for the scatter and swap kernels, from
+    // its bins' counts: stable, its left count; Hoare, its misplaced right
+    // (right-side bins before the midpoint) and misplaced left counts. The
+    // one chunk that holds the midpoint counts its positions.
+    const uint32_t plane = s_plan.plane, mid = t.b + s_plan.m;
+    for (uint32_t g = threadIdx.x; g < t.nchunks; g += kPlaneThreads) {
+        uint32_t lo, hi;
+        chunk_range(t, g, lo, hi);
+        const uint32_t* cb = src + (size_t)g * words;  // bin k's count at cb[k * kBinWords]
+        uint32_t left = 0, all = 0;
+        for (uint32_t k0 = 0; k0 < a.nb; k0 += 16) {
+            uint32_t v[16];
+#pragma unroll
+            for (uint32_t k = 0; k < 16; k++) v[k] = k0 + k < a.nb ? cb[(k0 + k) * kBinWords] : 0u;
+#pragma unroll
+            for (uint32_t k = 0; k < 16; k++) {
+                all += v[k];
+                left += k0 + k <= plane ? v[k] : 0u;
+            }
+        }
+        uint32_t c0 = left, c1 = 0;
+        if (a.part == 0) {
+            if (hi <= mid) {
+                c0 = all - left;
+            } else if (lo >= mid) {
+                c0 = 0;
+                c1 = left;
+            } else {
+                s_mid = g;  // the one chunk with lo < mid < hi
+                continue;
+            }
+        }
+        a.chunk_cnt[2 * (t.chunk0 + g)] = c0;
+        a.chunk_cnt[2 * (t.chunk0 + g) + 1] = c1;
+    }
+    __syncthreads();
+    const uint32_t gm = s_mid;
+    if (a.part == 0 && gm != 0xFFFFFFFFu) {
+        uint32_t lo, hi;
+        chunk_range(t, gm, lo, hi);
+        uint32_t c0 = 0, c1 = 0;
+        for (uint32_t i = lo + threadIdx.x; i < hi; i += kPlaneThreads) {
+            const bool right = a.bin_of[i] > plane;
             c0 += (right && i < mid) ? 1u : 0u;
             c1 += (!right && i >= mid) ? 1u : 0u;
         }
-    }
-    c0 = block_sum(c0, s_tmp);
-    c1 = block_sum(c1, s_tmp);
-    if (threadIdx.x == 0) {
-        a.chunk_cnt[2 * chunk] = c0;
-        a.chunk_cnt[2 * chunk + 1] = c1;
+        for (int off = 32; off > 0; off >>= 1) {
+            c0 += __shfl_xor(c0, off, 64);
+            c1 += __shfl_xor(c1, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            s_sum[0][threadIdx.x >> 6] = c0;
+            s_sum[1][threadIdx.x >> 6] = c1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t0 = 0, t1 = 0;
+            for (int w = 0; w < kPlaneThreads / 64; w++) {
+                t0 += s_sum[0][w];
+                t1 += s_sum[1][w];
+            }
+            a.chunk_cnt[2 * (t.chunk0 + gm)] = t0;
+            a.chunk_cnt[2 * (t.chunk0 + gm) + 1] = t1;
+        }
     }
 }
 
@@ -1636,7 +1683,6 @@ static int sah_levels(sq_ctx* c, SahArgs a, int in, const SahCaps& k, uint32_t* 
             const dim3 gc((uint32_t)nchunks), gt(nbig), bt(kSahThreads);
             hipLaunchKernelGGL(k_big_bin, gc, bt, 0, c->stream, a);
             hipLaunchKernelGGL(k_big_plane, gt, dim3(kPlaneThreads), 0, c->stream, a);
-            hipLaunchKernelGGL(k_big_count, gc, bt, 0, c->stream, a);
             hipLaunchKernelGGL(k_big_scatter, gc, bt, 0, c->stream, a);
             if (a.part == 0) hipLaunchKernelGGL(k_big_swap, gc, bt, 0, c->stream, a);
             hipLaunchKernelGGL(k_big_emit, gt, bt, 0, c->stream, a);
